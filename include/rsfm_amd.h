@@ -1,0 +1,138 @@
+/*
+ * rsfm_amd.h - C ABI of the MI355X (gfx950) two-view matching hot path.
+ *
+ * Every entry point enqueues work on the caller's HIP stream and returns an
+ * int status (RSFM_OK = 0, negative on error; rsfm_last_error() gives the
+ * message of the calling thread's last failure). Where the reference throws
+ * (cudaCheck -> cudapp::Exception) or asserts, these return an error code; the
+ * C++ adapters in rsfm_amd.hpp turn a nonzero status back into an exception.
+ * The caller owns every buffer. Descriptors are 128 B (u8 x 128, AoS) or 64 B
+ * in the 4-bit mode; BestMatch is 8 B {u32 index; u32 distance}, index in the
+ * low word, exactly BestMatchImpl<uint32_t> of the reference.
+ * No entry point allocates device memory; the ones that need scratch take a
+ * caller workspace and have a *_workspace_size query.
+ */
+#ifndef RSFM_AMD_H
+#define RSFM_AMD_H
+#include <stddef.h>
+#include <stdint.h>
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RSFM_OK 0
+#define RSFM_ERR_INVALID (-1)
+#define RSFM_ERR_HIP (-2)
+#define RSFM_ERR_UNSUPPORTED (-3)
+
+/* message of this thread's last failed call ("" if none) */
+const char* rsfm_last_error(void);
+/* library version string */
+const char* rsfm_version(void);
+
+/* ---------------------------------------------------------------------------
+ * Types (identical layouts to the reference)
+ * ------------------------------------------------------------------------ */
+
+/* BestMatchImpl<uint32_t>, bruteForceMatch.h:91-96 */
+typedef struct rsfm_best_match {
+    uint32_t index;
+    uint32_t distance;
+} rsfm_best_match;
+
+/* BruteForceMatchTask<kU8|kU4,128,kL2>, bruteForceMatch.h:98-113 (56 B).
+ * sqrNormDescA/B are accepted for layout compatibility: the kernels derive
+ * the distance offsets from the descriptor bytes themselves, so results equal
+ * the reference's whenever the norms passed are the true sums of squares. */
+typedef struct rsfm_bf_task {
+    uint32_t nbDescA;
+    uint32_t nbDescB;
+    const void* descA;
+    const void* descB;
+    rsfm_best_match* bestMatchA;
+    rsfm_best_match* bestMatchB; /* may be NULL: query direction only */
+    const uint32_t* sqrNormDescA;
+    const uint32_t* sqrNormDescB;
+} rsfm_bf_task;
+
+/* ValidMatch, bruteForceMatch.h:120-125 */
+typedef struct rsfm_valid_match {
+    uint32_t idxQuery;
+    uint32_t idxTrain;
+    float distance;
+} rsfm_valid_match;
+
+/* top-2 task: no reference counterpart. bestMatchA/secondMatchA get the best
+ * and second-best train index per query (second = best over j != best j,
+ * lexicographic (L2^2, index)); bestMatchB the best query per train. */
+typedef struct rsfm_bf_top2_task {
+    rsfm_bf_task base;
+    rsfm_best_match* secondMatchA;
+} rsfm_bf_top2_task;
+
+/* ---------------------------------------------------------------------------
+ * Brute-force matching
+ * ------------------------------------------------------------------------ */
+
+/* replaces rsfm::launchPreCompSiftSqrNorm (bruteForceMatch.h:135,
+ * bruteForceMatchIMMA.cu:743-750): sqrNorm[i] = sum_k desc[i][k]^2 */
+int rsfm_sqrnorm_u8(uint32_t* sqrNorm, const void* desc, uint32_t nbDesc, hipStream_t stream);
+
+/* replaces rsfm::launchBruteForceMatchIMMA(const SiftBruteForceMatchTask*, ...)
+ * (bruteForceMatch.h:136, bruteForceMatchIMMA.cu:626-633): bidirectional top-1
+ * L2^2, lowest index on ties; outputs start at {~0u, UINT32_MAX}. Any number
+ * of tasks (internally batched). */
+int rsfm_bf_match_u8(const rsfm_bf_task* tasks, size_t nbTasks, hipStream_t stream);
+
+/* replaces rsfm::launchBruteForceMatchIDP (bruteForceMatch.h:118,
+ * bruteForceMatch.cu:615-620): same contract on the vector ALUs (v_dot4),
+ * no matrix cores. */
+int rsfm_bf_match_idp_u8(const rsfm_bf_task* tasks, size_t nbTasks, hipStream_t stream);
+
+/* replaces rsfm::launchSiftDesc8bTo4b (bruteForceMatch.h:133,
+ * bruteForceMatchIMMA.cu:663-667): keep the high nibble of each byte,
+ * dst word i = ((src[2i] & 0xF0F0F0F0) >> 4) | (src[2i+1] & 0xF0F0F0F0) */
+int rsfm_desc8b_to_4b(void* dst, const void* src, uint32_t nbDesc, hipStream_t stream);
+
+/* replaces rsfm::launchPreCompSift4bSqrNorm (bruteForceMatch.h:134) */
+int rsfm_sqrnorm_u4(uint32_t* sqrNorm, const void* desc4, uint32_t nbDesc, hipStream_t stream);
+
+/* replaces rsfm::launchBruteForceMatchIMMA(const Sift4bBruteForceMatchTask*, ...)
+ * (bruteForceMatch.h:134): descA/descB point to 64-B 4-bit descriptors */
+int rsfm_bf_match_u4(const rsfm_bf_task* tasks, size_t nbTasks, hipStream_t stream);
+
+/* top-2 query side + top-1 train side (new). Needs a workspace of
+ * rsfm_bf_match_top2_workspace_size() bytes, 256-B aligned. */
+int rsfm_bf_match_top2_workspace_size(const rsfm_bf_top2_task* tasks, size_t nbTasks, size_t* bytes);
+int rsfm_bf_match_top2(const rsfm_bf_top2_task* tasks, size_t nbTasks, void* workspace,
+                       size_t workspaceBytes, hipStream_t stream);
+
+/* ---------------------------------------------------------------------------
+ * Match filtering
+ * ------------------------------------------------------------------------ */
+
+/* host: crossCheckMatches / removeMatchConflicts (bruteForceMatch.h:126-128,
+ * bruteForceMatch.cpp:23-66). out must hold min(nQ, nT) entries; returns the
+ * number written. */
+size_t rsfm_cross_check_host(const rsfm_best_match* fwd, uint32_t nbQueryDesc,
+                             const rsfm_best_match* bwd, uint32_t nbTrainDesc, rsfm_valid_match* out);
+size_t rsfm_remove_conflicts_host(const rsfm_best_match* fwd, uint32_t nbQueryDesc,
+                                  uint32_t nbTrainDesc, rsfm_valid_match* out);
+
+/* device: keep query i iff fwd[i] valid, (bwd == NULL or bwd[fwd[i].index]
+ * .index == i) and (second == NULL or den^2 d1 < num^2 d2, the exact integer
+ * ratio test sqrt(d1)/sqrt(d2) < num/den). Output in ascending query order,
+ * count written to *count (device pointer). out holds nbQueryDesc entries.
+ * workspace: rsfm_filter_matches_workspace_size() bytes. */
+int rsfm_filter_matches_workspace_size(uint32_t nbQueryDesc, size_t* bytes);
+int rsfm_filter_matches(const rsfm_best_match* fwd, const rsfm_best_match* second,
+                        uint32_t nbQueryDesc, const rsfm_best_match* bwd, uint32_t nbTrainDesc,
+                        uint32_t ratioNum, uint32_t ratioDen, rsfm_valid_match* out, uint32_t* count,
+                        void* workspace, size_t workspaceBytes, hipStream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

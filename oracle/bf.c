@@ -1,0 +1,237 @@
+/*
+ * bf.c - exact brute-force matching, cross-check and conflict removal.
+ * TEST INFRASTRUCTURE ONLY (see rsfm_oracle.h).
+ *
+ * Restates:
+ *  - kernelPreCompSift8bSqrNorm   bruteForceMatchIMMA.cu:704-731  (sum of squares)
+ *  - kernelSiftDesc8bTo4b         bruteForceMatchIMMA.cu:637-660  (high-nibble pack)
+ *  - kernelPreCompSift4bSqrNorm   bruteForceMatchIMMA.cu:671-701
+ *  - kernelBruteForceMatchIMMA    bruteForceMatchIMMA.cu:551-586 with the packed
+ *    keys of bruteForceMatch.cuh:108-261: per descriptor the lexicographic
+ *    minimum of (L2^2 distance, index), i.e. the lowest index wins ties;
+ *    results start at {~0u, UINT32_MAX} (bruteForceMatch.cuh:293-305).
+ *  - crossCheckMatches / removeMatchConflicts   bruteForceMatch.cpp:23-66
+ * New (no reference counterpart): top-2 for the query side, the exact
+ * integer ratio test and the dot-product ranking mode.
+ *
+ * Pinning: distances are the closed form sum (a_k - b_k)^2, checked in
+ * tests against numpy brute force (np.einsum over int64) including planted
+ * ties; the sampled optimality check of
+ * test/bruteForceMatchKernelTest.cpp:140-171 is mirrored as a property test.
+ */
+#include "rsfm_oracle.h"
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+void or_sqrnorm_u8(const uint8_t* desc, uint32_t n, uint32_t* out)
+{
+    for (uint32_t i = 0; i < n; i++) {
+        uint32_t s = 0;
+        for (int k = 0; k < 128; k++) s += (uint32_t)desc[(size_t)i * 128 + k] * desc[(size_t)i * 128 + k];
+        out[i] = s;
+    }
+}
+
+void or_desc8b_to4b(const uint8_t* src, uint32_t n, uint8_t* dst)
+{
+    for (uint32_t d = 0; d < n; d++) {
+        const uint8_t* s = src + (size_t)d * 128;
+        uint8_t* o = dst + (size_t)d * 64;
+        /* dst word i (4 bytes) from src words 2i and 2i+1 */
+        for (int i = 0; i < 16; i++)
+            for (int b = 0; b < 4; b++) {
+                const uint8_t lo = (uint8_t)(s[8 * i + b] >> 4);
+                const uint8_t hi = (uint8_t)(s[8 * i + 4 + b] & 0xF0);
+                o[4 * i + b] = (uint8_t)(lo | hi);
+            }
+    }
+}
+
+void or_sqrnorm_u4(const uint8_t* desc4, uint32_t n, uint32_t* out)
+{
+    for (uint32_t i = 0; i < n; i++) {
+        uint32_t s = 0;
+        for (int k = 0; k < 64; k++) {
+            const uint32_t v = desc4[(size_t)i * 64 + k];
+            s += (v & 15) * (v & 15) + (v >> 4) * (v >> 4);
+        }
+        out[i] = s;
+    }
+}
+
+static inline uint32_t l2_u8(const uint8_t* a, const uint8_t* b)
+{
+    int32_t s = 0;
+    for (int k = 0; k < 128; k++) {
+        const int32_t d = (int32_t)a[k] - (int32_t)b[k];
+        s += d * d;
+    }
+    return (uint32_t)s;
+}
+
+static inline uint32_t l2_u4(const uint8_t* a, const uint8_t* b)
+{
+    int32_t s = 0;
+    for (int k = 0; k < 64; k++) {
+        const int32_t d0 = (int32_t)(a[k] & 15) - (int32_t)(b[k] & 15);
+        const int32_t d1 = (int32_t)(a[k] >> 4) - (int32_t)(b[k] >> 4);
+        s += d0 * d0 + d1 * d1;
+    }
+    return (uint32_t)s;
+}
+
+static void set_threads(int nthreads)
+{
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+    (void)nthreads;
+#endif
+}
+
+typedef uint32_t (*dist_fn)(const uint8_t*, const uint8_t*);
+
+static void bf_generic(const uint8_t* A, uint32_t nA, const uint8_t* B, uint32_t nB, size_t stride,
+                       dist_fn dist, or_best_match* bestA, or_best_match* secondA,
+                       or_best_match* bestB, int nthreads)
+{
+    set_threads(nthreads);
+    /* A direction (and optional second best), parallel over rows */
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t i = 0; i < (int64_t)nA; i++) {
+        uint32_t d1 = UINT32_MAX, j1 = ~0u, d2 = UINT32_MAX, j2 = ~0u;
+        for (uint32_t j = 0; j < nB; j++) {
+            const uint32_t d = dist(A + (size_t)i * stride, B + (size_t)j * stride);
+            if (d < d1) {
+                d2 = d1; j2 = j1; d1 = d; j1 = j;
+            } else if (d < d2) {
+                d2 = d; j2 = j;
+            }
+        }
+        bestA[i].index = j1;
+        bestA[i].distance = d1;
+        if (secondA) {
+            secondA[i].index = j2;
+            secondA[i].distance = d2;
+        }
+    }
+    if (!bestB) return;
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t j = 0; j < (int64_t)nB; j++) {
+        uint32_t d1 = UINT32_MAX, i1 = ~0u;
+        for (uint32_t i = 0; i < nA; i++) {
+            const uint32_t d = dist(A + (size_t)i * stride, B + (size_t)j * stride);
+            if (d < d1) { d1 = d; i1 = i; }
+        }
+        bestB[j].index = i1;
+        bestB[j].distance = d1;
+    }
+}
+
+void or_bf_match_u8(const uint8_t* A, uint32_t nA, const uint8_t* B, uint32_t nB,
+                    or_best_match* bestA, or_best_match* bestB, int nthreads)
+{
+    bf_generic(A, nA, B, nB, 128, l2_u8, bestA, NULL, bestB, nthreads);
+}
+
+void or_bf_match_u4(const uint8_t* A4, uint32_t nA, const uint8_t* B4, uint32_t nB,
+                    or_best_match* bestA, or_best_match* bestB, int nthreads)
+{
+    bf_generic(A4, nA, B4, nB, 64, l2_u4, bestA, NULL, bestB, nthreads);
+}
+
+void or_bf_match_top2_u8(const uint8_t* A, uint32_t nA, const uint8_t* B, uint32_t nB,
+                         or_best_match* bestA, or_best_match* secondA, or_best_match* bestB,
+                         int nthreads)
+{
+    bf_generic(A, nA, B, nB, 128, l2_u8, bestA, secondA, bestB, nthreads);
+}
+
+void or_bf_match_dot_top2_u8(const uint8_t* A, uint32_t nA, const uint8_t* B, uint32_t nB,
+                             int32_t* best_dot, uint32_t* best_idx, int32_t* second_dot,
+                             uint32_t* second_idx, int nthreads)
+{
+    set_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t i = 0; i < (int64_t)nA; i++) {
+        int32_t d1 = INT32_MIN, d2 = INT32_MIN;
+        uint32_t j1 = ~0u, j2 = ~0u;
+        for (uint32_t j = 0; j < nB; j++) {
+            int32_t d = 0;
+            for (int k = 0; k < 128; k++) d += (int32_t)A[(size_t)i * 128 + k] * (int32_t)B[(size_t)j * 128 + k];
+            if (d > d1) {
+                d2 = d1; j2 = j1; d1 = d; j1 = j;
+            } else if (d > d2) {
+                d2 = d; j2 = j;
+            }
+        }
+        best_dot[i] = d1; best_idx[i] = j1;
+        second_dot[i] = d2; second_idx[i] = j2;
+    }
+}
+
+uint32_t or_cross_check(const or_best_match* fwd, uint32_t nQ, const or_best_match* bwd,
+                        uint32_t nT, or_valid_match* out)
+{
+    uint32_t cnt = 0;
+    for (uint32_t i = 0; i < nQ; i++) {
+        const uint32_t t = fwd[i].index;
+        if (t == ~0u) continue;
+        if (t >= nT) continue;
+        if (bwd[t].index == i) {
+            out[cnt].idxQuery = i;
+            out[cnt].idxTrain = t;
+            out[cnt].distance = (float)fwd[i].distance;
+            cnt++;
+        }
+    }
+    return cnt;
+}
+
+uint32_t or_remove_conflicts(const or_best_match* fwd, uint32_t nQ, uint32_t nT, or_valid_match* out)
+{
+    or_best_match* bwd = (or_best_match*)malloc(sizeof(or_best_match) * (nT ? nT : 1));
+    for (uint32_t j = 0; j < nT; j++) { bwd[j].index = ~0u; bwd[j].distance = UINT32_MAX; }
+    for (uint32_t i = 0; i < nQ; i++) {
+        const uint32_t t = fwd[i].index;
+        if (t == ~0u || t >= nT) continue;
+        if (fwd[i].distance < bwd[t].distance) { bwd[t].index = i; bwd[t].distance = fwd[i].distance; }
+    }
+    uint32_t cnt = 0;
+    for (uint32_t j = 0; j < nT; j++) {
+        if (bwd[j].index != ~0u) {
+            out[cnt].idxQuery = bwd[j].index;
+            out[cnt].idxTrain = j;
+            out[cnt].distance = (float)bwd[j].distance;
+            cnt++;
+        }
+    }
+    free(bwd);
+    return cnt;
+}
+
+int or_ratio_pass(uint32_t d1, uint32_t d2, uint32_t num, uint32_t den)
+{
+    return (uint64_t)den * den * d1 < (uint64_t)num * num * d2;
+}
+
+uint32_t or_filter_matches(const or_best_match* fwd, const or_best_match* second, uint32_t nQ,
+                           const or_best_match* bwd, uint32_t nT, uint32_t ratio_num,
+                           uint32_t ratio_den, or_valid_match* out)
+{
+    uint32_t cnt = 0;
+    for (uint32_t i = 0; i < nQ; i++) {
+        const uint32_t t = fwd[i].index;
+        if (t == ~0u || t >= nT) continue;
+        if (bwd && bwd[t].index != i) continue;
+        if (second && !or_ratio_pass(fwd[i].distance, second[i].distance, ratio_num, ratio_den)) continue;
+        out[cnt].idxQuery = i;
+        out[cnt].idxTrain = t;
+        out[cnt].distance = (float)fwd[i].distance;
+        cnt++;
+    }
+    return cnt;
+}

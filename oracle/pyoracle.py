@@ -1,0 +1,223 @@
+"""numpy/ctypes front end of the CPU oracle (liborsfm.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py. The product (rapidsfm_amd) never imports it.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liborsfm.so")
+REF_SAMPLER = os.path.join(HERE, "_ref", "ref_sampler")
+
+
+def build(ref_tree="/root/reference"):
+    """Compile the oracle (and, when the reference tree is present, _ref/)."""
+    subprocess.run(["make", "-s", "-C", HERE, "liborsfm.so"], check=True)
+    if os.path.exists(os.path.join(ref_tree, "ransac.hpp")):
+        subprocess.run(["make", "-s", "-C", HERE, "ref", "REF=" + ref_tree], check=True)
+
+
+if not os.path.exists(LIB_PATH):
+    build()
+_lib = ctypes.CDLL(LIB_PATH)
+
+P = ctypes.c_void_p
+U32 = ctypes.c_uint32
+U64 = ctypes.c_uint64
+I32 = ctypes.c_int
+F32 = ctypes.c_float
+F64 = ctypes.c_double
+
+
+def _decl(name, res, args):
+    f = getattr(_lib, name)
+    f.restype = res
+    f.argtypes = args
+    return f
+
+
+class MT64(ctypes.Structure):
+    _fields_ = [("mt", ctypes.c_uint64 * 312), ("mti", ctypes.c_int)]
+
+
+_decl("or_mt64_seed", None, [ctypes.POINTER(MT64), U64])
+_decl("or_mt64_next", U64, [ctypes.POINTER(MT64)])
+_decl("or_mt64_normal", F64, [ctypes.POINTER(MT64)])
+_decl("or_lcg_step", I32, [ctypes.POINTER(ctypes.c_int32)])
+_decl("or_gen_sift_pair", None, [U64, U32, U32, U32, F64, P, P, P])
+_decl("or_gen_random_u8", None, [U64, U32, P])
+_decl("or_gen_two_view", None, [U64, U32, U32, F64, F64, F64, F64, P, P, P])
+_decl("or_sample_indices", None, [U64, U32, U32, U32, P])
+_decl("or_sqrnorm_u8", None, [P, U32, P])
+_decl("or_desc8b_to4b", None, [P, U32, P])
+_decl("or_sqrnorm_u4", None, [P, U32, P])
+_decl("or_bf_match_u8", None, [P, U32, P, U32, P, P, I32])
+_decl("or_bf_match_u4", None, [P, U32, P, U32, P, P, I32])
+_decl("or_bf_match_top2_u8", None, [P, U32, P, U32, P, P, P, I32])
+_decl("or_bf_match_dot_top2_u8", None, [P, U32, P, U32, P, P, P, P, I32])
+_decl("or_cross_check", U32, [P, U32, P, U32, P])
+_decl("or_remove_conflicts", U32, [P, U32, U32, P])
+_decl("or_ratio_pass", I32, [U32, U32, U32, U32])
+_decl("or_filter_matches", U32, [P, P, U32, P, U32, U32, U32, P])
+_decl("or_cashash_accumulate", None, [P, U32, P])
+_decl("or_cashash_compute_bias", None, [P, P, P, U64, U32, P])
+_decl("or_cashash_compute_hash", None, [P, P, U32, P, U32, P, P])
+_decl("or_cashash_histogram", None, [P, U32, P])
+_decl("or_cashash_bucket_bounds", U32, [P, P, P])
+_decl("or_cashash_build_buckets", None, [P, U32, P, P])
+_decl("or_cashash_find_best", None, [P, P, U32, P, P, U32, U32, P, P])
+_decl("or_cashash_find_best_rerank", None, [P, P, P, U32, P, P, P, U32, U32, U32, P])
+_decl("or_cashash_gaussian_weights", None, [U64, U32, P, P])
+_decl("or_rmf_bin", U32, [I32, I32, U32, P, U32, P, P, P, P, P, P])
+_decl("or_rmf_filter", None, [P, P, P, U32, P, U32, U32, U32, F32, I32, U32])
+_decl("or_solve_gauss3", None, [P, P, P])
+_decl("or_f8_fit_f32", I32, [P, P])
+_decl("or_f8_fit_f64", I32, [P, P])
+_decl("or_f_inlier_f32", I32, [P, P, F32, I32])
+_decl("or_f_inlier_f64", I32, [P, P, F64, I32])
+_decl("or_f8_score", None, [P, U32, P, U32, F32, I32, F32, F32, P, P, I32])
+_decl("or_f8_ransac_adaptive", U32, [P, U32, F32, I32, F32, U32, U64, F32, F32, P, P])
+_decl("or_ransac_loop_hash", U32, [U32, F32, U32, U64, U32, P, P])
+_decl("or_ransac_required_iters", U32, [U32, U32, U32, F64, U32])
+_decl("or_mlp_forward", None, [P, P, U32, P])
+_decl("or_mlp_loss_grad", F64, [P, P, P, U32, P])
+
+MLP_NPARAM = 64 * 8 + 64 + 64 * 64 + 64 + 64 * 64 + 64 + 64 + 1
+
+
+def _p(a):
+    return a.ctypes.data_as(P) if a is not None else None
+
+
+def _c(a, dt):
+    return np.ascontiguousarray(a, dtype=dt)
+
+
+# ---------------- RNG ----------------
+def mt64_stream(seed, n):
+    s = MT64()
+    _lib.or_mt64_seed(ctypes.byref(s), seed)
+    return np.array([_lib.or_mt64_next(ctypes.byref(s)) for _ in range(n)], dtype=np.uint64)
+
+
+def mt64_normals(seed, n):
+    s = MT64()
+    _lib.or_mt64_seed(ctypes.byref(s), seed)
+    return np.array([_lib.or_mt64_normal(ctypes.byref(s)) for _ in range(n)])
+
+
+def lcg_stream(state, n):
+    st = ctypes.c_int32(state)
+    return np.array([_lib.or_lcg_step(ctypes.byref(st)) for _ in range(n)], dtype=np.int64)
+
+
+# ---------------- generators ----------------
+def gen_sift_pair(seed, n_train, n_query, n_true, noise_sigma=0.05):
+    train = np.zeros((n_train, 128), np.uint8)
+    query = np.zeros((n_query, 128), np.uint8)
+    gt = np.zeros(n_query, np.int32)
+    _lib.or_gen_sift_pair(seed, n_train, n_query, n_true, noise_sigma, _p(train), _p(query), _p(gt))
+    return train, query, gt
+
+
+def gen_random_u8(seed, n):
+    d = np.zeros((n, 128), np.uint8)
+    _lib.or_gen_random_u8(seed, n, _p(d))
+    return d
+
+
+def gen_two_view(seed, n, n_out, focal=2000.0, width=4000.0, height=3000.0, noise_px=0.5):
+    pts = np.zeros((n, 4), np.float32)
+    inl = np.zeros(n, np.uint8)
+    F = np.zeros(9, np.float64)
+    _lib.or_gen_two_view(seed, n, n_out, focal, width, height, noise_px, _p(pts), _p(inl), _p(F))
+    return pts, inl.astype(bool), F.reshape(3, 3)
+
+
+def sample_indices(seed, n, H, k=8):
+    idx = np.zeros((H, k), np.uint32)
+    _lib.or_sample_indices(seed, n, k, H, _p(idx))
+    return idx
+
+
+# ---------------- brute force ----------------
+def sqrnorm_u8(desc):
+    desc = _c(desc, np.uint8)
+    out = np.zeros(desc.shape[0], np.uint32)
+    _lib.or_sqrnorm_u8(_p(desc), desc.shape[0], _p(out))
+    return out
+
+
+def desc8b_to4b(desc):
+    desc = _c(desc, np.uint8)
+    out = np.zeros((desc.shape[0], 64), np.uint8)
+    _lib.or_desc8b_to4b(_p(desc), desc.shape[0], _p(out))
+    return out
+
+
+def sqrnorm_u4(desc4):
+    desc4 = _c(desc4, np.uint8)
+    out = np.zeros(desc4.shape[0], np.uint32)
+    _lib.or_sqrnorm_u4(_p(desc4), desc4.shape[0], _p(out))
+    return out
+
+
+def _bm(n):
+    return np.zeros((n, 2), np.uint32)
+
+
+def bf_match_u8(A, B, nthreads=0):
+    A = _c(A, np.uint8)
+    B = _c(B, np.uint8)
+    ba, bb = _bm(A.shape[0]), _bm(B.shape[0])
+    _lib.or_bf_match_u8(_p(A), A.shape[0], _p(B), B.shape[0], _p(ba), _p(bb), nthreads)
+    return ba, bb
+
+
+def bf_match_u4(A4, B4, nthreads=0):
+    A4 = _c(A4, np.uint8)
+    B4 = _c(B4, np.uint8)
+    ba, bb = _bm(A4.shape[0]), _bm(B4.shape[0])
+    _lib.or_bf_match_u4(_p(A4), A4.shape[0], _p(B4), B4.shape[0], _p(ba), _p(bb), nthreads)
+    return ba, bb
+
+
+def bf_match_top2_u8(A, B, nthreads=0):
+    A = _c(A, np.uint8)
+    B = _c(B, np.uint8)
+    ba, sa, bb = _bm(A.shape[0]), _bm(A.shape[0]), _bm(B.shape[0])
+    _lib.or_bf_match_top2_u8(_p(A), A.shape[0], _p(B), B.shape[0], _p(ba), _p(sa), _p(bb), nthreads)
+    return ba, sa, bb
+
+
+def cross_check(fwd, bwd):
+    fwd = _c(fwd, np.uint32)
+    bwd = _c(bwd, np.uint32)
+    out = np.zeros((max(1, fwd.shape[0]), 3), np.uint32)
+    n = _lib.or_cross_check(_p(fwd), fwd.shape[0], _p(bwd), bwd.shape[0], _p(out))
+    return out[:n]
+
+
+def remove_conflicts(fwd, n_train):
+    fwd = _c(fwd, np.uint32)
+    out = np.zeros((max(1, fwd.shape[0]), 3), np.uint32)
+    n = _lib.or_remove_conflicts(_p(fwd), fwd.shape[0], n_train, _p(out))
+    return out[:n]
+
+
+def filter_matches(fwd, second, bwd, ratio=(4, 5)):
+    fwd = _c(fwd, np.uint32)
+    second = _c(second, np.uint32) if second is not None else None
+    bwd = _c(bwd, np.uint32) if bwd is not None else None
+    out = np.zeros((max(1, fwd.shape[0]), 3), np.uint32)
+    nt = bwd.shape[0] if bwd is not None else 0xFFFFFFFF
+    n = _lib.or_filter_matches(_p(fwd), _p(second), fwd.shape[0], _p(bwd), nt, ratio[0], ratio[1], _p(out))
+    return out[:n]
+
+
+def ratio_pass(d1, d2, num=4, den=5):
+    return bool(_lib.or_ratio_pass(d1, d2, num, den))

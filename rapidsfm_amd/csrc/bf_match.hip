@@ -1,0 +1,549 @@
+// bf_match.hip - brute-force SIFT matching on gfx950 int8 matrix cores.
+//
+// Contract (reference: kernelBruteForceMatchIMMA, bruteForceMatchIMMA.cu:551-586,
+// packed keys bruteForceMatch.cuh:108-261): for every query i the lexicographic
+// minimum of (L2^2(i,j), j) over all train j, and for every train j the minimum
+// of (L2^2(i,j), i) over all queries i. Distances are exact integers.
+//
+// Design (own, for CDNA4):
+//  * u8 descriptors are mapped to signed bytes u = 127 - a (rows) and
+//    w = b - 128 (columns) so one v_mfma_i32_32x32x32_i8 chain gives
+//    dot = u.w and  L2^2 = NA' + NB' + 2 dot  with NA' = |a-128|^2 and
+//    NB' = |b-127|^2 - 128 (both in [-128, 2^21]).
+//  * A workgroup keeps 256 query rows resident in registers (kRB 32-row MFMA
+//    blocks per wave, K = 128 = 4 MFMAs each) and streams the train
+//    descriptors of its column segment through LDS in 64-column chunks
+//    (double buffered, XOR-swizzled 16-B slots).
+//  * Epilogue per accumulator: one v_lshl_add makes the row key
+//    ((NB'_j + 2dot) << 8 | column-block tag), one makes the column key
+//    ((NA'_i + 2dot) << 8 | local row); v_min3 folds top-1, v_med3 + v_min
+//    fold top-2. Row state lives in registers for the whole segment and is
+//    reduced at the end through an LDS transpose (one thread per row).
+//    Column partials are reduced across the waves through LDS and merged
+//    globally with one 64-bit atomicMin per column per workgroup.
+//  * Segments split the train range when one launch has too few row blocks
+//    to fill 256 CUs; row results then merge by 64-bit atomicMin (top-1) or
+//    through a small workspace and a merge kernel (top-2).
+#include "common.hpp"
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+namespace rsfm_amd {
+namespace bf {
+
+constexpr int kBM = 256;                // query rows per workgroup
+constexpr int kBN = 64;                 // train columns per staged chunk
+constexpr uint32_t kMaxSegCols = 8192;  // 256 blocks of 32: the 8-bit tag of a row key
+constexpr int kMaxTasks = 32;
+constexpr int32_t kNone = INT32_MAX;
+
+struct DevTask {
+    uint32_t nA, nB;
+    const uint8_t* A;
+    const uint8_t* B;
+    uint64_t* bestA;
+    uint64_t* secondA;  // top-2 only
+    uint64_t* bestB;    // may be null
+    uint64_t* part;     // top-2 with several segments: [nSeg][nA][2]
+};
+struct TaskBatch {
+    DevTask t[kMaxTasks];
+};
+
+// 16 bytes of the (expanded) descriptor at byte offset off of the 128-value row
+template <int kBits>
+__device__ inline v4i load16(const uint8_t* desc, int off)
+{
+    if constexpr (kBits == 8) {
+        return *reinterpret_cast<const v4i*>(desc + off);
+    } else {
+        // 4-bit: value 2b is the low nibble of byte b, value 2b+1 the high one
+        const uint2 x = *reinterpret_cast<const uint2*>(desc + off / 2);
+        v4i r;
+        const uint32_t w[2] = {x.x, x.y};
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const uint32_t lo = w[h] & 0x0F0F0F0Fu;
+            const uint32_t hn = (w[h] >> 4) & 0x0F0F0F0Fu;
+            r[2 * h + 0] = (int)__builtin_amdgcn_perm(hn, lo, 0x05010400u);
+            r[2 * h + 1] = (int)__builtin_amdgcn_perm(hn, lo, 0x07030602u);
+        }
+        return r;
+    }
+}
+
+__device__ inline int32_t imin(int32_t a, int32_t b) { return a < b ? a : b; }
+__device__ inline int32_t imax(int32_t a, int32_t b) { return a > b ? a : b; }
+__device__ inline uint64_t umin64(uint64_t a, uint64_t b) { return a < b ? a : b; }
+__device__ inline uint64_t umax64(uint64_t a, uint64_t b) { return a > b ? a : b; }
+
+// (a << 9) + b in one instruction
+__device__ inline int32_t shl9_add(int32_t a, int32_t b)
+{
+    int32_t r;
+    asm("v_lshl_add_u32 %0, %1, 9, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+// top-2 fold of x into (m1 <= m2): m2 = med3(m1, m2, x), m1 = min(m1, x),
+// both in place so the loop-carried registers need no copies
+__device__ inline void top2_update(int32_t& m1, int32_t& m2, int32_t x)
+{
+    asm("v_med3_i32 %0, %1, %0, %2\n\tv_min_i32 %1, %1, %2" : "+v"(m2), "+v"(m1) : "v"(x));
+}
+
+template <int kBits, bool kTop2, int kWaves, int kRB>
+__global__ __launch_bounds__(kWaves * 64) void match_kernel(const TaskBatch batch, uint32_t segCols)
+{
+    static_assert(kWaves * kRB * 32 == kBM, "tile must cover 256 rows");
+    constexpr int kThreads = kWaves * 64;
+    constexpr int kDescBytes = kBits * 16;
+    constexpr int kStageBytes = kBN * 128 / kThreads;  // bytes of one chunk staged per thread
+    constexpr int kThrPerCol = 128 / kStageBytes;
+    static_assert(kStageBytes == 16 || kStageBytes == 32, "staging split");
+
+    const DevTask& T = batch.t[blockIdx.z];
+    const uint32_t rowBase = blockIdx.x * kBM;
+    const uint32_t colBeg = blockIdx.y * segCols;
+    if (rowBase >= T.nA || colBeg >= T.nB) return;
+    const uint32_t colEnd = min(colBeg + segCols, T.nB);
+    const uint32_t nChunks = (colEnd - colBeg + kBN - 1) / kBN;
+
+    // sB (double-buffered chunk) and the row-tail transpose share memory
+    constexpr int kTailStride = 33;
+    constexpr int kUnionBytes = (2 * kBN * 128 > kBM * kTailStride * 4) ? 2 * kBN * 128 : kBM * kTailStride * 4;
+    __shared__ __attribute__((aligned(16))) uint8_t sU[kUnionBytes];
+    __shared__ int32_t sNB[3][kBN];
+    __shared__ int32_t sCol[3][kWaves][kBN];
+    __shared__ int32_t sNA[kBM];
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int l32 = lane & 31, hi = lane >> 5;
+
+    // ---- query rows: operands u = 127 - a, offsets NA' = |a - 128|^2
+    v4i aop[kRB][4];
+    int32_t Q[kRB][16];
+#pragma unroll
+    for (int rb = 0; rb < kRB; rb++) {
+        const int rloc = (wave * kRB + rb) * 32 + l32;
+        const uint32_t row = rowBase + rloc;
+        const bool valid = row < T.nA;
+        int32_t nap = 0;
+#pragma unroll
+        for (int kc = 0; kc < 4; kc++) {
+            v4i x = {0, 0, 0, 0};
+            if (valid) x = load16<kBits>(T.A + (size_t)row * kDescBytes, kc * 32 + hi * 16);
+            v4i u;
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int xa = x[q] ^ (int)0x80808080u;
+                nap = __builtin_amdgcn_sdot4(xa, xa, nap, false);
+                u[q] = valid ? (x[q] ^ 0x7F7F7F7F) : 0;
+            }
+            aop[rb][kc] = u;
+        }
+        const int32_t na = nap + __shfl_xor(nap, 32);
+        if (hi == 0) sNA[rloc] = na;
+#pragma unroll
+        for (int v = 0; v < 16; v++) {
+            const int r = 8 * (v >> 2) + 4 * hi + (v & 3);
+            const int32_t nav = __shfl(na, r);
+            const int rl = (wave * kRB + rb) * 32 + r;
+            Q[rb][v] = rowBase + rl < T.nA ? (nav * 256 + rl) : kNone;
+        }
+    }
+
+    int32_t m1[kRB][16], m2[kRB][16];
+#pragma unroll
+    for (int rb = 0; rb < kRB; rb++)
+#pragma unroll
+        for (int v = 0; v < 16; v++) {
+            m1[rb][v] = kNone;
+            m2[rb][v] = kNone;
+        }
+
+    // ---- staging: thread -> (column tid / kThrPerCol, kStageBytes bytes)
+    const int sColIdx = tid / kThrPerCol, sPart = tid % kThrPerCol;
+    v4i xs[kStageBytes / 16];
+    auto stage_load = [&](uint32_t c) {
+        const uint32_t gcol = colBeg + c * kBN + sColIdx;
+#pragma unroll
+        for (int h = 0; h < kStageBytes / 16; h++) {
+            if (gcol < colEnd)
+                xs[h] = load16<kBits>(T.B + (size_t)gcol * kDescBytes, sPart * kStageBytes + h * 16);
+            else
+                xs[h] = v4i{0, 0, 0, 0};
+        }
+    };
+    auto stage_store = [&](uint32_t c) {
+        const uint32_t gcol = colBeg + c * kBN + sColIdx;
+        const bool valid = gcol < colEnd;
+        int32_t acc = 0;
+        uint8_t* base = sU + (c & 1) * (kBN * 128) + sColIdx * 128;
+#pragma unroll
+        for (int h = 0; h < kStageBytes / 16; h++) {
+            v4i w;
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                w[q] = valid ? (xs[h][q] ^ (int)0x80808080u) : 0;
+                acc = __builtin_amdgcn_sdot4(w[q], w[q], acc, false);
+                acc = __builtin_amdgcn_sdot4(w[q], 0x02020202, acc, false);
+            }
+            const int s = sPart * (kStageBytes / 16) + h;
+            *reinterpret_cast<v4i*>(base + ((s ^ (sColIdx & 7)) << 4)) = w;
+        }
+#pragma unroll
+        for (int m = 1; m < kThrPerCol; m <<= 1) acc += __shfl_xor(acc, m);
+        // sum (w^2 + 2w) over the 128 values = |b - 127|^2 - 128 = NB'
+        if (sPart == 0) sNB[c % 3][sColIdx] = acc;
+    };
+
+    stage_load(0);
+    stage_store(0);
+    __syncthreads();
+
+    for (uint32_t c = 0; c < nChunks; c++) {
+        const bool more = c + 1 < nChunks;
+        if (more) stage_load(c + 1);
+        const uint8_t* sb = sU + (c & 1) * (kBN * 128);
+        const int32_t* nb = sNB[c % 3];
+        int32_t colOut[2];
+        v16i acc[2][kRB];
+#pragma unroll
+        for (int cb = 0; cb < 2; cb++) {
+            const int cl = cb * 32 + l32;
+            v4i bop[4];
+#pragma unroll
+            for (int kc = 0; kc < 4; kc++) {
+                const int s = 2 * kc + hi;
+                bop[kc] = *reinterpret_cast<const v4i*>(sb + cl * 128 + ((s ^ (cl & 7)) << 4));
+            }
+#pragma unroll
+            for (int rb = 0; rb < kRB; rb++) {
+                acc[cb][rb] = v16i{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+                for (int kc = 0; kc < 4; kc++)
+                    acc[cb][rb] = __builtin_amdgcn_mfma_i32_32x32x32_i8(aop[rb][kc], bop[kc], acc[cb][rb], 0, 0, 0);
+            }
+        }
+        int32_t P[2];
+#pragma unroll
+        for (int cb = 0; cb < 2; cb++) {
+            const int cl = cb * 32 + l32;
+            P[cb] = colBeg + c * kBN + cl < colEnd ? (nb[cl] * 256 + (int32_t)(c * 2 + cb)) : kNone;
+        }
+#pragma unroll
+        for (int cb = 0; cb < 2; cb++) {
+            int32_t cm = kNone;
+#pragma unroll
+            for (int rb = 0; rb < kRB; rb++)
+#pragma unroll
+                for (int v = 0; v < 16; v++) cm = imin(cm, shl9_add(acc[cb][rb][v], Q[rb][v]));
+            cm = imin(cm, __shfl_xor(cm, 32));
+            colOut[cb] = cm;
+        }
+#pragma unroll
+        for (int rb = 0; rb < kRB; rb++)
+#pragma unroll
+            for (int v = 0; v < 16; v++) {
+                const int32_t r0 = shl9_add(acc[0][rb][v], P[0]);
+                const int32_t r1 = shl9_add(acc[1][rb][v], P[1]);
+                if constexpr (kTop2) {
+                    top2_update(m1[rb][v], m2[rb][v], r0);
+                    top2_update(m1[rb][v], m2[rb][v], r1);
+                } else {
+                    m1[rb][v] = imin(imin(m1[rb][v], r0), r1);
+                }
+            }
+        sCol[c % 3][wave][hi * 32 + l32] = hi ? colOut[1] : colOut[0];
+        if (more) stage_store(c + 1);
+        __syncthreads();
+        if (wave == (int)(c % kWaves) && T.bestB != nullptr) {
+            int32_t k = sCol[c % 3][0][lane];
+#pragma unroll
+            for (int w = 1; w < kWaves; w++) k = imin(k, sCol[c % 3][w][lane]);
+            const uint32_t gcol = colBeg + c * kBN + lane;
+            if (gcol < colEnd && k != kNone) {
+                const uint32_t dist = (uint32_t)((k >> 8) + sNB[c % 3][lane]);
+                const uint32_t row = rowBase + (uint32_t)(k & 0xFF);
+                atomicMin(reinterpret_cast<unsigned long long*>(T.bestB + gcol), (unsigned long long)make_key(dist, row));
+            }
+        }
+    }
+
+    // ---- row results: transpose the per-lane (row, column-class) bests through LDS
+    int32_t* L = reinterpret_cast<int32_t*>(sU);
+    __syncthreads();
+#pragma unroll
+    for (int rb = 0; rb < kRB; rb++)
+#pragma unroll
+        for (int v = 0; v < 16; v++) {
+            const int rl = (wave * kRB + rb) * 32 + 8 * (v >> 2) + 4 * hi + (v & 3);
+            L[rl * kTailStride + l32] = m1[rb][v];
+        }
+    __syncthreads();
+    // thread tid owns local row tid
+    int32_t b1 = kNone, b2 = kNone;
+    int c1 = 0, c2 = 0;
+    if (tid < kBM) {
+        for (int cc = 0; cc < 32; cc++) {
+            const int32_t x = L[tid * kTailStride + cc];
+            if (x < b1) {
+                b2 = b1; c2 = c1;
+                b1 = x; c1 = cc;
+            } else if (x < b2) {
+                b2 = x; c2 = cc;
+            }
+        }
+    }
+    if constexpr (kTop2) {
+        __syncthreads();
+#pragma unroll
+        for (int rb = 0; rb < kRB; rb++)
+#pragma unroll
+            for (int v = 0; v < 16; v++) {
+                const int rl = (wave * kRB + rb) * 32 + 8 * (v >> 2) + 4 * hi + (v & 3);
+                L[rl * kTailStride + l32] = m2[rb][v];
+            }
+        __syncthreads();
+        if (tid < kBM) {
+            // second best = min(second best over classes, second best inside the winning class)
+            const int32_t x = L[tid * kTailStride + c1];
+            if (x < b2 || (x == b2 && c1 < c2)) {
+                b2 = x;
+                c2 = c1;
+            }
+        }
+    }
+    if (tid < kBM) {
+        const uint32_t row = rowBase + tid;
+        if (row < T.nA) {
+            const int32_t na = sNA[tid];
+            auto to_key = [&](int32_t m, int cls) -> uint64_t {
+                if (m == kNone) return kNoMatch;
+                return make_key((uint32_t)((m >> 8) + na), colBeg + (uint32_t)(m & 0xFF) * 32u + (uint32_t)cls);
+            };
+            const uint64_t k1 = to_key(b1, c1);
+            const bool direct = gridDim.y == 1;
+            if constexpr (kTop2) {
+                const uint64_t k2 = to_key(b2, c2);
+                if (direct) {
+                    T.bestA[row] = k1;
+                    T.secondA[row] = k2;
+                } else {
+                    uint64_t* p = T.part + ((size_t)blockIdx.y * T.nA + row) * 2;
+                    p[0] = k1;
+                    p[1] = k2;
+                }
+            } else {
+                if (direct)
+                    T.bestA[row] = k1;
+                else if (k1 != kNoMatch)
+                    atomicMin(reinterpret_cast<unsigned long long*>(T.bestA + row), (unsigned long long)k1);
+            }
+        }
+    }
+}
+
+__global__ void init_kernel(const TaskBatch batch, bool initA)
+{
+    const DevTask& T = batch.t[blockIdx.z];
+    const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (initA && x < T.nA) T.bestA[x] = kNoMatch;
+    if (T.bestB != nullptr && x < T.nB) T.bestB[x] = kNoMatch;
+}
+
+__global__ void merge_top2_kernel(const TaskBatch batch, uint32_t nSeg, uint32_t segCols)
+{
+    const DevTask& T = batch.t[blockIdx.z];
+    const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= T.nA) return;
+    uint64_t k1 = kNoMatch, k2 = kNoMatch;
+    // segments at or past this task's train count were never written
+    const uint32_t nUsed = min(nSeg, (T.nB + segCols - 1) / segCols);
+    for (uint32_t s = 0; s < nUsed; s++) {
+        const uint64_t* p = T.part + ((size_t)s * T.nA + row) * 2;
+        const uint64_t o1 = p[0], o2 = p[1];
+        k2 = umin64(umax64(k1, o1), umin64(k2, o2));
+        k1 = umin64(k1, o1);
+    }
+    T.bestA[row] = k1;
+    T.secondA[row] = k2;
+}
+
+// tile configuration of the matcher (waves per workgroup x 32-row blocks per wave)
+#ifndef RSFM_BF_WAVES
+#define RSFM_BF_WAVES 8
+#endif
+constexpr int kCfgWaves = RSFM_BF_WAVES;
+constexpr int kCfgRB = kBM / 32 / kCfgWaves;
+
+// Launch geometry shared by the size query and the launch.
+struct Plan {
+    uint32_t gridX = 0, nSeg = 1, segCols = kBN;
+    uint32_t maxA = 0, maxB = 0;
+};
+
+static Plan make_plan(const DevTask* t, size_t n)
+{
+    Plan p;
+    for (size_t i = 0; i < n; i++) {
+        p.maxA = std::max(p.maxA, t[i].nA);
+        p.maxB = std::max(p.maxB, t[i].nB);
+    }
+    p.gridX = div_up(p.maxA, kBM);
+    if (p.maxA == 0 || p.maxB == 0) return p;
+    // aim for >= 512 workgroups (2 per CU) but keep >= 1024 columns per segment
+    const uint32_t wg = p.gridX * (uint32_t)n;
+    uint32_t nSeg = div_up(512, wg);
+    nSeg = std::min(nSeg, std::max(1u, p.maxB / 1024));
+    nSeg = std::max(nSeg, div_up(p.maxB, kMaxSegCols));
+    uint32_t seg = div_up(p.maxB, nSeg);
+    seg = div_up(seg, kBN) * kBN;
+    p.segCols = std::min(seg, kMaxSegCols);
+    p.nSeg = div_up(p.maxB, p.segCols);
+    return p;
+}
+
+static int validate_task(const rsfm_bf_task& t, int bits)
+{
+    const size_t align = bits == 8 ? 16 : 8;
+    RSFM_RETURN_IF(t.nbDescA > 0 && (t.descA == nullptr || t.bestMatchA == nullptr), RSFM_ERR_INVALID,
+                   "bf task: descA/bestMatchA must be set when nbDescA > 0");
+    RSFM_RETURN_IF(t.nbDescB > 0 && t.descB == nullptr, RSFM_ERR_INVALID, "bf task: descB must be set when nbDescB > 0");
+    RSFM_RETURN_IF((reinterpret_cast<uintptr_t>(t.descA) | reinterpret_cast<uintptr_t>(t.descB)) % align != 0,
+                   RSFM_ERR_INVALID, "bf task: descriptor pointers must be 16-byte aligned (8 for 4-bit)");
+    RSFM_RETURN_IF((reinterpret_cast<uintptr_t>(t.bestMatchA) | reinterpret_cast<uintptr_t>(t.bestMatchB)) % 8 != 0,
+                   RSFM_ERR_INVALID, "bf task: result pointers must be 8-byte aligned");
+    return RSFM_OK;
+}
+
+static DevTask to_dev(const rsfm_bf_task& t)
+{
+    DevTask d{};
+    d.nA = t.nbDescA;
+    d.nB = t.nbDescB;
+    d.A = static_cast<const uint8_t*>(t.descA);
+    d.B = static_cast<const uint8_t*>(t.descB);
+    d.bestA = reinterpret_cast<uint64_t*>(t.bestMatchA);
+    d.bestB = reinterpret_cast<uint64_t*>(t.bestMatchB);
+    return d;
+}
+
+template <int kBits>
+static int launch_top1(const rsfm_bf_task* tasks, size_t nbTasks, hipStream_t stream)
+{
+    RSFM_RETURN_IF(nbTasks > 0 && tasks == nullptr, RSFM_ERR_INVALID, "bf match: tasks is null");
+    for (size_t i = 0; i < nbTasks; i++) RSFM_CHECK(validate_task(tasks[i], kBits));
+    for (size_t b = 0; b < nbTasks; b += kMaxTasks) {
+        const size_t n = std::min<size_t>(kMaxTasks, nbTasks - b);
+        TaskBatch batch{};
+        for (size_t i = 0; i < n; i++) batch.t[i] = to_dev(tasks[b + i]);
+        const Plan p = make_plan(batch.t, n);
+        const uint32_t initX = div_up(std::max(p.maxA, p.maxB), 256);
+        if (initX == 0) continue;
+        init_kernel<<<dim3(initX, 1, (uint32_t)n), 256, 0, stream>>>(batch, true);
+        RSFM_CHECK(check_launch("bf init"));
+        if (p.gridX == 0 || p.maxB == 0) continue;
+        match_kernel<kBits, false, kCfgWaves, kCfgRB>
+            <<<dim3(p.gridX, p.nSeg, (uint32_t)n), kCfgWaves * 64, 0, stream>>>(batch, p.segCols);
+        RSFM_CHECK(check_launch("bf match"));
+    }
+    return RSFM_OK;
+}
+
+static size_t top2_workspace(const rsfm_bf_top2_task* tasks, size_t nbTasks)
+{
+    size_t bytes = 0;
+    for (size_t b = 0; b < nbTasks; b += kMaxTasks) {
+        const size_t n = std::min<size_t>(kMaxTasks, nbTasks - b);
+        DevTask t[kMaxTasks];
+        for (size_t i = 0; i < n; i++) t[i] = to_dev(tasks[b + i].base);
+        const Plan p = make_plan(t, n);
+        if (p.nSeg <= 1) continue;
+        for (size_t i = 0; i < n; i++) bytes += align_up((size_t)p.nSeg * t[i].nA * 16, 256);
+    }
+    return bytes;
+}
+
+}  // namespace bf
+}  // namespace rsfm_amd
+
+using namespace rsfm_amd;
+
+extern "C" int rsfm_bf_match_u8(const rsfm_bf_task* tasks, size_t nbTasks, hipStream_t stream)
+{
+    return bf::launch_top1<8>(tasks, nbTasks, stream);
+}
+
+extern "C" int rsfm_bf_match_u4(const rsfm_bf_task* tasks, size_t nbTasks, hipStream_t stream)
+{
+    return bf::launch_top1<4>(tasks, nbTasks, stream);
+}
+
+extern "C" int rsfm_bf_match_top2_workspace_size(const rsfm_bf_top2_task* tasks, size_t nbTasks, size_t* bytes)
+{
+    RSFM_RETURN_IF(bytes == nullptr, RSFM_ERR_INVALID, "bytes is null");
+    RSFM_RETURN_IF(nbTasks > 0 && tasks == nullptr, RSFM_ERR_INVALID, "tasks is null");
+    *bytes = bf::top2_workspace(tasks, nbTasks);
+    return RSFM_OK;
+}
+
+extern "C" int rsfm_bf_match_top2(const rsfm_bf_top2_task* tasks, size_t nbTasks, void* workspace,
+                                  size_t workspaceBytes, hipStream_t stream)
+{
+    using namespace bf;
+    RSFM_RETURN_IF(nbTasks > 0 && tasks == nullptr, RSFM_ERR_INVALID, "bf top2: tasks is null");
+    for (size_t i = 0; i < nbTasks; i++) {
+        RSFM_CHECK(validate_task(tasks[i].base, 8));
+        RSFM_RETURN_IF(tasks[i].base.nbDescA > 0 && tasks[i].secondMatchA == nullptr, RSFM_ERR_INVALID,
+                       "bf top2: secondMatchA must be set when nbDescA > 0");
+        RSFM_RETURN_IF(reinterpret_cast<uintptr_t>(tasks[i].secondMatchA) % 8 != 0, RSFM_ERR_INVALID,
+                       "bf top2: secondMatchA must be 8-byte aligned");
+    }
+    const size_t need = top2_workspace(tasks, nbTasks);
+    RSFM_RETURN_IF(workspaceBytes < need, RSFM_ERR_INVALID,
+                   "bf top2: workspace too small (" + std::to_string(workspaceBytes) + " < " + std::to_string(need) + ")");
+    RSFM_RETURN_IF(need > 0 && (workspace == nullptr || reinterpret_cast<uintptr_t>(workspace) % 256 != 0),
+                   RSFM_ERR_INVALID, "bf top2: workspace must be non-null and 256-byte aligned");
+    uint8_t* ws = static_cast<uint8_t*>(workspace);
+    for (size_t b = 0; b < nbTasks; b += kMaxTasks) {
+        const size_t n = std::min<size_t>(kMaxTasks, nbTasks - b);
+        TaskBatch batch{};
+        for (size_t i = 0; i < n; i++) {
+            batch.t[i] = to_dev(tasks[b + i].base);
+            batch.t[i].secondA = reinterpret_cast<uint64_t*>(tasks[b + i].secondMatchA);
+        }
+        const Plan p = make_plan(batch.t, n);
+        if (p.nSeg > 1) {
+            for (size_t i = 0; i < n; i++) {
+                batch.t[i].part = reinterpret_cast<uint64_t*>(ws);
+                ws += align_up((size_t)p.nSeg * batch.t[i].nA * 16, 256);
+            }
+        }
+        const uint32_t initX = div_up(std::max(p.maxA, p.maxB), 256);
+        if (initX == 0) continue;
+        // with no train descriptors the query side must still read "no match"
+        init_kernel<<<dim3(initX, 1, (uint32_t)n), 256, 0, stream>>>(batch, true);
+        RSFM_CHECK(check_launch("bf top2 init"));
+        if (p.maxA > 0) {
+            TaskBatch b2 = batch;
+            for (size_t i = 0; i < n; i++) {
+                b2.t[i].bestA = batch.t[i].secondA;
+                b2.t[i].bestB = nullptr;
+            }
+            init_kernel<<<dim3(div_up(p.maxA, 256), 1, (uint32_t)n), 256, 0, stream>>>(b2, true);
+            RSFM_CHECK(check_launch("bf top2 init2"));
+        }
+        if (p.gridX == 0 || p.maxB == 0) continue;
+        match_kernel<8, true, kCfgWaves, kCfgRB>
+            <<<dim3(p.gridX, p.nSeg, (uint32_t)n), kCfgWaves * 64, 0, stream>>>(batch, p.segCols);
+        RSFM_CHECK(check_launch("bf top2 match"));
+        if (p.nSeg > 1) {
+            merge_top2_kernel<<<dim3(div_up(p.maxA, 256), 1, (uint32_t)n), 256, 0, stream>>>(batch, p.nSeg, p.segCols);
+            RSFM_CHECK(check_launch("bf top2 merge"));
+        }
+    }
+    return RSFM_OK;
+}

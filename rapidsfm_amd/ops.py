@@ -1,0 +1,115 @@
+"""Torch-tensor helpers over the C ABI (device memory and streams only).
+
+Descriptors are uint8 CUDA tensors of shape (n, 128) (or (n, 64) for 4-bit);
+best-match outputs are int32 tensors of shape (n, 2) holding the reference's
+BestMatch {index, distance} (as uint32 bit patterns; no match = (-1, -1)).
+"""
+import ctypes
+
+import torch
+
+from ._abi import BfTask, BfTop2Task, check, lib
+
+
+def _stream(stream):
+    if stream is None:
+        stream = torch.cuda.current_stream()
+    return ctypes.c_void_p(stream.cuda_stream)
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None and t.numel() > 0 else ctypes.c_void_p(0)
+
+
+def _check_desc(d, width=128):
+    if d.dtype != torch.uint8 or d.dim() != 2 or d.shape[1] != width or not d.is_cuda or not d.is_contiguous():
+        raise ValueError("descriptors must be contiguous CUDA uint8 tensors of shape (n, %d)" % width)
+
+
+def sqrnorm_u8(desc, stream=None):
+    _check_desc(desc)
+    out = torch.empty(desc.shape[0], dtype=torch.int32, device=desc.device)
+    check(lib.rsfm_sqrnorm_u8(_ptr(out), _ptr(desc), desc.shape[0], _stream(stream)), "rsfm_sqrnorm_u8")
+    return out
+
+
+def sqrnorm_u4(desc4, stream=None):
+    _check_desc(desc4, 64)
+    out = torch.empty(desc4.shape[0], dtype=torch.int32, device=desc4.device)
+    check(lib.rsfm_sqrnorm_u4(_ptr(out), _ptr(desc4), desc4.shape[0], _stream(stream)), "rsfm_sqrnorm_u4")
+    return out
+
+
+def desc8b_to_4b(desc, stream=None):
+    _check_desc(desc)
+    out = torch.empty((desc.shape[0], 64), dtype=torch.uint8, device=desc.device)
+    check(lib.rsfm_desc8b_to_4b(_ptr(out), _ptr(desc), desc.shape[0], _stream(stream)), "rsfm_desc8b_to_4b")
+    return out
+
+
+def bf_match(pairs, impl="mfma", bidirectional=True, stream=None):
+    """Top-1 matching of a list of (descA, descB) pairs in one call.
+
+    impl: "mfma" (rsfm_bf_match_u8), "idp" (rsfm_bf_match_idp_u8) or "u4"
+    (rsfm_bf_match_u4 on 64-byte 4-bit descriptors). Returns a list of
+    (bestA, bestB) int32 (n, 2) tensors (bestB None if not bidirectional).
+    """
+    width = 64 if impl == "u4" else 128
+    tasks = (BfTask * max(1, len(pairs)))()
+    outs = []
+    for i, (a, b) in enumerate(pairs):
+        _check_desc(a, width)
+        _check_desc(b, width)
+        ba = torch.empty((a.shape[0], 2), dtype=torch.int32, device=a.device)
+        bb = torch.empty((b.shape[0], 2), dtype=torch.int32, device=a.device) if bidirectional else None
+        tasks[i] = BfTask(a.shape[0], b.shape[0], _ptr(a), _ptr(b), _ptr(ba), _ptr(bb) if bb is not None else None, None, None)
+        outs.append((ba, bb))
+    fn = {"mfma": lib.rsfm_bf_match_u8, "idp": lib.rsfm_bf_match_idp_u8, "u4": lib.rsfm_bf_match_u4}[impl]
+    check(fn(ctypes.cast(tasks, ctypes.c_void_p), len(pairs), _stream(stream)), "bf_match(%s)" % impl)
+    return outs
+
+
+def bf_match_top2(pairs, stream=None, workspace=None):
+    """Top-2 query side + top-1 train side for a list of (descA, descB).
+
+    Returns a list of (bestA, secondA, bestB) int32 (n, 2) tensors.
+    """
+    tasks = (BfTop2Task * max(1, len(pairs)))()
+    outs = []
+    for i, (a, b) in enumerate(pairs):
+        _check_desc(a)
+        _check_desc(b)
+        ba = torch.empty((a.shape[0], 2), dtype=torch.int32, device=a.device)
+        sa = torch.empty((a.shape[0], 2), dtype=torch.int32, device=a.device)
+        bb = torch.empty((b.shape[0], 2), dtype=torch.int32, device=a.device)
+        tasks[i] = BfTop2Task(BfTask(a.shape[0], b.shape[0], _ptr(a), _ptr(b), _ptr(ba), _ptr(bb), None, None), _ptr(sa))
+        outs.append((ba, sa, bb))
+    tp = ctypes.cast(tasks, ctypes.c_void_p)
+    need = ctypes.c_size_t(0)
+    check(lib.rsfm_bf_match_top2_workspace_size(tp, len(pairs), ctypes.byref(need)), "top2 workspace size")
+    if need.value > 0 and (workspace is None or workspace.numel() < need.value):
+        dev = pairs[0][0].device
+        workspace = torch.empty(need.value + 256, dtype=torch.uint8, device=dev)
+    ws_ptr = 0
+    if need.value > 0:
+        ws_ptr = (workspace.data_ptr() + 255) // 256 * 256
+    check(lib.rsfm_bf_match_top2(tp, len(pairs), ctypes.c_void_p(ws_ptr), need.value, _stream(stream)), "bf_match_top2")
+    return outs
+
+
+def filter_matches(fwd, second, bwd, n_train, ratio=(4, 5), stream=None):
+    """Device cross-check (+ ratio test when second is given).
+
+    Returns (matches int32 (nQ, 3) [query, train, distance-float-bits], count tensor).
+    """
+    nq = fwd.shape[0]
+    out = torch.empty((max(1, nq), 3), dtype=torch.int32, device=fwd.device)
+    count = torch.zeros(1, dtype=torch.int32, device=fwd.device)
+    num, den = ratio
+    check(
+        lib.rsfm_filter_matches(_ptr(fwd), _ptr(second) if second is not None else None, nq,
+                                _ptr(bwd) if bwd is not None else None, n_train, num, den,
+                                _ptr(out), _ptr(count), None, 0, _stream(stream)),
+        "rsfm_filter_matches",
+    )
+    return out, count

@@ -1,0 +1,191 @@
+"""GPU parity of the brute-force matcher against the CPU oracle (bit-exact).
+
+Every case prints its name and sizes before touching the device.
+"""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print("[case]", *a, flush=True)
+
+
+def _np(t):
+    return t.cpu().numpy().view(np.uint32)
+
+
+def _to_dev(a, dev):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+
+
+def test_sqrnorm_and_4bit_precompute(rs, orc, dev):
+    _say("sqrnorm/8b->4b n=1037 full-range u8")
+    d = orc.gen_random_u8(11, 1037)
+    dd = _to_dev(d, dev)
+    assert np.array_equal(_np(rs.sqrnorm_u8(dd)), orc.sqrnorm_u8(d))
+    d4 = rs.desc8b_to_4b(dd)
+    ref4 = orc.desc8b_to4b(d)
+    assert np.array_equal(d4.cpu().numpy(), ref4)
+    assert np.array_equal(_np(rs.sqrnorm_u4(d4)), orc.sqrnorm_u4(ref4))
+
+
+@pytest.mark.parametrize("impl", ["mfma", "idp"])
+def test_config0_top1_bidirectional(rs, orc, dev, impl):
+    _say("config0 2048x2048 top1", impl)
+    train, query, gt = orc.gen_sift_pair(0, 2048, 2048, 819, 0.05)
+    ra, rb = orc.bf_match_u8(query, train)
+    (ga, gb), = rs.bf_match([(_to_dev(query, dev), _to_dev(train, dev))], impl=impl)
+    torch.cuda.synchronize()
+    assert np.array_equal(_np(ga), ra)
+    assert np.array_equal(_np(gb), rb)
+    # the noisy copies are found
+    hit = (ra[:, 0].astype(np.int64) == gt) & (gt >= 0)
+    assert hit.sum() > 0.95 * (gt >= 0).sum()
+
+
+def test_config0_top2_ratio_crosscheck(rs, orc, dev):
+    _say("config0 2048x2048 top2 + ratio 0.8 + cross-check")
+    train, query, _ = orc.gen_sift_pair(0, 2048, 2048, 819, 0.05)
+    ra, sa, rb = orc.bf_match_top2_u8(query, train)
+    (ga, gs, gb), = rs.bf_match_top2([(_to_dev(query, dev), _to_dev(train, dev))])
+    assert np.array_equal(_np(ga), ra)
+    assert np.array_equal(_np(gs), sa)
+    assert np.array_equal(_np(gb), rb)
+    ref = orc.filter_matches(ra, sa, rb, (4, 5))
+    out, cnt = rs.filter_matches(ga, gs, gb, 2048, (4, 5))
+    n = int(cnt.item())
+    assert n == ref.shape[0]
+    assert np.array_equal(out[:n].cpu().numpy().view(np.uint32), ref)
+    # cross-check only
+    ref_cc = orc.cross_check(ra, rb)
+    out2, cnt2 = rs.filter_matches(ga, None, gb, 2048)
+    n2 = int(cnt2.item())
+    assert n2 == ref_cc.shape[0]
+    assert np.array_equal(out2[:n2].cpu().numpy().view(np.uint32), ref_cc)
+
+
+@pytest.mark.parametrize("nA,nB", [(1, 1), (31, 65), (897, 1000), (257, 63), (300, 2049), (5, 9000)])
+@pytest.mark.parametrize("impl", ["mfma", "idp"])
+def test_ragged_full_range(rs, orc, dev, nA, nB, impl):
+    _say("ragged full-range u8", nA, nB, impl)
+    A = orc.gen_random_u8(100 + nA, nA)
+    B = orc.gen_random_u8(200 + nB, nB)
+    ra, rb = orc.bf_match_u8(A, B)
+    (ga, gb), = rs.bf_match([(_to_dev(A, dev), _to_dev(B, dev))], impl=impl)
+    assert np.array_equal(_np(ga), ra)
+    assert np.array_equal(_np(gb), rb)
+    if impl == "mfma":
+        qa, qs, qb = orc.bf_match_top2_u8(A, B)
+        (ha, hs, hb), = rs.bf_match_top2([(_to_dev(A, dev), _to_dev(B, dev))])
+        assert np.array_equal(_np(ha), qa)
+        assert np.array_equal(_np(hs), qs)
+        assert np.array_equal(_np(hb), qb)
+
+
+def test_extreme_values_and_ties(rs, orc, dev):
+    _say("extreme bytes 0/255 and planted duplicate descriptors")
+    rng = np.random.default_rng(5)
+    A = rng.choice(np.array([0, 255], np.uint8), size=(300, 128))
+    B = np.concatenate([A[::3], A[::3], rng.integers(0, 256, (50, 128), dtype=np.uint8), A[::5]])
+    A[7] = 255
+    B[3] = 255
+    B[40] = 255
+    ra, sa, rb = orc.bf_match_top2_u8(A, B)
+    # duplicates: the lowest index must win
+    assert ra[7, 0] == 3 and ra[7, 1] == 0 and sa[7, 0] == 40 and sa[7, 1] == 0
+    (ga, gs, gb), = rs.bf_match_top2([(_to_dev(A, dev), _to_dev(B, dev))])
+    assert np.array_equal(_np(ga), ra)
+    assert np.array_equal(_np(gs), sa)
+    assert np.array_equal(_np(gb), rb)
+    (ta, tb), = rs.bf_match([(_to_dev(A, dev), _to_dev(B, dev))])
+    assert np.array_equal(_np(ta), ra)
+    assert np.array_equal(_np(tb), rb)
+
+
+def test_empty_sides(rs, orc, dev):
+    _say("empty query/train sides")
+    A = _to_dev(orc.gen_random_u8(1, 40), dev)
+    E = torch.empty((0, 128), dtype=torch.uint8, device=dev)
+    (ga, gb), (ha, hb) = rs.bf_match([(A, E), (E, A)])
+    assert (_np(ga) == 0xFFFFFFFF).all() and gb.numel() == 0
+    assert ha.numel() == 0 and (_np(hb) == 0xFFFFFFFF).all()
+    (ta, ts, tb), = rs.bf_match_top2([(A, E)])
+    assert (_np(ta) == 0xFFFFFFFF).all() and (_np(ts) == 0xFFFFFFFF).all()
+    # cross-check skips index ~0u (bruteForceMatch.cpp:29-31)
+    out, cnt = rs.filter_matches(ga, None, tb, 0)
+    assert int(cnt.item()) == 0
+
+
+def test_many_uneven_tasks(rs, orc, dev):
+    _say("40 tasks of uneven sizes in one call (crosses the 32-task batch)")
+    rng = np.random.default_rng(7)
+    pairs, refs = [], []
+    for t in range(40):
+        nA, nB = int(rng.integers(1, 700)), int(rng.integers(1, 900))
+        A = orc.gen_random_u8(1000 + t, nA)
+        B = orc.gen_random_u8(2000 + t, nB)
+        pairs.append((_to_dev(A, dev), _to_dev(B, dev)))
+        refs.append(orc.bf_match_top2_u8(A, B))
+    outs = rs.bf_match([(a, b) for a, b in pairs])
+    outs2 = rs.bf_match_top2(pairs)
+    for (ga, gb), (ha, hs, hb), (ra, sa, rb) in zip(outs, outs2, refs):
+        assert np.array_equal(_np(ga), ra) and np.array_equal(_np(gb), rb)
+        assert np.array_equal(_np(ha), ra) and np.array_equal(_np(hs), sa) and np.array_equal(_np(hb), rb)
+
+
+def test_4bit_mode(rs, orc, dev):
+    _say("4-bit mode 700x900")
+    A = orc.gen_random_u8(31, 700)
+    B = orc.gen_random_u8(32, 900)
+    A4, B4 = orc.desc8b_to4b(A), orc.desc8b_to4b(B)
+    ra, rb = orc.bf_match_u4(A4, B4)
+    (ga, gb), = rs.bf_match([(_to_dev(A4, dev), _to_dev(B4, dev))], impl="u4")
+    assert np.array_equal(_np(ga), ra)
+    assert np.array_equal(_np(gb), rb)
+
+
+def test_segmented_train_range(rs, orc, dev):
+    _say("one query block vs 20000 train (several segments)")
+    train, query, _ = orc.gen_sift_pair(3, 20000, 600, 240, 0.05)
+    ra, sa, rb = orc.bf_match_top2_u8(query, train)
+    (ga, gs, gb), = rs.bf_match_top2([(_to_dev(query, dev), _to_dev(train, dev))])
+    assert np.array_equal(_np(ga), ra)
+    assert np.array_equal(_np(gs), sa)
+    assert np.array_equal(_np(gb), rb)
+    (ta, tb), = rs.bf_match([(_to_dev(query, dev), _to_dev(train, dev))])
+    assert np.array_equal(_np(ta), ra)
+    assert np.array_equal(_np(tb), rb)
+
+
+def test_8k_properties(rs, orc, dev):
+    """8K x 8K (the benchmark size): size-independent properties plus an exact
+    check of sampled rows/columns against the oracle."""
+    _say("8192x8192 properties")
+    train, query, gt = orc.gen_sift_pair(9, 8192, 8192, 2458, 0.05)
+    qd, td = _to_dev(query, dev), _to_dev(train, dev)
+    (ga, gs, gb), = rs.bf_match_top2([(qd, td)])
+    (ta, tb), = rs.bf_match([(qd, td)])
+    a, s, b = _np(ga), _np(gs), _np(gb)
+    assert np.array_equal(a, _np(ta)) and np.array_equal(b, _np(tb))
+    # determinism
+    (ga2, gs2, gb2), = rs.bf_match_top2([(qd, td)])
+    assert np.array_equal(a, _np(ga2)) and np.array_equal(s, _np(gs2)) and np.array_equal(b, _np(gb2))
+    # best <= second, distances recomputed exactly
+    assert ((a[:, 1] < s[:, 1]) | ((a[:, 1] == s[:, 1]) & (a[:, 0] < s[:, 0]))).all()
+    qi = query.astype(np.int64)
+    ti = train.astype(np.int64)
+    d_best = ((qi - ti[a[:, 0]]) ** 2).sum(1)
+    assert np.array_equal(d_best, a[:, 1].astype(np.int64))
+    d_b = ((ti - qi[b[:, 0]]) ** 2).sum(1)
+    assert np.array_equal(d_b, b[:, 1].astype(np.int64))
+    # exact on sampled rows and columns
+    rng = np.random.default_rng(0)
+    rows = rng.choice(8192, 96, replace=False)
+    ra, sa, _ = orc.bf_match_top2_u8(query[rows], train)
+    assert np.array_equal(a[rows], ra) and np.array_equal(s[rows], sa)
+    cols = rng.choice(8192, 96, replace=False)
+    _, rbc = orc.bf_match_u8(query, train[cols])
+    assert np.array_equal(b[cols], rbc)
