@@ -38,11 +38,16 @@ DEVICE = [
 ]
 
 
+def _defines():
+    # optional tile-configuration overrides, e.g. RSFM_DEFINES="RSFM_BF_WAVES=4"
+    return ["-D" + d for d in os.environ.get("RSFM_DEFINES", "").split() if d]
+
+
 def _compile(src):
     path = os.path.join(CSRC, src)
     obj = os.path.join(OBJDIR, src + ".o")
     if src.endswith(".hip"):
-        cmd = [HIPCC, "-x", "hip"] + COMMON + DEVICE + ["-c", path, "-o", obj]
+        cmd = [HIPCC, "-x", "hip"] + COMMON + _defines() + DEVICE + ["-c", path, "-o", obj]
     else:
         cmd = [HIPCC] + COMMON + ["-c", path, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -56,19 +61,21 @@ def _compile(src):
     return obj
 
 
-def build(verbose=False):
+def build(verbose=False, out=None):
+    """Compile and link; `out` overrides the library path (default in-tree)."""
     os.makedirs(OBJDIR, exist_ok=True)
     with ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as ex:
         objs = list(ex.map(_compile, SOURCES))
-    tmp = LIB + ".tmp"
+    lib = out or LIB
+    tmp = lib + ".tmp"
     cmd = [HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs + ["-lamdhip64"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("link failed: %s\n%s" % (" ".join(cmd), r.stderr))
-    os.replace(tmp, LIB)
+    os.replace(tmp, lib)
     if verbose:
-        print("built", LIB)
-    return LIB
+        print("built", lib)
+    return lib
 
 
 if __name__ == "__main__":

@@ -92,6 +92,12 @@ __device__ inline void top2_update(int32_t& m1, int32_t& m2, int32_t x)
     asm("v_med3_i32 %0, %1, %0, %2\n\tv_min_i32 %1, %1, %2" : "+v"(m2), "+v"(m1) : "v"(x));
 }
 
+// 16-B slot swizzle of a staged column: columns c and c^1 sit in different
+// 128-B halves of a 256-B bank row, and the eight columns of one parity map
+// slot s to eight different slots, so both 8- and 16-lane b128 phases are
+// bank-conflict free
+__device__ inline int swz(int c) { return ((c >> 1) & 7) ^ ((c & 1) << 2); }
+
 template <int kBits, bool kTop2, int kWaves, int kRB>
 __global__ __launch_bounds__(kWaves * 64) void match_kernel(const TaskBatch batch, uint32_t segCols)
 {
@@ -191,7 +197,7 @@ __global__ __launch_bounds__(kWaves * 64) void match_kernel(const TaskBatch batc
                 acc = __builtin_amdgcn_sdot4(w[q], 0x02020202, acc, false);
             }
             const int s = sPart * (kStageBytes / 16) + h;
-            *reinterpret_cast<v4i*>(base + ((s ^ (sColIdx & 7)) << 4)) = w;
+            *reinterpret_cast<v4i*>(base + ((s ^ swz(sColIdx)) << 4)) = w;
         }
 #pragma unroll
         for (int m = 1; m < kThrPerCol; m <<= 1) acc += __shfl_xor(acc, m);
@@ -217,7 +223,7 @@ __global__ __launch_bounds__(kWaves * 64) void match_kernel(const TaskBatch batc
 #pragma unroll
             for (int kc = 0; kc < 4; kc++) {
                 const int s = 2 * kc + hi;
-                bop[kc] = *reinterpret_cast<const v4i*>(sb + cl * 128 + ((s ^ (cl & 7)) << 4));
+                bop[kc] = *reinterpret_cast<const v4i*>(sb + cl * 128 + ((s ^ swz(cl)) << 4));
             }
 #pragma unroll
             for (int rb = 0; rb < kRB; rb++) {
