@@ -132,6 +132,46 @@ int rsfm_filter_matches(const rsfm_best_match* fwd, const rsfm_best_match* secon
                         uint32_t ratioNum, uint32_t ratioDen, rsfm_valid_match* out, uint32_t* count,
                         void* workspace, size_t workspaceBytes, hipStream_t stream);
 
+/* ---------------------------------------------------------------------------
+ * Fundamental-matrix RANSAC (8-point hypotheses)
+ * ------------------------------------------------------------------------ */
+
+/* one correspondence: (x0, y0) in image 0, (x1, y1) in image 1; the AoS
+ * form of the reference's ptPairs (4 x ArrayXf, ImagePair.cpp:148-151) */
+typedef struct rsfm_pt_pair {
+    float x0, y0, x1, y1;
+} rsfm_pt_pair;
+
+typedef struct rsfm_ransac_task {
+    const rsfm_pt_pair* pts;    /* nbPts correspondences, 16-byte aligned */
+    uint32_t nbPts;             /* capacity / count when nbPtsDev is NULL */
+    const uint32_t* nbPtsDev;   /* optional device count (capped at nbPts) */
+    const uint32_t* sampleIdx;  /* [nbHyp][8] indices into pts */
+    uint32_t nbHyp;
+    uint32_t* counts;           /* [nbHyp] inlier count, 0 for rejected hypotheses */
+    float* models;              /* optional [nbHyp][9] row-major F (unit Frobenius) */
+    uint32_t* best;             /* optional [2]: best hypothesis (most inliers, lowest index), its count */
+    uint8_t* inlierMask;        /* optional [nbPts]: inliers of the best hypothesis */
+} rsfm_ransac_task;
+
+/* Replaces the per-hypothesis body of ransac<F,8> in
+ * findEpipolarityRansacWithInsufficientTiePts<0> (ImagePair.cpp:148-204;
+ * ransac.hpp:115-163) with a batched scorer over given sample indices:
+ * sanity gate (squared separations minSepSq0/minSepSq1 in each image, 0 =
+ * off; ImagePair.cpp:160-177), FP32 8-point fit (legacy/geometry.hpp:216-259)
+ * and inlier count. mode 0: checkEpipolarTransformation's two-sided
+ * point-to-line test (legacy/geometry.cpp:55-86) with threshold t pixels;
+ * mode 1: Sampson distance < t. */
+int rsfm_ransac_f8(const rsfm_ransac_task* tasks, size_t nbTasks, float threshold, int mode, float minSepSq0,
+                   float minSepSq1, hipStream_t stream);
+
+/* deterministic device sampler (no reference counterpart: the reference's
+ * engine is random_device-seeded, ransac.hpp:117): hypothesis h draws 8
+ * distinct indices from splitmix64 seeded with seed ^ h * 0xD1B54A32D192ED03,
+ * rejecting repeats; fewer than 8 points give ~0u entries */
+int rsfm_ransac_sample_indices(uint32_t* sampleIdx, uint32_t nbHyp, uint32_t nbPts, const uint32_t* nbPtsDev,
+                               uint64_t seed, hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -328,7 +328,10 @@ static int sample_sane(const float* pts, const uint32_t* idx, float min_sep_sq0,
 static uint32_t score_one(const float* pts, uint32_t n, const uint32_t* idx, float t, int mode,
                           float min_sep_sq0, float min_sep_sq1, float* model)
 {
-    if (!sample_sane(pts, idx, min_sep_sq0, min_sep_sq1)) {
+    /* fewer than 8 points or an index out of range: the hypothesis is rejected */
+    int in_range = n >= 8;
+    for (int i = 0; i < 8; i++) in_range &= idx[i] < n;
+    if (!in_range || !sample_sane(pts, idx, min_sep_sq0, min_sep_sq1)) {
         if (model) for (int q = 0; q < 9; q++) model[q] = 0.f;
         return 0;
     }

@@ -220,3 +220,35 @@ void or_sample_indices(uint64_t seed, uint32_t n, uint32_t k, uint32_t H, uint32
     }
     free(mark);
 }
+
+/* the device sampler of rsfm_ransac_sample_indices: hypothesis h draws from
+ * splitmix64 seeded with seed ^ h * 0xD1B54A32D192ED03, value % n, repeats
+ * rejected; n < 8 gives ~0u entries */
+static uint64_t splitmix_next(uint64_t* s)
+{
+    *s += 0x9E3779B97F4A7C15ULL;
+    uint64_t z = *s;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+void or_sample_indices_hash(uint64_t seed, uint32_t n, uint32_t H, uint32_t* idx)
+{
+    for (uint32_t h = 0; h < H; h++) {
+        uint32_t* out = idx + (size_t)h * 8;
+        if (n < 8) {
+            for (int i = 0; i < 8; i++) out[i] = ~0u;
+            continue;
+        }
+        uint64_t s = seed ^ ((uint64_t)h * 0xD1B54A32D192ED03ULL);
+        for (int i = 0; i < 8; i++) {
+            for (;;) {
+                const uint32_t c = (uint32_t)(splitmix_next(&s) % n);
+                int dup = 0;
+                for (int j = 0; j < i; j++) dup |= out[j] == c;
+                if (!dup) { out[i] = c; break; }
+            }
+        }
+    }
+}

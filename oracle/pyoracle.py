@@ -221,3 +221,62 @@ def filter_matches(fwd, second, bwd, ratio=(4, 5)):
 
 def ratio_pass(d1, d2, num=4, den=5):
     return bool(_lib.or_ratio_pass(d1, d2, num, den))
+
+_decl("or_sample_indices_hash", None, [U64, U32, U32, P])
+
+
+def sample_indices_hash(seed, n, H):
+    idx = np.zeros((H, 8), np.uint32)
+    _lib.or_sample_indices_hash(seed, n, H, _p(idx))
+    return idx
+
+
+# ---------------- F-RANSAC ----------------
+def f8_fit_f32(p8):
+    p8 = _c(p8, np.float32).reshape(8, 4)
+    F = np.zeros(9, np.float32)
+    ok = _lib.or_f8_fit_f32(_p(p8), _p(F))
+    return bool(ok), F.reshape(3, 3)
+
+
+def f8_fit_f64(p8):
+    p8 = _c(p8, np.float64).reshape(8, 4)
+    F = np.zeros(9, np.float64)
+    ok = _lib.or_f8_fit_f64(_p(p8), _p(F))
+    return bool(ok), F.reshape(3, 3)
+
+
+def f_inlier_f32(F, pts, t, mode):
+    F = _c(F, np.float32).reshape(9)
+    pts = _c(pts, np.float32).reshape(-1, 4)
+    return np.array([_lib.or_f_inlier_f32(_p(F), _p(pts[i]), t, mode) for i in range(pts.shape[0])], bool)
+
+
+def f_inlier_f64(F, pts, t, mode):
+    F = _c(F, np.float64).reshape(9)
+    pts = _c(pts, np.float64).reshape(-1, 4)
+    return np.array([_lib.or_f_inlier_f64(_p(F), _p(pts[i]), t, mode) for i in range(pts.shape[0])], bool)
+
+
+def f8_score(pts, idx, t, mode, min_sep_sq0=0.0, min_sep_sq1=0.0, nthreads=0, with_models=True):
+    pts = _c(pts, np.float32).reshape(-1, 4)
+    idx = _c(idx, np.uint32).reshape(-1, 8)
+    H = idx.shape[0]
+    counts = np.zeros(H, np.uint32)
+    models = np.zeros((H, 9), np.float32) if with_models else None
+    _lib.or_f8_score(_p(pts), pts.shape[0], _p(idx), H, t, mode, min_sep_sq0, min_sep_sq1, _p(counts), _p(models),
+                     nthreads)
+    return counts, (models.reshape(H, 3, 3) if with_models else None)
+
+
+def f8_ransac_adaptive(pts, t, mode, confidence, max_iter, seed, min_sep_sq0=0.0, min_sep_sq1=0.0):
+    pts = _c(pts, np.float32).reshape(-1, 4)
+    bi = np.zeros(8, np.uint32)
+    draws = ctypes.c_uint32(0)
+    best = _lib.or_f8_ransac_adaptive(_p(pts), pts.shape[0], t, mode, confidence, max_iter, seed, min_sep_sq0,
+                                      min_sep_sq1, _p(bi), ctypes.byref(draws))
+    return best, bi, draws.value
+
+
+def required_iters(best, n, k, conf, max_iter):
+    return _lib.or_ransac_required_iters(best, n, k, conf, max_iter)

@@ -50,6 +50,8 @@ void or_gen_two_view(uint64_t seed, uint32_t n, uint32_t n_out, double focal, do
 /* sample index sets: H sets of k distinct indices in [0,n), non-duplicate
  * TableSampler semantics (ransac.hpp:52-82) driven by mt19937_64(seed). */
 void or_sample_indices(uint64_t seed, uint32_t n, uint32_t k, uint32_t H, uint32_t* idx);
+/* restatement of the GPU's splitmix64 sampler (rsfm_ransac_sample_indices) */
+void or_sample_indices_hash(uint64_t seed, uint32_t n, uint32_t H, uint32_t* idx);
 
 /* ---------------- brute-force matching (bf.c) ---------------- */
 typedef struct { uint32_t index; uint32_t distance; } or_best_match;   /* BestMatchImpl<u32> */

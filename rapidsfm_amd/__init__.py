@@ -22,6 +22,8 @@ from .ops import (  # noqa: F401
     bf_match_top2,
     desc8b_to_4b,
     filter_matches,
+    ransac_f8,
+    ransac_sample_indices,
     sqrnorm_u4,
     sqrnorm_u8,
 )
@@ -37,6 +39,8 @@ __all__ = [
     "bf_match_top2",
     "desc8b_to_4b",
     "filter_matches",
+    "ransac_f8",
+    "ransac_sample_indices",
     "sqrnorm_u8",
     "sqrnorm_u4",
 ]

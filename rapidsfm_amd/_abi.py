@@ -50,6 +50,20 @@ class BfTop2Task(ctypes.Structure):
     _fields_ = [("base", BfTask), ("secondMatchA", c_void_p)]
 
 
+class RansacTask(ctypes.Structure):
+    _fields_ = [
+        ("pts", c_void_p),
+        ("nbPts", c_u32),
+        ("nbPtsDev", c_void_p),
+        ("sampleIdx", c_void_p),
+        ("nbHyp", c_u32),
+        ("counts", c_void_p),
+        ("models", c_void_p),
+        ("best", c_void_p),
+        ("inlierMask", c_void_p),
+    ]
+
+
 assert ctypes.sizeof(BfTask) == 56
 assert ctypes.sizeof(BestMatch) == 8
 assert ctypes.sizeof(ValidMatch) == 12
@@ -72,6 +86,8 @@ _SIGS = {
         c_int,
         [c_void_p, c_void_p, c_u32, c_void_p, c_u32, c_u32, c_u32, c_void_p, c_void_p, c_void_p, c_size, c_void_p],
     ),
+    "rsfm_ransac_f8": (c_int, [c_void_p, c_size, c_float, c_int, c_float, c_float, c_void_p]),
+    "rsfm_ransac_sample_indices": (c_int, [c_void_p, c_u32, c_u32, c_void_p, ctypes.c_uint64, c_void_p]),
 }
 
 

@@ -21,6 +21,7 @@ SOURCES = [
     "api.cpp",
     "bf_match.hip",
     "bf_aux.hip",
+    "ransac.hip",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

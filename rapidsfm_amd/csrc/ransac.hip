@@ -1,0 +1,263 @@
+// ransac.hip - batched fundamental-matrix RANSAC hypothesis fitting and scoring.
+//
+// Reference path: ransac<F,8> (ransac.hpp:115-163) driving
+// findEpipolarityRansacWithInsufficientTiePts<0> (ImagePair.cpp:148-204):
+// per hypothesis the sample sanity gate (ImagePair.cpp:160-177), the 8-point
+// fit (legacy/geometry.hpp:216-259) and the inlier count of
+// checkEpipolarTransformation (legacy/geometry.cpp:55-86) over all points.
+// Sample indices are an input (the reference draws them from a
+// random_device-seeded engine), so a fixed index set gives one defined
+// inlier count per hypothesis.
+//
+// GPU layout: a 256-thread workgroup owns 256 hypotheses of one pair. Each
+// lane fits one hypothesis in registers (fgeom::fit8) into LDS; then each
+// wavefront scores its 64 hypotheses one at a time, lanes striding over the
+// correspondences staged in LDS tiles of 2048, counting inliers with
+// ballot + popcount on the scalar unit.
+#include "common.hpp"
+#include "fgeom.hpp"
+#include <algorithm>
+
+namespace rsfm_amd {
+namespace ransac {
+
+constexpr int kThreads = 256;
+constexpr int kTile = 2048;
+constexpr int kMaxTasks = 16;
+
+struct DevTask {
+    const float4* pts;
+    const uint32_t* nDev;
+    uint32_t nHost;
+    uint32_t H;
+    const uint32_t* idx;
+    uint32_t* counts;
+    float* models;
+    uint32_t* best;
+    uint8_t* mask;
+};
+struct Batch {
+    DevTask t[kMaxTasks];
+};
+
+__device__ inline uint32_t task_n(const DevTask& T) { return T.nDev ? min(*T.nDev, T.nHost) : T.nHost; }
+
+// sanity gate of ImagePair.cpp:167-177: any two samples closer than the
+// threshold in either image reject the hypothesis
+__device__ inline bool sane(const float4 p[8], float s0, float s1)
+{
+    if (s0 <= 0.f && s1 <= 0.f) return true;
+    bool ok = true;
+#pragma unroll
+    for (int i = 0; i < 8; i++)
+#pragma unroll
+        for (int j = 0; j < i; j++) {
+            const float dx0 = p[i].x - p[j].x, dy0 = p[i].y - p[j].y;
+            const float dx1 = p[i].z - p[j].z, dy1 = p[i].w - p[j].w;
+            if (dx0 * dx0 + dy0 * dy0 < s0 || dx1 * dx1 + dy1 * dy1 < s1) ok = false;
+        }
+    return ok;
+}
+
+// fit of hypothesis h; false when out of range, insane or degenerate
+__device__ inline bool fit_hypothesis(const DevTask& T, uint32_t n, uint32_t h, float s0, float s1, float F[9])
+{
+    if (h >= T.H || n < 8) return false;
+    float4 p[8];
+    bool inRange = true;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        const uint32_t k = T.idx[(size_t)h * 8 + i];
+        inRange = inRange && k < n;
+        p[i] = T.pts[k < n ? k : 0];
+    }
+    if (!inRange || !sane(p, s0, s1)) return false;
+    return fgeom::fit8(p, F);
+}
+
+template <int kMode>
+__global__ __launch_bounds__(kThreads) void score_kernel(const Batch batch, float t2, float s0, float s1)
+{
+    const DevTask& T = batch.t[blockIdx.y];
+    const uint32_t hBase = blockIdx.x * kThreads;
+    if (hBase >= T.H) return;
+    const uint32_t n = task_n(T);
+    __shared__ float sF[kThreads][9];
+    __shared__ uint32_t sValid[kThreads];
+    __shared__ uint32_t sCnt[kThreads];
+    __shared__ float4 sP[kTile];
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    {
+        float F[9];
+        const uint32_t h = hBase + tid;
+        const bool ok = fit_hypothesis(T, n, h, s0, s1, F);
+#pragma unroll
+        for (int q = 0; q < 9; q++) sF[tid][q] = ok ? F[q] : 0.f;
+        sValid[tid] = ok;
+        sCnt[tid] = 0;
+        if (T.models && h < T.H) {
+#pragma unroll
+            for (int q = 0; q < 9; q++) T.models[(size_t)h * 9 + q] = ok ? F[q] : 0.f;
+        }
+    }
+    for (uint32_t base = 0; base < n; base += kTile) {
+        const uint32_t tn = min((uint32_t)kTile, n - base);
+        __syncthreads();
+        for (uint32_t i = tid; i < tn; i += kThreads) sP[i] = T.pts[base + i];
+        __syncthreads();
+        for (int hh = 0; hh < 64; hh++) {
+            const int hyp = wave * 64 + hh;
+            if (!sValid[hyp]) continue;
+            float F[9];
+#pragma unroll
+            for (int q = 0; q < 9; q++) F[q] = sF[hyp][q];
+            uint32_t cnt = 0;
+            for (uint32_t p = lane; p < tn; p += 64) {
+                const bool in = fgeom::inlier<kMode>(F, sP[p], t2);
+                cnt += (uint32_t)__popcll(__ballot(in));
+            }
+            // the loop bound is lane-dependent only in the last partial round;
+            // inactive lanes vote 0, so cnt is the same in every lane
+            if (lane == 0) sCnt[hyp] += cnt;
+        }
+    }
+    __syncthreads();
+    const uint32_t h = hBase + tid;
+    if (h < T.H) T.counts[h] = sValid[tid] ? sCnt[tid] : 0u;
+}
+
+// best hypothesis (largest count, lowest index among equals) and its inlier mask
+template <int kMode>
+__global__ __launch_bounds__(kThreads) void best_kernel(const Batch batch, float t2, float s0, float s1)
+{
+    const DevTask& T = batch.t[blockIdx.x];
+    const uint32_t n = task_n(T);
+    __shared__ uint64_t sKey[kThreads];
+    __shared__ float sF[9];
+    __shared__ uint32_t sOk;
+    uint64_t key = 0;  // (count << 32) | ~h : max count, then lowest h
+    for (uint32_t h = threadIdx.x; h < T.H; h += kThreads) {
+        const uint64_t k = ((uint64_t)T.counts[h] << 32) | (uint32_t)~h;
+        key = k > key ? k : key;
+    }
+    sKey[threadIdx.x] = key;
+    __syncthreads();
+    for (int s = kThreads / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) sKey[threadIdx.x] = max(sKey[threadIdx.x], sKey[threadIdx.x + s]);
+        __syncthreads();
+    }
+    const uint64_t bk = sKey[0];
+    const uint32_t bestH = T.H ? ~(uint32_t)bk : 0u;
+    const uint32_t bestC = (uint32_t)(bk >> 32);
+    if (threadIdx.x == 0) {
+        float F[9];
+        const bool ok = T.H > 0 && bestC > 0 && fit_hypothesis(T, n, bestH, s0, s1, F);
+        for (int q = 0; q < 9; q++) sF[q] = ok ? F[q] : 0.f;
+        sOk = ok;
+        if (T.best) {
+            T.best[0] = T.H ? bestH : ~0u;
+            T.best[1] = bestC;
+        }
+    }
+    __syncthreads();
+    if (!T.mask) return;
+    float F[9];
+    for (int q = 0; q < 9; q++) F[q] = sF[q];
+    for (uint32_t i = threadIdx.x; i < T.nHost; i += kThreads)
+        T.mask[i] = (sOk && i < n) ? (uint8_t)fgeom::inlier<kMode>(F, T.pts[i], t2) : 0;
+}
+
+// ---------------------------------------------------------------------------
+// device sampler: hypothesis h of stream `seed` draws 8 distinct indices in
+// [0, n) from a splitmix64 sequence seeded by (seed, h), rejecting repeats
+__device__ inline uint64_t splitmix(uint64_t& s)
+{
+    s += 0x9E3779B97F4A7C15ull;
+    uint64_t z = s;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+__global__ void sample_kernel(uint32_t* idx, uint32_t H, uint32_t nHost, const uint32_t* nDev, uint64_t seed)
+{
+    const uint32_t h = blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= H) return;
+    const uint32_t n = nDev ? min(*nDev, nHost) : nHost;
+    uint32_t* out = idx + (size_t)h * 8;
+    if (n < 8) {
+        for (int i = 0; i < 8; i++) out[i] = ~0u;
+        return;
+    }
+    uint64_t s = seed ^ ((uint64_t)h * 0xD1B54A32D192ED03ull);
+    uint32_t v[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        for (;;) {
+            const uint32_t c = (uint32_t)(splitmix(s) % n);
+            bool dup = false;
+#pragma unroll
+            for (int j = 0; j < i; j++) dup |= v[j] == c;
+            if (!dup) {
+                v[i] = c;
+                break;
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 8; i++) out[i] = v[i];
+}
+
+}  // namespace ransac
+}  // namespace rsfm_amd
+
+using namespace rsfm_amd;
+
+extern "C" int rsfm_ransac_f8(const rsfm_ransac_task* tasks, size_t nbTasks, float threshold, int mode,
+                              float minSepSq0, float minSepSq1, hipStream_t stream)
+{
+    using namespace ransac;
+    RSFM_RETURN_IF(nbTasks > 0 && tasks == nullptr, RSFM_ERR_INVALID, "ransac: tasks is null");
+    RSFM_RETURN_IF(mode != 0 && mode != 1, RSFM_ERR_INVALID, "ransac: mode must be 0 (two-sided line) or 1 (Sampson)");
+    RSFM_RETURN_IF(!(threshold > 0.f), RSFM_ERR_INVALID, "ransac: threshold must be > 0");
+    const float t2 = threshold * threshold;
+    for (size_t b = 0; b < nbTasks; b += kMaxTasks) {
+        const size_t n = std::min<size_t>(kMaxTasks, nbTasks - b);
+        Batch batch{};
+        uint32_t maxH = 0;
+        for (size_t i = 0; i < n; i++) {
+            const rsfm_ransac_task& t = tasks[b + i];
+            RSFM_RETURN_IF(t.nbHyp > 0 && (t.sampleIdx == nullptr || t.counts == nullptr), RSFM_ERR_INVALID,
+                           "ransac: sampleIdx/counts must be set when nbHyp > 0");
+            RSFM_RETURN_IF(t.nbPts > 0 && t.pts == nullptr, RSFM_ERR_INVALID, "ransac: pts is null");
+            RSFM_RETURN_IF(reinterpret_cast<uintptr_t>(t.pts) % 16, RSFM_ERR_INVALID, "ransac: pts must be 16-byte aligned");
+            batch.t[i] = DevTask{reinterpret_cast<const float4*>(t.pts), t.nbPtsDev, t.nbPts, t.nbHyp, t.sampleIdx,
+                                 t.counts, t.models, t.best, t.inlierMask};
+            maxH = std::max(maxH, t.nbHyp);
+        }
+        if (maxH > 0) {
+            const dim3 grid(div_up(maxH, kThreads), (uint32_t)n);
+            if (mode == 0) score_kernel<0><<<grid, kThreads, 0, stream>>>(batch, t2, minSepSq0, minSepSq1);
+            else score_kernel<1><<<grid, kThreads, 0, stream>>>(batch, t2, minSepSq0, minSepSq1);
+            RSFM_CHECK(check_launch("ransac score"));
+        }
+        bool anyBest = false;
+        for (size_t i = 0; i < n; i++) anyBest |= batch.t[i].best != nullptr || batch.t[i].mask != nullptr;
+        if (anyBest) {
+            if (mode == 0) best_kernel<0><<<(uint32_t)n, kThreads, 0, stream>>>(batch, t2, minSepSq0, minSepSq1);
+            else best_kernel<1><<<(uint32_t)n, kThreads, 0, stream>>>(batch, t2, minSepSq0, minSepSq1);
+            RSFM_CHECK(check_launch("ransac best"));
+        }
+    }
+    return RSFM_OK;
+}
+
+extern "C" int rsfm_ransac_sample_indices(uint32_t* sampleIdx, uint32_t nbHyp, uint32_t nbPts,
+                                          const uint32_t* nbPtsDev, uint64_t seed, hipStream_t stream)
+{
+    if (nbHyp == 0) return RSFM_OK;
+    RSFM_RETURN_IF(sampleIdx == nullptr, RSFM_ERR_INVALID, "sample_indices: sampleIdx is null");
+    ransac::sample_kernel<<<div_up(nbHyp, 256), 256, 0, stream>>>(sampleIdx, nbHyp, nbPts, nbPtsDev, seed);
+    return check_launch("ransac sample_indices");
+}

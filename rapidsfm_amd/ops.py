@@ -8,7 +8,7 @@ import ctypes
 
 import torch
 
-from ._abi import BfTask, BfTop2Task, check, lib
+from ._abi import BfTask, BfTop2Task, RansacTask, check, lib
 
 
 def _stream(stream):
@@ -95,6 +95,44 @@ def bf_match_top2(pairs, stream=None, workspace=None):
         ws_ptr = (workspace.data_ptr() + 255) // 256 * 256
     check(lib.rsfm_bf_match_top2(tp, len(pairs), ctypes.c_void_p(ws_ptr), need.value, _stream(stream)), "bf_match_top2")
     return outs
+
+
+def ransac_f8(problems, threshold, mode=1, min_sep_sq=(0.0, 0.0), want_models=False, stream=None):
+    """Score 8-point hypotheses for a list of (pts, sampleIdx[, nDev]) problems.
+
+    pts: float32 CUDA (n, 4) [x0, y0, x1, y1]; sampleIdx: int32/uint32 CUDA (H, 8);
+    nDev: optional int32 CUDA (1,) count. mode 0 = reference two-sided line test,
+    1 = Sampson. Returns a list of dicts with counts (H,), best (2,), mask (n,) and
+    models (H, 3, 3) when asked.
+    """
+    tasks = (RansacTask * max(1, len(problems)))()
+    outs = []
+    for i, prob in enumerate(problems):
+        pts, idx = prob[0], prob[1]
+        ndev = prob[2] if len(prob) > 2 else None
+        n, H = pts.shape[0], idx.shape[0]
+        if pts.dtype != torch.float32 or pts.dim() != 2 or pts.shape[1] != 4 or not pts.is_contiguous():
+            raise ValueError("pts must be contiguous float32 (n, 4)")
+        dev = pts.device
+        o = {
+            "counts": torch.empty(H, dtype=torch.int32, device=dev),
+            "best": torch.empty(2, dtype=torch.int32, device=dev),
+            "mask": torch.empty(n, dtype=torch.uint8, device=dev),
+            "models": torch.empty((H, 3, 3), dtype=torch.float32, device=dev) if want_models else None,
+        }
+        tasks[i] = RansacTask(_ptr(pts), n, _ptr(ndev) if ndev is not None else None, _ptr(idx), H, _ptr(o["counts"]),
+                              _ptr(o["models"]) if want_models else None, _ptr(o["best"]), _ptr(o["mask"]))
+        outs.append(o)
+    check(lib.rsfm_ransac_f8(ctypes.cast(tasks, ctypes.c_void_p), len(problems), float(threshold), int(mode),
+                             float(min_sep_sq[0]), float(min_sep_sq[1]), _stream(stream)), "rsfm_ransac_f8")
+    return outs
+
+
+def ransac_sample_indices(n, H, seed, n_dev=None, device=None, stream=None):
+    idx = torch.empty((H, 8), dtype=torch.int32, device=device or "cuda")
+    check(lib.rsfm_ransac_sample_indices(_ptr(idx), H, n, _ptr(n_dev) if n_dev is not None else None, seed,
+                                         _stream(stream)), "rsfm_ransac_sample_indices")
+    return idx
 
 
 def filter_matches(fwd, second, bwd, n_train, ratio=(4, 5), stream=None):
