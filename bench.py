@@ -1,0 +1,217 @@
+"""Two-view matching throughput: pairs/s at 8K keypoints (match + RANSAC) on one MI355X.
+
+One step = one pass of the hot path over a batch of image pairs already
+resident in HBM, all on one stream:
+  1. rsfm_bf_match_top2           int8-MFMA brute force, 8192 x 8192 per pair
+                                  (top-2 query side, top-1 train side)
+  2. rsfm_filter_matches_batch    ratio test 0.8 + cross-check + keypoint gather
+  3. rsfm_ransac_sample_indices_batch   H seeded 8-point samples per pair
+  4. rsfm_ransac_f8               FP32 8-point fits + Sampson scoring (1 px),
+                                  best model + inlier mask per pair
+Synthetic data: SIFT-like descriptors (30% noisy true matches) with keypoints
+from one two-view geometry per pair (30% of true matches geometric outliers).
+
+Prints one JSON line (metric, value, ..., roofline, cpu_baseline).
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+N_KPTS = 8192
+INT8_PEAK_TOPS = 5000.0  # MI355X dense int8 MFMA, 5.0 POPS
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--pairs", type=int, default=64, help="image pairs per step")
+    ap.add_argument("--hyp", type=int, default=4096, help="RANSAC hypotheses per pair")
+    ap.add_argument("--n", type=int, default=N_KPTS, help="keypoints per image")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="bound of the CPU baseline sample")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="also run one 256 MiB device copy (HBM-counter calibration for rocprofv3 --pmc)")
+    return ap.parse_args()
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path (every pair is independent; "
+              "the sharding design is described in DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import rapidsfm_amd as rs
+    from rapidsfm_amd._abi import BfTask, BfTop2Task, FilterTask, RansacTask, SampleTask, check, lib
+
+    dev = torch.device("cuda:0")
+    n, P, H = args.n, args.pairs, args.hyp
+    stream = torch.cuda.current_stream()
+    sp = ctypes.c_void_p(stream.cuda_stream)
+
+    # ---- synthetic pairs (generated once on the host, then resident in HBM)
+    from oracle import pyoracle as orc  # generator only; the timed path is the HIP library
+
+    n_unique = min(P, 8)
+    uniq = [orc.gen_match_pair(1000 + u, n, n, int(0.3 * n), 0.3) for u in range(n_unique)]
+    bufs = []
+    for p in range(P):
+        dA, dB, kA, kB, gt, geo = uniq[p % n_unique]
+        b = {
+            "dA": torch.from_numpy(dA).to(dev), "dB": torch.from_numpy(dB).to(dev),
+            "kA": torch.from_numpy(kA).to(dev), "kB": torch.from_numpy(kB).to(dev),
+            "bestA": torch.empty((n, 2), dtype=torch.int32, device=dev),
+            "secA": torch.empty((n, 2), dtype=torch.int32, device=dev),
+            "bestB": torch.empty((n, 2), dtype=torch.int32, device=dev),
+            "pts": torch.empty((n, 4), dtype=torch.float32, device=dev),
+            "count": torch.zeros(1, dtype=torch.int32, device=dev),
+            "idx": torch.empty((H, 8), dtype=torch.int32, device=dev),
+            "counts": torch.empty(H, dtype=torch.int32, device=dev),
+            "best": torch.empty(2, dtype=torch.int32, device=dev),
+            "mask": torch.empty(n, dtype=torch.uint8, device=dev),
+        }
+        bufs.append(b)
+    ptr = lambda t: t.data_ptr()  # noqa: E731
+    top2 = (BfTop2Task * P)()
+    filt = (FilterTask * P)()
+    samp = (SampleTask * P)()
+    rtask = (RansacTask * P)()
+    for p, b in enumerate(bufs):
+        top2[p] = BfTop2Task(BfTask(n, n, ptr(b["dA"]), ptr(b["dB"]), ptr(b["bestA"]), ptr(b["bestB"]), None, None),
+                             ptr(b["secA"]))
+        filt[p] = FilterTask(ptr(b["bestA"]), ptr(b["secA"]), n, ptr(b["bestB"]), n, None, ptr(b["count"]),
+                             ptr(b["kA"]), ptr(b["kB"]), ptr(b["pts"]))
+        samp[p] = SampleTask(ptr(b["idx"]), H, n, ptr(b["count"]), 77 + p)
+        rtask[p] = RansacTask(ptr(b["pts"]), n, ptr(b["count"]), ptr(b["idx"]), H, ptr(b["counts"]), None,
+                              ptr(b["best"]), ptr(b["mask"]))
+    ws_need = ctypes.c_size_t(0)
+    check(lib.rsfm_bf_match_top2_workspace_size(ctypes.cast(top2, ctypes.c_void_p), P, ctypes.byref(ws_need)), "ws")
+    ws = torch.empty(ws_need.value + 256, dtype=torch.uint8, device=dev)
+    ws_ptr = (ws.data_ptr() + 255) // 256 * 256 if ws_need.value else 0
+
+    ev_m0 = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    ev_m1 = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+
+    def step(i=None):
+        if i is not None:
+            ev_m0[i].record(stream)
+        check(lib.rsfm_bf_match_top2(ctypes.cast(top2, ctypes.c_void_p), P, ctypes.c_void_p(ws_ptr), ws_need.value, sp),
+              "match")
+        if i is not None:
+            ev_m1[i].record(stream)
+        check(lib.rsfm_filter_matches_batch(ctypes.cast(filt, ctypes.c_void_p), P, 4, 5, sp), "filter")
+        check(lib.rsfm_ransac_sample_indices_batch(ctypes.cast(samp, ctypes.c_void_p), P, sp), "sample")
+        check(lib.rsfm_ransac_f8(ctypes.cast(rtask, ctypes.c_void_p), P, 1.0, 1, 0.0, 0.0, sp), "ransac")
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = torch.cuda.Event(enable_timing=True)
+    t1 = torch.cuda.Event(enable_timing=True)
+    t0.record(stream)
+    for i in range(args.steps):
+        step(i)
+    t1.record(stream)
+    torch.cuda.synchronize()
+    ms_total = t0.elapsed_time(t1)
+    ms_step = ms_total / args.steps
+    value = P * args.steps / (ms_total / 1e3)
+    match_ms = sum(a.elapsed_time(b) for a, b in zip(ev_m0, ev_m1)) / args.steps
+    ops = 2.0 * n * n * 128 * P
+    achieved = ops / (match_ms / 1e3) / 1e12
+
+    # ---- correctness of the measured pass (pair 0 against the CPU oracle)
+    dA, dB, kA, kB, gt, geo = uniq[0]
+    b0 = bufs[0]
+    ra, sa, rb = orc.bf_match_top2_fast(dA, dB)
+    u32 = lambda t: t.cpu().numpy().view(np.uint32)  # noqa: E731
+    ok_match = (np.array_equal(u32(b0["bestA"]), ra) and np.array_equal(u32(b0["secA"]), sa)
+                and np.array_equal(u32(b0["bestB"]), rb))
+    m = orc.filter_matches(ra, sa, rb, (4, 5))
+    cnt = int(b0["count"].item())
+    pts_ref = np.concatenate([kA[m[:, 0]], kB[m[:, 1]]], 1).astype(np.float32)
+    ok_pts = cnt == len(m) and np.array_equal(b0["pts"][:cnt].cpu().numpy(), pts_ref)
+    idx_ref = orc.sample_indices_hash(77, cnt, H)
+    rc, _ = orc.f8_score(pts_ref, idx_ref, 1.0, 1, with_models=False)
+    ok_ransac = np.array_equal(u32(b0["counts"]), rc) and int(u32(b0["best"])[1]) == int(rc.max())
+    parity = bool(ok_match and ok_pts and ok_ransac)
+
+    result = {
+        "metric": "pairs/s at 8K kpts (match+RANSAC)",
+        "value": round(value, 1),
+        "unit": "pairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int8 (match, int32 accumulate) + fp32 (RANSAC)",
+        "data": "synthetic",
+        "config": {
+            "workload": "bf-match top2+ratio0.8+crosscheck, then F 8-point RANSAC (Sampson 1px) per pair",
+            "keypoints_per_image": n,
+            "pairs_per_step": P,
+            "ransac_hypotheses": H,
+            "matches_per_pair": cnt,
+            "parallelism": "single GPU",
+        },
+        "parity_pair0": parity,
+        "roofline": {
+            "kernel": "match_kernel<8,true,8,1> (rsfm_bf_match_top2)",
+            "bound": "mfma",
+            "achieved": round(achieved, 1),
+            "peak": INT8_PEAK_TOPS,
+            "unit": "TOP/s",
+            "frac": round(achieved / INT8_PEAK_TOPS, 4),
+            "traffic": None,
+            "avg_launch_ms": round(match_ms, 4),
+        },
+        "cpu_baseline": None,
+    }
+
+    if args.calib_copy:
+        src = torch.empty(256 << 20, dtype=torch.uint8, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+
+    if not args.no_cpu:
+        cores = len(os.sched_getaffinity(0))
+        t_cpu = 0.0
+        k = 0
+        while t_cpu < args.cpu_seconds and k < 4096:
+            dA, dB, kA, kB, _, _ = uniq[k % n_unique]
+            t = time.perf_counter()
+            ra, sa, rb = orc.bf_match_top2_fast(dA, dB, cores)
+            mm = orc.filter_matches(ra, sa, rb, (4, 5))
+            pts = np.concatenate([kA[mm[:, 0]], kB[mm[:, 1]]], 1).astype(np.float32)
+            idx = orc.sample_indices_hash(77, len(mm), H)
+            orc.f8_score(pts, idx, 1.0, 1, nthreads=cores, with_models=False)
+            t_cpu += time.perf_counter() - t
+            k += 1
+        result["cpu_baseline"] = {
+            "value": round(k / t_cpu, 3),
+            "unit": "pairs/s",
+            "cores": cores,
+            "kind": "port",
+            "sample": "%d pairs of the same workload (%dx%d top-2 match + filter + %d-hypothesis F-RANSAC), "
+                      "%.1f s, OpenMP over %d threads" % (k, n, n, H, t_cpu, cores),
+        }
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -132,15 +132,35 @@ int rsfm_filter_matches(const rsfm_best_match* fwd, const rsfm_best_match* secon
                         uint32_t ratioNum, uint32_t ratioDen, rsfm_valid_match* out, uint32_t* count,
                         void* workspace, size_t workspaceBytes, hipStream_t stream);
 
-/* ---------------------------------------------------------------------------
- * Fundamental-matrix RANSAC (8-point hypotheses)
- * ------------------------------------------------------------------------ */
-
 /* one correspondence: (x0, y0) in image 0, (x1, y1) in image 1; the AoS
  * form of the reference's ptPairs (4 x ArrayXf, ImagePair.cpp:148-151) */
 typedef struct rsfm_pt_pair {
     float x0, y0, x1, y1;
 } rsfm_pt_pair;
+
+/* batched device filter (no reference counterpart; it moves the per-candidate
+ * host cross-check of BuilderMethods.cpp:145-202 onto the device): same rule as
+ * rsfm_filter_matches per task, one workgroup per task, and optionally gathers
+ * the kept keypoint pairs: pts[k] = (kpQuery[q], kpTrain[t]) of the k-th kept
+ * match, ready for rsfm_ransac_f8 with nbPtsDev = count. */
+typedef struct rsfm_filter_task {
+    const rsfm_best_match* fwd;
+    const rsfm_best_match* second; /* NULL: no ratio test */
+    uint32_t nbQueryDesc;
+    const rsfm_best_match* bwd;    /* NULL: no cross-check */
+    uint32_t nbTrainDesc;
+    rsfm_valid_match* out;         /* optional, nbQueryDesc entries */
+    uint32_t* count;               /* device */
+    const float* kpQuery;          /* optional (x, y) per query descriptor */
+    const float* kpTrain;          /* optional (x, y) per train descriptor */
+    rsfm_pt_pair* pts;             /* optional, nbQueryDesc entries */
+} rsfm_filter_task;
+int rsfm_filter_matches_batch(const rsfm_filter_task* tasks, size_t nbTasks, uint32_t ratioNum, uint32_t ratioDen,
+                              hipStream_t stream);
+
+/* ---------------------------------------------------------------------------
+ * Fundamental-matrix RANSAC (8-point hypotheses)
+ * ------------------------------------------------------------------------ */
 
 typedef struct rsfm_ransac_task {
     const rsfm_pt_pair* pts;    /* nbPts correspondences, 16-byte aligned */
@@ -171,6 +191,16 @@ int rsfm_ransac_f8(const rsfm_ransac_task* tasks, size_t nbTasks, float threshol
  * rejecting repeats; fewer than 8 points give ~0u entries */
 int rsfm_ransac_sample_indices(uint32_t* sampleIdx, uint32_t nbHyp, uint32_t nbPts, const uint32_t* nbPtsDev,
                                uint64_t seed, hipStream_t stream);
+
+/* the same sampler for many pairs in one launch */
+typedef struct rsfm_sample_task {
+    uint32_t* sampleIdx; /* [nbHyp][8] */
+    uint32_t nbHyp;
+    uint32_t nbPts;
+    const uint32_t* nbPtsDev; /* optional device count (capped at nbPts) */
+    uint64_t seed;
+} rsfm_sample_task;
+int rsfm_ransac_sample_indices_batch(const rsfm_sample_task* tasks, size_t nbTasks, hipStream_t stream);
 
 #ifdef __cplusplus
 }

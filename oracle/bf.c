@@ -150,6 +150,57 @@ void or_bf_match_top2_u8(const uint8_t* A, uint32_t nA, const uint8_t* B, uint32
     bf_generic(A, nA, B, nB, 128, l2_u8, bestA, secondA, bestB, nthreads);
 }
 
+/* Same results as or_bf_match_top2_u8, organised for CPU throughput: each
+ * distance is computed once and feeds both directions (row top-2 in
+ * registers, column top-1 in a per-thread array merged at the end). This is
+ * the "reference CPU path" timed by the benchmark's cpu_baseline leg. */
+void or_bf_match_top2_fast(const uint8_t* A, uint32_t nA, const uint8_t* B, uint32_t nB,
+                           or_best_match* bestA, or_best_match* secondA, or_best_match* bestB, int nthreads)
+{
+    set_threads(nthreads);
+    int nt = 1;
+#ifdef _OPENMP
+    nt = nthreads > 0 ? nthreads : omp_get_max_threads();
+#endif
+    uint64_t* colbest = (uint64_t*)malloc(sizeof(uint64_t) * (size_t)nt * (nB ? nB : 1));
+    for (size_t i = 0; i < (size_t)nt * nB; i++) colbest[i] = ~0ULL;
+#pragma omp parallel num_threads(nt)
+    {
+        int tid = 0;
+#ifdef _OPENMP
+        tid = omp_get_thread_num();
+#endif
+        uint64_t* cb = colbest + (size_t)tid * nB;
+#pragma omp for schedule(dynamic, 1)
+        for (int64_t i0 = 0; i0 < (int64_t)nA; i0 += 8) {
+            const uint32_t rn = (uint32_t)((int64_t)nA - i0 < 8 ? (int64_t)nA - i0 : 8);
+            uint32_t d1[8], j1[8], d2[8], j2[8];
+            for (uint32_t r = 0; r < rn; r++) { d1[r] = d2[r] = UINT32_MAX; j1[r] = j2[r] = ~0u; }
+            for (uint32_t j = 0; j < nB; j++) {
+                const uint8_t* b = B + (size_t)j * 128;
+                for (uint32_t r = 0; r < rn; r++) {
+                    const uint32_t d = l2_u8(A + (size_t)(i0 + r) * 128, b);
+                    if (d < d1[r]) { d2[r] = d1[r]; j2[r] = j1[r]; d1[r] = d; j1[r] = j; }
+                    else if (d < d2[r]) { d2[r] = d; j2[r] = j; }
+                    const uint64_t key = ((uint64_t)d << 32) | (uint64_t)(i0 + r);
+                    if (key < cb[j]) cb[j] = key;
+                }
+            }
+            for (uint32_t r = 0; r < rn; r++) {
+                bestA[i0 + r].index = j1[r]; bestA[i0 + r].distance = d1[r];
+                secondA[i0 + r].index = j2[r]; secondA[i0 + r].distance = d2[r];
+            }
+        }
+    }
+    for (uint32_t j = 0; j < nB; j++) {
+        uint64_t k = ~0ULL;
+        for (int t = 0; t < nt; t++) k = colbest[(size_t)t * nB + j] < k ? colbest[(size_t)t * nB + j] : k;
+        bestB[j].index = (uint32_t)k;
+        bestB[j].distance = (uint32_t)(k >> 32);
+    }
+    free(colbest);
+}
+
 void or_bf_match_dot_top2_u8(const uint8_t* A, uint32_t nA, const uint8_t* B, uint32_t nB,
                              int32_t* best_dot, uint32_t* best_idx, int32_t* second_dot,
                              uint32_t* second_idx, int nthreads)

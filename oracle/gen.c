@@ -199,6 +199,68 @@ void or_gen_two_view(uint64_t seed, uint32_t n, uint32_t n_out, double focal, do
     free(outl);
 }
 
+/* one image pair for the end-to-end workload: descriptors as or_gen_sift_pair
+ * (query = A, train = B, n_true noisy copies), keypoints from the two-view
+ * geometry of or_gen_two_view: a fraction (1 - outlier_frac) of the true
+ * descriptor pairs are projections of one 3-D point (+ noise_px), the other
+ * keypoints are uniform over the image. kp arrays are (x, y) float32. */
+void or_gen_match_pair(uint64_t seed, uint32_t nA, uint32_t nB, uint32_t n_true, double outlier_frac,
+                       double focal, double width, double height, double noise_px, uint8_t* descA,
+                       uint8_t* descB, float* kpA, float* kpB, int32_t* gtA, uint8_t* geo_inlier)
+{
+    or_gen_sift_pair(seed, nB, nA, n_true, 0.05, descB, descA, gtA);
+    or_mt64 rng;
+    or_mt64_seed(&rng, seed ^ 0x5DEECE66DULL);
+    const double cx = width / 2, cy = height / 2;
+    double axis[3], t[3], an, tn;
+    do {
+        for (int i = 0; i < 3; i++) axis[i] = or_mt64_normal(&rng);
+        an = sqrt(axis[0] * axis[0] + axis[1] * axis[1] + axis[2] * axis[2]);
+    } while (an < 1e-6);
+    const double ang = (15.0 * M_PI / 180.0) * or_mt64_uniform(&rng);
+    const double w[3] = {axis[0] / an * ang, axis[1] / an * ang, axis[2] / an * ang};
+    double R[9];
+    rodrigues(w, R);
+    do {
+        for (int i = 0; i < 3; i++) t[i] = or_mt64_normal(&rng);
+        tn = sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
+    } while (tn < 1e-6);
+    const double base = 1.0 + 2.0 * or_mt64_uniform(&rng);
+    for (int i = 0; i < 3; i++) t[i] = t[i] / tn * base;
+    for (uint32_t j = 0; j < nB; j++) {
+        kpB[2 * j] = (float)(width * or_mt64_uniform(&rng));
+        kpB[2 * j + 1] = (float)(height * or_mt64_uniform(&rng));
+    }
+    for (uint32_t i = 0; i < nA; i++) {
+        if (geo_inlier) geo_inlier[i] = 0;
+        const int32_t g = gtA[i];
+        const int geo = g >= 0 && or_mt64_uniform(&rng) >= outlier_frac;
+        if (!geo) {
+            kpA[2 * i] = (float)(width * or_mt64_uniform(&rng));
+            kpA[2 * i + 1] = (float)(height * or_mt64_uniform(&rng));
+            continue;
+        }
+        for (;;) {
+            const double u = width * or_mt64_uniform(&rng);
+            const double v = height * or_mt64_uniform(&rng);
+            const double depth = 10.0 + 40.0 * or_mt64_uniform(&rng);
+            const double X[3] = {(u - cx) / focal * depth, (v - cy) / focal * depth, depth};
+            double Y[3];
+            for (int r = 0; r < 3; r++) Y[r] = R[r * 3] * X[0] + R[r * 3 + 1] * X[1] + R[r * 3 + 2] * X[2] + t[r];
+            if (Y[2] <= 1e-3) continue;
+            const double u1 = focal * Y[0] / Y[2] + cx;
+            const double v1 = focal * Y[1] / Y[2] + cy;
+            if (u1 < 0 || u1 >= width || v1 < 0 || v1 >= height) continue;
+            kpA[2 * i] = (float)(u + noise_px * or_mt64_normal(&rng));
+            kpA[2 * i + 1] = (float)(v + noise_px * or_mt64_normal(&rng));
+            kpB[2 * g] = (float)(u1 + noise_px * or_mt64_normal(&rng));
+            kpB[2 * g + 1] = (float)(v1 + noise_px * or_mt64_normal(&rng));
+            break;
+        }
+        if (geo_inlier) geo_inlier[i] = 1;
+    }
+}
+
 /* TableSampler::operator() of ransac.hpp:59-72 with Picker (ransac.hpp:31-38,
  * no focused tail): val = engine() % n, redraw while already taken. */
 void or_sample_indices(uint64_t seed, uint32_t n, uint32_t k, uint32_t H, uint32_t* idx)

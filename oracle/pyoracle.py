@@ -223,6 +223,29 @@ def ratio_pass(d1, d2, num=4, den=5):
     return bool(_lib.or_ratio_pass(d1, d2, num, den))
 
 _decl("or_sample_indices_hash", None, [U64, U32, U32, P])
+_decl("or_gen_match_pair", None, [U64, U32, U32, U32, F64, F64, F64, F64, F64, P, P, P, P, P, P])
+_decl("or_bf_match_top2_fast", None, [P, U32, P, U32, P, P, P, I32])
+
+
+def gen_match_pair(seed, nA, nB, n_true, outlier_frac, focal=2000.0, width=4000.0, height=3000.0, noise_px=0.5):
+    """descA (nA,128), descB (nB,128), kpA (nA,2), kpB (nB,2), gtA (nA,), geo_inlier (nA,)"""
+    dA = np.zeros((nA, 128), np.uint8)
+    dB = np.zeros((nB, 128), np.uint8)
+    kA = np.zeros((nA, 2), np.float32)
+    kB = np.zeros((nB, 2), np.float32)
+    gt = np.zeros(nA, np.int32)
+    geo = np.zeros(nA, np.uint8)
+    _lib.or_gen_match_pair(seed, nA, nB, n_true, outlier_frac, focal, width, height, noise_px, _p(dA), _p(dB), _p(kA),
+                           _p(kB), _p(gt), _p(geo))
+    return dA, dB, kA, kB, gt, geo.astype(bool)
+
+
+def bf_match_top2_fast(A, B, nthreads=0):
+    A = _c(A, np.uint8)
+    B = _c(B, np.uint8)
+    ba, sa, bb = _bm(A.shape[0]), _bm(A.shape[0]), _bm(B.shape[0])
+    _lib.or_bf_match_top2_fast(_p(A), A.shape[0], _p(B), B.shape[0], _p(ba), _p(sa), _p(bb), nthreads)
+    return ba, sa, bb
 
 
 def sample_indices_hash(seed, n, H):

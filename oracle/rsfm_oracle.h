@@ -47,6 +47,13 @@ void or_gen_random_u8(uint64_t seed, uint32_t n, uint8_t* desc);
 void or_gen_two_view(uint64_t seed, uint32_t n, uint32_t n_out, double focal, double width,
                      double height, double noise_px, float* pts, uint8_t* is_inlier, double* F_true);
 
+/* one end-to-end image pair: descriptors (query A with n_true noisy copies of
+ * train B) and keypoints (x,y) whose true pairs follow one two-view geometry
+ * except an outlier_frac of them; gtA[i] = train index or -1. */
+void or_gen_match_pair(uint64_t seed, uint32_t nA, uint32_t nB, uint32_t n_true, double outlier_frac,
+                       double focal, double width, double height, double noise_px, uint8_t* descA,
+                       uint8_t* descB, float* kpA, float* kpB, int32_t* gtA, uint8_t* geo_inlier);
+
 /* sample index sets: H sets of k distinct indices in [0,n), non-duplicate
  * TableSampler semantics (ransac.hpp:52-82) driven by mt19937_64(seed). */
 void or_sample_indices(uint64_t seed, uint32_t n, uint32_t k, uint32_t H, uint32_t* idx);
@@ -71,6 +78,9 @@ void or_bf_match_u4(const uint8_t* A4, uint32_t nA, const uint8_t* B4, uint32_t 
 void or_bf_match_top2_u8(const uint8_t* A, uint32_t nA, const uint8_t* B, uint32_t nB,
                          or_best_match* bestA, or_best_match* secondA, or_best_match* bestB,
                          int nthreads);
+/* same results, each distance computed once (CPU baseline path) */
+void or_bf_match_top2_fast(const uint8_t* A, uint32_t nA, const uint8_t* B, uint32_t nB,
+                           or_best_match* bestA, or_best_match* secondA, or_best_match* bestB, int nthreads);
 /* dot-product ranking mode (higher is better, ties -> lowest index), top-2 */
 void or_bf_match_dot_top2_u8(const uint8_t* A, uint32_t nA, const uint8_t* B, uint32_t nB,
                              int32_t* best_dot, uint32_t* best_idx, int32_t* second_dot,

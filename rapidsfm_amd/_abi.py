@@ -12,7 +12,7 @@ class RsfmError(RuntimeError):
 
 if not os.path.exists(LIB_PATH):
     raise ImportError(
-        "librsfm_amd.so is not built (%s); run `python -m rapidsfm_amd.build` "
+        "librsfm_amd.so is not built (%s); run `python rapidsfm_amd/build.py` "
         "(or __graft_entry__.build()) first" % LIB_PATH
     )
 
@@ -64,6 +64,31 @@ class RansacTask(ctypes.Structure):
     ]
 
 
+class FilterTask(ctypes.Structure):
+    _fields_ = [
+        ("fwd", c_void_p),
+        ("second", c_void_p),
+        ("nbQueryDesc", c_u32),
+        ("bwd", c_void_p),
+        ("nbTrainDesc", c_u32),
+        ("out", c_void_p),
+        ("count", c_void_p),
+        ("kpQuery", c_void_p),
+        ("kpTrain", c_void_p),
+        ("pts", c_void_p),
+    ]
+
+
+class SampleTask(ctypes.Structure):
+    _fields_ = [
+        ("sampleIdx", c_void_p),
+        ("nbHyp", c_u32),
+        ("nbPts", c_u32),
+        ("nbPtsDev", c_void_p),
+        ("seed", ctypes.c_uint64),
+    ]
+
+
 assert ctypes.sizeof(BfTask) == 56
 assert ctypes.sizeof(BestMatch) == 8
 assert ctypes.sizeof(ValidMatch) == 12
@@ -88,6 +113,8 @@ _SIGS = {
     ),
     "rsfm_ransac_f8": (c_int, [c_void_p, c_size, c_float, c_int, c_float, c_float, c_void_p]),
     "rsfm_ransac_sample_indices": (c_int, [c_void_p, c_u32, c_u32, c_void_p, ctypes.c_uint64, c_void_p]),
+    "rsfm_filter_matches_batch": (c_int, [c_void_p, c_size, c_u32, c_u32, c_void_p]),
+    "rsfm_ransac_sample_indices_batch": (c_int, [c_void_p, c_size, c_void_p]),
 }
 
 

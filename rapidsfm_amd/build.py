@@ -1,6 +1,6 @@
 """Builds librsfm_amd.so (HIP, gfx950) in-tree with hipcc.
 
-Usage: python -m rapidsfm_amd.build  (or rapidsfm_amd.build.build()).
+Usage: python rapidsfm_amd/build.py  (or __graft_entry__.build()).
 Every source is compiled with contraction of multiply-add off, so the FP32
 RANSAC kernels round exactly like their CPU restatement, and with the
 packed-FP32 instructions disabled.

@@ -160,6 +160,63 @@ __device__ inline bool keep_match(const uint64_t* fwd, const uint64_t* second, c
     return true;
 }
 
+struct FilterTask {
+    const uint64_t* fwd;
+    const uint64_t* second;
+    uint32_t nQ;
+    const uint64_t* bwd;
+    uint32_t nT;
+    rsfm_valid_match* out;
+    uint32_t* count;
+    const float2* kpQ;
+    const float2* kpT;
+    float4* pts;
+};
+constexpr int kFilterMaxTasks = 32;
+struct FilterBatch {
+    FilterTask t[kFilterMaxTasks];
+};
+
+// one workgroup per task: flags, block-wide exclusive scan, ordered scatter of
+// the kept matches and (optionally) of their keypoint pairs
+__global__ __launch_bounds__(kFilterThreads) void filter_batch_kernel(const FilterBatch batch, uint32_t num, uint32_t den)
+{
+    const FilterTask& T = batch.t[blockIdx.x];
+    __shared__ uint32_t sScan[kFilterThreads];
+    const uint32_t nQ = T.nQ;
+    const uint32_t per = (nQ + kFilterThreads - 1) / kFilterThreads;
+    const uint32_t beg = min(nQ, threadIdx.x * per), end = min(nQ, beg + per);
+    uint32_t local = 0;
+    for (uint32_t i = beg; i < end; i++) local += keep_match(T.fwd, T.second, T.bwd, T.nT, i, num, den);
+    sScan[threadIdx.x] = local;
+    __syncthreads();
+    for (uint32_t off = 1; off < kFilterThreads; off <<= 1) {
+        const uint32_t v = threadIdx.x >= off ? sScan[threadIdx.x - off] : 0u;
+        __syncthreads();
+        sScan[threadIdx.x] += v;
+        __syncthreads();
+    }
+    uint32_t pos = sScan[threadIdx.x] - local;
+    for (uint32_t i = beg; i < end; i++) {
+        if (!keep_match(T.fwd, T.second, T.bwd, T.nT, i, num, den)) continue;
+        const uint64_t f = T.fwd[i];
+        const uint32_t t = (uint32_t)f;
+        if (T.out) {
+            rsfm_valid_match m;
+            m.idxQuery = i;
+            m.idxTrain = t;
+            m.distance = (float)(uint32_t)(f >> 32);
+            T.out[pos] = m;
+        }
+        if (T.pts) {
+            const float2 a = T.kpQ[i], b = T.kpT[t];
+            T.pts[pos] = make_float4(a.x, a.y, b.x, b.y);
+        }
+        pos++;
+    }
+    if (threadIdx.x == kFilterThreads - 1) *T.count = sScan[threadIdx.x];
+}
+
 __global__ __launch_bounds__(kFilterThreads) void filter_kernel(const uint64_t* __restrict__ fwd,
                                                                 const uint64_t* __restrict__ second, uint32_t nQ,
                                                                 const uint64_t* __restrict__ bwd, uint32_t nT,
@@ -282,4 +339,33 @@ extern "C" int rsfm_filter_matches(const rsfm_best_match* fwd, const rsfm_best_m
         reinterpret_cast<const uint64_t*>(fwd), reinterpret_cast<const uint64_t*>(second), nbQueryDesc,
         reinterpret_cast<const uint64_t*>(bwd), nbTrainDesc, ratioNum, ratioDen, out, count);
     return check_launch("filter_matches");
+}
+
+extern "C" int rsfm_filter_matches_batch(const rsfm_filter_task* tasks, size_t nbTasks, uint32_t ratioNum,
+                                         uint32_t ratioDen, hipStream_t stream)
+{
+    using namespace aux;
+    RSFM_RETURN_IF(nbTasks > 0 && tasks == nullptr, RSFM_ERR_INVALID, "filter batch: tasks is null");
+    RSFM_RETURN_IF(ratioNum == 0 || ratioDen == 0, RSFM_ERR_INVALID, "filter batch: ratio must be num/den > 0");
+    for (size_t b = 0; b < nbTasks; b += kFilterMaxTasks) {
+        const size_t n = std::min<size_t>(kFilterMaxTasks, nbTasks - b);
+        FilterBatch batch{};
+        for (size_t i = 0; i < n; i++) {
+            const rsfm_filter_task& t = tasks[b + i];
+            RSFM_RETURN_IF(t.count == nullptr, RSFM_ERR_INVALID, "filter batch: count is null");
+            RSFM_RETURN_IF(t.nbQueryDesc > 0 && t.fwd == nullptr, RSFM_ERR_INVALID, "filter batch: fwd is null");
+            RSFM_RETURN_IF(t.pts != nullptr && (t.kpQuery == nullptr || t.kpTrain == nullptr), RSFM_ERR_INVALID,
+                           "filter batch: pts needs kpQuery and kpTrain");
+            RSFM_RETURN_IF(reinterpret_cast<uintptr_t>(t.pts) % 16 || reinterpret_cast<uintptr_t>(t.kpQuery) % 8 ||
+                               reinterpret_cast<uintptr_t>(t.kpTrain) % 8,
+                           RSFM_ERR_INVALID, "filter batch: pts must be 16-byte and keypoints 8-byte aligned");
+            batch.t[i] = FilterTask{reinterpret_cast<const uint64_t*>(t.fwd), reinterpret_cast<const uint64_t*>(t.second),
+                                    t.nbQueryDesc, reinterpret_cast<const uint64_t*>(t.bwd), t.nbTrainDesc, t.out,
+                                    t.count, reinterpret_cast<const float2*>(t.kpQuery),
+                                    reinterpret_cast<const float2*>(t.kpTrain), reinterpret_cast<float4*>(t.pts)};
+        }
+        filter_batch_kernel<<<(uint32_t)n, kFilterThreads, 0, stream>>>(batch, ratioNum, ratioDen);
+        RSFM_CHECK(check_launch("filter_matches_batch"));
+    }
+    return RSFM_OK;
 }

@@ -180,6 +180,52 @@ __device__ inline uint64_t splitmix(uint64_t& s)
     return z ^ (z >> 31);
 }
 
+struct SampleTask {
+    uint32_t* idx;
+    uint32_t H;
+    uint32_t nHost;
+    const uint32_t* nDev;
+    uint64_t seed;
+};
+constexpr int kSampleMaxTasks = 32;
+struct SampleBatch {
+    SampleTask t[kSampleMaxTasks];
+};
+
+__global__ void sample_kernel(uint32_t* idx, uint32_t H, uint32_t nHost, const uint32_t* nDev, uint64_t seed);
+
+__global__ void sample_batch_kernel(const SampleBatch batch)
+{
+    const SampleTask& T = batch.t[blockIdx.y];
+    if (blockIdx.x * blockDim.x >= T.H) return;
+    // same draw as sample_kernel, task taken from the batch
+    const uint32_t h = blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= T.H) return;
+    const uint32_t n = T.nDev ? min(*T.nDev, T.nHost) : T.nHost;
+    uint32_t* out = T.idx + (size_t)h * 8;
+    if (n < 8) {
+        for (int i = 0; i < 8; i++) out[i] = ~0u;
+        return;
+    }
+    uint64_t s = T.seed ^ ((uint64_t)h * 0xD1B54A32D192ED03ull);
+    uint32_t v[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        for (;;) {
+            const uint32_t c = (uint32_t)(splitmix(s) % n);
+            bool dup = false;
+#pragma unroll
+            for (int j = 0; j < i; j++) dup |= v[j] == c;
+            if (!dup) {
+                v[i] = c;
+                break;
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 8; i++) out[i] = v[i];
+}
+
 __global__ void sample_kernel(uint32_t* idx, uint32_t H, uint32_t nHost, const uint32_t* nDev, uint64_t seed)
 {
     const uint32_t h = blockIdx.x * blockDim.x + threadIdx.x;
@@ -260,4 +306,25 @@ extern "C" int rsfm_ransac_sample_indices(uint32_t* sampleIdx, uint32_t nbHyp, u
     RSFM_RETURN_IF(sampleIdx == nullptr, RSFM_ERR_INVALID, "sample_indices: sampleIdx is null");
     ransac::sample_kernel<<<div_up(nbHyp, 256), 256, 0, stream>>>(sampleIdx, nbHyp, nbPts, nbPtsDev, seed);
     return check_launch("ransac sample_indices");
+}
+
+extern "C" int rsfm_ransac_sample_indices_batch(const rsfm_sample_task* tasks, size_t nbTasks, hipStream_t stream)
+{
+    using namespace ransac;
+    RSFM_RETURN_IF(nbTasks > 0 && tasks == nullptr, RSFM_ERR_INVALID, "sample batch: tasks is null");
+    for (size_t b = 0; b < nbTasks; b += kSampleMaxTasks) {
+        const size_t n = std::min<size_t>(kSampleMaxTasks, nbTasks - b);
+        SampleBatch batch{};
+        uint32_t maxH = 0;
+        for (size_t i = 0; i < n; i++) {
+            const rsfm_sample_task& t = tasks[b + i];
+            RSFM_RETURN_IF(t.nbHyp > 0 && t.sampleIdx == nullptr, RSFM_ERR_INVALID, "sample batch: sampleIdx is null");
+            batch.t[i] = SampleTask{t.sampleIdx, t.nbHyp, t.nbPts, t.nbPtsDev, t.seed};
+            maxH = std::max(maxH, t.nbHyp);
+        }
+        if (maxH == 0) continue;
+        sample_batch_kernel<<<dim3(div_up(maxH, 256), (uint32_t)n), 256, 0, stream>>>(batch);
+        RSFM_CHECK(check_launch("ransac sample_indices_batch"));
+    }
+    return RSFM_OK;
 }
