@@ -108,9 +108,10 @@ __global__ __launch_bounds__(kWaves * 64) void match_kernel(const TaskBatch batc
     constexpr int kThrPerCol = 128 / kStageBytes;
     static_assert(kStageBytes == 16 || kStageBytes == 32, "staging split");
 
-    const DevTask& T = batch.t[blockIdx.z];
-    const uint32_t rowBase = blockIdx.x * kBM;
-    const uint32_t colBeg = blockIdx.y * segCols;
+    const Wg3 wg = xcd_local_block();
+    const DevTask& T = batch.t[wg.z];
+    const uint32_t rowBase = wg.x * kBM;
+    const uint32_t colBeg = wg.y * segCols;
     if (rowBase >= T.nA || colBeg >= T.nB) return;
     const uint32_t colEnd = min(colBeg + segCols, T.nB);
     const uint32_t nChunks = (colEnd - colBeg + kBN - 1) / kBN;
@@ -338,7 +339,7 @@ __global__ __launch_bounds__(kWaves * 64) void match_kernel(const TaskBatch batc
                     T.bestA[row] = k1;
                     T.secondA[row] = k2;
                 } else {
-                    uint64_t* p = T.part + ((size_t)blockIdx.y * T.nA + row) * 2;
+                    uint64_t* p = T.part + ((size_t)wg.y * T.nA + row) * 2;
                     p[0] = k1;
                     p[1] = k2;
                 }

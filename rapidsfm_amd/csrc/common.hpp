@@ -56,4 +56,21 @@ __host__ __device__ inline uint64_t make_key(uint32_t dist, uint32_t idx)
 }
 constexpr uint64_t kNoMatch = ~0ull;
 
+// The dispatcher hands consecutive workgroups to the 8 XCDs round-robin, and
+// each XCD has its own L2. Remapping the linear workgroup id so that runs of
+// consecutive logical ids execute on one XCD keeps the workgroups that share
+// an operand (all row blocks of one image pair) on the same L2.
+constexpr uint32_t kNumXcd = 8;
+struct Wg3 {
+    uint32_t x, y, z;
+};
+__device__ inline Wg3 xcd_local_block()
+{
+    const uint32_t gx = gridDim.x, gy = gridDim.y;
+    const uint32_t total = gx * gy * gridDim.z;
+    uint32_t L = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+    if (total % kNumXcd == 0) L = (L % kNumXcd) * (total / kNumXcd) + L / kNumXcd;
+    return Wg3{L % gx, (L / gx) % gy, L / (gx * gy)};
+}
+
 }  // namespace rsfm_amd

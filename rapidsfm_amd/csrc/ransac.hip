@@ -78,8 +78,9 @@ __device__ inline bool fit_hypothesis(const DevTask& T, uint32_t n, uint32_t h, 
 template <int kMode>
 __global__ __launch_bounds__(kThreads) void score_kernel(const Batch batch, float t2, float s0, float s1)
 {
-    const DevTask& T = batch.t[blockIdx.y];
-    const uint32_t hBase = blockIdx.x * kThreads;
+    const Wg3 wg = xcd_local_block();  // all hypothesis blocks of a pair share one L2
+    const DevTask& T = batch.t[wg.y];
+    const uint32_t hBase = wg.x * kThreads;
     if (hBase >= T.H) return;
     const uint32_t n = task_n(T);
     __shared__ float sF[kThreads][9];
