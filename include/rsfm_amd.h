@@ -110,6 +110,68 @@ int rsfm_bf_match_top2(const rsfm_bf_top2_task* tasks, size_t nbTasks, void* wor
                        size_t workspaceBytes, hipStream_t stream);
 
 /* ---------------------------------------------------------------------------
+ * Cascade hashing (rsfm::cashash, casHash/cascadeHash.h:32-107)
+ * Weights are int8 [nbChannels][128] (the reference's KArray<int32_t,64,32>
+ * bytes); codes ("bucketIndices") are u8[8] per descriptor, code byte s =
+ * channels 8s..8s+7 (bit b = channel 8s+b); remap is u32[4] per descriptor,
+ * word r bit b = channel 64+32r+b. Histograms and bounds are u32[8][256].
+ * ------------------------------------------------------------------------ */
+
+/* replaces cashash::accumuateDesc (accumulateDesc.cu:117-131): acc[k] (+)=
+ * sum over descriptors of (byte_k >> 1); sqrAcc (optional) the squares */
+int rsfm_cashash_accumulate_desc(uint64_t* acc, uint64_t* sqrAcc, int initAcc, const void* desc, uint32_t nbDesc,
+                                 hipStream_t stream);
+/* replaces cashash::computeBias (accumulateDesc.cu:176-182): bias[c] =
+ * int(roundf(sum_k w[c][k] * (-(float)acc[k] / (float)nbDesc))) + biasBase[c],
+ * FP32 with fused multiply-add in k order per 16-term lane and an xor tree over
+ * 8 lanes. nbChannels = 64 (reference) or 192 (bucket + remap channels). */
+int rsfm_cashash_compute_bias(int32_t* bias, const int8_t* weights, const int32_t* biasBase, const uint64_t* acc,
+                              uint64_t nbDesc, uint32_t nbChannels, hipStream_t stream);
+/* replaces cashash::computeHashForBucketOnly (computeHash.cu:498-520): 64 channels;
+ * histogram (optional) is cleared and filled */
+int rsfm_cashash_compute_hash_bucket_only(const int8_t* weights, const int32_t* bias, const void* desc, uint32_t nbDesc,
+                                          uint8_t* bucketIndices, uint32_t* histogram, hipStream_t stream);
+/* replaces cashash::computeHashForBucketAndRemap (computeHash.cu:474-496): 192 channels */
+int rsfm_cashash_compute_hash_bucket_and_remap(const int8_t* weights, const int32_t* bias, const void* desc,
+                                               uint32_t nbDesc, uint8_t* bucketIndices, uint32_t* remap,
+                                               uint32_t* histogram, hipStream_t stream);
+/* replaces cashash::buildHistogram (buildBuckets.cu:107-114) */
+int rsfm_cashash_build_histogram(uint32_t* histogram, const uint8_t* bucketIndices, uint32_t nbDesc, hipStream_t stream);
+/* replaces cashash::computeBucketBound (buildBuckets.cu:169-172): lower =
+ * exclusive, upper = inclusive prefix per scheme; *maxBucketSize (device,
+ * optional) the largest bucket */
+int rsfm_cashash_compute_bucket_bound(uint32_t* lower, uint32_t* upper, uint32_t* maxBucketSize,
+                                      const uint32_t* histogram, hipStream_t stream);
+/* replaces cashash::buildBuckets (buildBuckets.cu:262-269): buckets[s][slot]
+ * = descriptor index; idxAllocator starts as `lower` and is consumed; order
+ * inside a bucket is unspecified, as in the reference */
+int rsfm_cashash_build_buckets(const uint8_t* bucketIndices, uint32_t nbDesc, uint32_t* idxAllocator, uint32_t* buckets,
+                               hipStream_t stream);
+
+/* cashash::ImageDesc (cascadeHash.h:60-92), 32 B */
+typedef struct rsfm_cashash_image {
+    const void* desc;
+    const uint32_t* bucketEnds; /* [8][256] inclusive prefix (upper bounds) */
+    const uint32_t* buckets;    /* [8][nbDesc] */
+    const uint32_t* sqrNormDesc;
+} rsfm_cashash_image;
+/* cashash::MatchResult (cascadeHash.h:95-99), 16 B */
+typedef struct rsfm_cashash_result {
+    rsfm_best_match* query; /* indices into train */
+    rsfm_best_match* train; /* indices into query, may be NULL */
+} rsfm_cashash_result;
+/* replaces cashash::findBestMatch (findBestMatch.cu:654-676): for every query
+ * descriptor the lexicographic minimum of (L2^2, train index) over the train
+ * descriptors sharing a bucket in any of the first nbSchemes schemes (8 in the
+ * reference), and symmetrically for train; no candidate -> {~0u, UINT32_MAX}.
+ * The max bucket sizes are accepted for interface compatibility and unused. */
+int rsfm_cashash_find_best_match(const rsfm_cashash_image* queries, const uint32_t* querySize,
+                                 const rsfm_cashash_image* train, uint32_t trainSize,
+                                 const rsfm_cashash_result* results, uint32_t nbTasks,
+                                 const uint32_t* maxQueryBucketSize, uint32_t maxTrainBucketSize, uint32_t nbSchemes,
+                                 hipStream_t stream);
+
+/* ---------------------------------------------------------------------------
  * Match filtering
  * ------------------------------------------------------------------------ */
 

@@ -227,6 +227,84 @@ _decl("or_gen_match_pair", None, [U64, U32, U32, U32, F64, F64, F64, F64, F64, P
 _decl("or_bf_match_top2_fast", None, [P, U32, P, U32, P, P, P, I32])
 
 
+# ---------------- cascade hashing ----------------
+def cashash_accumulate(desc, acc=None):
+    desc = _c(desc, np.uint8)
+    acc = np.zeros(128, np.uint64) if acc is None else acc.copy()
+    _lib.or_cashash_accumulate(_p(desc), desc.shape[0], _p(acc))
+    return acc
+
+
+def cashash_compute_bias(w, bias_base, acc, n_desc):
+    w = _c(w, np.int8).reshape(-1, 128)
+    bias_base = _c(bias_base, np.int32)
+    acc = _c(acc, np.uint64)
+    out = np.zeros(w.shape[0], np.int32)
+    _lib.or_cashash_compute_bias(_p(w), _p(bias_base), _p(acc), n_desc, w.shape[0], _p(out))
+    return out
+
+
+def cashash_compute_hash(w, bias, desc):
+    w = _c(w, np.int8).reshape(-1, 128)
+    bias = _c(bias, np.int32)
+    desc = _c(desc, np.uint8)
+    n = desc.shape[0]
+    codes = np.zeros((n, 8), np.uint8)
+    remap = np.zeros((n, 4), np.uint32) if w.shape[0] > 64 else None
+    _lib.or_cashash_compute_hash(_p(w), _p(bias), w.shape[0], _p(desc), n, _p(codes), _p(remap))
+    return codes, remap
+
+
+def cashash_histogram(codes):
+    codes = _c(codes, np.uint8)
+    h = np.zeros((8, 256), np.uint32)
+    _lib.or_cashash_histogram(_p(codes), codes.shape[0], _p(h))
+    return h
+
+
+def cashash_bucket_bounds(hist):
+    hist = _c(hist, np.uint32)
+    lo = np.zeros((8, 256), np.uint32)
+    up = np.zeros((8, 256), np.uint32)
+    mx = _lib.or_cashash_bucket_bounds(_p(hist), _p(lo), _p(up))
+    return lo, up, mx
+
+
+def cashash_build_buckets(codes, lower):
+    codes = _c(codes, np.uint8)
+    lower = _c(lower, np.uint32)
+    n = codes.shape[0]
+    b = np.zeros((8, n), np.uint32)
+    _lib.or_cashash_build_buckets(_p(codes), n, _p(lower), _p(b))
+    return b
+
+
+def cashash_find_best(qdesc, qcodes, tdesc, tcodes, nscheme=8, both=True):
+    qdesc, tdesc = _c(qdesc, np.uint8), _c(tdesc, np.uint8)
+    qcodes, tcodes = _c(qcodes, np.uint8), _c(tcodes, np.uint8)
+    bq = _bm(qdesc.shape[0])
+    bt = _bm(tdesc.shape[0]) if both else None
+    _lib.or_cashash_find_best(_p(qdesc), _p(qcodes), qdesc.shape[0], _p(tdesc), _p(tcodes), tdesc.shape[0], nscheme,
+                              _p(bq), _p(bt))
+    return bq, bt
+
+
+def cashash_find_best_rerank(qdesc, qcodes, qremap, tdesc, tcodes, tremap, nscheme=6, keep=8):
+    args = [_c(a, dt) for a, dt in ((qdesc, np.uint8), (qcodes, np.uint8), (qremap, np.uint32), (tdesc, np.uint8),
+                                    (tcodes, np.uint8), (tremap, np.uint32))]
+    bq = _bm(args[0].shape[0])
+    _lib.or_cashash_find_best_rerank(_p(args[0]), _p(args[1]), _p(args[2]), args[0].shape[0], _p(args[3]),
+                                     _p(args[4]), _p(args[5]), args[3].shape[0], nscheme, keep, _p(bq))
+    return bq
+
+
+def cashash_gaussian_weights(seed, nch):
+    w = np.zeros((nch, 128), np.int8)
+    b = np.zeros(nch, np.int32)
+    _lib.or_cashash_gaussian_weights(seed, nch, _p(w), _p(b))
+    return w, b
+
+
 def gen_match_pair(seed, nA, nB, n_true, outlier_frac, focal=2000.0, width=4000.0, height=3000.0, noise_px=0.5):
     """descA (nA,128), descB (nB,128), kpA (nA,2), kpB (nB,2), gtA (nA,), geo_inlier (nA,)"""
     dA = np.zeros((nA, 128), np.uint8)

@@ -22,6 +22,7 @@ SOURCES = [
     "bf_match.hip",
     "bf_aux.hip",
     "ransac.hip",
+    "cashash.hip",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -1,0 +1,542 @@
+// cashash.hip - cascade hashing on gfx950.
+//
+// Reference (rsfm::cashash, casHash/cascadeHash.h:32-107):
+//  accumuateDesc     casHash/accumulateDesc.cu:33-131   acc[k] += byte_k >> 1 over all descriptors
+//  computeBias       casHash/accumulateDesc.cu:136-182  bias[c] = round(sum_k w[c][k] * -acc[k]/n) + biasBase[c]
+//  computeHash*      casHash/computeHash.cu:437-520     bit c = bias[c] + sum_k w[c][k] (byte_k >> 1) >= 0
+//  buildHistogram    casHash/buildBuckets.cu:29-114
+//  computeBucketBound casHash/buildBuckets.cu:116-172
+//  buildBuckets      casHash/buildBuckets.cu:178-269    bucket CSR (order inside a bucket unspecified)
+//  findBestMatch     casHash/findBestMatch.cu:570-676   exact L2^2 over the union of same-bucket candidates
+//
+// GPU design (own):
+//  * the hash projection is an [n x 128] x [128 x nch] int8 GEMM on
+//    v_mfma_i32_32x32x32_i8; after adding the bias, one wave ballot per
+//    accumulator register yields 32 channel bits for two descriptors at once,
+//    already in the code-byte / remap-word bit order.
+//  * histogram, bounds and buckets use LDS atomics and one global reservation
+//    per (scheme, bucket) per workgroup.
+//  * matching packs each (scheme, bucket) candidate block into 32 x 32 MFMA
+//    tiles (average bucket size at 8K descriptors is 32, so a tile is mostly
+//    full, against 128 x 128 tiles in the reference); one wave per
+//    (task, scheme, bucket); the distance tile is transposed through LDS and
+//    each lane owns one query row or one train column, merged with 64-bit
+//    atomicMin on (distance, global index) keys.
+#include "common.hpp"
+#include <algorithm>
+
+namespace rsfm_amd {
+namespace cashash {
+
+constexpr uint32_t kSchemes = 8;
+constexpr uint32_t kBuckets = 256;
+
+// ---------------------------------------------------------------------------
+// accumulate (byte >> 1) per dimension into u64[128]
+__global__ void zero_u64_kernel(uint64_t* p, uint32_t n)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+constexpr int kAccThreads = 256;
+constexpr int kAccDescPerBlock = 512;
+
+__global__ __launch_bounds__(kAccThreads) void accumulate_kernel(uint64_t* __restrict__ acc, uint64_t* __restrict__ sqrAcc,
+                                                                 const uint8_t* __restrict__ desc, uint32_t n)
+{
+    // thread -> (16-byte column group tid % 8, descriptor subset tid / 8)
+    __shared__ uint32_t sAcc[128];
+    __shared__ uint32_t sSqr[128];
+    if (threadIdx.x < 128) {
+        sAcc[threadIdx.x] = 0;
+        sSqr[threadIdx.x] = 0;
+    }
+    __syncthreads();
+    const uint32_t part = threadIdx.x & 7, sub = threadIdx.x >> 3;
+    uint32_t a[16], q[16];
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+        a[k] = 0;
+        q[k] = 0;
+    }
+    const uint32_t d0 = blockIdx.x * kAccDescPerBlock;
+    for (uint32_t d = d0 + sub; d < min(n, d0 + kAccDescPerBlock); d += kAccThreads / 8) {
+        const uint4 x = *reinterpret_cast<const uint4*>(desc + (size_t)d * 128 + part * 16);
+        const uint32_t w[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+#pragma unroll
+            for (int b = 0; b < 4; b++) {
+                const uint32_t v = ((w[j] >> (8 * b)) & 0xFF) >> 1;
+                a[j * 4 + b] += v;
+                q[j * 4 + b] += v * v;
+            }
+    }
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+        atomicAdd(&sAcc[part * 16 + k], a[k]);
+        if (sqrAcc) atomicAdd(&sSqr[part * 16 + k], q[k]);
+    }
+    __syncthreads();
+    if (threadIdx.x < 128) {
+        atomicAdd(reinterpret_cast<unsigned long long*>(acc + threadIdx.x), (unsigned long long)sAcc[threadIdx.x]);
+        if (sqrAcc)
+            atomicAdd(reinterpret_cast<unsigned long long*>(sqrAcc + threadIdx.x), (unsigned long long)sSqr[threadIdx.x]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// bias: FP32, 8 lanes per channel each summing 16 products in k order with
+// fused multiply-add, then an xor tree over the 8 lanes (4, 2, 1)
+__global__ void bias_kernel(int32_t* __restrict__ bias, const int8_t* __restrict__ w, const int32_t* __restrict__ biasBase,
+                            const uint64_t* __restrict__ acc, uint64_t n, uint32_t nch)
+{
+    __shared__ float sNeg[128];
+    if (threadIdx.x < 128) sNeg[threadIdx.x] = -(float)acc[threadIdx.x] / (float)n;
+    __syncthreads();
+    const uint32_t c = blockIdx.x * (blockDim.x >> 3) + (threadIdx.x >> 3), L = threadIdx.x & 7;
+    if (c >= nch) return;  // whole groups of 8 lanes leave together
+    float v = 0.f;
+#pragma unroll
+    for (int k = 0; k < 16; k++) v = __builtin_fmaf(sNeg[L * 16 + k], (float)w[c * 128 + L * 16 + k], v);
+    v = v + __shfl_xor(v, 4);
+    v = v + __shfl_xor(v, 2);
+    v = v + __shfl_xor(v, 1);
+    if (L == 0) bias[c] = (int32_t)roundf(v) + biasBase[c];
+}
+
+// ---------------------------------------------------------------------------
+// hash: one wave per 32 descriptors, all channel blocks of 32
+constexpr int kHashWaves = 4;
+
+template <int kChBlocks>
+__global__ __launch_bounds__(kHashWaves * 64) void hash_kernel(const int8_t* __restrict__ w, const int32_t* __restrict__ bias,
+                                                                const uint8_t* __restrict__ desc, uint32_t n,
+                                                                uint32_t* __restrict__ codes, uint32_t* __restrict__ remap,
+                                                                uint32_t* __restrict__ hist)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int l32 = lane & 31, hi = lane >> 5;
+    const uint32_t rowBase = (blockIdx.x * kHashWaves + wave) * 32;
+    if (rowBase >= n) return;
+    const uint32_t row = rowBase + l32;
+    const bool valid = row < n;
+    v4i aop[4];
+#pragma unroll
+    for (int kc = 0; kc < 4; kc++) {
+        v4i x = {0, 0, 0, 0};
+        if (valid) x = *reinterpret_cast<const v4i*>(desc + (size_t)row * 128 + kc * 32 + hi * 16);
+#pragma unroll
+        for (int q = 0; q < 4; q++) x[q] = (int)(((uint32_t)x[q] >> 1) & 0x7F7F7F7Fu);
+        aop[kc] = x;
+    }
+    uint32_t myWord[kChBlocks];
+#pragma unroll
+    for (int cb = 0; cb < kChBlocks; cb++) {
+        const int ch = cb * 32 + l32;
+        v16i acc = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+        for (int kc = 0; kc < 4; kc++) {
+            const v4i bop = *reinterpret_cast<const v4i*>(w + (size_t)ch * 128 + kc * 32 + hi * 16);
+            acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(aop[kc], bop, acc, 0, 0, 0);
+        }
+        const int32_t bc = bias[ch];
+        uint32_t mine = 0;
+#pragma unroll
+        for (int v = 0; v < 16; v++) {
+            // lanes 0..31 hold row r0 = 8(v/4)+(v%4), lanes 32..63 row r0 + 4;
+            // lane%32 is the channel inside this block
+            const uint64_t b = __ballot(acc[v] + bc >= 0);
+            const int r0 = 8 * (v >> 2) + (v & 3);
+            if (l32 == r0) mine = (uint32_t)b;
+            if (l32 == r0 + 4) mine = (uint32_t)(b >> 32);
+        }
+        myWord[cb] = mine;
+    }
+    if (valid && hi == 0) {
+        codes[(size_t)row * 2 + 0] = myWord[0];
+        codes[(size_t)row * 2 + 1] = myWord[1];
+        if constexpr (kChBlocks > 2) {
+            if (remap) {
+#pragma unroll
+                for (int r = 0; r < 4; r++) remap[(size_t)row * 4 + r] = myWord[2 + r];
+            }
+        }
+        if (hist) {
+#pragma unroll
+            for (int s = 0; s < 8; s++) atomicAdd(&hist[s * 256 + ((myWord[s >> 2] >> (8 * (s & 3))) & 0xFF)], 1u);
+        }
+    }
+}
+
+__global__ void zero_u32_kernel(uint32_t* p, uint32_t n)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+// ---------------------------------------------------------------------------
+constexpr int kHistThreads = 256;
+constexpr int kHistPerBlock = 2048;
+
+__global__ __launch_bounds__(kHistThreads) void histogram_kernel(uint32_t* __restrict__ hist, const uint8_t* __restrict__ codes,
+                                                                 uint32_t n)
+{
+    __shared__ uint32_t sH[kSchemes * kBuckets];
+    for (uint32_t i = threadIdx.x; i < kSchemes * kBuckets; i += kHistThreads) sH[i] = 0;
+    __syncthreads();
+    const uint32_t d0 = blockIdx.x * kHistPerBlock;
+    for (uint32_t d = d0 + threadIdx.x; d < min(n, d0 + kHistPerBlock); d += kHistThreads) {
+        const uint2 c = *reinterpret_cast<const uint2*>(codes + (size_t)d * 8);
+#pragma unroll
+        for (int s = 0; s < 8; s++) atomicAdd(&sH[s * 256 + (((s < 4 ? c.x : c.y) >> (8 * (s & 3))) & 0xFF)], 1u);
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < kSchemes * kBuckets; i += kHistThreads)
+        if (sH[i]) atomicAdd(&hist[i], sH[i]);
+}
+
+// exclusive (lower) and inclusive (upper) prefix per scheme, largest bucket
+__global__ __launch_bounds__(256) void bounds_kernel(uint32_t* __restrict__ lower, uint32_t* __restrict__ upper,
+                                                     uint32_t* __restrict__ maxBucket, const uint32_t* __restrict__ hist)
+{
+    __shared__ uint32_t sScan[256];
+    __shared__ uint32_t sMax[256];
+    uint32_t mx = 0;
+    for (int s = 0; s < 8; s++) {
+        const uint32_t h = hist[s * 256 + threadIdx.x];
+        mx = max(mx, h);
+        sScan[threadIdx.x] = h;
+        __syncthreads();
+        for (int off = 1; off < 256; off <<= 1) {
+            const uint32_t v = threadIdx.x >= (uint32_t)off ? sScan[threadIdx.x - off] : 0u;
+            __syncthreads();
+            sScan[threadIdx.x] += v;
+            __syncthreads();
+        }
+        upper[s * 256 + threadIdx.x] = sScan[threadIdx.x];
+        lower[s * 256 + threadIdx.x] = sScan[threadIdx.x] - h;
+        __syncthreads();
+    }
+    sMax[threadIdx.x] = mx;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < (uint32_t)off) sMax[threadIdx.x] = max(sMax[threadIdx.x], sMax[threadIdx.x + off]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && maxBucket) *maxBucket = sMax[0];
+}
+
+// two-pass scatter: block histogram, one global slot reservation per
+// (scheme, bucket), then LDS slot counters
+constexpr int kBuildThreads = 256;
+constexpr int kBuildPerBlock = 1024;
+
+__global__ __launch_bounds__(kBuildThreads) void buckets_kernel(const uint8_t* __restrict__ codes, uint32_t n,
+                                                               uint32_t* __restrict__ alloc, uint32_t* __restrict__ buckets)
+{
+    __shared__ uint32_t sH[kSchemes * kBuckets];
+    for (uint32_t i = threadIdx.x; i < kSchemes * kBuckets; i += kBuildThreads) sH[i] = 0;
+    __syncthreads();
+    const uint32_t d0 = blockIdx.x * kBuildPerBlock;
+    const uint32_t d1 = min(n, d0 + kBuildPerBlock);
+    for (uint32_t d = d0 + threadIdx.x; d < d1; d += kBuildThreads) {
+        const uint2 c = *reinterpret_cast<const uint2*>(codes + (size_t)d * 8);
+#pragma unroll
+        for (int s = 0; s < 8; s++) atomicAdd(&sH[s * 256 + (((s < 4 ? c.x : c.y) >> (8 * (s & 3))) & 0xFF)], 1u);
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < kSchemes * kBuckets; i += kBuildThreads)
+        sH[i] = sH[i] ? atomicAdd(&alloc[i], sH[i]) : 0u;
+    __syncthreads();
+    for (uint32_t d = d0 + threadIdx.x; d < d1; d += kBuildThreads) {
+        const uint2 c = *reinterpret_cast<const uint2*>(codes + (size_t)d * 8);
+#pragma unroll
+        for (int s = 0; s < 8; s++) {
+            const uint32_t b = ((s < 4 ? c.x : c.y) >> (8 * (s & 3))) & 0xFF;
+            const uint32_t slot = atomicAdd(&sH[s * 256 + b], 1u);
+            buckets[(size_t)s * n + slot] = d;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// matching
+struct DevImage {
+    const uint8_t* desc;
+    const uint32_t* ends;     // [8][256] inclusive prefix
+    const uint32_t* buckets;  // [8][nDesc]
+    uint32_t n;
+};
+constexpr int kMatchMaxTasks = 16;
+struct MatchBatch {
+    DevImage q[kMatchMaxTasks];
+    uint64_t* resQ[kMatchMaxTasks];
+    uint64_t* resT[kMatchMaxTasks];
+    DevImage t;
+};
+
+__device__ inline void bucket_range(const DevImage& im, uint32_t s, uint32_t b, uint32_t& beg, uint32_t& cnt)
+{
+    const uint32_t e = im.ends[s * 256 + b];
+    beg = b == 0 ? 0u : im.ends[s * 256 + b - 1];
+    cnt = e - beg;
+}
+
+// 16-byte chunk of the (index-gathered) descriptor, operand-transformed
+__device__ inline v4i gather16(const DevImage& im, uint32_t s, uint32_t beg, uint32_t idx, uint32_t cnt, int off, int xorMask,
+                               bool& valid, uint32_t& gid)
+{
+    valid = idx < cnt;
+    v4i x = {0, 0, 0, 0};
+    gid = valid ? im.buckets[(size_t)s * im.n + beg + idx] : 0u;
+    if (valid) {
+        x = *reinterpret_cast<const v4i*>(im.desc + (size_t)gid * 128 + off);
+#pragma unroll
+        for (int q = 0; q < 4; q++) x[q] ^= xorMask;
+    }
+    return x;
+}
+
+__global__ __launch_bounds__(64) void match_kernel(const MatchBatch batch, uint32_t nSchemes)
+{
+    const uint32_t b = blockIdx.x, s = blockIdx.y, task = blockIdx.z;
+    const DevImage& Q = batch.q[task];
+    const DevImage& T = batch.t;
+    uint64_t* resQ = batch.resQ[task];
+    uint64_t* resT = batch.resT[task];
+    if (s >= nSchemes || Q.n == 0 || T.n == 0) return;
+    uint32_t qBeg, qCnt, tBeg, tCnt;
+    bucket_range(Q, s, b, qBeg, qCnt);
+    bucket_range(T, s, b, tBeg, tCnt);
+    if (qCnt == 0 || tCnt == 0) return;
+    __shared__ uint32_t sD[32][33];
+    __shared__ uint32_t sQid[32], sTid[32];
+    const int lane = threadIdx.x, l32 = lane & 31, hi = lane >> 5;
+    for (uint32_t q0 = 0; q0 < qCnt; q0 += 32) {
+        // query rows: u = 127 - a, NA' = |a - 128|^2
+        v4i aop[4];
+        bool qv;
+        uint32_t qid;
+        int32_t nap = 0;
+#pragma unroll
+        for (int kc = 0; kc < 4; kc++) {
+            aop[kc] = gather16(Q, s, qBeg, q0 + l32, qCnt, kc * 32 + hi * 16, 0x7F7F7F7F, qv, qid);
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int xa = aop[kc][q] ^ (int)0xFFFFFFFFu;  // (a ^ 0x7F) ^ 0xFF = a ^ 0x80
+                nap = qv ? __builtin_amdgcn_sdot4(xa, xa, nap, false) : 0;
+            }
+        }
+        const int32_t na = nap + __shfl_xor(nap, 32);
+        for (uint32_t t0 = 0; t0 < tCnt; t0 += 32) {
+            v4i bop[4];
+            bool tv;
+            uint32_t tid;
+            int32_t nbp = 0;
+#pragma unroll
+            for (int kc = 0; kc < 4; kc++) {
+                bop[kc] = gather16(T, s, tBeg, t0 + l32, tCnt, kc * 32 + hi * 16, (int)0x80808080u, tv, tid);
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    nbp = __builtin_amdgcn_sdot4(bop[kc][q], bop[kc][q], nbp, false);
+                    nbp = __builtin_amdgcn_sdot4(bop[kc][q], 0x02020202, nbp, false);
+                }
+            }
+            const int32_t nb = nbp + __shfl_xor(nbp, 32);  // NB' = |b - 127|^2 - 128
+            v16i acc = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+            for (int kc = 0; kc < 4; kc++) acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(aop[kc], bop[kc], acc, 0, 0, 0);
+            __syncthreads();
+            if (hi == 0) {
+                sQid[l32] = qv ? qid : ~0u;
+                sTid[l32] = tv ? tid : ~0u;
+            }
+            // lane holds column l32 (train), rows r(v); NA' of row r from lane r
+#pragma unroll
+            for (int v = 0; v < 16; v++) {
+                const int r = 8 * (v >> 2) + 4 * hi + (v & 3);
+                const int32_t nar = __shfl(na, r);
+                sD[r][l32] = (uint32_t)(nar + nb + 2 * acc[v]);
+            }
+            __syncthreads();
+            if (hi == 0) {
+                // lane l32 < 32: query row l32 -> best train column
+                const uint32_t gq = sQid[l32];
+                if (gq != ~0u) {
+                    uint64_t best = kNoMatch;
+                    for (int c = 0; c < 32; c++) {
+                        const uint32_t gt = sTid[c];
+                        if (gt == ~0u) continue;
+                        best = min(best, make_key(sD[l32][c], gt));
+                    }
+                    if (best != kNoMatch) atomicMin(reinterpret_cast<unsigned long long*>(resQ + gq), (unsigned long long)best);
+                }
+            } else if (resT) {
+                // lane 32 + c: train column c -> best query row
+                const uint32_t gt = sTid[l32];
+                if (gt != ~0u) {
+                    uint64_t best = kNoMatch;
+                    for (int r = 0; r < 32; r++) {
+                        const uint32_t gq = sQid[r];
+                        if (gq == ~0u) continue;
+                        best = min(best, make_key(sD[r][l32], gq));
+                    }
+                    if (best != kNoMatch) atomicMin(reinterpret_cast<unsigned long long*>(resT + gt), (unsigned long long)best);
+                }
+            }
+        }
+    }
+}
+
+__global__ void match_init_kernel(const MatchBatch batch)
+{
+    const uint32_t task = blockIdx.z;
+    const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x < batch.q[task].n) batch.resQ[task][x] = kNoMatch;
+    if (batch.resT[task] != nullptr && x < batch.t.n) batch.resT[task][x] = kNoMatch;
+}
+
+}  // namespace cashash
+}  // namespace rsfm_amd
+
+using namespace rsfm_amd;
+
+extern "C" int rsfm_cashash_accumulate_desc(uint64_t* acc, uint64_t* sqrAcc, int initAcc, const void* desc, uint32_t nbDesc,
+                                            hipStream_t stream)
+{
+    RSFM_RETURN_IF(acc == nullptr, RSFM_ERR_INVALID, "accumulate_desc: acc is null");
+    RSFM_RETURN_IF(nbDesc > 0 && (desc == nullptr || reinterpret_cast<uintptr_t>(desc) % 16), RSFM_ERR_INVALID,
+                   "accumulate_desc: desc must be non-null and 16-byte aligned");
+    if (initAcc) {
+        cashash::zero_u64_kernel<<<1, 128, 0, stream>>>(acc, 128);
+        if (sqrAcc) cashash::zero_u64_kernel<<<1, 128, 0, stream>>>(sqrAcc, 128);
+        RSFM_CHECK(check_launch("accumulate_desc init"));
+    }
+    if (nbDesc == 0) return RSFM_OK;
+    cashash::accumulate_kernel<<<div_up(nbDesc, cashash::kAccDescPerBlock), cashash::kAccThreads, 0, stream>>>(
+        acc, sqrAcc, static_cast<const uint8_t*>(desc), nbDesc);
+    return check_launch("accumulate_desc");
+}
+
+extern "C" int rsfm_cashash_compute_bias(int32_t* bias, const int8_t* weights, const int32_t* biasBase, const uint64_t* acc,
+                                         uint64_t nbDesc, uint32_t nbChannels, hipStream_t stream)
+{
+    RSFM_RETURN_IF(bias == nullptr || weights == nullptr || biasBase == nullptr || acc == nullptr, RSFM_ERR_INVALID,
+                   "compute_bias: null pointer");
+    RSFM_RETURN_IF(nbChannels != 64 && nbChannels != 192, RSFM_ERR_INVALID, "compute_bias: nbChannels must be 64 or 192");
+    RSFM_RETURN_IF(nbDesc == 0, RSFM_ERR_INVALID, "compute_bias: nbDesc must be > 0");
+    cashash::bias_kernel<<<nbChannels / 64, 512, 0, stream>>>(bias, weights, biasBase, acc, nbDesc, nbChannels);
+    return check_launch("compute_bias");
+}
+
+static int launch_hash(const int8_t* weights, const int32_t* bias, const void* desc, uint32_t nbDesc, uint8_t* bucketIndices,
+                       uint32_t* remap, uint32_t* histogram, hipStream_t stream, bool withRemap)
+{
+    RSFM_RETURN_IF(weights == nullptr || bias == nullptr, RSFM_ERR_INVALID, "compute_hash: weights/bias null");
+    RSFM_RETURN_IF(nbDesc > 0 && (desc == nullptr || bucketIndices == nullptr), RSFM_ERR_INVALID,
+                   "compute_hash: desc/bucketIndices null");
+    RSFM_RETURN_IF(withRemap && nbDesc > 0 && remap == nullptr, RSFM_ERR_INVALID, "compute_hash: remap null");
+    RSFM_RETURN_IF((reinterpret_cast<uintptr_t>(desc) | reinterpret_cast<uintptr_t>(weights)) % 16 ||
+                       reinterpret_cast<uintptr_t>(bucketIndices) % 8 || reinterpret_cast<uintptr_t>(remap) % 16,
+                   RSFM_ERR_INVALID, "compute_hash: misaligned pointer");
+    if (histogram) {
+        cashash::zero_u32_kernel<<<8, 256, 0, stream>>>(histogram, 2048);
+        RSFM_CHECK(check_launch("compute_hash histogram init"));
+    }
+    if (nbDesc == 0) return RSFM_OK;
+    const uint32_t blocks = div_up(nbDesc, 32 * cashash::kHashWaves);
+    uint32_t* codes = reinterpret_cast<uint32_t*>(bucketIndices);
+    if (withRemap)
+        cashash::hash_kernel<6><<<blocks, cashash::kHashWaves * 64, 0, stream>>>(
+            weights, bias, static_cast<const uint8_t*>(desc), nbDesc, codes, remap, histogram);
+    else
+        cashash::hash_kernel<2><<<blocks, cashash::kHashWaves * 64, 0, stream>>>(
+            weights, bias, static_cast<const uint8_t*>(desc), nbDesc, codes, nullptr, histogram);
+    return check_launch("compute_hash");
+}
+
+extern "C" int rsfm_cashash_compute_hash_bucket_only(const int8_t* weights, const int32_t* bias, const void* desc,
+                                                     uint32_t nbDesc, uint8_t* bucketIndices, uint32_t* histogram,
+                                                     hipStream_t stream)
+{
+    return launch_hash(weights, bias, desc, nbDesc, bucketIndices, nullptr, histogram, stream, false);
+}
+
+extern "C" int rsfm_cashash_compute_hash_bucket_and_remap(const int8_t* weights, const int32_t* bias, const void* desc,
+                                                          uint32_t nbDesc, uint8_t* bucketIndices, uint32_t* remap,
+                                                          uint32_t* histogram, hipStream_t stream)
+{
+    return launch_hash(weights, bias, desc, nbDesc, bucketIndices, remap, histogram, stream, true);
+}
+
+extern "C" int rsfm_cashash_build_histogram(uint32_t* histogram, const uint8_t* bucketIndices, uint32_t nbDesc,
+                                            hipStream_t stream)
+{
+    RSFM_RETURN_IF(histogram == nullptr, RSFM_ERR_INVALID, "build_histogram: histogram null");
+    RSFM_RETURN_IF(nbDesc > 0 && (bucketIndices == nullptr || reinterpret_cast<uintptr_t>(bucketIndices) % 8),
+                   RSFM_ERR_INVALID, "build_histogram: bucketIndices null or misaligned");
+    cashash::zero_u32_kernel<<<8, 256, 0, stream>>>(histogram, 2048);
+    RSFM_CHECK(check_launch("build_histogram init"));
+    if (nbDesc == 0) return RSFM_OK;
+    cashash::histogram_kernel<<<div_up(nbDesc, cashash::kHistPerBlock), cashash::kHistThreads, 0, stream>>>(
+        histogram, bucketIndices, nbDesc);
+    return check_launch("build_histogram");
+}
+
+extern "C" int rsfm_cashash_compute_bucket_bound(uint32_t* lower, uint32_t* upper, uint32_t* maxBucketSize,
+                                                 const uint32_t* histogram, hipStream_t stream)
+{
+    RSFM_RETURN_IF(lower == nullptr || upper == nullptr || histogram == nullptr, RSFM_ERR_INVALID,
+                   "compute_bucket_bound: null pointer");
+    cashash::bounds_kernel<<<1, 256, 0, stream>>>(lower, upper, maxBucketSize, histogram);
+    return check_launch("compute_bucket_bound");
+}
+
+extern "C" int rsfm_cashash_build_buckets(const uint8_t* bucketIndices, uint32_t nbDesc, uint32_t* idxAllocator,
+                                          uint32_t* buckets, hipStream_t stream)
+{
+    if (nbDesc == 0) return RSFM_OK;
+    RSFM_RETURN_IF(bucketIndices == nullptr || idxAllocator == nullptr || buckets == nullptr, RSFM_ERR_INVALID,
+                   "build_buckets: null pointer");
+    cashash::buckets_kernel<<<div_up(nbDesc, cashash::kBuildPerBlock), cashash::kBuildThreads, 0, stream>>>(
+        bucketIndices, nbDesc, idxAllocator, buckets);
+    return check_launch("build_buckets");
+}
+
+extern "C" int rsfm_cashash_find_best_match(const rsfm_cashash_image* queries, const uint32_t* querySize,
+                                            const rsfm_cashash_image* train, uint32_t trainSize,
+                                            const rsfm_cashash_result* results, uint32_t nbTasks,
+                                            const uint32_t* maxQueryBucketSize, uint32_t maxTrainBucketSize,
+                                            uint32_t nbSchemes, hipStream_t stream)
+{
+    (void)maxQueryBucketSize;
+    (void)maxTrainBucketSize;
+    using namespace cashash;
+    RSFM_RETURN_IF(nbTasks > 0 && (queries == nullptr || querySize == nullptr || results == nullptr || train == nullptr),
+                   RSFM_ERR_INVALID, "find_best_match: null pointer");
+    RSFM_RETURN_IF(nbSchemes == 0 || nbSchemes > kSchemes, RSFM_ERR_INVALID, "find_best_match: nbSchemes must be 1..8");
+    for (uint32_t b = 0; b < nbTasks; b += kMatchMaxTasks) {
+        const uint32_t n = std::min<uint32_t>(kMatchMaxTasks, nbTasks - b);
+        MatchBatch batch{};
+        batch.t = DevImage{static_cast<const uint8_t*>(train->desc), train->bucketEnds, train->buckets, trainSize};
+        uint32_t maxN = trainSize;
+        for (uint32_t i = 0; i < n; i++) {
+            const rsfm_cashash_image& q = queries[b + i];
+            RSFM_RETURN_IF(querySize[b + i] > 0 && (q.desc == nullptr || q.bucketEnds == nullptr || q.buckets == nullptr ||
+                                                    results[b + i].query == nullptr),
+                           RSFM_ERR_INVALID, "find_best_match: query image incomplete");
+            batch.q[i] = DevImage{static_cast<const uint8_t*>(q.desc), q.bucketEnds, q.buckets, querySize[b + i]};
+            batch.resQ[i] = reinterpret_cast<uint64_t*>(results[b + i].query);
+            batch.resT[i] = reinterpret_cast<uint64_t*>(results[b + i].train);
+            maxN = std::max(maxN, querySize[b + i]);
+        }
+        if (maxN == 0) continue;
+        match_init_kernel<<<dim3(div_up(maxN, 256), 1, n), 256, 0, stream>>>(batch);
+        RSFM_CHECK(check_launch("find_best_match init"));
+        match_kernel<<<dim3(kBuckets, nbSchemes, n), 64, 0, stream>>>(batch, nbSchemes);
+        RSFM_CHECK(check_launch("find_best_match"));
+    }
+    return RSFM_OK;
+}
