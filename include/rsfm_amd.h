@@ -221,6 +221,29 @@ int rsfm_filter_matches_batch(const rsfm_filter_task* tasks, size_t nbTasks, uin
                               hipStream_t stream);
 
 /* ---------------------------------------------------------------------------
+ * Affine RANSAC match filter (rsfm::rmf, RansacMatchFilter.cuh:21-34)
+ * ------------------------------------------------------------------------ */
+
+/* replaces rmf::cudaRansacMatchFilter (RansacMatchFilter.cu:394-419): matches
+ * (x0,y0 = left, x1,y1 = right) reordered by cell with CSR bounds[cols*rows+1];
+ * per cell the best of nbRansacTests 3-point affine hypotheses over its 3x3
+ * neighbourhood, then (tryOtherAffines) the best of all cells' affines, and a
+ * +1 vote for every neighbourhood inlier of a cell whose best affine has >= 6
+ * inliers. votes: nbMatches bytes, 4-byte aligned, padded to a multiple of 4.
+ * affineScratch: cols*rows*6 floats. seed replaces the reference's clock()
+ * seed of the glibc LCG: the thread g of the grid starts at seed + g*22695477. */
+int rsfm_rmf_filter(uint8_t* votes, float* affineScratch, const rsfm_pt_pair* matches, uint32_t nbMatches,
+                    const uint32_t* bounds, uint32_t cols, uint32_t rows, uint32_t nbRansacTests, float threshold,
+                    int tryOtherAffines, uint32_t seed, hipStream_t stream);
+/* host: the cell binning of RansacMatchFilter::getInlierMask
+ * (RansacMatchFilter.cpp:42-67): cellWidth = ceil(max(w,h)/cellCols), cell =
+ * (ceil(y0/cw), ceil(x0/cw)) clamped; reordered/order/bounds are filled
+ * (order[k] = original index of reordered[k]); bounds holds cols*rows+1 */
+int rsfm_rmf_bin_host(int imgWidth, int imgHeight, uint32_t cellCols, const rsfm_pt_pair* matches, uint32_t nbMatches,
+                      uint32_t* cols, uint32_t* rows, uint32_t* cellWidth, rsfm_pt_pair* reordered, uint32_t* bounds,
+                      uint32_t* order);
+
+/* ---------------------------------------------------------------------------
  * Fundamental-matrix RANSAC (8-point hypotheses)
  * ------------------------------------------------------------------------ */
 

@@ -227,6 +227,40 @@ _decl("or_gen_match_pair", None, [U64, U32, U32, U32, F64, F64, F64, F64, F64, P
 _decl("or_bf_match_top2_fast", None, [P, U32, P, U32, P, P, P, I32])
 
 
+# ---------------- affine match filter ----------------
+def rmf_bin(img_w, img_h, matches, cell_cols=16):
+    m = _c(matches, np.float32).reshape(-1, 4)
+    n = m.shape[0]
+    cols, rows, cw = ctypes.c_uint32(0), ctypes.c_uint32(0), ctypes.c_uint32(0)
+    mx = max(img_w, img_h)
+    cwv = (mx + cell_cols - 1) // cell_cols
+    ncell = ((img_w + cwv - 1) // cwv) * ((img_h + cwv - 1) // cwv)
+    reordered = np.zeros_like(m)
+    bounds = np.zeros(ncell + 1, np.uint32)
+    order = np.zeros(n, np.uint32)
+    _lib.or_rmf_bin(img_w, img_h, cell_cols, _p(m), n, ctypes.byref(cols), ctypes.byref(rows), ctypes.byref(cw),
+                    _p(reordered), _p(bounds), _p(order))
+    return reordered, bounds, order, cols.value, rows.value, cw.value
+
+
+def rmf_filter(reordered, bounds, cols, rows, n_tests, threshold, try_other, seed):
+    m = _c(reordered, np.float32).reshape(-1, 4)
+    bounds = _c(bounds, np.uint32)
+    votes = np.zeros(m.shape[0], np.uint8)
+    aff = np.zeros((cols * rows, 6), np.float32)
+    _lib.or_rmf_filter(_p(votes), _p(aff), _p(m), m.shape[0], _p(bounds), cols, rows, n_tests, threshold,
+                       int(try_other), seed)
+    return votes, aff
+
+
+def solve_gauss3(A, b):
+    A = _c(A, np.float32).reshape(9)
+    b = _c(b, np.float32).reshape(6)
+    x = np.zeros(6, np.float32)
+    _lib.or_solve_gauss3(_p(A), _p(b), _p(x))
+    return x.reshape(3, 2)
+
+
 # ---------------- cascade hashing ----------------
 def cashash_accumulate(desc, acc=None):
     desc = _c(desc, np.uint8)

@@ -23,6 +23,7 @@ SOURCES = [
     "bf_aux.hip",
     "ransac.hip",
     "cashash.hip",
+    "rmf.hip",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
