@@ -1,0 +1,439 @@
+// rsfm_amd.hpp - C++ adapters with the reference's interfaces over the C ABI.
+//
+// A RapidSFM build switches from its CUDA kernels to the MI355X library by
+// including this header in place of bruteForceMatch.h, casHash/cascadeHash.h,
+// RansacMatchFilter.cuh/.h and bruteForceMatcher.h: the function names,
+// argument order and meaning are the reference's, streams are hipStream_t, and
+// failures throw rsfm::Exception (the reference throws cudapp::Exception from
+// cudaCheck). Header-only; link against librsfm_amd.so.
+#pragma once
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdint>
+#include <future>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <unordered_map>
+#include <utility>
+#include <vector>
+
+#include "rsfm_amd.h"
+
+namespace rsfm {
+
+struct Exception : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+inline void rsfmCheck(int rc, const char* what)
+{
+    if (rc != RSFM_OK) throw Exception(std::string(what) + ": " + rsfm_last_error());
+}
+inline void hipCheck(hipError_t e, const char* what)
+{
+    if (e != hipSuccess) throw Exception(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// ---- bruteForceMatch.h:76-136 ----------------------------------------------
+using BestMatch = rsfm_best_match;
+using ValidMatch = rsfm_valid_match;
+using SiftBruteForceMatchTask = rsfm_bf_task;
+// same 56-B layout; a distinct type so the 4-bit overload resolves as in the reference
+struct Sift4bBruteForceMatchTask : rsfm_bf_task {};
+static_assert(sizeof(SiftBruteForceMatchTask) == 56 && sizeof(Sift4bBruteForceMatchTask) == 56, "task layout");
+static_assert(sizeof(BestMatch) == 8 && sizeof(ValidMatch) == 12, "result layout");
+
+inline void launchPreCompSiftSqrNorm(uint32_t* sqrNorm, const void* src, uint32_t nbDesc, hipStream_t stream)
+{
+    rsfmCheck(rsfm_sqrnorm_u8(sqrNorm, src, nbDesc, stream), "launchPreCompSiftSqrNorm");
+}
+inline void launchBruteForceMatchIMMA(const SiftBruteForceMatchTask* tasks, size_t nbTasks, hipStream_t stream)
+{
+    rsfmCheck(rsfm_bf_match_u8(tasks, nbTasks, stream), "launchBruteForceMatchIMMA");
+}
+inline void launchBruteForceMatchIDP(const SiftBruteForceMatchTask* tasks, size_t nbTasks, hipStream_t stream)
+{
+    rsfmCheck(rsfm_bf_match_idp_u8(tasks, nbTasks, stream), "launchBruteForceMatchIDP");
+}
+inline void launchSiftDesc8bTo4b(void* dst, const void* src, uint32_t nbDesc, hipStream_t stream)
+{
+    rsfmCheck(rsfm_desc8b_to_4b(dst, src, nbDesc, stream), "launchSiftDesc8bTo4b");
+}
+inline void launchPreCompSift4bSqrNorm(uint32_t* sqrNorm, const void* src, uint32_t nbDesc, hipStream_t stream)
+{
+    rsfmCheck(rsfm_sqrnorm_u4(sqrNorm, src, nbDesc, stream), "launchPreCompSift4bSqrNorm");
+}
+inline void launchBruteForceMatchIMMA(const Sift4bBruteForceMatchTask* tasks, size_t nbTasks, hipStream_t stream)
+{
+    std::vector<rsfm_bf_task> t(tasks, tasks + nbTasks);
+    rsfmCheck(rsfm_bf_match_u4(t.data(), nbTasks, stream), "launchBruteForceMatchIMMA(4b)");
+}
+
+// bruteForceMatch.cpp:23-66
+inline std::vector<ValidMatch> crossCheckMatches(const BestMatch* matches, uint32_t nbQueryDesc, const BestMatch* matchesBwd,
+                                                 uint32_t nbTrainDesc)
+{
+    std::vector<ValidMatch> out(std::min(nbQueryDesc, nbTrainDesc));
+    out.resize(rsfm_cross_check_host(matches, nbQueryDesc, matchesBwd, nbTrainDesc, out.data()));
+    return out;
+}
+inline std::vector<ValidMatch> removeMatchConflicts(const BestMatch* matches, uint32_t nbQueryDesc, uint32_t nbTrainDesc)
+{
+    std::vector<ValidMatch> out(std::min(nbQueryDesc, nbTrainDesc));
+    out.resize(rsfm_remove_conflicts_host(matches, nbQueryDesc, nbTrainDesc, out.data()));
+    return out;
+}
+
+// ---- casHash/cascadeHash.h:23-107 ------------------------------------------
+namespace cashash {
+static constexpr uint32_t nbScheme = 8;
+static constexpr uint32_t nbBuckets = 256;
+using ImageDesc = rsfm_cashash_image;
+using MatchResult = rsfm_cashash_result;
+static_assert(sizeof(ImageDesc) == 32 && sizeof(MatchResult) == 16, "cashash layouts");
+
+// acc / sqrAcc: device uint64_t[128]; weights: device int8[64][128]
+inline void accumuateDesc(uint64_t* acc, uint64_t* sqrAcc, bool initAcc, const void* desc, uint32_t nbDesc,
+                          hipStream_t stream)
+{
+    rsfmCheck(rsfm_cashash_accumulate_desc(acc, sqrAcc, initAcc, desc, nbDesc, stream), "cashash::accumuateDesc");
+}
+inline void computeBias(int32_t* bias, const int8_t* weights, const int32_t* biasBase, const uint64_t* acc, uint64_t nbDesc,
+                        hipStream_t stream)
+{
+    rsfmCheck(rsfm_cashash_compute_bias(bias, weights, biasBase, acc, nbDesc, 64, stream), "cashash::computeBias");
+}
+inline void computeHashForBucketAndRemap(const int8_t* weights, const int32_t* bias, const void* desc, uint32_t nbDesc,
+                                         uint8_t* bucketIndices, uint32_t* remap, uint32_t* histogram, hipStream_t stream)
+{
+    rsfmCheck(rsfm_cashash_compute_hash_bucket_and_remap(weights, bias, desc, nbDesc, bucketIndices, remap, histogram, stream),
+              "cashash::computeHashForBucketAndRemap");
+}
+inline void computeHashForBucketOnly(const int8_t* weights, const int32_t* bias, const void* desc, uint32_t nbDesc,
+                                     uint8_t* bucketIndices, uint32_t* histogram, hipStream_t stream)
+{
+    rsfmCheck(rsfm_cashash_compute_hash_bucket_only(weights, bias, desc, nbDesc, bucketIndices, histogram, stream),
+              "cashash::computeHashForBucketOnly");
+}
+inline void buildHistogram(uint32_t* histogram, const uint8_t* bucketIndices, uint32_t nbDesc, hipStream_t stream)
+{
+    rsfmCheck(rsfm_cashash_build_histogram(histogram, bucketIndices, nbDesc, stream), "cashash::buildHistogram");
+}
+inline void computeBucketBound(uint32_t* lower, uint32_t* upper, uint32_t* maxBucketSize, const uint32_t* histogram,
+                               hipStream_t stream)
+{
+    rsfmCheck(rsfm_cashash_compute_bucket_bound(lower, upper, maxBucketSize, histogram, stream), "cashash::computeBucketBound");
+}
+inline void buildBuckets(const uint8_t* bucketIndices, uint32_t nbDesc, uint32_t* idxAllocator, uint32_t* buckets,
+                         hipStream_t stream)
+{
+    rsfmCheck(rsfm_cashash_build_buckets(bucketIndices, nbDesc, idxAllocator, buckets, stream), "cashash::buildBuckets");
+}
+inline void findBestMatch(const ImageDesc* queries, const uint32_t* querySize, const ImageDesc& train, uint32_t trainSize,
+                          const MatchResult* results, uint32_t nbTasks, const uint32_t* maxQueryBucketSize,
+                          uint32_t maxTrainBucketSize, hipStream_t stream)
+{
+    rsfmCheck(rsfm_cashash_find_best_match(queries, querySize, &train, trainSize, results, nbTasks, maxQueryBucketSize,
+                                           maxTrainBucketSize, nbScheme, stream),
+              "cashash::findBestMatch");
+}
+}  // namespace cashash
+
+// ---- RansacMatchFilter.cuh:21-34 -------------------------------------------
+namespace rmf {
+using PtPair = rsfm_pt_pair;
+// kmat<float,2,3> per cell is 6 floats
+inline void cudaRansacMatchFilter(uint8_t* votes, float* affineScratch, const PtPair* matches, uint32_t nbMatches,
+                                  const uint32_t* bounds, uint32_t cols, uint32_t rows, uint32_t nbRansacTests,
+                                  float threshold, bool tryOtherSolutions, hipStream_t stream, uint32_t seed = 0)
+{
+    rsfmCheck(rsfm_rmf_filter(votes, affineScratch, matches, nbMatches, bounds, cols, rows, nbRansacTests, threshold,
+                              tryOtherSolutions, seed, stream),
+              "rmf::cudaRansacMatchFilter");
+}
+}  // namespace rmf
+
+// ---- device buffer used by the two operator classes -------------------------
+template <typename T>
+struct DeviceBuffer {
+    T* ptr = nullptr;
+    size_t count = 0;
+    DeviceBuffer() = default;
+    explicit DeviceBuffer(size_t n) { reserve(n); }
+    DeviceBuffer(const DeviceBuffer&) = delete;
+    DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+    DeviceBuffer(DeviceBuffer&& o) noexcept : ptr(o.ptr), count(o.count) { o.ptr = nullptr; o.count = 0; }
+    DeviceBuffer& operator=(DeviceBuffer&& o) noexcept
+    {
+        std::swap(ptr, o.ptr);
+        std::swap(count, o.count);
+        return *this;
+    }
+    ~DeviceBuffer()
+    {
+        if (ptr) (void)hipFree(ptr);
+    }
+    void reserve(size_t n)
+    {
+        if (n <= count) return;
+        if (ptr) hipCheck(hipFree(ptr), "hipFree");
+        ptr = nullptr;
+        hipCheck(hipMalloc(&ptr, std::max<size_t>(n, 1) * sizeof(T)), "hipMalloc");
+        count = n;
+    }
+};
+
+// ---- RansacMatchFilter.h:28-44 -----------------------------------------------
+using Vec2f = std::array<float, 2>;
+
+class RansacMatchFilter {
+public:
+    RansacMatchFilter() = default;
+    // threshold is relative to the cell width (RansacMatchFilter.h:34); seed
+    // replaces the reference's clock() seed
+    std::vector<bool> getInlierMask(std::array<int, 2> imgSize0, const std::vector<std::pair<Vec2f, Vec2f>>& matches,
+                                    uint32_t minVotes = 2, uint32_t nbRansacTests = 256, float threshold = 0.15f,
+                                    uint32_t cellCols = 16, bool tryOtherAffines = false, hipStream_t stream = nullptr,
+                                    uint32_t seed = 0)
+    {
+        std::lock_guard<std::mutex> lk(mMutex);
+        const uint32_t n = static_cast<uint32_t>(matches.size());
+        std::vector<rsfm_pt_pair> in(n);
+        for (uint32_t k = 0; k < n; k++)
+            in[k] = rsfm_pt_pair{matches[k].first[0], matches[k].first[1], matches[k].second[0], matches[k].second[1]};
+        const uint32_t cw0 = (uint32_t(std::max(imgSize0[0], imgSize0[1])) + cellCols - 1) / cellCols;
+        const uint32_t nCells = ((uint32_t(imgSize0[0]) + cw0 - 1) / cw0) * ((uint32_t(imgSize0[1]) + cw0 - 1) / cw0);
+        std::vector<rsfm_pt_pair> reordered(n);
+        std::vector<uint32_t> bounds(nCells + 1), order(n);
+        uint32_t cols = 0, rows = 0, cw = 0;
+        rsfmCheck(rsfm_rmf_bin_host(imgSize0[0], imgSize0[1], cellCols, in.data(), n, &cols, &rows, &cw, reordered.data(),
+                                    bounds.data(), order.data()),
+                  "RansacMatchFilter binning");
+        const size_t voteBytes = (n + 3) / 4 * 4;
+        mVotes.reserve(voteBytes);
+        mAffine.reserve(size_t(nCells) * 6);
+        mMatches.reserve(n);
+        mBounds.reserve(nCells + 1);
+        hipCheck(hipMemcpyAsync(mMatches.ptr, reordered.data(), n * sizeof(rsfm_pt_pair), hipMemcpyHostToDevice, stream),
+                 "H2D matches");
+        hipCheck(hipMemcpyAsync(mBounds.ptr, bounds.data(), bounds.size() * 4, hipMemcpyHostToDevice, stream), "H2D bounds");
+        rmf::cudaRansacMatchFilter(mVotes.ptr, mAffine.ptr, mMatches.ptr, n, mBounds.ptr, cols, rows, nbRansacTests,
+                                   float(cw) * threshold, tryOtherAffines, stream, seed);
+        std::vector<uint8_t> votes(n);
+        hipCheck(hipMemcpyAsync(votes.data(), mVotes.ptr, n, hipMemcpyDeviceToHost, stream), "D2H votes");
+        hipCheck(hipStreamSynchronize(stream), "sync");
+        std::vector<bool> mask(n);
+        for (uint32_t i = 0; i < n; i++) mask[order[i]] = votes[i] >= minVotes;
+        return mask;
+    }
+
+private:
+    std::mutex mMutex;
+    DeviceBuffer<uint8_t> mVotes;
+    DeviceBuffer<float> mAffine;
+    DeviceBuffer<rsfm_pt_pair> mMatches;
+    DeviceBuffer<uint32_t> mBounds;
+};
+
+// ---- bruteForceMatcher.h:28-102 ----------------------------------------------
+using ImageHandle = uint32_t;
+
+// Per-image precompute cache and batched matching of queries against one
+// train image (bruteForceMatcher.cpp:163-443). Results come back as futures;
+// the cascade-hash method keeps the running descriptor mean in registration
+// order, as the reference does (bruteForceMatcher.cpp:97-100).
+class BruteForceMatcher {
+public:
+    using Descriptor = std::array<uint32_t, 32>;
+    struct Config {
+        bool useCascadeHash = false;
+        bool useSift4b = false;
+        // cascade hash projection (int8 [64][128]) and bias base, host memory;
+        // the reference draws them from default_random_engine{11}
+        std::vector<int8_t> casHashWeights;
+        std::vector<int32_t> casHashBiasBase;
+    };
+    struct Img {
+        const Descriptor* desc;  // device
+        uint32_t nbDesc;
+        ImageHandle hImage;
+    };
+
+    BruteForceMatcher() : BruteForceMatcher(Config()) {}
+    explicit BruteForceMatcher(Config cfg) : mCfg(std::move(cfg))
+    {
+        hipCheck(hipStreamCreate(&mStream), "hipStreamCreate");
+        if (mCfg.useCascadeHash) {
+            if (mCfg.casHashWeights.size() != 64 * 128 || mCfg.casHashBiasBase.size() != 64)
+                throw Exception("BruteForceMatcher: cascade hash needs 64x128 weights and 64 bias values");
+            mWeights.reserve(64 * 128);
+            mBiasBase.reserve(64);
+            mAcc.reserve(128);
+            hipCheck(hipMemcpy(mWeights.ptr, mCfg.casHashWeights.data(), 64 * 128, hipMemcpyHostToDevice), "H2D weights");
+            hipCheck(hipMemcpy(mBiasBase.ptr, mCfg.casHashBiasBase.data(), 64 * 4, hipMemcpyHostToDevice), "H2D bias base");
+        }
+    }
+    BruteForceMatcher(const BruteForceMatcher&) = delete;
+    BruteForceMatcher& operator=(const BruteForceMatcher&) = delete;
+    ~BruteForceMatcher()
+    {
+        if (mStream) (void)hipStreamDestroy(mStream);
+    }
+
+    void registerImage(ImageHandle hImage, const Descriptor* desc, uint32_t nbDesc, hipStream_t userStream)
+    {
+        std::lock_guard<std::mutex> lk(mLock);
+        join(userStream, mStream);
+        PreComp pc;
+        pc.nbDesc = nbDesc;
+        if (mCfg.useSift4b) {
+            pc.desc4.reserve(size_t(nbDesc) * 64);
+            launchSiftDesc8bTo4b(pc.desc4.ptr, desc, nbDesc, mStream);
+        }
+        if (mCfg.useCascadeHash) {
+            const bool first = mAccNbDesc == 0;
+            cashash::accumuateDesc(mAcc.ptr, nullptr, first, desc, nbDesc, mStream);
+            mAccNbDesc += nbDesc;
+            DeviceBuffer<int32_t> bias(64);
+            cashash::computeBias(bias.ptr, mWeights.ptr, mBiasBase.ptr, mAcc.ptr, mAccNbDesc, mStream);
+            DeviceBuffer<uint8_t> codes(size_t(nbDesc) * 8);
+            DeviceBuffer<uint32_t> hist(2048), lower(2048);
+            pc.ends.reserve(2048);
+            pc.buckets.reserve(size_t(nbDesc) * 8);
+            cashash::computeHashForBucketOnly(mWeights.ptr, bias.ptr, desc, nbDesc, codes.ptr, nullptr, mStream);
+            cashash::buildHistogram(hist.ptr, codes.ptr, nbDesc, mStream);
+            cashash::computeBucketBound(lower.ptr, pc.ends.ptr, nullptr, hist.ptr, mStream);
+            cashash::buildBuckets(codes.ptr, nbDesc, lower.ptr, pc.buckets.ptr, mStream);
+            hipCheck(hipStreamSynchronize(mStream), "registerImage");
+        }
+        mPreComp[hImage] = std::move(pc);
+        join(mStream, userStream);
+    }
+
+    using Result = std::vector<std::pair<std::vector<BestMatch>, std::vector<BestMatch>>>;
+
+    std::future<Result> match(const std::vector<Img>& queries, const Img& train, bool bidirectional, hipStream_t userStream)
+    {
+        std::lock_guard<std::mutex> lk(mLock);
+        join(userStream, mStream);
+        (void)bidirectional;  // bidirectional is forced on, as in bruteForceMatcher.cpp:207
+        auto dev = matchImpl(queries, train);
+        auto res = std::make_shared<Result>(queries.size());
+        for (size_t i = 0; i < queries.size(); i++) {
+            (*res)[i].first.resize(queries[i].nbDesc);
+            (*res)[i].second.resize(train.nbDesc);
+            if (queries[i].nbDesc)
+                hipCheck(hipMemcpyAsync((*res)[i].first.data(), dev[i].first.ptr, queries[i].nbDesc * 8,
+                                        hipMemcpyDeviceToHost, mStream), "D2H fwd");
+            if (train.nbDesc)
+                hipCheck(hipMemcpyAsync((*res)[i].second.data(), dev[i].second.ptr, train.nbDesc * 8,
+                                        hipMemcpyDeviceToHost, mStream), "D2H bwd");
+        }
+        hipEvent_t done;
+        hipCheck(hipEventCreateWithFlags(&done, hipEventDisableTiming), "event");
+        hipCheck(hipEventRecord(done, mStream), "record");
+        join(mStream, userStream);
+        auto keep = std::make_shared<std::vector<std::pair<DeviceBuffer<BestMatch>, DeviceBuffer<BestMatch>>>>(std::move(dev));
+        return std::async(std::launch::deferred, [done, res, keep]() {
+            hipCheck(hipEventSynchronize(done), "wait");
+            (void)hipEventDestroy(done);
+            return std::move(*res);
+        });
+    }
+
+    std::future<std::vector<std::vector<ValidMatch>>> matchAndFilter(const std::vector<Img>& queries, const Img& train,
+                                                                    bool crossCheck, hipStream_t userStream)
+    {
+        auto f = std::make_shared<std::future<Result>>(match(queries, train, true, userStream));
+        const uint32_t nT = train.nbDesc;
+        return std::async(std::launch::deferred, [f, crossCheck, nT]() {
+            Result r = f->get();
+            std::vector<std::vector<ValidMatch>> out(r.size());
+            for (size_t i = 0; i < r.size(); i++) {
+                const auto& fwd = r[i].first;
+                out[i] = crossCheck ? crossCheckMatches(fwd.data(), uint32_t(fwd.size()), r[i].second.data(), nT)
+                                    : removeMatchConflicts(fwd.data(), uint32_t(fwd.size()), nT);
+            }
+            return out;
+        });
+    }
+
+private:
+    struct PreComp {
+        uint32_t nbDesc = 0;
+        DeviceBuffer<uint8_t> desc4;
+        DeviceBuffer<uint32_t> ends;
+        DeviceBuffer<uint32_t> buckets;
+    };
+
+    static void join(hipStream_t from, hipStream_t to)
+    {
+        if (from == to) return;
+        hipEvent_t e;
+        hipCheck(hipEventCreateWithFlags(&e, hipEventDisableTiming), "event");
+        hipCheck(hipEventRecord(e, from), "record");
+        hipCheck(hipStreamWaitEvent(to, e, 0), "wait");
+        hipCheck(hipEventDestroy(e), "destroy");
+    }
+
+    std::vector<std::pair<DeviceBuffer<BestMatch>, DeviceBuffer<BestMatch>>> matchImpl(const std::vector<Img>& queries,
+                                                                                      const Img& train)
+    {
+        std::vector<std::pair<DeviceBuffer<BestMatch>, DeviceBuffer<BestMatch>>> out;
+        out.reserve(queries.size());
+        for (const auto& q : queries) out.emplace_back(DeviceBuffer<BestMatch>(q.nbDesc), DeviceBuffer<BestMatch>(train.nbDesc));
+        if (mCfg.useCascadeHash) {
+            std::vector<cashash::ImageDesc> qi(queries.size());
+            std::vector<uint32_t> qs(queries.size());
+            std::vector<cashash::MatchResult> res(queries.size());
+            const PreComp& tp = mPreComp.at(train.hImage);
+            for (size_t i = 0; i < queries.size(); i++) {
+                const PreComp& p = mPreComp.at(queries[i].hImage);
+                qi[i] = cashash::ImageDesc{queries[i].desc, p.ends.ptr, p.buckets.ptr, nullptr};
+                qs[i] = queries[i].nbDesc;
+                res[i] = cashash::MatchResult{out[i].first.ptr, out[i].second.ptr};
+            }
+            const cashash::ImageDesc ti{train.desc, tp.ends.ptr, tp.buckets.ptr, nullptr};
+            cashash::findBestMatch(qi.data(), qs.data(), ti, train.nbDesc, res.data(), uint32_t(queries.size()), nullptr, 0,
+                                   mStream);
+        } else if (mCfg.useSift4b) {
+            std::vector<Sift4bBruteForceMatchTask> tasks(queries.size());
+            const PreComp& tp = mPreComp.at(train.hImage);
+            for (size_t i = 0; i < queries.size(); i++) {
+                const PreComp& p = mPreComp.at(queries[i].hImage);
+                tasks[i].nbDescA = queries[i].nbDesc;
+                tasks[i].nbDescB = train.nbDesc;
+                tasks[i].descA = p.desc4.ptr;
+                tasks[i].descB = tp.desc4.ptr;
+                tasks[i].bestMatchA = out[i].first.ptr;
+                tasks[i].bestMatchB = out[i].second.ptr;
+                tasks[i].sqrNormDescA = nullptr;
+                tasks[i].sqrNormDescB = nullptr;
+            }
+            launchBruteForceMatchIMMA(tasks.data(), tasks.size(), mStream);
+        } else {
+            std::vector<SiftBruteForceMatchTask> tasks(queries.size());
+            for (size_t i = 0; i < queries.size(); i++)
+                tasks[i] = SiftBruteForceMatchTask{queries[i].nbDesc, train.nbDesc, queries[i].desc, train.desc,
+                                                   out[i].first.ptr, out[i].second.ptr, nullptr, nullptr};
+            launchBruteForceMatchIMMA(tasks.data(), tasks.size(), mStream);
+        }
+        return out;
+    }
+
+    Config mCfg;
+    hipStream_t mStream = nullptr;
+    std::mutex mLock;
+    std::unordered_map<ImageHandle, PreComp> mPreComp;
+    DeviceBuffer<int8_t> mWeights;
+    DeviceBuffer<int32_t> mBiasBase;
+    DeviceBuffer<uint64_t> mAcc;
+    uint64_t mAccNbDesc = 0;
+};
+
+}  // namespace rsfm

@@ -78,7 +78,27 @@ def build(verbose=False, out=None):
     os.replace(tmp, lib)
     if verbose:
         print("built", lib)
+    if out is None:
+        build_cpp_tests(verbose)
     return lib
+
+
+def build_cpp_tests(verbose=False):
+    """The C++ adapter test (tests/cpp/test_adapters): host code over
+    include/rsfm_amd.hpp, linked against the in-tree library."""
+    src = os.path.join(ROOT, "tests", "cpp", "test_adapters.cpp")
+    if not os.path.exists(src):
+        return None
+    exe = os.path.join(ROOT, "tests", "cpp", "test_adapters")
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", "-I" + INCLUDE, src,
+           "-L" + HERE, "-lrsfm_amd", "-L/opt/rocm/lib", "-lamdhip64", "-Wl,-rpath,$ORIGIN/../../rapidsfm_amd",
+           "-Wl,-rpath,/opt/rocm/lib", "-o", exe]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("C++ adapter test failed to build: %s\n%s" % (" ".join(cmd), r.stderr))
+    if verbose:
+        print("built", exe)
+    return exe
 
 
 if __name__ == "__main__":
