@@ -78,19 +78,34 @@ __device__ inline int32_t imax(int32_t a, int32_t b) { return a > b ? a : b; }
 __device__ inline uint64_t umin64(uint64_t a, uint64_t b) { return a < b ? a : b; }
 __device__ inline uint64_t umax64(uint64_t a, uint64_t b) { return a > b ? a : b; }
 
-// (a << 9) + b in one instruction
-__device__ inline int32_t shl9_add(int32_t a, int32_t b)
+// (a << 9) + b: one v_lshl_add_u32
+__device__ inline int32_t shl9_add(int32_t a, int32_t b) { return (int32_t)(((uint32_t)a << 9) + (uint32_t)b); }
+// The same as asm. The row and the column key shift the same accumulator; with
+// both in plain code the compiler shares one shift and adds twice (three
+// instructions for two keys), so the column key uses this form.
+__device__ inline int32_t shl9_add_asm(int32_t a, int32_t b)
 {
     int32_t r;
     asm("v_lshl_add_u32 %0, %1, 9, %2" : "=v"(r) : "v"(a), "v"(b));
     return r;
 }
-// top-2 fold of x into (m1 <= m2): m2 = med3(m1, m2, x), m1 = min(m1, x),
-// both in place so the loop-carried registers need no copies
-__device__ inline void top2_update(int32_t& m1, int32_t& m2, int32_t x)
+// middle of three: one v_med3_i32
+__device__ inline int32_t med3(int32_t a, int32_t b, int32_t c) { return imax(imin(a, b), imin(imax(a, b), c)); }
+// Fold two new keys x0, x1 into a running top-2 (m1 < m2). The smallest of
+// {m1, x0, x1} is the new m1; the second smallest of {m1, m2, x0, x1} is
+// min(m2, med3(m1, x0, x1)) because m2 >= m1. Three instructions per pair.
+__device__ inline void top2_fold2(int32_t& m1, int32_t& m2, int32_t x0, int32_t x1)
 {
-    asm("v_med3_i32 %0, %1, %0, %2\n\tv_min_i32 %1, %1, %2" : "+v"(m2), "+v"(m1) : "v"(x));
+    const int32_t t = med3(m1, x0, x1);
+    // asm: the compiler would reuse med3's inner min(m1, x0) and spend two v_min
+    asm("v_min3_i32 %0, %0, %1, %2" : "+v"(m1) : "v"(x0), "v"(x1));
+    m2 = imin(m2, t);
 }
+// DPP lane exchanges inside groups of 8 consecutive lanes
+__device__ inline int32_t dpp_xor1(int32_t x) { return __builtin_amdgcn_update_dpp(0, x, 0xB1, 0xF, 0xF, false); }
+__device__ inline int32_t dpp_xor2(int32_t x) { return __builtin_amdgcn_update_dpp(0, x, 0x4E, 0xF, 0xF, false); }
+// lane i <-> lane 7 - i of each group of 8 (row_half_mirror)
+__device__ inline int32_t dpp_mirror8(int32_t x) { return __builtin_amdgcn_update_dpp(0, x, 0x141, 0xF, 0xF, false); }
 
 // 16-B slot swizzle of a staged column: columns c and c^1 sit in different
 // 128-B halves of a 256-B bank row, and the eight columns of one parity map
@@ -98,15 +113,23 @@ __device__ inline void top2_update(int32_t& m1, int32_t& m2, int32_t x)
 // bank-conflict free
 __device__ inline int swz(int c) { return ((c >> 1) & 7) ^ ((c & 1) << 2); }
 
-template <int kBits, bool kTop2, int kWaves, int kRB>
-__global__ __launch_bounds__(kWaves * 64) void match_kernel(const TaskBatch batch, uint32_t segCols)
+// sCol row stride in words: the flush reads 8 consecutive columns from each of
+// the 8 waves; a stride of 72 puts those 64 words in distinct banks
+constexpr int kColStride = 72;
+
+// One workgroup = 8 waves x 32 query rows. The column segment streams through
+// LDS in 64-column chunks. The loop is software pipelined: in iteration c the
+// wave issues the 8 MFMAs of chunk c into one accumulator pair while the VALU
+// folds the keys of chunk c-1 from the other pair, so matrix and vector work of
+// one wave overlap; the body has no branches, so the scheduler can interleave
+// them. Column partials of chunk c-1 are written to LDS in iteration c and
+// merged to global memory by all waves (8 columns each) in iteration c+1.
+template <int kBits, bool kTop2>
+__global__ __launch_bounds__(512) void match_kernel(const TaskBatch batch, uint32_t segCols)
 {
-    static_assert(kWaves * kRB * 32 == kBM, "tile must cover 256 rows");
-    constexpr int kThreads = kWaves * 64;
+    constexpr int kWaves = 8;
     constexpr int kDescBytes = kBits * 16;
-    constexpr int kStageBytes = kBN * 128 / kThreads;  // bytes of one chunk staged per thread
-    constexpr int kThrPerCol = 128 / kStageBytes;
-    static_assert(kStageBytes == 16 || kStageBytes == 32, "staging split");
+    constexpr int kChunkBytes = kBN * 128;
 
     const Wg3 wg = xcd_local_block();
     const DevTask& T = batch.t[wg.z];
@@ -116,12 +139,12 @@ __global__ __launch_bounds__(kWaves * 64) void match_kernel(const TaskBatch batc
     const uint32_t colEnd = min(colBeg + segCols, T.nB);
     const uint32_t nChunks = (colEnd - colBeg + kBN - 1) / kBN;
 
-    // sB (double-buffered chunk) and the row-tail transpose share memory
+    // the double-buffered chunk and the row-tail transpose share memory
     constexpr int kTailStride = 33;
-    constexpr int kUnionBytes = (2 * kBN * 128 > kBM * kTailStride * 4) ? 2 * kBN * 128 : kBM * kTailStride * 4;
+    constexpr int kUnionBytes = (2 * kChunkBytes > kBM * kTailStride * 4) ? 2 * kChunkBytes : kBM * kTailStride * 4;
     __shared__ __attribute__((aligned(16))) uint8_t sU[kUnionBytes];
-    __shared__ int32_t sNB[3][kBN];
-    __shared__ int32_t sCol[3][kWaves][kBN];
+    __shared__ int32_t sP[4][kBN];  // per column: NB' << 8 | column block (kNone past the end)
+    __shared__ int32_t sCol[2][kWaves][kColStride];
     __shared__ int32_t sNA[kBM];
 
     const int tid = threadIdx.x;
@@ -129,11 +152,10 @@ __global__ __launch_bounds__(kWaves * 64) void match_kernel(const TaskBatch batc
     const int l32 = lane & 31, hi = lane >> 5;
 
     // ---- query rows: operands u = 127 - a, offsets NA' = |a - 128|^2
-    v4i aop[kRB][4];
-    int32_t Q[kRB][16];
-#pragma unroll
-    for (int rb = 0; rb < kRB; rb++) {
-        const int rloc = (wave * kRB + rb) * 32 + l32;
+    v4i aop[4];
+    int32_t Q[16];
+    {
+        const int rloc = wave * 32 + l32;
         const uint32_t row = rowBase + rloc;
         const bool valid = row < T.nA;
         int32_t nap = 0;
@@ -148,7 +170,7 @@ __global__ __launch_bounds__(kWaves * 64) void match_kernel(const TaskBatch batc
                 nap = __builtin_amdgcn_sdot4(xa, xa, nap, false);
                 u[q] = valid ? (x[q] ^ 0x7F7F7F7F) : 0;
             }
-            aop[rb][kc] = u;
+            aop[kc] = u;
         }
         const int32_t na = nap + __shfl_xor(nap, 32);
         if (hi == 0) sNA[rloc] = na;
@@ -156,139 +178,169 @@ __global__ __launch_bounds__(kWaves * 64) void match_kernel(const TaskBatch batc
         for (int v = 0; v < 16; v++) {
             const int r = 8 * (v >> 2) + 4 * hi + (v & 3);
             const int32_t nav = __shfl(na, r);
-            const int rl = (wave * kRB + rb) * 32 + r;
-            Q[rb][v] = rowBase + rl < T.nA ? (nav * 256 + rl) : kNone;
+            const int rl = wave * 32 + r;
+            Q[v] = rowBase + rl < T.nA ? (nav * 256 + rl) : kNone;
         }
     }
 
-    int32_t m1[kRB][16], m2[kRB][16];
+    int32_t m1[16], m2[16];
 #pragma unroll
-    for (int rb = 0; rb < kRB; rb++)
-#pragma unroll
-        for (int v = 0; v < 16; v++) {
-            m1[rb][v] = kNone;
-            m2[rb][v] = kNone;
-        }
+    for (int v = 0; v < 16; v++) {
+        m1[v] = kNone;
+        m2[v] = kNone;
+    }
 
-    // ---- staging: thread -> (column tid / kThrPerCol, kStageBytes bytes)
-    const int sColIdx = tid / kThrPerCol, sPart = tid % kThrPerCol;
-    v4i xs[kStageBytes / 16];
-    auto stage_load = [&](uint32_t c) {
-        const uint32_t gcol = colBeg + c * kBN + sColIdx;
-#pragma unroll
-        for (int h = 0; h < kStageBytes / 16; h++) {
-            if (gcol < colEnd)
-                xs[h] = load16<kBits>(T.B + (size_t)gcol * kDescBytes, sPart * kStageBytes + h * 16);
-            else
-                xs[h] = v4i{0, 0, 0, 0};
-        }
+    // ---- staging: thread -> 16 bytes (slot sp) of chunk column sc
+    const int sc = tid >> 3, sp = tid & 7;
+    auto stage_load = [&](uint32_t c) -> v4i {
+        // past the segment end: any valid column (the value is masked at the store)
+        const uint32_t gcol = min(colBeg + c * kBN + sc, colEnd - 1);
+        return load16<kBits>(T.B + (size_t)gcol * kDescBytes, sp * 16);
     };
-    auto stage_store = [&](uint32_t c) {
-        const uint32_t gcol = colBeg + c * kBN + sColIdx;
-        const bool valid = gcol < colEnd;
+    auto stage_store = [&](uint32_t c, const v4i& x) {
+        const bool valid = colBeg + c * kBN + sc < colEnd;
         int32_t acc = 0;
-        uint8_t* base = sU + (c & 1) * (kBN * 128) + sColIdx * 128;
+        v4i w;
 #pragma unroll
-        for (int h = 0; h < kStageBytes / 16; h++) {
-            v4i w;
-#pragma unroll
-            for (int q = 0; q < 4; q++) {
-                w[q] = valid ? (xs[h][q] ^ (int)0x80808080u) : 0;
-                acc = __builtin_amdgcn_sdot4(w[q], w[q], acc, false);
-                acc = __builtin_amdgcn_sdot4(w[q], 0x02020202, acc, false);
-            }
-            const int s = sPart * (kStageBytes / 16) + h;
-            *reinterpret_cast<v4i*>(base + ((s ^ swz(sColIdx)) << 4)) = w;
+        for (int q = 0; q < 4; q++) {
+            w[q] = valid ? (x[q] ^ (int)0x80808080u) : 0;
+            acc = __builtin_amdgcn_sdot4(w[q], w[q], acc, false);
+            acc = __builtin_amdgcn_sdot4(w[q], 0x02020202, acc, false);
         }
-#pragma unroll
-        for (int m = 1; m < kThrPerCol; m <<= 1) acc += __shfl_xor(acc, m);
-        // sum (w^2 + 2w) over the 128 values = |b - 127|^2 - 128 = NB'
-        if (sPart == 0) sNB[c % 3][sColIdx] = acc;
+        *reinterpret_cast<v4i*>(sU + (c & 1) * kChunkBytes + sc * 128 + ((sp ^ swz(sc)) << 4)) = w;
+        // sum over the 8 threads of the column: (w^2 + 2w) over 128 values = |b - 127|^2 - 128 = NB'
+        acc += dpp_xor1(acc);
+        acc += dpp_xor2(acc);
+        acc += dpp_mirror8(acc);
+        // row-key offset NB' << 8 | column block; columns past the end never win
+        sP[c & 3][sc] = valid ? (acc * 256 + (int32_t)(c * 2 + (sc >> 5))) : kNone;  // 8 threads, same value
     };
-
-    stage_load(0);
-    stage_store(0);
-    __syncthreads();
-
-    for (uint32_t c = 0; c < nChunks; c++) {
-        const bool more = c + 1 < nChunks;
-        if (more) stage_load(c + 1);
-        const uint8_t* sb = sU + (c & 1) * (kBN * 128);
-        const int32_t* nb = sNB[c % 3];
-        int32_t colOut[2];
-        v16i acc[2][kRB];
+    // merged column minima of chunk k -> one 64-bit atomicMin per column (8 columns per wave)
+    auto flush = [&](uint32_t k) {
+        const int col = wave * 8 + (lane >> 3);
+        int32_t x = sCol[k & 1][lane & 7][col];
+        x = imin(x, dpp_xor1(x));
+        x = imin(x, dpp_xor2(x));
+        x = imin(x, dpp_mirror8(x));
+        const uint32_t gcol = colBeg + k * kBN + col;
+        if ((lane & 7) == 0 && gcol < colEnd && x != kNone) {
+            const uint32_t dist = (uint32_t)((x >> 8) + (sP[k & 3][col] >> 8));
+            const uint32_t row = rowBase + (uint32_t)(x & 0xFF);
+            atomicMin(reinterpret_cast<unsigned long long*>(T.bestB + gcol), (unsigned long long)make_key(dist, row));
+        }
+    };
+    // Keys of one chunk from its accumulators: row keys (NB'_j + 2 dot) << 8 |
+    // column block fold into the row state; column keys (NA'_i + 2 dot) << 8 |
+    // local row are reduced over the lane's 16 rows and written to sCol[slot].
+    // keys of row v of one chunk: the row keys fold into the row state, the
+    // column keys into the running column minima cm0, cm1 of the lane
+    auto fold_row = [&](int v, const v16i (&acc)[2], const int32_t (&P)[2], int32_t& cm0, int32_t& cm1) {
+        const int32_t a0 = acc[0][v], a1 = acc[1][v];
+        const int32_t r0 = shl9_add(a0, P[0]);
+        const int32_t r1 = shl9_add(a1, P[1]);
+        if constexpr (kTop2)
+            top2_fold2(m1[v], m2[v], r0, r1);
+        else
+            m1[v] = imin(m1[v], imin(r0, r1));
+        cm0 = imin(cm0, shl9_add_asm(a0, Q[v]));
+        cm1 = imin(cm1, shl9_add_asm(a1, Q[v]));
+    };
+    // lanes l and l + 32 hold the same column: exchange the upper half of
+    // block 0 with the lower half of block 1, then lane l holds column l
+    auto put_cols = [&](int32_t cm0, int32_t cm1, uint32_t slot) {
+        const auto sw = __builtin_amdgcn_permlane32_swap((uint32_t)cm0, (uint32_t)cm1, false, false);
+        sCol[slot][wave][lane] = imin((int32_t)sw[0], (int32_t)sw[1]);
+    };
+    auto fold_keys = [&](const v16i (&acc)[2], const int32_t (&P)[2], uint32_t slot) {
+        int32_t cm0 = kNone, cm1 = kNone;
 #pragma unroll
-        for (int cb = 0; cb < 2; cb++) {
-            const int cl = cb * 32 + l32;
-            v4i bop[4];
+        for (int v = 0; v < 16; v++) fold_row(v, acc, P, cm0, cm1);
+        put_cols(cm0, cm1, slot);
+    };
+    const bool doCols = T.bestB != nullptr;
+
+    // iteration c: MFMAs of chunk c -> accN; keys of chunk c-1 from accP (row keys Pp)
+    auto body = [&](uint32_t c, v16i (&accN)[2], int32_t (&Pn)[2], const v16i (&accP)[2], const int32_t (&Pp)[2],
+                    const v4i& xStore, v4i& xLoad) {
+        const uint8_t* sb = sU + (c & 1) * kChunkBytes;
+        v4i bop[2][4];
+#pragma unroll
+        for (int cb = 0; cb < 2; cb++)
 #pragma unroll
             for (int kc = 0; kc < 4; kc++) {
+                const int cl = cb * 32 + l32;
                 const int s = 2 * kc + hi;
-                bop[kc] = *reinterpret_cast<const v4i*>(sb + cl * 128 + ((s ^ swz(cl)) << 4));
+                bop[cb][kc] = *reinterpret_cast<const v4i*>(sb + cl * 128 + ((s ^ swz(cl)) << 4));
             }
+        xLoad = stage_load(c + 2);
+        // MFMA i is followed by the keys of rows 2i, 2i+1 of the previous chunk;
+        // the scheduling fences keep that order, so each wave overlaps its own
+        // matrix and vector work and dependent MFMAs are ~18 VALU apart
+        int32_t cm0 = kNone, cm1 = kNone;
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-            for (int rb = 0; rb < kRB; rb++) {
-                acc[cb][rb] = v16i{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-#pragma unroll
-                for (int kc = 0; kc < 4; kc++)
-                    acc[cb][rb] = __builtin_amdgcn_mfma_i32_32x32x32_i8(aop[rb][kc], bop[kc], acc[cb][rb], 0, 0, 0);
-            }
+        for (int i = 0; i < 8; i++) {
+            const int cb = i >> 2, kc = i & 3;
+            const v16i zero = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+            accN[cb] = __builtin_amdgcn_mfma_i32_32x32x32_i8(aop[kc], bop[cb][kc], kc == 0 ? zero : accN[cb], 0, 0, 0);
+            fold_row(2 * i, accP, Pp, cm0, cm1);
+            fold_row(2 * i + 1, accP, Pp, cm0, cm1);
+            __builtin_amdgcn_sched_barrier(0);
         }
-        int32_t P[2];
+        put_cols(cm0, cm1, (c - 1) & 1);
 #pragma unroll
         for (int cb = 0; cb < 2; cb++) {
             const int cl = cb * 32 + l32;
-            P[cb] = colBeg + c * kBN + cl < colEnd ? (nb[cl] * 256 + (int32_t)(c * 2 + cb)) : kNone;
+            Pn[cb] = sP[c & 3][cl];
         }
+        stage_store(c + 1, xStore);
+        // keep this chunk's row folding in this iteration (the compiler would
+        // otherwise sink it past the barrier into the next one)
 #pragma unroll
-        for (int cb = 0; cb < 2; cb++) {
-            int32_t cm = kNone;
-#pragma unroll
-            for (int rb = 0; rb < kRB; rb++)
-#pragma unroll
-                for (int v = 0; v < 16; v++) cm = imin(cm, shl9_add(acc[cb][rb][v], Q[rb][v]));
-            cm = imin(cm, __shfl_xor(cm, 32));
-            colOut[cb] = cm;
-        }
-#pragma unroll
-        for (int rb = 0; rb < kRB; rb++)
-#pragma unroll
-            for (int v = 0; v < 16; v++) {
-                const int32_t r0 = shl9_add(acc[0][rb][v], P[0]);
-                const int32_t r1 = shl9_add(acc[1][rb][v], P[1]);
-                if constexpr (kTop2) {
-                    top2_update(m1[rb][v], m2[rb][v], r0);
-                    top2_update(m1[rb][v], m2[rb][v], r1);
-                } else {
-                    m1[rb][v] = imin(imin(m1[rb][v], r0), r1);
-                }
-            }
-        sCol[c % 3][wave][hi * 32 + l32] = hi ? colOut[1] : colOut[0];
-        if (more) stage_store(c + 1);
+        for (int v = 0; v < 16; v++) asm volatile("" : "+v"(m1[v]), "+v"(m2[v]));
         __syncthreads();
-        if (wave == (int)(c % kWaves) && T.bestB != nullptr) {
-            int32_t k = sCol[c % 3][0][lane];
+    };
+
+    v16i accA[2], accB[2];
+    int32_t PA[2], PB[2];
 #pragma unroll
-            for (int w = 1; w < kWaves; w++) k = imin(k, sCol[c % 3][w][lane]);
-            const uint32_t gcol = colBeg + c * kBN + lane;
-            if (gcol < colEnd && k != kNone) {
-                const uint32_t dist = (uint32_t)((k >> 8) + sNB[c % 3][lane]);
-                const uint32_t row = rowBase + (uint32_t)(k & 0xFF);
-                atomicMin(reinterpret_cast<unsigned long long*>(T.bestB + gcol), (unsigned long long)make_key(dist, row));
-            }
-        }
+    for (int cb = 0; cb < 2; cb++) {
+        accB[cb] = v16i{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+        PB[cb] = kNone;  // the keys of "chunk -1" never win
+    }
+    v4i xs0 = stage_load(0), xs1 = stage_load(1);
+    stage_store(0, xs0);
+    __syncthreads();
+
+    // two iterations per trip so the accumulator pairs swap roles without copies
+    uint32_t c = 0;
+    for (; c + 2 <= nChunks; c += 2) {
+        if (c >= 2 && doCols) flush(c - 2);
+        body(c, accA, PA, accB, PB, xs1, xs0);
+        if (c >= 2 && doCols) flush(c - 1);
+        body(c + 1, accB, PB, accA, PA, xs0, xs1);
+    }
+    if (c < nChunks) {
+        if (c >= 2 && doCols) flush(c - 2);
+        body(c, accA, PA, accB, PB, xs1, xs0);
+        fold_keys(accA, PA, (nChunks - 1) & 1);
+    } else {
+        fold_keys(accB, PB, (nChunks - 1) & 1);
+    }
+    __syncthreads();
+    if (doCols) {
+        if (nChunks >= 2) flush(nChunks - 2);
+        flush(nChunks - 1);
     }
 
     // ---- row results: transpose the per-lane (row, column-class) bests through LDS
     int32_t* L = reinterpret_cast<int32_t*>(sU);
     __syncthreads();
 #pragma unroll
-    for (int rb = 0; rb < kRB; rb++)
-#pragma unroll
-        for (int v = 0; v < 16; v++) {
-            const int rl = (wave * kRB + rb) * 32 + 8 * (v >> 2) + 4 * hi + (v & 3);
-            L[rl * kTailStride + l32] = m1[rb][v];
-        }
+    for (int v = 0; v < 16; v++) {
+        const int rl = wave * 32 + 8 * (v >> 2) + 4 * hi + (v & 3);
+        L[rl * kTailStride + l32] = m1[v];
+    }
     __syncthreads();
     // thread tid owns local row tid
     int32_t b1 = kNone, b2 = kNone;
@@ -307,12 +359,10 @@ __global__ __launch_bounds__(kWaves * 64) void match_kernel(const TaskBatch batc
     if constexpr (kTop2) {
         __syncthreads();
 #pragma unroll
-        for (int rb = 0; rb < kRB; rb++)
-#pragma unroll
-            for (int v = 0; v < 16; v++) {
-                const int rl = (wave * kRB + rb) * 32 + 8 * (v >> 2) + 4 * hi + (v & 3);
-                L[rl * kTailStride + l32] = m2[rb][v];
-            }
+        for (int v = 0; v < 16; v++) {
+            const int rl = wave * 32 + 8 * (v >> 2) + 4 * hi + (v & 3);
+            L[rl * kTailStride + l32] = m2[v];
+        }
         __syncthreads();
         if (tid < kBM) {
             // second best = min(second best over classes, second best inside the winning class)
@@ -378,13 +428,6 @@ __global__ void merge_top2_kernel(const TaskBatch batch, uint32_t nSeg, uint32_t
     T.bestA[row] = k1;
     T.secondA[row] = k2;
 }
-
-// tile configuration of the matcher (waves per workgroup x 32-row blocks per wave)
-#ifndef RSFM_BF_WAVES
-#define RSFM_BF_WAVES 8
-#endif
-constexpr int kCfgWaves = RSFM_BF_WAVES;
-constexpr int kCfgRB = kBM / 32 / kCfgWaves;
 
 // Launch geometry shared by the size query and the launch.
 struct Plan {
@@ -453,8 +496,7 @@ static int launch_top1(const rsfm_bf_task* tasks, size_t nbTasks, hipStream_t st
         init_kernel<<<dim3(initX, 1, (uint32_t)n), 256, 0, stream>>>(batch, true);
         RSFM_CHECK(check_launch("bf init"));
         if (p.gridX == 0 || p.maxB == 0) continue;
-        match_kernel<kBits, false, kCfgWaves, kCfgRB>
-            <<<dim3(p.gridX, p.nSeg, (uint32_t)n), kCfgWaves * 64, 0, stream>>>(batch, p.segCols);
+        match_kernel<kBits, false><<<dim3(p.gridX, p.nSeg, (uint32_t)n), 512, 0, stream>>>(batch, p.segCols);
         RSFM_CHECK(check_launch("bf match"));
     }
     return RSFM_OK;
@@ -544,8 +586,7 @@ extern "C" int rsfm_bf_match_top2(const rsfm_bf_top2_task* tasks, size_t nbTasks
             RSFM_CHECK(check_launch("bf top2 init2"));
         }
         if (p.gridX == 0 || p.maxB == 0) continue;
-        match_kernel<8, true, kCfgWaves, kCfgRB>
-            <<<dim3(p.gridX, p.nSeg, (uint32_t)n), kCfgWaves * 64, 0, stream>>>(batch, p.segCols);
+        match_kernel<8, true><<<dim3(p.gridX, p.nSeg, (uint32_t)n), 512, 0, stream>>>(batch, p.segCols);
         RSFM_CHECK(check_launch("bf top2 match"));
         if (p.nSeg > 1) {
             merge_top2_kernel<<<dim3(div_up(p.maxA, 256), 1, (uint32_t)n), 256, 0, stream>>>(batch, p.nSeg, p.segCols);
