@@ -287,6 +287,42 @@ typedef struct rsfm_sample_task {
 } rsfm_sample_task;
 int rsfm_ransac_sample_indices_batch(const rsfm_sample_task* tasks, size_t nbTasks, hipStream_t stream);
 
+/* ---- learned proposals (no reference counterpart: the reference samples
+ * uniformly, NonDuplicateSampler in ransac.hpp:23-82) ---- */
+
+/* weighted sampler: hypothesis h draws 8 distinct indices, each draw with
+ * probability proportional to q_i = round(clamp(weights[i], 0, 1) * 65536)
+ * among the points not drawn yet (same splitmix64 stream as
+ * rsfm_ransac_sample_indices; 512 draws at most, then ~0u). All-zero weights
+ * fall back to the uniform draw. cdf: device workspace of nbPts u64. */
+typedef struct rsfm_weighted_sample_task {
+    uint32_t* sampleIdx; /* [nbHyp][8] */
+    uint32_t nbHyp;
+    uint32_t nbPts;
+    const uint32_t* nbPtsDev; /* optional device count (capped at nbPts) */
+    const float* weights;     /* [nbPts] */
+    uint64_t* cdf;            /* [nbPts] workspace */
+    uint64_t seed;
+} rsfm_weighted_sample_task;
+int rsfm_ransac_sample_weighted_batch(const rsfm_weighted_sample_task* tasks, size_t nbTasks, hipStream_t stream);
+
+/* proposal network 8 -> 64 -> 64 -> 64 -> 1 (ReLU, sigmoid) on bf16 matrix
+ * cores with FP32 accumulation. params: FP32 vector of rsfm_mlp_param_count()
+ * entries, W1[64x8] b1[64] W2[64x64] b2[64] W3[64x64] b3[64] W4[64] b4,
+ * row-major W[out][in]; weights and inputs are rounded to bf16 on load.
+ * x: [n][8] FP32 features, 16-byte aligned. */
+int rsfm_mlp_param_count(void);
+int rsfm_mlp_forward(const float* params, const float* x, uint32_t n, float* prob, hipStream_t stream);
+/* gradient of the mean binary cross-entropy over the n samples (labels y in
+ * {0, 1}) into grad (same layout as params) and the mean loss into *loss
+ * (optional, device). Deterministic: per-workgroup partials summed in order. */
+int rsfm_mlp_grad_workspace_size(uint32_t n, size_t* bytes);
+int rsfm_mlp_grad(const float* params, const float* x, const float* y, uint32_t n, float* grad, float* loss,
+                  void* workspace, size_t workspaceBytes, hipStream_t stream);
+/* one Adam step (step counts from 1) on the FP32 master parameters */
+int rsfm_mlp_adam_step(float* params, const float* grad, float* m, float* v, uint32_t step, float lr, float beta1,
+                       float beta2, float eps, hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -314,3 +314,56 @@ void or_sample_indices_hash(uint64_t seed, uint32_t n, uint32_t H, uint32_t* idx
         }
     }
 }
+
+/* Weighted draw of 8 distinct indices per hypothesis (the learned-proposal
+ * sampler; no reference counterpart). Integer weights q_i = round(clamp(w_i,
+ * 0, 1) * 65536), exact u64 prefix; r = splitmix % total picks the first i
+ * with cdf[i] > r; repeats rejected, 512 draws at most (then all ~0u);
+ * total 0 falls back to the uniform draw of or_sample_indices_hash. */
+void or_sample_weighted(const float* w, uint32_t n, uint32_t H, uint64_t seed, uint32_t* idx)
+{
+    uint64_t* cdf = (uint64_t*)malloc(sizeof(uint64_t) * (n ? n : 1));
+    uint64_t acc = 0;
+    for (uint32_t i = 0; i < n; i++) {
+        float v = w[i];
+        v = v > 0.f ? v : 0.f; /* NaN -> 0 */
+        v = v < 1.f ? v : 1.f;
+        acc += (uint32_t)(v * 65536.f + 0.5f);
+        cdf[i] = acc;
+    }
+    for (uint32_t h = 0; h < H; h++) {
+        uint32_t* out = idx + (size_t)h * 8;
+        if (n < 8) {
+            for (int i = 0; i < 8; i++) out[i] = ~0u;
+            continue;
+        }
+        const uint64_t total = cdf[n - 1];
+        uint64_t s = seed ^ ((uint64_t)h * 0xD1B54A32D192ED03ULL);
+        uint32_t draws = 0;
+        int failed = 0;
+        for (int i = 0; i < 8 && !failed; i++) {
+            for (;;) {
+                uint32_t c;
+                if (total == 0) {
+                    c = (uint32_t)(splitmix_next(&s) % n);
+                } else {
+                    const uint64_t r = splitmix_next(&s) % total;
+                    uint32_t lo = 0, hi = n - 1;
+                    while (lo < hi) {
+                        const uint32_t mid = (lo + hi) >> 1;
+                        if (cdf[mid] > r) hi = mid;
+                        else lo = mid + 1;
+                    }
+                    c = lo;
+                }
+                int dup = 0;
+                for (int j = 0; j < i; j++) dup |= out[j] == c;
+                if (!dup) { out[i] = c; break; }
+                if (++draws >= 512) { failed = 1; break; }
+            }
+        }
+        if (failed)
+            for (int i = 0; i < 8; i++) out[i] = ~0u;
+    }
+    free(cdf);
+}

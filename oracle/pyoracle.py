@@ -83,8 +83,12 @@ _decl("or_f8_score", None, [P, U32, P, U32, F32, I32, F32, F32, P, P, I32])
 _decl("or_f8_ransac_adaptive", U32, [P, U32, F32, I32, F32, U32, U64, F32, F32, P, P])
 _decl("or_ransac_loop_hash", U32, [U32, F32, U32, U64, U32, P, P])
 _decl("or_ransac_required_iters", U32, [U32, U32, U32, F64, U32])
+_decl("or_sample_weighted", None, [P, U32, U32, U64, P])
 _decl("or_mlp_forward", None, [P, P, U32, P])
 _decl("or_mlp_loss_grad", F64, [P, P, P, U32, P])
+_decl("or_mlp_forward_bf16", None, [P, P, U32, P])
+_decl("or_mlp_loss_grad_bf16", F64, [P, P, P, U32, P])
+_decl("or_bf16_round", F64, [F64])
 
 MLP_NPARAM = 64 * 8 + 64 + 64 * 64 + 64 + 64 * 64 + 64 + 64 + 1
 
@@ -360,6 +364,13 @@ def bf_match_top2_fast(A, B, nthreads=0):
     return ba, sa, bb
 
 
+def sample_weighted(w, H, seed):
+    w = _c(w, np.float32)
+    out = np.zeros((H, 8), np.uint32)
+    _lib.or_sample_weighted(_p(w), len(w), H, seed, _p(out))
+    return out
+
+
 def sample_indices_hash(seed, n, H):
     idx = np.zeros((H, 8), np.uint32)
     _lib.or_sample_indices_hash(seed, n, H, _p(idx))
@@ -415,3 +426,29 @@ def f8_ransac_adaptive(pts, t, mode, confidence, max_iter, seed, min_sep_sq0=0.0
 
 def required_iters(best, n, k, conf, max_iter):
     return _lib.or_ransac_required_iters(best, n, k, conf, max_iter)
+
+
+# ---------------- learned proposal MLP ----------------
+def mlp_forward(params, x, bf16=False):
+    """Sigmoid outputs of the 8-64-64-64-1 network for x (n x 8)."""
+    params = _c(params, np.float64)
+    x = _c(x, np.float32)
+    n = x.shape[0]
+    out = np.zeros(n, np.float64)
+    (_lib.or_mlp_forward_bf16 if bf16 else _lib.or_mlp_forward)(_p(params), _p(x), n, _p(out))
+    return out
+
+
+def mlp_loss_grad(params, x, y, bf16=False, with_grad=True):
+    """(mean BCE loss, gradient over the parameter vector)."""
+    params = _c(params, np.float64)
+    x = _c(x, np.float32)
+    y = _c(y, np.float32)
+    g = np.zeros(MLP_NPARAM, np.float64) if with_grad else None
+    fn = _lib.or_mlp_loss_grad_bf16 if bf16 else _lib.or_mlp_loss_grad
+    loss = fn(_p(params), _p(x), _p(y), x.shape[0], _p(g))
+    return loss, g
+
+
+def bf16_round(v):
+    return _lib.or_bf16_round(float(v))

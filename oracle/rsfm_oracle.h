@@ -59,6 +59,8 @@ void or_gen_match_pair(uint64_t seed, uint32_t nA, uint32_t nB, uint32_t n_true,
 void or_sample_indices(uint64_t seed, uint32_t n, uint32_t k, uint32_t H, uint32_t* idx);
 /* restatement of the GPU's splitmix64 sampler (rsfm_ransac_sample_indices) */
 void or_sample_indices_hash(uint64_t seed, uint32_t n, uint32_t H, uint32_t* idx);
+/* learned-proposal weighted draw (gen.c) */
+void or_sample_weighted(const float* w, uint32_t n, uint32_t H, uint64_t seed, uint32_t* idx);
 
 /* ---------------- brute-force matching (bf.c) ---------------- */
 typedef struct { uint32_t index; uint32_t distance; } or_best_match;   /* BestMatchImpl<u32> */
@@ -172,6 +174,12 @@ void   or_mlp_forward(const double* params, const float* x /*n x 8*/, uint32_t n
 /* mean binary cross-entropy loss and gradient over the batch */
 double or_mlp_loss_grad(const double* params, const float* x, const float* y, uint32_t n,
                         double* grad);
+/* the same with the GPU kernels' bf16 roundings emulated (mlp.c header) */
+void   or_mlp_forward_bf16(const double* params, const float* x, uint32_t n, double* prob);
+double or_mlp_loss_grad_bf16(const double* params, const float* x, const float* y, uint32_t n,
+                             double* grad);
+/* float -> bf16 -> float, round to nearest even */
+double or_bf16_round(double v);
 
 #ifdef __cplusplus
 }

@@ -89,6 +89,18 @@ class SampleTask(ctypes.Structure):
     ]
 
 
+class WSampleTask(ctypes.Structure):
+    _fields_ = [
+        ("sampleIdx", c_void_p),
+        ("nbHyp", c_u32),
+        ("nbPts", c_u32),
+        ("nbPtsDev", c_void_p),
+        ("weights", c_void_p),
+        ("cdf", c_void_p),
+        ("seed", ctypes.c_uint64),
+    ]
+
+
 assert ctypes.sizeof(BfTask) == 56
 assert ctypes.sizeof(BestMatch) == 8
 assert ctypes.sizeof(ValidMatch) == 12
@@ -115,6 +127,13 @@ _SIGS = {
     "rsfm_ransac_sample_indices": (c_int, [c_void_p, c_u32, c_u32, c_void_p, ctypes.c_uint64, c_void_p]),
     "rsfm_filter_matches_batch": (c_int, [c_void_p, c_size, c_u32, c_u32, c_void_p]),
     "rsfm_ransac_sample_indices_batch": (c_int, [c_void_p, c_size, c_void_p]),
+    "rsfm_ransac_sample_weighted_batch": (c_int, [c_void_p, c_size, c_void_p]),
+    "rsfm_mlp_param_count": (c_int, []),
+    "rsfm_mlp_forward": (c_int, [c_void_p, c_void_p, c_u32, c_void_p, c_void_p]),
+    "rsfm_mlp_grad_workspace_size": (c_int, [c_u32, ctypes.POINTER(c_size)]),
+    "rsfm_mlp_grad": (c_int, [c_void_p, c_void_p, c_void_p, c_u32, c_void_p, c_void_p, c_void_p, c_size, c_void_p]),
+    "rsfm_mlp_adam_step": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_u32, c_float, c_float, c_float, c_float,
+                                   c_void_p]),
 }
 
 

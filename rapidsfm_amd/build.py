@@ -24,6 +24,7 @@ SOURCES = [
     "ransac.hip",
     "cashash.hip",
     "rmf.hip",
+    "mlp.hip",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -139,12 +139,13 @@ __global__ __launch_bounds__(512) void match_kernel(const TaskBatch batch, uint3
     const uint32_t colEnd = min(colBeg + segCols, T.nB);
     const uint32_t nChunks = (colEnd - colBeg + kBN - 1) / kBN;
 
-    // the double-buffered chunk and the row-tail transpose share memory
+    // ring of staged chunks; the row-tail transpose reuses its memory
+    constexpr int kRing = 4, kColSlots = 4, kPSlots = 8;
     constexpr int kTailStride = 33;
-    constexpr int kUnionBytes = (2 * kChunkBytes > kBM * kTailStride * 4) ? 2 * kChunkBytes : kBM * kTailStride * 4;
+    constexpr int kUnionBytes = (kRing * kChunkBytes > kBM * kTailStride * 4) ? kRing * kChunkBytes : kBM * kTailStride * 4;
     __shared__ __attribute__((aligned(16))) uint8_t sU[kUnionBytes];
-    __shared__ int32_t sP[4][kBN];  // per column: NB' << 8 | column block (kNone past the end)
-    __shared__ int32_t sCol[2][kWaves][kColStride];
+    __shared__ int32_t sP[kPSlots][kBN];  // per column: NB' << 8 | column block (kNone past the end)
+    __shared__ int32_t sCol[kColSlots][kWaves][kColStride];
     __shared__ int32_t sNA[kBM];
 
     const int tid = threadIdx.x;
@@ -190,7 +191,11 @@ __global__ __launch_bounds__(512) void match_kernel(const TaskBatch batch, uint3
         m2[v] = kNone;
     }
 
-    // ---- staging: thread -> 16 bytes (slot sp) of chunk column sc
+    // ---- staging: thread -> 16 bytes (slot sp) of chunk column sc. Chunks live
+    // in a ring of kRing LDS slots; chunk c is staged while chunk c-2 is being
+    // processed and the workgroup synchronises after every odd chunk. Between
+    // the staging of a slot and its next read, and between its read and its
+    // next staging, there is then always one barrier.
     const int sc = tid >> 3, sp = tid & 7;
     auto stage_load = [&](uint32_t c) -> v4i {
         // past the segment end: any valid column (the value is masked at the store)
@@ -207,33 +212,31 @@ __global__ __launch_bounds__(512) void match_kernel(const TaskBatch batch, uint3
             acc = __builtin_amdgcn_sdot4(w[q], w[q], acc, false);
             acc = __builtin_amdgcn_sdot4(w[q], 0x02020202, acc, false);
         }
-        *reinterpret_cast<v4i*>(sU + (c & 1) * kChunkBytes + sc * 128 + ((sp ^ swz(sc)) << 4)) = w;
+        *reinterpret_cast<v4i*>(sU + (c % kRing) * kChunkBytes + sc * 128 + ((sp ^ swz(sc)) << 4)) = w;
         // sum over the 8 threads of the column: (w^2 + 2w) over 128 values = |b - 127|^2 - 128 = NB'
         acc += dpp_xor1(acc);
         acc += dpp_xor2(acc);
         acc += dpp_mirror8(acc);
         // row-key offset NB' << 8 | column block; columns past the end never win
-        sP[c & 3][sc] = valid ? (acc * 256 + (int32_t)(c * 2 + (sc >> 5))) : kNone;  // 8 threads, same value
+        sP[c % kPSlots][sc] = valid ? (acc * 256 + (int32_t)(c * 2 + (sc >> 5))) : kNone;  // 8 threads, same value
     };
     // merged column minima of chunk k -> one 64-bit atomicMin per column (8 columns per wave)
     auto flush = [&](uint32_t k) {
         const int col = wave * 8 + (lane >> 3);
-        int32_t x = sCol[k & 1][lane & 7][col];
+        int32_t x = sCol[k % kColSlots][lane & 7][col];
         x = imin(x, dpp_xor1(x));
         x = imin(x, dpp_xor2(x));
         x = imin(x, dpp_mirror8(x));
         const uint32_t gcol = colBeg + k * kBN + col;
         if ((lane & 7) == 0 && gcol < colEnd && x != kNone) {
-            const uint32_t dist = (uint32_t)((x >> 8) + (sP[k & 3][col] >> 8));
+            const uint32_t dist = (uint32_t)((x >> 8) + (sP[k % kPSlots][col] >> 8));
             const uint32_t row = rowBase + (uint32_t)(x & 0xFF);
             atomicMin(reinterpret_cast<unsigned long long*>(T.bestB + gcol), (unsigned long long)make_key(dist, row));
         }
     };
-    // Keys of one chunk from its accumulators: row keys (NB'_j + 2 dot) << 8 |
-    // column block fold into the row state; column keys (NA'_i + 2 dot) << 8 |
-    // local row are reduced over the lane's 16 rows and written to sCol[slot].
-    // keys of row v of one chunk: the row keys fold into the row state, the
-    // column keys into the running column minima cm0, cm1 of the lane
+    // keys of row v of one chunk: the row keys (NB'_j + 2 dot) << 8 | column
+    // block fold into the row state, the column keys (NA'_i + 2 dot) << 8 |
+    // local row into the running column minima cm0, cm1 of the lane
     auto fold_row = [&](int v, const v16i (&acc)[2], const int32_t (&P)[2], int32_t& cm0, int32_t& cm1) {
         const int32_t a0 = acc[0][v], a1 = acc[1][v];
         const int32_t r0 = shl9_add(a0, P[0]);
@@ -247,22 +250,24 @@ __global__ __launch_bounds__(512) void match_kernel(const TaskBatch batch, uint3
     };
     // lanes l and l + 32 hold the same column: exchange the upper half of
     // block 0 with the lower half of block 1, then lane l holds column l
-    auto put_cols = [&](int32_t cm0, int32_t cm1, uint32_t slot) {
+    auto put_cols = [&](int32_t cm0, int32_t cm1, uint32_t k) {
         const auto sw = __builtin_amdgcn_permlane32_swap((uint32_t)cm0, (uint32_t)cm1, false, false);
-        sCol[slot][wave][lane] = imin((int32_t)sw[0], (int32_t)sw[1]);
+        sCol[k % kColSlots][wave][lane] = imin((int32_t)sw[0], (int32_t)sw[1]);
     };
-    auto fold_keys = [&](const v16i (&acc)[2], const int32_t (&P)[2], uint32_t slot) {
+    auto fold_keys = [&](const v16i (&acc)[2], const int32_t (&P)[2], uint32_t k) {
         int32_t cm0 = kNone, cm1 = kNone;
 #pragma unroll
         for (int v = 0; v < 16; v++) fold_row(v, acc, P, cm0, cm1);
-        put_cols(cm0, cm1, slot);
+        put_cols(cm0, cm1, k);
     };
     const bool doCols = T.bestB != nullptr;
 
-    // iteration c: MFMAs of chunk c -> accN; keys of chunk c-1 from accP (row keys Pp)
+    // chunk c: MFMAs of chunk c -> accN (row offsets Pn); keys of chunk c-1 from
+    // accP (row offsets Pp); stages chunk c+2 from xs and loads chunk c+4 into xs
     auto body = [&](uint32_t c, v16i (&accN)[2], int32_t (&Pn)[2], const v16i (&accP)[2], const int32_t (&Pp)[2],
-                    const v4i& xStore, v4i& xLoad) {
-        const uint8_t* sb = sU + (c & 1) * kChunkBytes;
+                    v4i& xs, bool sync) {
+        stage_store(c + 2, xs);
+        const uint8_t* sb = sU + (c % kRing) * kChunkBytes;
         v4i bop[2][4];
 #pragma unroll
         for (int cb = 0; cb < 2; cb++)
@@ -272,33 +277,38 @@ __global__ __launch_bounds__(512) void match_kernel(const TaskBatch batch, uint3
                 const int s = 2 * kc + hi;
                 bop[cb][kc] = *reinterpret_cast<const v4i*>(sb + cl * 128 + ((s ^ swz(cl)) << 4));
             }
-        xLoad = stage_load(c + 2);
-        // MFMA i is followed by the keys of rows 2i, 2i+1 of the previous chunk;
-        // the scheduling fences keep that order, so each wave overlaps its own
-        // matrix and vector work and dependent MFMAs are ~18 VALU apart
+        xs = stage_load(c + 4);
+        // the keys of rows 2i, 2i+1 of the previous chunk, then MFMA i: the
+        // scheduling fences keep that order, so each wave overlaps its own
+        // vector and matrix work, the first MFMA finds its LDS operand loaded
+        // and dependent MFMAs are ~18 VALU apart
         int32_t cm0 = kNone, cm1 = kNone;
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int i = 0; i < 8; i++) {
             const int cb = i >> 2, kc = i & 3;
-            const v16i zero = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-            accN[cb] = __builtin_amdgcn_mfma_i32_32x32x32_i8(aop[kc], bop[cb][kc], kc == 0 ? zero : accN[cb], 0, 0, 0);
             fold_row(2 * i, accP, Pp, cm0, cm1);
             fold_row(2 * i + 1, accP, Pp, cm0, cm1);
             __builtin_amdgcn_sched_barrier(0);
+            const v16i zero = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+            accN[cb] = __builtin_amdgcn_mfma_i32_32x32x32_i8(aop[kc], bop[cb][kc], kc == 0 ? zero : accN[cb], 0, 0, 0);
+            __builtin_amdgcn_sched_barrier(0);
         }
-        put_cols(cm0, cm1, (c - 1) & 1);
+        put_cols(cm0, cm1, c - 1);
 #pragma unroll
-        for (int cb = 0; cb < 2; cb++) {
-            const int cl = cb * 32 + l32;
-            Pn[cb] = sP[c & 3][cl];
-        }
-        stage_store(c + 1, xStore);
-        // keep this chunk's row folding in this iteration (the compiler would
-        // otherwise sink it past the barrier into the next one)
+        for (int cb = 0; cb < 2; cb++) Pn[cb] = sP[c % kPSlots][cb * 32 + l32];
+        // keep this chunk's row folding here (the compiler would otherwise
+        // sink it past the barrier into the next chunk)
 #pragma unroll
         for (int v = 0; v < 16; v++) asm volatile("" : "+v"(m1[v]), "+v"(m2[v]));
-        __syncthreads();
+        if (sync) __syncthreads();
+    };
+    // column partials of chunks k-3 and k-2 are complete after the barrier that
+    // precedes the even chunk k
+    auto flush_before = [&](uint32_t k) {
+        if (!doCols) return;
+        if (k >= 3) flush(k - 3);
+        if (k >= 2) flush(k - 2);
     };
 
     v16i accA[2], accB[2];
@@ -310,28 +320,33 @@ __global__ __launch_bounds__(512) void match_kernel(const TaskBatch batch, uint3
     }
     v4i xs0 = stage_load(0), xs1 = stage_load(1);
     stage_store(0, xs0);
+    stage_store(1, xs1);
+    xs0 = stage_load(2);
+    xs1 = stage_load(3);
     __syncthreads();
 
-    // two iterations per trip so the accumulator pairs swap roles without copies
+    // two chunks per trip: the accumulator pairs swap roles without copies and
+    // the barrier follows the odd chunk
     uint32_t c = 0;
     for (; c + 2 <= nChunks; c += 2) {
-        if (c >= 2 && doCols) flush(c - 2);
-        body(c, accA, PA, accB, PB, xs1, xs0);
-        if (c >= 2 && doCols) flush(c - 1);
-        body(c + 1, accB, PB, accA, PA, xs0, xs1);
+        flush_before(c);
+        body(c, accA, PA, accB, PB, xs0, false);
+        body(c + 1, accB, PB, accA, PA, xs1, true);
     }
+    uint32_t lastEven;  // the last even chunk
     if (c < nChunks) {
-        if (c >= 2 && doCols) flush(c - 2);
-        body(c, accA, PA, accB, PB, xs1, xs0);
-        fold_keys(accA, PA, (nChunks - 1) & 1);
+        flush_before(c);
+        body(c, accA, PA, accB, PB, xs0, false);
+        fold_keys(accA, PA, nChunks - 1);
+        lastEven = c;
     } else {
-        fold_keys(accB, PB, (nChunks - 1) & 1);
+        fold_keys(accB, PB, nChunks - 1);
+        lastEven = c - 2;
     }
     __syncthreads();
-    if (doCols) {
-        if (nChunks >= 2) flush(nChunks - 2);
-        flush(nChunks - 1);
-    }
+    // chunks from lastEven - 1 on are not flushed yet
+    if (doCols)
+        for (uint32_t k = lastEven >= 1 ? lastEven - 1 : 0; k < nChunks; k++) flush(k);
 
     // ---- row results: transpose the per-lane (row, column-class) bests through LDS
     int32_t* L = reinterpret_cast<int32_t*>(sU);

@@ -227,6 +227,122 @@ __global__ void sample_batch_kernel(const SampleBatch batch)
     for (int i = 0; i < 8; i++) out[i] = v[i];
 }
 
+// ---- weighted sampler for learned proposals (no reference counterpart; the
+// reference draws uniformly, ransac.hpp:23-82). Weights become integers
+// q_i = round(clamp(w_i, 0, 1) * 65536) with an exact u64 inclusive prefix, so
+// the draw is reproducible bit for bit on the CPU. Hypothesis h uses the same
+// splitmix64 stream as the uniform sampler: r = next % total selects the first
+// i with cdf[i] > r (probability q_i / total); repeats are rejected (draws
+// without replacement), at most kMaxWeightedDraws draws per hypothesis, after
+// which it is marked ~0u. total == 0 falls back to the uniform draw.
+constexpr uint32_t kMaxWeightedDraws = 512;
+
+struct WSampleTask {
+    uint32_t* idx;
+    uint32_t H;
+    uint32_t nHost;
+    const uint32_t* nDev;
+    const float* w;
+    uint64_t* cdf;
+    uint64_t seed;
+};
+struct WSampleBatch {
+    WSampleTask t[kSampleMaxTasks];
+};
+
+__device__ inline uint32_t quant_weight(float w) { return (uint32_t)(fminf(fmaxf(w, 0.f), 1.f) * 65536.f + 0.5f); }
+
+// one 1024-thread workgroup per task: cdf[i] = sum_{k <= i} q_k
+__global__ __launch_bounds__(1024) void weight_scan_kernel(const WSampleBatch batch)
+{
+    const WSampleTask& T = batch.t[blockIdx.x];
+    const uint32_t n = T.nDev ? min(*T.nDev, T.nHost) : T.nHost;
+    __shared__ uint64_t sWave[16];
+    __shared__ uint64_t sCarry;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (tid == 0) sCarry = 0;
+    __syncthreads();
+    for (uint32_t base = 0; base < n; base += 4096) {
+        // each thread owns 4 consecutive entries
+        const uint32_t i0 = base + 4 * tid;
+        uint64_t q[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) q[k] = i0 + k < n ? quant_weight(T.w[i0 + k]) : 0;
+        q[1] += q[0];
+        q[2] += q[1];
+        q[3] += q[2];
+        // inclusive scan of the thread totals across the wave
+        uint64_t x = q[3];
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint64_t y = __shfl_up(x, o);
+            if (lane >= o) x += y;
+        }
+        if (lane == 63) sWave[wave] = x;
+        __syncthreads();
+        uint64_t waveOff = 0;
+        for (int w = 0; w < wave; w++) waveOff += sWave[w];
+        const uint64_t off = sCarry + waveOff + x - q[3];
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            if (i0 + k < n) T.cdf[i0 + k] = off + q[k];
+        __syncthreads();
+        if (tid == 1023) sCarry = off + q[3];
+        __syncthreads();
+    }
+}
+
+__global__ void weighted_sample_kernel(const WSampleBatch batch)
+{
+    const WSampleTask& T = batch.t[blockIdx.y];
+    const uint32_t h = blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= T.H) return;
+    const uint32_t n = T.nDev ? min(*T.nDev, T.nHost) : T.nHost;
+    uint32_t* out = T.idx + (size_t)h * 8;
+    if (n < 8) {
+        for (int i = 0; i < 8; i++) out[i] = ~0u;
+        return;
+    }
+    const uint64_t total = T.cdf[n - 1];
+    uint64_t s = T.seed ^ ((uint64_t)h * 0xD1B54A32D192ED03ull);
+    uint32_t v[8];
+    uint32_t draws = 0;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        for (;;) {
+            uint32_t c;
+            if (total == 0) {
+                c = (uint32_t)(splitmix(s) % n);
+            } else {
+                const uint64_t r = splitmix(s) % total;
+                // first index with cdf > r
+                uint32_t lo = 0, hi = n - 1;
+                while (lo < hi) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (T.cdf[mid] > r)
+                        hi = mid;
+                    else
+                        lo = mid + 1;
+                }
+                c = lo;
+            }
+            bool dup = false;
+#pragma unroll
+            for (int j = 0; j < i; j++) dup |= v[j] == c;
+            if (!dup) {
+                v[i] = c;
+                break;
+            }
+            if (++draws >= kMaxWeightedDraws) {
+                for (int k = 0; k < 8; k++) out[k] = ~0u;
+                return;
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 8; i++) out[i] = v[i];
+}
+
 __global__ void sample_kernel(uint32_t* idx, uint32_t H, uint32_t nHost, const uint32_t* nDev, uint64_t seed)
 {
     const uint32_t h = blockIdx.x * blockDim.x + threadIdx.x;
@@ -326,6 +442,35 @@ extern "C" int rsfm_ransac_sample_indices_batch(const rsfm_sample_task* tasks, s
         if (maxH == 0) continue;
         sample_batch_kernel<<<dim3(div_up(maxH, 256), (uint32_t)n), 256, 0, stream>>>(batch);
         RSFM_CHECK(check_launch("ransac sample_indices_batch"));
+    }
+    return RSFM_OK;
+}
+
+extern "C" int rsfm_ransac_sample_weighted_batch(const rsfm_weighted_sample_task* tasks, size_t nbTasks,
+                                                 hipStream_t stream)
+{
+    using namespace ransac;
+    RSFM_RETURN_IF(nbTasks > 0 && tasks == nullptr, RSFM_ERR_INVALID, "weighted sample: tasks is null");
+    for (size_t b = 0; b < nbTasks; b += kSampleMaxTasks) {
+        const size_t n = std::min<size_t>(kSampleMaxTasks, nbTasks - b);
+        WSampleBatch batch{};
+        uint32_t maxH = 0;
+        for (size_t i = 0; i < n; i++) {
+            const rsfm_weighted_sample_task& t = tasks[b + i];
+            RSFM_RETURN_IF(t.nbHyp > 0 && t.sampleIdx == nullptr, RSFM_ERR_INVALID,
+                           "weighted sample: sampleIdx is null");
+            RSFM_RETURN_IF(t.nbPts > 0 && (t.weights == nullptr || t.cdf == nullptr), RSFM_ERR_INVALID,
+                           "weighted sample: weights/cdf null");
+            RSFM_RETURN_IF(reinterpret_cast<uintptr_t>(t.cdf) % 8 != 0, RSFM_ERR_INVALID,
+                           "weighted sample: cdf must be 8-byte aligned");
+            batch.t[i] = WSampleTask{t.sampleIdx, t.nbHyp, t.nbPts, t.nbPtsDev, t.weights, t.cdf, t.seed};
+            maxH = std::max(maxH, t.nbHyp);
+        }
+        weight_scan_kernel<<<(uint32_t)n, 1024, 0, stream>>>(batch);
+        RSFM_CHECK(check_launch("weighted sample scan"));
+        if (maxH == 0) continue;
+        weighted_sample_kernel<<<dim3(div_up(maxH, 256), (uint32_t)n), 256, 0, stream>>>(batch);
+        RSFM_CHECK(check_launch("weighted sample"));
     }
     return RSFM_OK;
 }
