@@ -8,8 +8,9 @@ resident in HBM, all on one stream:
   3. rsfm_ransac_sample_indices_batch   H seeded 8-point samples per pair
   4. rsfm_ransac_f8               FP32 8-point fits + Sampson scoring (1 px),
                                   best model + inlier mask per pair
-Synthetic data: SIFT-like descriptors (30% noisy true matches) with keypoints
-from one two-view geometry per pair (30% of true matches geometric outliers).
+Synthetic data (rapidsfm_amd.synth, drawn on the device): SIFT-like
+descriptors (30% noisy true matches) with keypoints from one two-view geometry
+per pair (30% of true matches geometric outliers).
 
 Prints one JSON line (metric, value, ..., roofline, cpu_baseline).
 """
@@ -65,17 +66,18 @@ def main():
     stream = torch.cuda.current_stream()
     sp = ctypes.c_void_p(stream.cuda_stream)
 
-    # ---- synthetic pairs (generated once on the host, then resident in HBM)
-    from oracle import pyoracle as orc  # generator only; the timed path is the HIP library
+    # ---- synthetic pairs (generated on the device, resident in HBM)
+    from rapidsfm_amd import synth
 
     n_unique = min(P, 8)
-    uniq = [orc.gen_match_pair(1000 + u, n, n, int(0.3 * n), 0.3) for u in range(n_unique)]
+    uniq = [synth.make_pair(1000 + u, n, n, copy_frac=0.3, geo_frac=0.7, device=dev) for u in range(n_unique)]
     bufs = []
     for p in range(P):
-        dA, dB, kA, kB, gt, geo = uniq[p % n_unique]
+        u = uniq[p % n_unique]
         b = {
-            "dA": torch.from_numpy(dA).to(dev), "dB": torch.from_numpy(dB).to(dev),
-            "kA": torch.from_numpy(kA).to(dev), "kB": torch.from_numpy(kB).to(dev),
+            "dA": u["descA"], "dB": u["descB"],
+            "kA": torch.from_numpy(np.ascontiguousarray(u["kpA"][:, :2])).to(dev),
+            "kB": torch.from_numpy(np.ascontiguousarray(u["kpB"][:, :2])).to(dev),
             "bestA": torch.empty((n, 2), dtype=torch.int32, device=dev),
             "secA": torch.empty((n, 2), dtype=torch.int32, device=dev),
             "bestB": torch.empty((n, 2), dtype=torch.int32, device=dev),
@@ -138,22 +140,8 @@ def main():
     ops = 2.0 * n * n * 128 * P / launches
     achieved = ops / (match_ms / 1e3) / 1e12
 
-    # ---- correctness of the measured pass (pair 0 against the CPU oracle)
-    dA, dB, kA, kB, gt, geo = uniq[0]
     b0 = bufs[0]
-    ra, sa, rb = orc.bf_match_top2_fast(dA, dB)
-    u32 = lambda t: t.cpu().numpy().view(np.uint32)  # noqa: E731
-    ok_match = (np.array_equal(u32(b0["bestA"]), ra) and np.array_equal(u32(b0["secA"]), sa)
-                and np.array_equal(u32(b0["bestB"]), rb))
-    m = orc.filter_matches(ra, sa, rb, (4, 5))
     cnt = int(b0["count"].item())
-    pts_ref = np.concatenate([kA[m[:, 0]], kB[m[:, 1]]], 1).astype(np.float32)
-    ok_pts = cnt == len(m) and np.array_equal(b0["pts"][:cnt].cpu().numpy(), pts_ref)
-    idx_ref = orc.sample_indices_hash(77, cnt, H)
-    rc, _ = orc.f8_score(pts_ref, idx_ref, 1.0, 1, with_models=False)
-    ok_ransac = np.array_equal(u32(b0["counts"]), rc) and int(u32(b0["best"])[1]) == int(rc.max())
-    parity = bool(ok_match and ok_pts and ok_ransac)
-
     result = {
         "metric": "pairs/s at 8K kpts (match+RANSAC)",
         "value": round(value, 1),
@@ -175,7 +163,7 @@ def main():
             "matches_per_pair": cnt,
             "parallelism": "single GPU",
         },
-        "parity_pair0": parity,
+        "parity_pair0": None,
         "roofline": {
             "kernel": "match_kernel<8,true,8,1> (rsfm_bf_match_top2)",
             "bound": "mfma",
@@ -200,11 +188,29 @@ def main():
         torch.cuda.synchronize()
 
     if not args.no_cpu:
+        # ---- CPU baseline: the oracle's restatement of the same pass on the
+        # host cores, which also checks pair 0 of the measured GPU pass
+        from oracle import pyoracle as orc
+
+        host = [(u["descA"].cpu().numpy(), u["descB"].cpu().numpy(), u["kpA"][:, :2], u["kpB"][:, :2]) for u in uniq]
+        dA, dB, kA, kB = host[0]
+        ra, sa, rb = orc.bf_match_top2_fast(dA, dB)
+        u32 = lambda t: t.cpu().numpy().view(np.uint32)  # noqa: E731
+        ok_match = (np.array_equal(u32(b0["bestA"]), ra) and np.array_equal(u32(b0["secA"]), sa)
+                    and np.array_equal(u32(b0["bestB"]), rb))
+        m = orc.filter_matches(ra, sa, rb, (4, 5))
+        pts_ref = np.concatenate([kA[m[:, 0]], kB[m[:, 1]]], 1).astype(np.float32)
+        ok_pts = cnt == len(m) and np.array_equal(b0["pts"][:cnt].cpu().numpy(), pts_ref)
+        idx_ref = orc.sample_indices_hash(77, cnt, H)
+        rc, _ = orc.f8_score(pts_ref, idx_ref, 1.0, 1, with_models=False)
+        ok_ransac = np.array_equal(u32(b0["counts"]), rc) and int(u32(b0["best"])[1]) == int(rc.max())
+        result["parity_pair0"] = bool(ok_match and ok_pts and ok_ransac)
+
         cores = len(os.sched_getaffinity(0))
         t_cpu = 0.0
         k = 0
         while t_cpu < args.cpu_seconds and k < 4096:
-            dA, dB, kA, kB, _, _ = uniq[k % n_unique]
+            dA, dB, kA, kB = host[k % n_unique]
             t = time.perf_counter()
             ra, sa, rb = orc.bf_match_top2_fast(dA, dB, cores)
             mm = orc.filter_matches(ra, sa, rb, (4, 5))

@@ -287,6 +287,61 @@ typedef struct rsfm_sample_task {
 } rsfm_sample_task;
 int rsfm_ransac_sample_indices_batch(const rsfm_sample_task* tasks, size_t nbTasks, hipStream_t stream);
 
+/* ---- batched pipeline (no reference counterpart: the reference matches each
+ * candidate pair with BruteForceMatcher::match, bruteForceMatcher.cpp:163-443,
+ * and verifies it on a host fiber, BuilderMethods.cpp:141-202 /
+ * ImagePair.cpp:148-204). Many pairs of uneven size are matched (top-2 query
+ * side, top-1 train side), filtered (ratio test + cross-check + keypoint
+ * gather), sampled and scored by one HIP graph launch. All buffers are the
+ * caller's except the pipeline's own task tables and partials. Inputs must be
+ * complete when rsfm_pipeline_create is called: it times candidate matcher
+ * tilings on them (autotuning) on a private stream before capturing the graph.
+ * Results are the same as the individual entry points' (rsfm_bf_match_top2,
+ * rsfm_filter_matches_batch, rsfm_ransac_sample_indices_batch with seed + pair
+ * index, rsfm_ransac_f8 with nbPtsDev = count). ---- */
+typedef struct rsfm_pipeline_pair {
+    uint32_t nA, nB;
+    const void* descA;          /* device u8 [nA][128], 16-byte aligned */
+    const void* descB;          /* device u8 [nB][128], 16-byte aligned */
+    const float* kpA;           /* device (x, y) per query keypoint */
+    const float* kpB;           /* device (x, y) per train keypoint */
+    rsfm_best_match* bestA;     /* [nA] */
+    rsfm_best_match* secondA;   /* [nA] */
+    rsfm_best_match* bestB;     /* [nB] */
+    rsfm_valid_match* matches;  /* optional [nA] kept matches */
+    uint32_t* count;            /* [1] kept matches */
+    rsfm_pt_pair* pts;          /* [nA] kept keypoint pairs, 16-byte aligned */
+    uint32_t* sampleIdx;        /* [nbHyp][8] */
+    uint32_t* counts;           /* [nbHyp] inlier counts */
+    uint32_t* best;             /* [2] best hypothesis, its count */
+    uint8_t* inlierMask;        /* optional [nA] */
+} rsfm_pipeline_pair;
+typedef struct rsfm_pipeline_config {
+    uint32_t nbHyp;            /* hypotheses per pair */
+    float threshold;           /* inlier threshold in pixels */
+    int mode;                  /* 0 two-sided line test, 1 Sampson */
+    uint32_t ratioNum, ratioDen; /* ratio test num/den (4, 5 = 0.8) */
+    uint64_t seed;             /* pair p samples with seed + p */
+    float minSepSq0, minSepSq1;  /* sample sanity gate, 0 = off */
+    uint32_t segCols;          /* matcher tile width (multiple of 64, <= 8192); 0 = autotune */
+    uint32_t wgPerXcd;         /* persistent matcher workgroups per XCD; 0 = autotune */
+} rsfm_pipeline_config;
+typedef struct rsfm_pipeline_tuning_entry {
+    uint32_t segCols, wgPerXcd;
+    float matchMs;             /* fill + match + merge, best of three */
+} rsfm_pipeline_tuning_entry;
+typedef struct rsfm_pipeline_info_t {
+    uint32_t segCols, wgPerXcd, nbTiles;
+    float tunedMatchMs;        /* 0 when nothing was tuned */
+} rsfm_pipeline_info_t;
+typedef struct rsfm_pipeline rsfm_pipeline;
+int rsfm_pipeline_create(const rsfm_pipeline_pair* pairs, size_t nbPairs, const rsfm_pipeline_config* cfg,
+                         rsfm_pipeline** out);
+int rsfm_pipeline_run(rsfm_pipeline* pipeline, hipStream_t stream);
+int rsfm_pipeline_info(const rsfm_pipeline* pipeline, rsfm_pipeline_info_t* info, rsfm_pipeline_tuning_entry* entries,
+                       size_t maxEntries, size_t* nbEntries);
+int rsfm_pipeline_destroy(rsfm_pipeline* pipeline);
+
 /* ---- learned proposals (no reference counterpart: the reference samples
  * uniformly, NonDuplicateSampler in ransac.hpp:23-82) ---- */
 
@@ -305,6 +360,25 @@ typedef struct rsfm_weighted_sample_task {
     uint64_t seed;
 } rsfm_weighted_sample_task;
 int rsfm_ransac_sample_weighted_batch(const rsfm_weighted_sample_task* tasks, size_t nbTasks, hipStream_t stream);
+
+/* input features of the proposal network for each kept match k < *count:
+ * f0 = sqrt(d1 / d2) of the ratio test (0 without a second match, 1 if d2 =
+ * 0), f1 = sqrt(d1) / 512, f2..f5 = query and train keypoints mapped to
+ * [-1, 1] (x / width * 2 - 1, y / height * 2 - 1), f6 = log2(train scale /
+ * query scale) clamped to [-8, 8], f7 = orientation difference wrapped to
+ * [-pi, pi) over pi. Keypoints are (x, y, scale, orientation) float4. Rows
+ * count .. maxMatches-1 are zero. */
+typedef struct rsfm_feature_task {
+    const rsfm_valid_match* matches; /* from rsfm_filter_matches_batch */
+    const uint32_t* count;           /* device */
+    uint32_t maxMatches;
+    const rsfm_best_match* second;   /* second best per query */
+    const float* kpQuery;            /* [nQuery][4] */
+    const float* kpTrain;            /* [nTrain][4] */
+    float width, height;
+    float* x;                        /* [maxMatches][8] */
+} rsfm_feature_task;
+int rsfm_proposal_features_batch(const rsfm_feature_task* tasks, size_t nbTasks, hipStream_t stream);
 
 /* proposal network 8 -> 64 -> 64 -> 64 -> 1 (ReLU, sigmoid) on bf16 matrix
  * cores with FP32 accumulation. params: FP32 vector of rsfm_mlp_param_count()

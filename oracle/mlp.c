@@ -180,3 +180,37 @@ double or_mlp_loss_grad_bf16(const double* params, const float* x, const float* 
 {
     return loss_grad_all(params, x, y, n, grad, 1);
 }
+
+/* Input features of the proposal network (rsfm_proposal_features_batch). FP32
+ * like the kernel; the kernel's sqrtf/log2f may differ in the last ulp. */
+void or_proposal_features(const or_valid_match* m, uint32_t n, const or_best_match* second, const float* kpQ,
+                          const float* kpT, float width, float height, float* x)
+{
+    const float pi = 3.14159265358979f;
+    for (uint32_t k = 0; k < n; k++) {
+        const uint32_t q = m[k].idxQuery, t = m[k].idxTrain;
+        const float d1 = m[k].distance;
+        const uint32_t d2 = second[q].distance;
+        float* f = x + (size_t)k * 8;
+        if (second[q].index == 0xFFFFFFFFu)
+            f[0] = 0.f;
+        else if (d2 == 0)
+            f[0] = 1.f;
+        else {
+            const float r = sqrtf(d1 / (float)d2);
+            f[0] = r < 1.f ? r : 1.f;
+        }
+        f[1] = sqrtf(d1) * (1.f / 512.f);
+        const float* a = kpQ + (size_t)q * 4;
+        const float* b = kpT + (size_t)t * 4;
+        f[2] = a[0] * (1.f / width) * 2.f - 1.f;
+        f[3] = a[1] * (1.f / height) * 2.f - 1.f;
+        f[4] = b[0] * (1.f / width) * 2.f - 1.f;
+        f[5] = b[1] * (1.f / height) * 2.f - 1.f;
+        float ls = log2f(b[2] / a[2]);
+        f[6] = ls < -8.f ? -8.f : (ls > 8.f ? 8.f : ls);
+        float d = b[3] - a[3];
+        d = d - 2.f * pi * floorf((d + pi) * (0.5f / pi));
+        f[7] = d * (1.f / pi);
+    }
+}

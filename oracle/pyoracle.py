@@ -89,6 +89,7 @@ _decl("or_mlp_loss_grad", F64, [P, P, P, U32, P])
 _decl("or_mlp_forward_bf16", None, [P, P, U32, P])
 _decl("or_mlp_loss_grad_bf16", F64, [P, P, P, U32, P])
 _decl("or_bf16_round", F64, [F64])
+_decl("or_proposal_features", None, [P, U32, P, P, P, F32, F32, P])
 
 MLP_NPARAM = 64 * 8 + 64 + 64 * 64 + 64 + 64 * 64 + 64 + 64 + 1
 
@@ -452,3 +453,15 @@ def mlp_loss_grad(params, x, y, bf16=False, with_grad=True):
 
 def bf16_round(v):
     return _lib.or_bf16_round(float(v))
+
+
+def proposal_features(matches, second, kpQ, kpT, width, height):
+    """matches: (n, 3) [query, train, distance float bits] (uint32/int32 view);
+    second: (nQ, 2) BestMatch; kp*: (n, 4) float32 -> (n, 8) float32."""
+    m = _c(matches, np.uint32)
+    s = _c(second, np.uint32)
+    kq = _c(kpQ, np.float32)
+    kt = _c(kpT, np.float32)
+    out = np.zeros((m.shape[0], 8), np.float32)
+    _lib.or_proposal_features(_p(m), m.shape[0], _p(s), _p(kq), _p(kt), width, height, _p(out))
+    return out

@@ -178,6 +178,9 @@ double or_mlp_loss_grad(const double* params, const float* x, const float* y, ui
 void   or_mlp_forward_bf16(const double* params, const float* x, uint32_t n, double* prob);
 double or_mlp_loss_grad_bf16(const double* params, const float* x, const float* y, uint32_t n,
                              double* grad);
+/* per-match features of the proposal network (rsfm_proposal_features_batch) */
+void   or_proposal_features(const or_valid_match* m, uint32_t n, const or_best_match* second, const float* kpQ,
+                            const float* kpT, float width, float height, float* x);
 /* float -> bf16 -> float, round to nearest even */
 double or_bf16_round(double v);
 

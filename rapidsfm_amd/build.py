@@ -25,6 +25,7 @@ SOURCES = [
     "cashash.hip",
     "rmf.hip",
     "mlp.hip",
+    "pipeline.hip",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -1,5 +1,5 @@
 // bf_aux.hip - descriptor precompute, the vector-ALU matcher and match filtering.
-#include "common.hpp"
+#include "internal.hpp"
 #include <algorithm>
 #include <vector>
 
@@ -160,28 +160,15 @@ __device__ inline bool keep_match(const uint64_t* fwd, const uint64_t* second, c
     return true;
 }
 
-struct FilterTask {
-    const uint64_t* fwd;
-    const uint64_t* second;
-    uint32_t nQ;
-    const uint64_t* bwd;
-    uint32_t nT;
-    rsfm_valid_match* out;
-    uint32_t* count;
-    const float2* kpQ;
-    const float2* kpT;
-    float4* pts;
-};
 constexpr int kFilterMaxTasks = 32;
-struct FilterBatch {
-    FilterTask t[kFilterMaxTasks];
-};
+typedef ByValue<FilterTask, kFilterMaxTasks> FilterBatch;
 
 // one workgroup per task: flags, block-wide exclusive scan, ordered scatter of
 // the kept matches and (optionally) of their keypoint pairs
-__global__ __launch_bounds__(kFilterThreads) void filter_batch_kernel(const FilterBatch batch, uint32_t num, uint32_t den)
+template <class Src>
+__global__ __launch_bounds__(kFilterThreads) void filter_batch_kernel(const Src batch, uint32_t num, uint32_t den)
 {
-    const FilterTask& T = batch.t[blockIdx.x];
+    const FilterTask& T = batch[blockIdx.x];
     __shared__ uint32_t sScan[kFilterThreads];
     const uint32_t nQ = T.nQ;
     const uint32_t per = (nQ + kFilterThreads - 1) / kFilterThreads;
@@ -248,6 +235,13 @@ __global__ __launch_bounds__(kFilterThreads) void filter_kernel(const uint64_t* 
         out[pos++] = m;
     }
     if (threadIdx.x == kFilterThreads - 1) *count = sScan[threadIdx.x];
+}
+
+int launch_filter_table(const FilterTask* tasks, uint32_t nTasks, uint32_t num, uint32_t den, hipStream_t stream)
+{
+    if (nTasks == 0) return RSFM_OK;
+    filter_batch_kernel<<<nTasks, kFilterThreads, 0, stream>>>(InTable<FilterTask>{tasks}, num, den);
+    return check_launch("filter table");
 }
 
 }  // namespace aux

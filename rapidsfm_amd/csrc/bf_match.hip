@@ -24,7 +24,7 @@
 //  * Segments split the train range when one launch has too few row blocks
 //    to fill 256 CUs; row results then merge by 64-bit atomicMin (top-1) or
 //    through a small workspace and a merge kernel (top-2).
-#include "common.hpp"
+#include "internal.hpp"
 #include <algorithm>
 #include <cstring>
 #include <vector>
@@ -32,24 +32,7 @@
 namespace rsfm_amd {
 namespace bf {
 
-constexpr int kBM = 256;                // query rows per workgroup
-constexpr int kBN = 64;                 // train columns per staged chunk
-constexpr uint32_t kMaxSegCols = 8192;  // 256 blocks of 32: the 8-bit tag of a row key
-constexpr int kMaxTasks = 32;
 constexpr int32_t kNone = INT32_MAX;
-
-struct DevTask {
-    uint32_t nA, nB;
-    const uint8_t* A;
-    const uint8_t* B;
-    uint64_t* bestA;
-    uint64_t* secondA;  // top-2 only
-    uint64_t* bestB;    // may be null
-    uint64_t* part;     // top-2 with several segments: [nSeg][nA][2]
-};
-struct TaskBatch {
-    DevTask t[kMaxTasks];
-};
 
 // 16 bytes of the (expanded) descriptor at byte offset off of the 128-value row
 template <int kBits>
@@ -125,18 +108,13 @@ constexpr int kColStride = 72;
 // them. Column partials of chunk c-1 are written to LDS in iteration c and
 // merged to global memory by all waves (8 columns each) in iteration c+1.
 template <int kBits, bool kTop2>
-__global__ __launch_bounds__(512) void match_kernel(const TaskBatch batch, uint32_t segCols)
+__device__ __forceinline__ void match_tile(const DevTask& T, uint32_t rowBase, uint32_t colBeg, uint32_t seg)
 {
     constexpr int kWaves = 8;
     constexpr int kDescBytes = kBits * 16;
     constexpr int kChunkBytes = kBN * 128;
 
-    const Wg3 wg = xcd_local_block();
-    const DevTask& T = batch.t[wg.z];
-    const uint32_t rowBase = wg.x * kBM;
-    const uint32_t colBeg = wg.y * segCols;
-    if (rowBase >= T.nA || colBeg >= T.nB) return;
-    const uint32_t colEnd = min(colBeg + segCols, T.nB);
+    const uint32_t colEnd = min(colBeg + T.segCols, T.nB);
     const uint32_t nChunks = (colEnd - colBeg + kBN - 1) / kBN;
 
     // ring of staged chunks; the row-tail transpose reuses its memory
@@ -397,14 +375,14 @@ __global__ __launch_bounds__(512) void match_kernel(const TaskBatch batch, uint3
                 return make_key((uint32_t)((m >> 8) + na), colBeg + (uint32_t)(m & 0xFF) * 32u + (uint32_t)cls);
             };
             const uint64_t k1 = to_key(b1, c1);
-            const bool direct = gridDim.y == 1;
+            const bool direct = T.nSeg == 1;
             if constexpr (kTop2) {
                 const uint64_t k2 = to_key(b2, c2);
                 if (direct) {
                     T.bestA[row] = k1;
                     T.secondA[row] = k2;
                 } else {
-                    uint64_t* p = T.part + ((size_t)wg.y * T.nA + row) * 2;
+                    uint64_t* p = T.part + ((size_t)seg * T.nA + row) * 2;
                     p[0] = k1;
                     p[1] = k2;
                 }
@@ -418,22 +396,56 @@ __global__ __launch_bounds__(512) void match_kernel(const TaskBatch batch, uint3
     }
 }
 
-__global__ void init_kernel(const TaskBatch batch, bool initA)
+// grid launch: x = row block, y = segment, z = task (XCD-local order)
+template <int kBits, bool kTop2>
+__global__ __launch_bounds__(512) void match_kernel(const TaskBatch batch)
 {
-    const DevTask& T = batch.t[blockIdx.z];
+    const Wg3 wg = xcd_local_block();
+    const DevTask& T = batch.t[wg.z];
+    const uint32_t rowBase = wg.x * kBM;
+    const uint32_t colBeg = wg.y * T.segCols;
+    if (rowBase >= T.nA || colBeg >= T.nB) return;
+    match_tile<kBits, kTop2>(T, rowBase, colBeg, wg.y);
+}
+
+// Persistent launch over a tile table: the tiles are split into kNumXcd lists
+// (xcdOff[x] .. xcdOff[x+1]), one per XCD; workgroup b serves list b % kNumXcd
+// (the XCD the hardware dispatches it to), striding by the workgroups of that
+// XCD, so all tiles of one task share one L2. gridDim.x is a multiple of
+// kNumXcd.
+template <int kBits, bool kTop2>
+__global__ __launch_bounds__(512) void match_tiles_kernel(const DevTask* __restrict__ tasks,
+                                                          const TileDesc* __restrict__ tiles,
+                                                          const uint32_t* __restrict__ xcdOff)
+{
+    const uint32_t xcd = blockIdx.x % kNumXcd, per = gridDim.x / kNumXcd;
+    const uint32_t end = xcdOff[xcd + 1];
+    for (uint32_t i = xcdOff[xcd] + blockIdx.x / kNumXcd; i < end; i += per) {
+        const TileDesc d = tiles[i];
+        const DevTask T = tasks[d.task];
+        __syncthreads();  // the previous tile's last LDS reads
+        match_tile<kBits, kTop2>(T, d.rowBase, d.colBeg, d.seg);
+    }
+}
+
+template <class Src>
+__global__ void init_kernel(const Src batch, bool initA)
+{
+    const DevTask& T = batch[blockIdx.z];
     const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
     if (initA && x < T.nA) T.bestA[x] = kNoMatch;
     if (T.bestB != nullptr && x < T.nB) T.bestB[x] = kNoMatch;
 }
 
-__global__ void merge_top2_kernel(const TaskBatch batch, uint32_t nSeg, uint32_t segCols)
+template <class Src>
+__global__ void merge_top2_kernel(const Src batch)
 {
-    const DevTask& T = batch.t[blockIdx.z];
+    const DevTask& T = batch[blockIdx.z];
     const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
-    if (row >= T.nA) return;
+    if (row >= T.nA || T.nSeg <= 1) return;
     uint64_t k1 = kNoMatch, k2 = kNoMatch;
     // segments at or past this task's train count were never written
-    const uint32_t nUsed = min(nSeg, (T.nB + segCols - 1) / segCols);
+    const uint32_t nUsed = min(T.nSeg, (T.nB + T.segCols - 1) / T.segCols);
     for (uint32_t s = 0; s < nUsed; s++) {
         const uint64_t* p = T.part + ((size_t)s * T.nA + row) * 2;
         const uint64_t o1 = p[0], o2 = p[1];
@@ -506,12 +518,16 @@ static int launch_top1(const rsfm_bf_task* tasks, size_t nbTasks, hipStream_t st
         TaskBatch batch{};
         for (size_t i = 0; i < n; i++) batch.t[i] = to_dev(tasks[b + i]);
         const Plan p = make_plan(batch.t, n);
+        for (size_t i = 0; i < n; i++) {
+            batch.t[i].nSeg = p.nSeg;
+            batch.t[i].segCols = p.segCols;
+        }
         const uint32_t initX = div_up(std::max(p.maxA, p.maxB), 256);
         if (initX == 0) continue;
         init_kernel<<<dim3(initX, 1, (uint32_t)n), 256, 0, stream>>>(batch, true);
         RSFM_CHECK(check_launch("bf init"));
         if (p.gridX == 0 || p.maxB == 0) continue;
-        match_kernel<kBits, false><<<dim3(p.gridX, p.nSeg, (uint32_t)n), 512, 0, stream>>>(batch, p.segCols);
+        match_kernel<kBits, false><<<dim3(p.gridX, p.nSeg, (uint32_t)n), 512, 0, stream>>>(batch);
         RSFM_CHECK(check_launch("bf match"));
     }
     return RSFM_OK;
@@ -529,6 +545,21 @@ static size_t top2_workspace(const rsfm_bf_top2_task* tasks, size_t nbTasks)
         for (size_t i = 0; i < n; i++) bytes += align_up((size_t)p.nSeg * t[i].nA * 16, 256);
     }
     return bytes;
+}
+
+int launch_match_tiles_top2(const DevTask* tasks, const TileDesc* tiles, const uint32_t* xcdOff, uint32_t grid,
+                            hipStream_t stream)
+{
+    RSFM_RETURN_IF(grid == 0 || grid % kNumXcd != 0, RSFM_ERR_INVALID, "match tiles: grid must be a multiple of 8");
+    match_tiles_kernel<8, true><<<grid, 512, 0, stream>>>(tasks, tiles, xcdOff);
+    return check_launch("bf match tiles");
+}
+
+int launch_merge_top2_table(const DevTask* tasks, uint32_t nTasks, uint32_t maxA, hipStream_t stream)
+{
+    if (nTasks == 0 || maxA == 0) return RSFM_OK;
+    merge_top2_kernel<<<dim3(div_up(maxA, 256), 1, nTasks), 256, 0, stream>>>(TaskTable{tasks});
+    return check_launch("bf merge table");
 }
 
 }  // namespace bf
@@ -580,6 +611,10 @@ extern "C" int rsfm_bf_match_top2(const rsfm_bf_top2_task* tasks, size_t nbTasks
             batch.t[i].secondA = reinterpret_cast<uint64_t*>(tasks[b + i].secondMatchA);
         }
         const Plan p = make_plan(batch.t, n);
+        for (size_t i = 0; i < n; i++) {
+            batch.t[i].nSeg = p.nSeg;
+            batch.t[i].segCols = p.segCols;
+        }
         if (p.nSeg > 1) {
             for (size_t i = 0; i < n; i++) {
                 batch.t[i].part = reinterpret_cast<uint64_t*>(ws);
@@ -601,10 +636,10 @@ extern "C" int rsfm_bf_match_top2(const rsfm_bf_top2_task* tasks, size_t nbTasks
             RSFM_CHECK(check_launch("bf top2 init2"));
         }
         if (p.gridX == 0 || p.maxB == 0) continue;
-        match_kernel<8, true><<<dim3(p.gridX, p.nSeg, (uint32_t)n), 512, 0, stream>>>(batch, p.segCols);
+        match_kernel<8, true><<<dim3(p.gridX, p.nSeg, (uint32_t)n), 512, 0, stream>>>(batch);
         RSFM_CHECK(check_launch("bf top2 match"));
         if (p.nSeg > 1) {
-            merge_top2_kernel<<<dim3(div_up(p.maxA, 256), 1, (uint32_t)n), 256, 0, stream>>>(batch, p.nSeg, p.segCols);
+            merge_top2_kernel<<<dim3(div_up(p.maxA, 256), 1, (uint32_t)n), 256, 0, stream>>>(batch);
             RSFM_CHECK(check_launch("bf top2 merge"));
         }
     }

@@ -1,0 +1,93 @@
+// internal.hpp - task descriptors shared between the kernels of the separate
+// translation units and the batched pipeline (library-internal).
+#pragma once
+#include "common.hpp"
+
+namespace rsfm_amd {
+
+// task source of a kernel: a by-value batch (kernel argument) or a table in
+// device memory (batched pipeline); kernels index either with src[i]
+template <class T, int kMax>
+struct ByValue {
+    T t[kMax];
+    __device__ const T& operator[](uint32_t i) const { return t[i]; }
+};
+template <class T>
+struct InTable {
+    const T* p;
+    __device__ const T& operator[](uint32_t i) const { return p[i]; }
+};
+
+namespace bf {
+constexpr int kBM = 256;                // query rows per workgroup tile
+constexpr int kBN = 64;                 // train columns per staged chunk
+constexpr uint32_t kMaxSegCols = 8192;  // 256 blocks of 32: the 8-bit tag of a row key
+constexpr int kMaxTasks = 32;
+
+struct DevTask {
+    uint32_t nA, nB;
+    const uint8_t* A;
+    const uint8_t* B;
+    uint64_t* bestA;
+    uint64_t* secondA;  // top-2 only
+    uint64_t* bestB;    // may be null
+    uint64_t* part;     // top-2 with several segments: [nSeg][nA][2]
+    uint32_t nSeg;      // column segments of this task (1: results written directly)
+    uint32_t segCols;   // columns per segment (multiple of kBN, <= kMaxSegCols)
+};
+typedef ByValue<DevTask, kMaxTasks> TaskBatch;
+typedef InTable<DevTask> TaskTable;
+
+// one workgroup tile of the batched pipeline: kBM query rows x one segment
+struct TileDesc {
+    uint32_t task, rowBase, colBeg, seg;
+};
+
+// persistent top-2 matcher over a tile table split into kNumXcd lists
+int launch_match_tiles_top2(const DevTask* tasks, const TileDesc* tiles, const uint32_t* xcdOff, uint32_t grid,
+                            hipStream_t stream);
+// merge of the segment partials of the listed tasks (all with nSeg > 1)
+int launch_merge_top2_table(const DevTask* tasks, uint32_t nTasks, uint32_t maxA, hipStream_t stream);
+}  // namespace bf
+
+namespace aux {
+struct FilterTask {
+    const uint64_t* fwd;
+    const uint64_t* second;
+    uint32_t nQ;
+    const uint64_t* bwd;
+    uint32_t nT;
+    rsfm_valid_match* out;
+    uint32_t* count;
+    const float2* kpQ;
+    const float2* kpT;
+    float4* pts;
+};
+int launch_filter_table(const FilterTask* tasks, uint32_t nTasks, uint32_t num, uint32_t den, hipStream_t stream);
+}  // namespace aux
+
+namespace ransac {
+struct DevTask {
+    const float4* pts;
+    const uint32_t* nDev;
+    uint32_t nHost;
+    uint32_t H;
+    const uint32_t* idx;
+    uint32_t* counts;
+    float* models;
+    uint32_t* best;
+    uint8_t* mask;
+};
+struct SampleTask {
+    uint32_t* idx;
+    uint32_t H;
+    uint32_t nHost;
+    const uint32_t* nDev;
+    uint64_t seed;
+};
+int launch_sample_table(const SampleTask* tasks, uint32_t nTasks, uint32_t maxH, hipStream_t stream);
+int launch_score_table(const DevTask* tasks, uint32_t nTasks, uint32_t maxH, float threshold, int mode,
+                       float minSepSq0, float minSepSq1, hipStream_t stream);
+}  // namespace ransac
+
+}  // namespace rsfm_amd

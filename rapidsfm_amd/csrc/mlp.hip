@@ -434,6 +434,55 @@ __global__ void adam_kernel(float* __restrict__ P, const float* __restrict__ G, 
     P[p] -= lr * (m / c1) / (sqrtf(v / c2) + eps);
 }
 
+// ---- per-match input features of the proposal network
+struct FeatTask {
+    const uint32_t* matches;  // ValidMatch {idxQuery, idxTrain, float distance} x count
+    const uint32_t* count;
+    uint32_t maxMatches;
+    const uint64_t* second;   // BestMatch per query (distance in the high word)
+    const float4* kpQ;        // (x, y, scale, orientation)
+    const float4* kpT;
+    float invW, invH;
+    float* x;                 // [maxMatches][8]
+};
+constexpr int kFeatMaxTasks = 32;
+struct FeatBatch {
+    FeatTask t[kFeatMaxTasks];
+};
+
+// f0 ratio sqrt(d1/d2) (0 without a second match, 1 for d2 = 0), f1 sqrt(d1)/512,
+// f2..f5 keypoints mapped to [-1, 1], f6 log2 scale ratio (clamped to +-8),
+// f7 orientation difference wrapped to [-pi, pi) over pi; rows past count are 0
+__global__ void features_kernel(const FeatBatch batch)
+{
+    const FeatTask& T = batch.t[blockIdx.y];
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= T.maxMatches) return;
+    const uint32_t cnt = min(*T.count, T.maxMatches);
+    float f[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (k < cnt) {
+        const uint32_t q = T.matches[3 * k], t = T.matches[3 * k + 1];
+        const float d1 = __uint_as_float(T.matches[3 * k + 2]);
+        const uint32_t d2 = (uint32_t)(T.second[q] >> 32);
+        const uint32_t i2 = (uint32_t)T.second[q];
+        f[0] = i2 == ~0u ? 0.f : (d2 == 0u ? 1.f : fminf(sqrtf(d1 / (float)d2), 1.f));
+        f[1] = sqrtf(d1) * (1.f / 512.f);
+        const float4 a = T.kpQ[q], b = T.kpT[t];
+        f[2] = a.x * T.invW * 2.f - 1.f;
+        f[3] = a.y * T.invH * 2.f - 1.f;
+        f[4] = b.x * T.invW * 2.f - 1.f;
+        f[5] = b.y * T.invH * 2.f - 1.f;
+        f[6] = fminf(fmaxf(log2f(b.z / a.z), -8.f), 8.f);
+        float d = b.w - a.w;
+        const float pi = 3.14159265358979f;
+        d = d - 2.f * pi * floorf((d + pi) * (0.5f / pi));
+        f[7] = d * (1.f / pi);
+    }
+    float4* out = reinterpret_cast<float4*>(T.x + (size_t)k * 8);
+    out[0] = make_float4(f[0], f[1], f[2], f[3]);
+    out[1] = make_float4(f[4], f[5], f[6], f[7]);
+}
+
 static uint32_t grad_grid(uint32_t n)
 {
     // enough workgroups to fill the chip, at least one step each
@@ -493,4 +542,34 @@ extern "C" int rsfm_mlp_adam_step(float* params, const float* grad, float* m, fl
     const float c1 = 1.f - powf(beta1, (float)step), c2 = 1.f - powf(beta2, (float)step);
     mlp::adam_kernel<<<div_up(mlp::kNParam, 256), 256, 0, stream>>>(params, grad, m, v, lr, beta1, beta2, eps, c1, c2);
     return check_launch("mlp adam");
+}
+
+extern "C" int rsfm_proposal_features_batch(const rsfm_feature_task* tasks, size_t nbTasks, hipStream_t stream)
+{
+    using namespace mlp;
+    RSFM_RETURN_IF(nbTasks > 0 && tasks == nullptr, RSFM_ERR_INVALID, "features: tasks is null");
+    for (size_t b = 0; b < nbTasks; b += kFeatMaxTasks) {
+        const size_t n = std::min<size_t>(kFeatMaxTasks, nbTasks - b);
+        FeatBatch batch{};
+        uint32_t maxM = 0;
+        for (size_t i = 0; i < n; i++) {
+            const rsfm_feature_task& t = tasks[b + i];
+            RSFM_RETURN_IF(t.maxMatches > 0 && (t.matches == nullptr || t.count == nullptr || t.second == nullptr ||
+                                                t.kpQuery == nullptr || t.kpTrain == nullptr || t.x == nullptr),
+                           RSFM_ERR_INVALID, "features: null pointer");
+            RSFM_RETURN_IF((reinterpret_cast<uintptr_t>(t.kpQuery) | reinterpret_cast<uintptr_t>(t.kpTrain) |
+                            reinterpret_cast<uintptr_t>(t.x)) % 16,
+                           RSFM_ERR_INVALID, "features: keypoints and x must be 16-byte aligned");
+            RSFM_RETURN_IF(!(t.width > 0.f && t.height > 0.f), RSFM_ERR_INVALID, "features: image size must be > 0");
+            batch.t[i] = FeatTask{reinterpret_cast<const uint32_t*>(t.matches), t.count, t.maxMatches,
+                                  reinterpret_cast<const uint64_t*>(t.second),
+                                  reinterpret_cast<const float4*>(t.kpQuery), reinterpret_cast<const float4*>(t.kpTrain),
+                                  1.f / t.width, 1.f / t.height, t.x};
+            maxM = std::max(maxM, t.maxMatches);
+        }
+        if (maxM == 0) continue;
+        features_kernel<<<dim3(div_up(maxM, 256), (uint32_t)n), 256, 0, stream>>>(batch);
+        RSFM_CHECK(check_launch("proposal features"));
+    }
+    return RSFM_OK;
 }

@@ -14,7 +14,7 @@
 // wavefront scores its 64 hypotheses one at a time, lanes striding over the
 // correspondences staged in LDS tiles of 2048, counting inliers with
 // ballot + popcount on the scalar unit.
-#include "common.hpp"
+#include "internal.hpp"
 #include "fgeom.hpp"
 #include <algorithm>
 
@@ -25,20 +25,7 @@ constexpr int kThreads = 256;
 constexpr int kTile = 2048;
 constexpr int kMaxTasks = 16;
 
-struct DevTask {
-    const float4* pts;
-    const uint32_t* nDev;
-    uint32_t nHost;
-    uint32_t H;
-    const uint32_t* idx;
-    uint32_t* counts;
-    float* models;
-    uint32_t* best;
-    uint8_t* mask;
-};
-struct Batch {
-    DevTask t[kMaxTasks];
-};
+typedef ByValue<DevTask, kMaxTasks> Batch;
 
 __device__ inline uint32_t task_n(const DevTask& T) { return T.nDev ? min(*T.nDev, T.nHost) : T.nHost; }
 
@@ -75,11 +62,11 @@ __device__ inline bool fit_hypothesis(const DevTask& T, uint32_t n, uint32_t h, 
     return fgeom::fit8(p, F);
 }
 
-template <int kMode>
-__global__ __launch_bounds__(kThreads) void score_kernel(const Batch batch, float t2, float s0, float s1)
+template <int kMode, class Src>
+__global__ __launch_bounds__(kThreads) void score_kernel(const Src batch, float t2, float s0, float s1)
 {
     const Wg3 wg = xcd_local_block();  // all hypothesis blocks of a pair share one L2
-    const DevTask& T = batch.t[wg.y];
+    const DevTask& T = batch[wg.y];
     const uint32_t hBase = wg.x * kThreads;
     if (hBase >= T.H) return;
     const uint32_t n = task_n(T);
@@ -129,10 +116,10 @@ __global__ __launch_bounds__(kThreads) void score_kernel(const Batch batch, floa
 }
 
 // best hypothesis (largest count, lowest index among equals) and its inlier mask
-template <int kMode>
-__global__ __launch_bounds__(kThreads) void best_kernel(const Batch batch, float t2, float s0, float s1)
+template <int kMode, class Src>
+__global__ __launch_bounds__(kThreads) void best_kernel(const Src batch, float t2, float s0, float s1)
 {
-    const DevTask& T = batch.t[blockIdx.x];
+    const DevTask& T = batch[blockIdx.x];
     const uint32_t n = task_n(T);
     __shared__ uint64_t sKey[kThreads];
     __shared__ float sF[9];
@@ -181,23 +168,15 @@ __device__ inline uint64_t splitmix(uint64_t& s)
     return z ^ (z >> 31);
 }
 
-struct SampleTask {
-    uint32_t* idx;
-    uint32_t H;
-    uint32_t nHost;
-    const uint32_t* nDev;
-    uint64_t seed;
-};
 constexpr int kSampleMaxTasks = 32;
-struct SampleBatch {
-    SampleTask t[kSampleMaxTasks];
-};
+typedef ByValue<SampleTask, kSampleMaxTasks> SampleBatch;
 
 __global__ void sample_kernel(uint32_t* idx, uint32_t H, uint32_t nHost, const uint32_t* nDev, uint64_t seed);
 
-__global__ void sample_batch_kernel(const SampleBatch batch)
+template <class Src>
+__global__ void sample_batch_kernel(const Src batch)
 {
-    const SampleTask& T = batch.t[blockIdx.y];
+    const SampleTask& T = batch[blockIdx.y];
     if (blockIdx.x * blockDim.x >= T.H) return;
     // same draw as sample_kernel, task taken from the batch
     const uint32_t h = blockIdx.x * blockDim.x + threadIdx.x;
@@ -370,6 +349,31 @@ __global__ void sample_kernel(uint32_t* idx, uint32_t H, uint32_t nHost, const u
     }
 #pragma unroll
     for (int i = 0; i < 8; i++) out[i] = v[i];
+}
+
+int launch_sample_table(const SampleTask* tasks, uint32_t nTasks, uint32_t maxH, hipStream_t stream)
+{
+    if (nTasks == 0 || maxH == 0) return RSFM_OK;
+    sample_batch_kernel<<<dim3(div_up(maxH, 256), nTasks), 256, 0, stream>>>(InTable<SampleTask>{tasks});
+    return check_launch("sample table");
+}
+
+int launch_score_table(const DevTask* tasks, uint32_t nTasks, uint32_t maxH, float threshold, int mode,
+                       float minSepSq0, float minSepSq1, hipStream_t stream)
+{
+    RSFM_RETURN_IF(mode != 0 && mode != 1, RSFM_ERR_INVALID, "ransac: mode must be 0 or 1");
+    if (nTasks == 0) return RSFM_OK;
+    const float t2 = threshold * threshold;
+    const InTable<DevTask> src{tasks};
+    if (maxH > 0) {
+        const dim3 grid(div_up(maxH, kThreads), nTasks);
+        if (mode == 0) score_kernel<0><<<grid, kThreads, 0, stream>>>(src, t2, minSepSq0, minSepSq1);
+        else score_kernel<1><<<grid, kThreads, 0, stream>>>(src, t2, minSepSq0, minSepSq1);
+        RSFM_CHECK(check_launch("ransac score table"));
+    }
+    if (mode == 0) best_kernel<0><<<nTasks, kThreads, 0, stream>>>(src, t2, minSepSq0, minSepSq1);
+    else best_kernel<1><<<nTasks, kThreads, 0, stream>>>(src, t2, minSepSq0, minSepSq1);
+    return check_launch("ransac best table");
 }
 
 }  // namespace ransac
