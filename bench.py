@@ -45,7 +45,108 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="bound of the CPU baseline sample")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run one 256 MiB device copy (HBM-counter calibration for rocprofv3 --pmc)")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched-pipeline leg")
+    ap.add_argument("--batched-pairs", type=int, default=256)
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-proposal leg")
+    ap.add_argument("--learned-train", type=int, default=10000, help="training pairs (1K-2K keypoints)")
+    ap.add_argument("--learned-eval", type=int, default=1024, help="held-out 8K-keypoint pairs at 70%% outliers")
     return ap.parse_args()
+
+
+def batched_leg(args, dev, log):
+    """256 pairs, keypoints U[1K,16K] per image, geometric outlier rate U[30%,80%]
+    among the true descriptor matches, one HIP-graph launch per batch."""
+    import numpy as np
+    import torch
+
+    from rapidsfm_amd import synth
+    from rapidsfm_amd.pipeline import Pipeline
+
+    rng = np.random.default_rng(2024)
+    P = args.batched_pairs
+    pairs, truth = [], []
+    for i in range(P):
+        nA, nB = (int(v) for v in rng.integers(1024, 16385, 2))
+        rate = float(rng.uniform(0.3, 0.8))
+        pr = synth.make_pair(5000 + i, nA, nB, copy_frac=0.5, geo_frac=1.0 - rate, device=dev)
+        kA = torch.from_numpy(np.ascontiguousarray(pr["kpA"][:, :2])).to(dev)
+        kB = torch.from_numpy(np.ascontiguousarray(pr["kpB"][:, :2])).to(dev)
+        pairs.append((pr["descA"], pr["descB"], kA, kB))
+        truth.append((pr["src"], pr["geo"]))
+    t0 = time.perf_counter()
+    pl = Pipeline(pairs, n_hyp=args.hyp, threshold=1.0, mode=1, seed=77)
+    t_create = time.perf_counter() - t0
+    for _ in range(2):
+        pl.run()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = max(3, args.steps // 2)
+    e0.record()
+    for _ in range(reps):
+        pl.run()
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    # realised outlier rate of the kept matches (ground truth of the generator)
+    kept = out = 0
+    for (src, geo), o in zip(truth, pl.out):
+        c = int(o["count"].item())
+        m = o["matches"][:c].cpu().numpy()
+        ok = (src[m[:, 0]] == m[:, 1]) & geo[m[:, 0]]
+        kept += c
+        out += int((~ok).sum())
+    info = pl.info()
+    ops = sum(2.0 * a.shape[0] * b.shape[0] * 128 for a, b, _, _ in pairs)
+    pl.close()
+    log("batched: %.3f ms per batch of %d" % (ms, P))
+    return {
+        "workload": "%d pairs, keypoints U[1024,16384] per image, top2+ratio0.8+crosscheck, %d-hypothesis "
+                    "F-RANSAC (Sampson 1px) per pair, one HIP graph launch" % (P, args.hyp),
+        "value": round(P / (ms / 1e3), 1),
+        "unit": "pairs/s",
+        "ms_per_batch": round(ms, 4),
+        "match_tops_equiv": round(ops / (ms / 1e3) / 1e12, 1),
+        "kept_matches": kept,
+        "outlier_rate_kept": round(out / max(1, kept), 4),
+        "tiling": info,
+        "create_s": round(t_create, 2),
+    }
+
+
+def learned_leg(args, dev, log):
+    """Learned proposals: train on plain-RANSAC labels, compare guided vs uniform
+    sampling on held-out 8K pairs at 70% outliers (plain scoring judges both)."""
+    import numpy as np
+
+    from rapidsfm_amd import proposal
+
+    t0 = time.perf_counter()
+    rng = np.random.default_rng(31)
+    n = args.learned_train
+    train = proposal.make_split(17, n, rng.integers(1024, 2049, n), rng.uniform(0.5, 0.9, n), dev)
+    proposal.label_with_plain_ransac(train, H=4096)
+    t_data = time.perf_counter() - t0
+    label_acc = float(np.mean(np.concatenate([(s["y"].cpu().numpy() > 0.5) == s["gt"] for s in train])))
+    t1 = time.perf_counter()
+    params, losses = proposal.train(train, epochs=4, batch=8192)
+    t_train = time.perf_counter() - t1
+    n_samples = sum(int(s["x"].shape[0]) for s in train)
+    del train
+    held = proposal.make_split(7777, args.learned_eval, 8192, 0.7, dev)
+    rep = proposal.evaluate(params, held, H_plain=65536, H_guided=4096)
+    log("learned: gain %.2f (h99 plain %s, guided %s)" % (rep["gain"], rep["h99_plain"], rep["h99_guided"]))
+    rep.update({
+        "train_pairs": n,
+        "train_samples": n_samples,
+        "label_agreement_with_truth": round(label_acc, 4),
+        "train_loss": [round(v, 4) for v in losses],
+        "train_s": round(t_train, 2),
+        "data_s": round(t_data, 2),
+        "held_out": "%d pairs, 8192 keypoints per image, 70%% outliers among the kept matches" % len(held),
+        "h_plain": 65536,
+        "h_guided": 4096,
+    })
+    return rep
 
 
 def main():
@@ -227,6 +328,19 @@ def main():
             "sample": "%d pairs of the same workload (%dx%d top-2 match + filter + %d-hypothesis F-RANSAC), "
                       "%.1f s, OpenMP over %d threads" % (k, n, n, H, t_cpu, cores),
         }
+    def log(msg):
+        print("[bench] " + msg, file=sys.stderr, flush=True)
+
+    if not args.no_batched:
+        try:
+            result["batched"] = batched_leg(args, dev, log)
+        except Exception as e:  # report, keep the main line
+            result["batched"] = {"error": repr(e)}
+    if not args.no_learned:
+        try:
+            result["learned"] = learned_leg(args, dev, log)
+        except Exception as e:
+            result["learned"] = {"error": repr(e)}
     print(json.dumps(result), flush=True)
 
 

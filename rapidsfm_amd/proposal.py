@@ -228,27 +228,33 @@ def h99(q):
 def evaluate(params, split, H_plain=65536, H_guided=4096, seed=9, batch=16, floor=0.02):
     """Guided vs uniform sampling on held-out pairs; returns a summary dict.
 
-    Guided weights: network probability + floor (every match keeps a small
-    chance, so the sampler never starves on a pair the network misjudges).
+    Per pair, an independent plain run of H_plain uniform hypotheses fixes the
+    plain method's inlier count C; a hypothesis succeeds if its count is >=
+    ceil(0.99 C). A second plain run (other seed, H_plain) and the guided run
+    (H_guided) give the success rates. Guided weights: network probability +
+    floor (every match keeps a small chance).
     """
     succ_p = succ_g = 0
     hyp_p = hyp_g = 0
     per_pair = []
     reach = 0
+    dev = split[0]["pts"].device
     for b0 in range(0, len(split), batch):
         part = split[b0:b0 + batch]
         ns = [s["pts"].shape[0] for s in part]
-        idx_p = uniform_samples(ns, H_plain, [seed + b0 + j for j in range(len(part))], part[0]["pts"].device)
+        idx_ref = uniform_samples(ns, H_plain, [seed + b0 + j for j in range(len(part))], dev)
+        idx_p = uniform_samples(ns, H_plain, [seed + 104729 + b0 + j for j in range(len(part))], dev)
         probs = [learned.forward(params, s["x"]) for s in part]
         idx_g = learned.sample_weighted([((p + floor).contiguous(), H_guided, seed + 7919 + b0 + j)
                                          for j, p in enumerate(probs)])
+        out_r = score([(s["pts"], i) for s, i in zip(part, idx_ref)])
         out_p = score([(s["pts"], i) for s, i in zip(part, idx_p)])
         out_g = score([(s["pts"], i) for s, i in zip(part, idx_g)])
-        for op, og in zip(out_p, out_g):
+        for orf, op, og in zip(out_r, out_p, out_g):
+            cref = int(orf["best"][1].item())
             cp = op["counts"].cpu().numpy()
             cg = og["counts"].cpu().numpy()
-            cstar = max(int(cp.max()), int(cg.max()))
-            thr = math.ceil(0.99 * cstar)
+            thr = math.ceil(0.99 * cref)
             sp, sg = int((cp >= thr).sum()), int((cg >= thr).sum())
             succ_p += sp
             succ_g += sg
@@ -266,5 +272,6 @@ def evaluate(params, split, H_plain=65536, H_guided=4096, seed=9, batch=16, floo
         "h99_guided": h99(qg),
         "gain": h99(qp) / h99(qg) if qg > 0 else 0.0,
         "gain_median_pair": float(np.median(ratios)) if ratios else None,
+        "pairs_with_both": len(ratios),
         "guided_reaches_99pct": reach / len(split),
     }
