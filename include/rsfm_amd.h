@@ -270,6 +270,16 @@ typedef struct rsfm_ransac_task {
 int rsfm_ransac_f8(const rsfm_ransac_task* tasks, size_t nbTasks, float threshold, int mode, float minSepSq0,
                    float minSepSq1, hipStream_t stream);
 
+/* 7-point hypotheses (the sampling model of findEpipolarityDegensac,
+ * degensac.cpp:113-150, and findEpipolarityWith7Points,
+ * legacy/geometry.cpp:221-307): sampleIdx is [nbHyp][7]; each sample gives up
+ * to three F from the cubic singularity constraint (own deterministic solver:
+ * FP32 Householder null space, FP64 cubic with bisection roots); a
+ * hypothesis's count and model are those of its best F (first on ties).
+ * best/inlierMask as for rsfm_ransac_f8. */
+int rsfm_ransac_f7(const rsfm_ransac_task* tasks, size_t nbTasks, float threshold, int mode, float minSepSq0,
+                   float minSepSq1, hipStream_t stream);
+
 /* deterministic device sampler (no reference counterpart: the reference's
  * engine is random_device-seeded, ransac.hpp:117): hypothesis h draws 8
  * distinct indices from splitmix64 seeded with seed ^ h * 0xD1B54A32D192ED03,

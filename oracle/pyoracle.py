@@ -84,6 +84,9 @@ _decl("or_f8_ransac_adaptive", U32, [P, U32, F32, I32, F32, U32, U64, F32, F32, 
 _decl("or_ransac_loop_hash", U32, [U32, F32, U32, U64, U32, P, P])
 _decl("or_ransac_required_iters", U32, [U32, U32, U32, F64, U32])
 _decl("or_sample_weighted", None, [P, U32, U32, U64, P])
+_decl("or_f7_fit_f32", I32, [P, P])
+_decl("or_f7_fit_f64", I32, [P, P])
+_decl("or_f7_score", None, [P, U32, P, U32, F32, I32, F32, F32, P, P, I32])
 _decl("or_mlp_forward", None, [P, P, U32, P])
 _decl("or_mlp_loss_grad", F64, [P, P, P, U32, P])
 _decl("or_mlp_forward_bf16", None, [P, P, U32, P])
@@ -465,3 +468,29 @@ def proposal_features(matches, second, kpQ, kpT, width, height):
     out = np.zeros((m.shape[0], 8), np.float32)
     _lib.or_proposal_features(_p(m), m.shape[0], _p(s), _p(kq), _p(kt), width, height, _p(out))
     return out
+
+
+# ---------------- 7-point hypotheses ----------------
+def f7_fit_f32(p7):
+    p = _c(p7, np.float32).reshape(7, 4)
+    Fs = np.zeros((3, 9), np.float32)
+    n = _lib.or_f7_fit_f32(_p(p), _p(Fs))
+    return Fs[:n].reshape(n, 3, 3)
+
+
+def f7_fit_f64(p7):
+    p = _c(p7, np.float64).reshape(7, 4)
+    Fs = np.zeros((3, 9), np.float64)
+    n = _lib.or_f7_fit_f64(_p(p), _p(Fs))
+    return Fs[:n].reshape(n, 3, 3)
+
+
+def f7_score(pts, idx, t, mode, min_sep_sq0=0.0, min_sep_sq1=0.0, nthreads=0, with_models=True):
+    pts = _c(pts, np.float32)
+    idx = _c(idx, np.uint32)
+    H = idx.shape[0]
+    counts = np.zeros(H, np.uint32)
+    models = np.zeros((H, 9), np.float32) if with_models else None
+    _lib.or_f7_score(_p(pts), pts.shape[0], _p(idx), H, t, mode, min_sep_sq0, min_sep_sq1, _p(counts), _p(models),
+                     nthreads)
+    return counts, models

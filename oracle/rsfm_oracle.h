@@ -166,6 +166,14 @@ uint32_t or_ransac_loop_hash(uint32_t n, float confidence, uint32_t max_iter, ui
                              uint32_t modulus, uint32_t best_idx[8], uint32_t* draws_out);
 uint32_t or_ransac_required_iters(uint32_t best, uint32_t n, uint32_t k, double conf, uint32_t max_iter);
 
+/* ---------------- 7-point hypotheses (f7.c) ---------------- */
+/* up to three F (row-major, unit Frobenius) from 7 correspondences; returns the count */
+int  or_f7_fit_f32(const float* pts7 /*7 x 4*/, float* Fs /*3 x 9*/);
+int  or_f7_fit_f64(const double* pts7, double* Fs);
+/* per hypothesis ([H][7] indices): count and model of its best F */
+void or_f7_score(const float* pts, uint32_t n, const uint32_t* idx, uint32_t H, float t, int mode, float min_sep_sq0,
+                 float min_sep_sq1, uint32_t* counts, float* models, int nthreads);
+
 /* ---------------- learned proposal MLP (mlp.c) ---------------- */
 /* 8 -> 64 -> 64 -> 64 -> 1, ReLU, sigmoid. params: W1[64x8] b1[64] W2[64x64]
  * b2[64] W3[64x64] b3[64] W4[64] b4[1] (float64 master copy). */

@@ -124,6 +124,7 @@ _SIGS = {
         [c_void_p, c_void_p, c_u32, c_void_p, c_u32, c_u32, c_u32, c_void_p, c_void_p, c_void_p, c_size, c_void_p],
     ),
     "rsfm_ransac_f8": (c_int, [c_void_p, c_size, c_float, c_int, c_float, c_float, c_void_p]),
+    "rsfm_ransac_f7": (c_int, [c_void_p, c_size, c_float, c_int, c_float, c_float, c_void_p]),
     "rsfm_ransac_sample_indices": (c_int, [c_void_p, c_u32, c_u32, c_void_p, ctypes.c_uint64, c_void_p]),
     "rsfm_filter_matches_batch": (c_int, [c_void_p, c_size, c_u32, c_u32, c_void_p]),
     "rsfm_ransac_sample_indices_batch": (c_int, [c_void_p, c_size, c_void_p]),

@@ -202,6 +202,243 @@ __device__ inline bool fit8(const float4 pts[8], float F[9])
     return true;
 }
 
+// ---- 7-point solver (findEpipolarityWith7Points, legacy/geometry.cpp:221-307)
+// Own formulation, deterministic bit for bit: Hartley normalisation and a
+// Householder QR of the transposed 7x9 system in FP32 give the two null
+// vectors f1, f2 (last two columns of Q); the singularity constraint
+// det(F2 + t (F1 - F2)) = 0 is a cubic whose coefficients follow exactly from
+// four FP64 determinants (t = 0, 1, -1, 2); its real roots are bracketed by the
+// critical points and the Cauchy bound and found by 80 bisection steps in
+// FP64 (no transcendental functions). Each root gives F = F2 + t (F1 - F2)
+// (FP64, rounded to FP32), denormalised as in fit8 and scaled to unit
+// Frobenius norm. Up to three solutions.
+__device__ inline double det3d(const double G[3][3])
+{
+    return G[0][0] * (G[1][1] * G[2][2] - G[1][2] * G[2][1]) - G[0][1] * (G[1][0] * G[2][2] - G[1][2] * G[2][0]) +
+           G[0][2] * (G[1][0] * G[2][1] - G[1][1] * G[2][0]);
+}
+
+__device__ inline double cubic_at(const double a[4], double t) { return ((a[3] * t + a[2]) * t + a[1]) * t + a[0]; }
+
+// root of the cubic in [lo, hi] with p(lo), p(hi) of opposite sign (or zero)
+__device__ inline double bisect(const double a[4], double lo, double hi)
+{
+    double plo = cubic_at(a, lo);
+    if (plo == 0.0) return lo;
+    if (cubic_at(a, hi) == 0.0) return hi;
+    for (int it = 0; it < 80; it++) {
+        const double mid = 0.5 * (lo + hi);
+        const double pm = cubic_at(a, mid);
+        if (pm == 0.0) return mid;
+        if ((pm < 0.0) == (plo < 0.0)) {
+            lo = mid;
+            plo = pm;
+        } else {
+            hi = mid;
+        }
+    }
+    return 0.5 * (lo + hi);
+}
+
+// real roots of a[3] t^3 + a[2] t^2 + a[1] t + a[0] (a[3] != 0), ascending
+__device__ inline int cubic_roots(const double a[4], double r[3])
+{
+    const double inv = 1.0 / fabs(a[3]);
+    double B = fabs(a[2]) * inv;
+    B = fabs(a[1]) * inv > B ? fabs(a[1]) * inv : B;
+    B = fabs(a[0]) * inv > B ? fabs(a[0]) * inv : B;
+    B = B + 1.0;
+    double e[4] = {-B, 0.0, 0.0, B};
+    int ne = 2;
+    const double disc = a[2] * a[2] - 3.0 * a[3] * a[1];
+    if (disc > 0.0) {
+        const double sq = sqrt(disc);
+        double c1 = (-a[2] - sq) / (3.0 * a[3]), c2 = (-a[2] + sq) / (3.0 * a[3]);
+        if (c2 < c1) {
+            const double tmp = c1;
+            c1 = c2;
+            c2 = tmp;
+        }
+        e[1] = c1;
+        e[2] = c2;
+        e[3] = B;
+        ne = 4;
+    } else {
+        e[1] = B;
+    }
+    int n = 0;
+    for (int i = 0; i + 1 < ne; i++) {
+        const double lo = e[i], hi = e[i + 1];
+        const double plo = cubic_at(a, lo), phi = cubic_at(a, hi);
+        if ((plo < 0.0 && phi < 0.0) || (plo > 0.0 && phi > 0.0)) continue;
+        const double x = bisect(a, lo, hi);
+        if (n > 0 && x == r[n - 1]) continue;
+        r[n++] = x;
+    }
+    return n;
+}
+
+// pts[i] = (x0, y0, x1, y1); Fs[k] row-major, unit Frobenius; returns the count
+__device__ inline int fit7(const float4 pts[7], float Fs[3][9])
+{
+    float c[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int i = 0; i < 7; i++) {
+        c[0] = c[0] + pts[i].x;
+        c[1] = c[1] + pts[i].y;
+        c[2] = c[2] + pts[i].z;
+        c[3] = c[3] + pts[i].w;
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) c[k] = c[k] / 7.f;
+    float n0 = 0.f, n1 = 0.f;
+#pragma unroll
+    for (int i = 0; i < 7; i++) {
+        const float d0 = pts[i].x - c[0], d1 = pts[i].y - c[1], d2 = pts[i].z - c[2], d3 = pts[i].w - c[3];
+        n0 = n0 + sqrtf(d0 * d0 + d1 * d1);
+        n1 = n1 + sqrtf(d2 * d2 + d3 * d3);
+    }
+    n0 = n0 / 7.f;
+    n1 = n1 / 7.f;
+    if (!(n0 > 0.f) || !(n1 > 0.f)) return 0;
+    const float sqrt2 = 1.41421354f;
+    const float s0 = sqrt2 / n0, s1 = sqrt2 / n1;
+    float M[9][7];
+#pragma unroll
+    for (int i = 0; i < 7; i++) {
+        const float x0 = (pts[i].x - c[0]) * s0, y0 = (pts[i].y - c[1]) * s0;
+        const float x1 = (pts[i].z - c[2]) * s1, y1 = (pts[i].w - c[3]) * s1;
+        M[0][i] = x1 * x0;
+        M[1][i] = y1 * x0;
+        M[2][i] = x0;
+        M[3][i] = x1 * y0;
+        M[4][i] = y1 * y0;
+        M[5][i] = y0;
+        M[6][i] = x1;
+        M[7][i] = y1;
+        M[8][i] = 1.f;
+    }
+    float vkk[7], beta[7];
+#pragma unroll
+    for (int k = 0; k < 7; k++) {
+        float nrm2 = 0.f;
+#pragma unroll
+        for (int r = k; r < 9; r++) nrm2 = nrm2 + M[r][k] * M[r][k];
+        const float nrm = sqrtf(nrm2);
+        vkk[k] = 0.f;
+        beta[k] = 0.f;
+        if (nrm == 0.f) {
+#pragma unroll
+            for (int r = k + 1; r < 9; r++) M[r][k] = 0.f;
+            continue;
+        }
+        const float alpha = (M[k][k] > 0.f) ? -nrm : nrm;
+        vkk[k] = M[k][k] - alpha;
+        float vtv = vkk[k] * vkk[k];
+#pragma unroll
+        for (int r = k + 1; r < 9; r++) vtv = vtv + M[r][k] * M[r][k];
+        if (vtv == 0.f) continue;
+        beta[k] = 2.f / vtv;
+#pragma unroll
+        for (int j = k + 1; j < 7; j++) {
+            float w = vkk[k] * M[k][j];
+#pragma unroll
+            for (int r = k + 1; r < 9; r++) w = w + M[r][k] * M[r][j];
+            const float bw = beta[k] * w;
+            M[k][j] = M[k][j] - bw * vkk[k];
+#pragma unroll
+            for (int r = k + 1; r < 9; r++) M[r][j] = M[r][j] - bw * M[r][k];
+        }
+    }
+    // f1 = Q e8, f2 = Q e7
+    float f[2][9];
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+#pragma unroll
+        for (int r = 0; r < 9; r++) f[q][r] = (r == 8 - q) ? 1.f : 0.f;
+#pragma unroll
+        for (int k = 6; k >= 0; k--) {
+            if (beta[k] == 0.f) continue;
+            float w = vkk[k] * f[q][k];
+#pragma unroll
+            for (int r = k + 1; r < 9; r++) w = w + M[r][k] * f[q][r];
+            const float bw = beta[k] * w;
+            f[q][k] = f[q][k] - bw * vkk[k];
+#pragma unroll
+            for (int r = k + 1; r < 9; r++) f[q][r] = f[q][r] - bw * M[r][k];
+        }
+    }
+    // G(t) = F2 + t D with F1 = mat(f[0]), F2 = mat(f[1]) (column-major vec)
+    double F2[3][3], D[3][3];
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int cc = 0; cc < 3; cc++) {
+            F2[r][cc] = (double)f[1][3 * cc + r];
+            D[r][cc] = (double)f[0][3 * cc + r] - F2[r][cc];
+        }
+    double d[4];
+    const double ts[4] = {0.0, 1.0, -1.0, 2.0};
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        double G[3][3];
+#pragma unroll
+        for (int r = 0; r < 3; r++)
+#pragma unroll
+            for (int cc = 0; cc < 3; cc++) G[r][cc] = F2[r][cc] + ts[q] * D[r][cc];
+        d[q] = det3d(G);
+    }
+    double a[4];
+    a[0] = d[0];
+    a[2] = 0.5 * (d[1] + d[2]) - d[0];
+    const double sOdd = 0.5 * (d[1] - d[2]);
+    a[3] = ((d[3] - d[0]) - 4.0 * a[2] - 2.0 * sOdd) / 6.0;
+    a[1] = sOdd - a[3];
+    double scale = fabs(a[0]);
+    for (int q = 1; q < 4; q++) scale = fabs(a[q]) > scale ? fabs(a[q]) : scale;
+    if (!(scale > 0.0) || !(fabs(a[3]) > 1e-12 * scale)) return 0;
+    double roots[3];
+    const int nr = cubic_roots(a, roots);
+    const float a0 = -(s0 * c[0]), b0 = -(s0 * c[1]);
+    const float a1 = -(s1 * c[2]), b1 = -(s1 * c[3]);
+    int out = 0;
+    for (int k = 0; k < nr; k++) {
+        float Fh[3][3];
+#pragma unroll
+        for (int r = 0; r < 3; r++)
+#pragma unroll
+            for (int cc = 0; cc < 3; cc++) Fh[r][cc] = (float)(F2[r][cc] + roots[k] * D[r][cc]);
+        float G[3][3];
+#pragma unroll
+        for (int r = 0; r < 3; r++) {
+            G[r][0] = Fh[r][0] * s0;
+            G[r][1] = Fh[r][1] * s0;
+            G[r][2] = (Fh[r][0] * a0 + Fh[r][1] * b0) + Fh[r][2];
+        }
+        float R[3][3];
+#pragma unroll
+        for (int cc = 0; cc < 3; cc++) {
+            R[0][cc] = s1 * G[0][cc];
+            R[1][cc] = s1 * G[1][cc];
+            R[2][cc] = (a1 * G[0][cc] + b1 * G[1][cc]) + G[2][cc];
+        }
+        float fro = 0.f;
+#pragma unroll
+        for (int r = 0; r < 3; r++)
+#pragma unroll
+            for (int cc = 0; cc < 3; cc++) fro = fro + R[r][cc] * R[r][cc];
+        fro = sqrtf(fro);
+        if (!(fro > 0.f) || !isfinite(fro)) continue;
+        const float invf = 1.f / fro;
+#pragma unroll
+        for (int r = 0; r < 3; r++)
+#pragma unroll
+            for (int cc = 0; cc < 3; cc++) Fs[out][r * 3 + cc] = R[r][cc] * invf;
+        out++;
+    }
+    return out;
+}
+
 // checkEpipolarTransformation (legacy/geometry.cpp:55-86), expression by
 // expression; kMode 0 = both point-to-line distances below t (reference),
 // kMode 1 = Sampson distance below t

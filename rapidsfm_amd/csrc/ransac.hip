@@ -157,6 +157,166 @@ __global__ __launch_bounds__(kThreads) void best_kernel(const Src batch, float t
 }
 
 // ---------------------------------------------------------------------------
+// 7-point hypotheses (the sampling model of findEpipolarityDegensac,
+// degensac.cpp:113-150): each sample of 7 indices gives up to three F
+// (fgeom::fit7); a hypothesis scores as its best model (most inliers, first
+// on ties), which is also the model reported for it.
+constexpr int kMaxF7 = 3;
+
+__device__ inline bool sane7(const float4 p[7], float s0, float s1)
+{
+    if (s0 <= 0.f && s1 <= 0.f) return true;
+    bool ok = true;
+#pragma unroll
+    for (int i = 0; i < 7; i++)
+#pragma unroll
+        for (int j = 0; j < i; j++) {
+            const float dx0 = p[i].x - p[j].x, dy0 = p[i].y - p[j].y;
+            const float dx1 = p[i].z - p[j].z, dy1 = p[i].w - p[j].w;
+            if (dx0 * dx0 + dy0 * dy0 < s0 || dx1 * dx1 + dy1 * dy1 < s1) ok = false;
+        }
+    return ok;
+}
+
+__device__ inline int fit_hypothesis7(const DevTask& T, uint32_t n, uint32_t h, float s0, float s1, float Fs[kMaxF7][9])
+{
+    if (h >= T.H || n < 7) return 0;
+    float4 p[7];
+    bool inRange = true;
+#pragma unroll
+    for (int i = 0; i < 7; i++) {
+        const uint32_t k = T.idx[(size_t)h * 7 + i];
+        inRange = inRange && k < n;
+        p[i] = T.pts[k < n ? k : 0];
+    }
+    if (!inRange || !sane7(p, s0, s1)) return 0;
+    return fgeom::fit7(p, Fs);
+}
+
+template <int kMode, class Src>
+__global__ __launch_bounds__(kThreads) void score7_kernel(const Src batch, float t2, float s0, float s1)
+{
+    const Wg3 wg = xcd_local_block();
+    const DevTask& T = batch[wg.y];
+    const uint32_t hBase = wg.x * kThreads;
+    if (hBase >= T.H) return;
+    const uint32_t n = task_n(T);
+    __shared__ float sF[kThreads][kMaxF7][9];
+    __shared__ uint32_t sNM[kThreads];
+    __shared__ uint32_t sCnt[kThreads][kMaxF7];
+    __shared__ float4 sP[kTile];
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    {
+        float Fs[kMaxF7][9];
+        const int nm = fit_hypothesis7(T, n, hBase + tid, s0, s1, Fs);
+#pragma unroll
+        for (int m = 0; m < kMaxF7; m++) {
+#pragma unroll
+            for (int q = 0; q < 9; q++) sF[tid][m][q] = m < nm ? Fs[m][q] : 0.f;
+            sCnt[tid][m] = 0;
+        }
+        sNM[tid] = (uint32_t)nm;
+    }
+    for (uint32_t base = 0; base < n; base += kTile) {
+        const uint32_t tn = min((uint32_t)kTile, n - base);
+        __syncthreads();
+        for (uint32_t i = tid; i < tn; i += kThreads) sP[i] = T.pts[base + i];
+        __syncthreads();
+        for (int hh = 0; hh < 64; hh++) {
+            const int hyp = wave * 64 + hh;
+            const uint32_t nm = sNM[hyp];
+            for (uint32_t m = 0; m < nm; m++) {
+                float F[9];
+#pragma unroll
+                for (int q = 0; q < 9; q++) F[q] = sF[hyp][m][q];
+                uint32_t cnt = 0;
+                for (uint32_t p = lane; p < tn; p += 64) {
+                    const bool in = fgeom::inlier<kMode>(F, sP[p], t2);
+                    cnt += (uint32_t)__popcll(__ballot(in));
+                }
+                if (lane == 0) sCnt[hyp][m] += cnt;
+            }
+        }
+    }
+    __syncthreads();
+    const uint32_t h = hBase + tid;
+    if (h >= T.H) return;
+    const uint32_t nm = sNM[tid];
+    uint32_t best = 0, bm = 0;
+    for (uint32_t m = 0; m < nm; m++)
+        if (m == 0 || sCnt[tid][m] > best) {
+            best = sCnt[tid][m];
+            bm = m;
+        }
+    T.counts[h] = nm ? best : 0u;
+    if (T.models) {
+#pragma unroll
+        for (int q = 0; q < 9; q++) T.models[(size_t)h * 9 + q] = nm ? sF[tid][bm][q] : 0.f;
+    }
+}
+
+// best 7-point hypothesis (largest count, lowest index), its best model and mask
+template <int kMode, class Src>
+__global__ __launch_bounds__(kThreads) void best7_kernel(const Src batch, float t2, float s0, float s1)
+{
+    const DevTask& T = batch[blockIdx.x];
+    const uint32_t n = task_n(T);
+    __shared__ uint64_t sKey[kThreads];
+    __shared__ uint32_t sRed[kMaxF7][kThreads];
+    __shared__ float sFs[kMaxF7][9];
+    __shared__ int sNm;
+    uint64_t key = 0;
+    for (uint32_t h = threadIdx.x; h < T.H; h += kThreads) {
+        const uint64_t k = ((uint64_t)T.counts[h] << 32) | (uint32_t)~h;
+        key = k > key ? k : key;
+    }
+    sKey[threadIdx.x] = key;
+    __syncthreads();
+    for (int s = kThreads / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) sKey[threadIdx.x] = max(sKey[threadIdx.x], sKey[threadIdx.x + s]);
+        __syncthreads();
+    }
+    const uint64_t bk = sKey[0];
+    const uint32_t bestH = T.H ? ~(uint32_t)bk : 0u;
+    const uint32_t bestC = (uint32_t)(bk >> 32);
+    if (threadIdx.x == 0) {
+        float Fs[kMaxF7][9];
+        const int nm = (T.H > 0 && bestC > 0) ? fit_hypothesis7(T, n, bestH, s0, s1, Fs) : 0;
+        for (int m = 0; m < kMaxF7; m++)
+            for (int q = 0; q < 9; q++) sFs[m][q] = m < nm ? Fs[m][q] : 0.f;
+        sNm = nm;
+    }
+    __syncthreads();
+    const int nm = sNm;
+    // counts of the (up to three) models of the best sample
+    uint32_t c[kMaxF7] = {0, 0, 0};
+    for (uint32_t i = threadIdx.x; i < n; i += kThreads) {
+        const float4 p = T.pts[i];
+        for (int m = 0; m < nm; m++) c[m] += fgeom::inlier<kMode>(sFs[m], p, t2) ? 1u : 0u;
+    }
+    for (int m = 0; m < kMaxF7; m++) sRed[m][threadIdx.x] = c[m];
+    __syncthreads();
+    for (int s = kThreads / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s)
+            for (int m = 0; m < kMaxF7; m++) sRed[m][threadIdx.x] += sRed[m][threadIdx.x + s];
+        __syncthreads();
+    }
+    int bm = 0;
+    for (int m = 1; m < nm; m++)
+        if (sRed[m][0] > sRed[bm][0]) bm = m;
+    if (threadIdx.x == 0 && T.best) {
+        T.best[0] = T.H ? bestH : ~0u;
+        T.best[1] = bestC;
+    }
+    if (!T.mask) return;
+    float F[9];
+    for (int q = 0; q < 9; q++) F[q] = sFs[bm][q];
+    for (uint32_t i = threadIdx.x; i < T.nHost; i += kThreads)
+        T.mask[i] = (nm > 0 && i < n) ? (uint8_t)fgeom::inlier<kMode>(F, T.pts[i], t2) : 0;
+}
+
+// ---------------------------------------------------------------------------
 // device sampler: hypothesis h of stream `seed` draws 8 distinct indices in
 // [0, n) from a splitmix64 sequence seeded by (seed, h), rejecting repeats
 __device__ inline uint64_t splitmix(uint64_t& s)
@@ -475,6 +635,45 @@ extern "C" int rsfm_ransac_sample_weighted_batch(const rsfm_weighted_sample_task
         if (maxH == 0) continue;
         weighted_sample_kernel<<<dim3(div_up(maxH, 256), (uint32_t)n), 256, 0, stream>>>(batch);
         RSFM_CHECK(check_launch("weighted sample"));
+    }
+    return RSFM_OK;
+}
+
+extern "C" int rsfm_ransac_f7(const rsfm_ransac_task* tasks, size_t nbTasks, float threshold, int mode, float minSepSq0,
+                              float minSepSq1, hipStream_t stream)
+{
+    using namespace ransac;
+    RSFM_RETURN_IF(nbTasks > 0 && tasks == nullptr, RSFM_ERR_INVALID, "ransac7: tasks is null");
+    RSFM_RETURN_IF(mode != 0 && mode != 1, RSFM_ERR_INVALID, "ransac7: mode must be 0 (two-sided line) or 1 (Sampson)");
+    RSFM_RETURN_IF(!(threshold > 0.f), RSFM_ERR_INVALID, "ransac7: threshold must be > 0");
+    const float t2 = threshold * threshold;
+    for (size_t b = 0; b < nbTasks; b += kMaxTasks) {
+        const size_t n = std::min<size_t>(kMaxTasks, nbTasks - b);
+        Batch batch{};
+        uint32_t maxH = 0;
+        for (size_t i = 0; i < n; i++) {
+            const rsfm_ransac_task& t = tasks[b + i];
+            RSFM_RETURN_IF(t.nbHyp > 0 && (t.sampleIdx == nullptr || t.counts == nullptr), RSFM_ERR_INVALID,
+                           "ransac7: sampleIdx/counts must be set when nbHyp > 0");
+            RSFM_RETURN_IF(t.nbPts > 0 && t.pts == nullptr, RSFM_ERR_INVALID, "ransac7: pts is null");
+            RSFM_RETURN_IF(reinterpret_cast<uintptr_t>(t.pts) % 16, RSFM_ERR_INVALID, "ransac7: pts must be 16-byte aligned");
+            batch.t[i] = DevTask{reinterpret_cast<const float4*>(t.pts), t.nbPtsDev, t.nbPts, t.nbHyp, t.sampleIdx,
+                                 t.counts, t.models, t.best, t.inlierMask};
+            maxH = std::max(maxH, t.nbHyp);
+        }
+        if (maxH > 0) {
+            const dim3 grid(div_up(maxH, kThreads), (uint32_t)n);
+            if (mode == 0) score7_kernel<0><<<grid, kThreads, 0, stream>>>(batch, t2, minSepSq0, minSepSq1);
+            else score7_kernel<1><<<grid, kThreads, 0, stream>>>(batch, t2, minSepSq0, minSepSq1);
+            RSFM_CHECK(check_launch("ransac7 score"));
+        }
+        bool anyBest = false;
+        for (size_t i = 0; i < n; i++) anyBest |= batch.t[i].best != nullptr || batch.t[i].mask != nullptr;
+        if (anyBest) {
+            if (mode == 0) best7_kernel<0><<<(uint32_t)n, kThreads, 0, stream>>>(batch, t2, minSepSq0, minSepSq1);
+            else best7_kernel<1><<<(uint32_t)n, kThreads, 0, stream>>>(batch, t2, minSepSq0, minSepSq1);
+            RSFM_CHECK(check_launch("ransac7 best"));
+        }
     }
     return RSFM_OK;
 }

@@ -97,6 +97,12 @@ def bf_match_top2(pairs, stream=None, workspace=None):
     return outs
 
 
+def ransac_f7(problems, threshold, mode=1, min_sep_sq=(0.0, 0.0), want_models=False, stream=None):
+    """7-point hypotheses: like ransac_f8 with sampleIdx of shape (H, 7); each
+    hypothesis reports its best of up to three F."""
+    return _ransac(lib.rsfm_ransac_f7, "rsfm_ransac_f7", problems, threshold, mode, min_sep_sq, want_models, stream)
+
+
 def ransac_f8(problems, threshold, mode=1, min_sep_sq=(0.0, 0.0), want_models=False, stream=None):
     """Score 8-point hypotheses for a list of (pts, sampleIdx[, nDev]) problems.
 
@@ -105,6 +111,10 @@ def ransac_f8(problems, threshold, mode=1, min_sep_sq=(0.0, 0.0), want_models=Fa
     1 = Sampson. Returns a list of dicts with counts (H,), best (2,), mask (n,) and
     models (H, 3, 3) when asked.
     """
+    return _ransac(lib.rsfm_ransac_f8, "rsfm_ransac_f8", problems, threshold, mode, min_sep_sq, want_models, stream)
+
+
+def _ransac(fn, name, problems, threshold, mode, min_sep_sq, want_models, stream):
     tasks = (RansacTask * max(1, len(problems)))()
     outs = []
     for i, prob in enumerate(problems):
@@ -123,8 +133,8 @@ def ransac_f8(problems, threshold, mode=1, min_sep_sq=(0.0, 0.0), want_models=Fa
         tasks[i] = RansacTask(_ptr(pts), n, _ptr(ndev) if ndev is not None else None, _ptr(idx), H, _ptr(o["counts"]),
                               _ptr(o["models"]) if want_models else None, _ptr(o["best"]), _ptr(o["mask"]))
         outs.append(o)
-    check(lib.rsfm_ransac_f8(ctypes.cast(tasks, ctypes.c_void_p), len(problems), float(threshold), int(mode),
-                             float(min_sep_sq[0]), float(min_sep_sq[1]), _stream(stream)), "rsfm_ransac_f8")
+    check(fn(ctypes.cast(tasks, ctypes.c_void_p), len(problems), float(threshold), int(mode), float(min_sep_sq[0]),
+             float(min_sep_sq[1]), _stream(stream)), name)
     return outs
 
 
