@@ -280,6 +280,42 @@ int rsfm_ransac_f8(const rsfm_ransac_task* tasks, size_t nbTasks, float threshol
 int rsfm_ransac_f7(const rsfm_ransac_task* tasks, size_t nbTasks, float threshold, int mode, float minSepSq0,
                    float minSepSq1, hipStream_t stream);
 
+/* DEGENSAC finisher, replacing the per-new-best body of
+ * findEpipolarityDegensac (degensac.cpp:126-231) after the 7-point samples
+ * were scored by rsfm_ransac_f7 (counts of the same sampleIdx [nbHyp][7]).
+ * The nbCandidates best samples (count, then lowest index; zero counts
+ * skipped) are replayed in that order: local optimisation of each of the
+ * sample's F (checkEpipolarityAndOptimise, degensac.cpp:65-78: FP64
+ * least-squares refit on at most 128 inliers evenly spaced by rank, kept when
+ * it has more inliers); for a sample beating the best so far, the degeneracy
+ * test (H from F and five sample triplets, degensac.cpp:151-187, homography
+ * threshold thresholdH); for >= 5 planar sample points, the optimised H
+ * (checkHomographyAndOptimise, degensac.cpp:87-100) and a plane-and-parallax
+ * search of nbPlaneSamples 2-point F among the points off H
+ * (degensac.cpp:195-228; sample s of candidate ordinal c draws two distinct
+ * ranks from splitmix64 seeded with seed ^ (c * 4096 + s) * 0xD1B54A32D192ED03).
+ * thresholdF and mode are those of rsfm_ransac_f7. Outputs: F and H (row-major,
+ * unit Frobenius, zero when none), result = {F inliers, H inliers, 1 when F
+ * came from plane and parallax else 0, sample index of the best 7-point F or
+ * ~0u}, and the optional inlier masks of F and H (DegensacSolution,
+ * degensac.h). At most 65536 correspondences per pair; one workgroup per pair. */
+typedef struct rsfm_degensac_task {
+    const rsfm_pt_pair* pts;    /* nbPts correspondences, 16-byte aligned */
+    uint32_t nbPts;             /* capacity / count when nbPtsDev is NULL */
+    const uint32_t* nbPtsDev;   /* optional device count (capped at nbPts) */
+    const uint32_t* sampleIdx;  /* [nbHyp][7] */
+    const uint32_t* counts;     /* [nbHyp] from rsfm_ransac_f7 */
+    uint32_t nbHyp;
+    float* F;                   /* [9] */
+    float* H;                   /* [9] */
+    uint32_t* result;           /* [4] */
+    uint8_t* inlierMaskF;       /* optional [nbPts] */
+    uint8_t* inlierMaskH;       /* optional [nbPts] */
+} rsfm_degensac_task;
+int rsfm_degensac_refine(const rsfm_degensac_task* tasks, size_t nbTasks, float thresholdF, float thresholdH,
+                         int mode, uint32_t nbCandidates, uint32_t nbPlaneSamples, uint64_t seed,
+                         hipStream_t stream);
+
 /* deterministic device sampler (no reference counterpart: the reference's
  * engine is random_device-seeded, ransac.hpp:117): hypothesis h draws 8
  * distinct indices from splitmix64 seeded with seed ^ h * 0xD1B54A32D192ED03,

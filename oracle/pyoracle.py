@@ -87,6 +87,12 @@ _decl("or_sample_weighted", None, [P, U32, U32, U64, P])
 _decl("or_f7_fit_f32", I32, [P, P])
 _decl("or_f7_fit_f64", I32, [P, P])
 _decl("or_f7_score", None, [P, U32, P, U32, F32, I32, F32, F32, P, P, I32])
+_decl("or_fit_ls_f", I32, [P, P, U32, P])
+_decl("or_fit_ls_h", I32, [P, P, U32, P])
+_decl("or_h_from_f", I32, [P, P, P])
+_decl("or_f_from_h", I32, [P, P, P, P])
+_decl("or_h_inlier_f32", I32, [P, P, F32])
+_decl("or_degensac_refine", None, [P, U32, P, P, U32, F32, F32, I32, U32, U32, U64, P, P, P, P, P])
 _decl("or_mlp_forward", None, [P, P, U32, P])
 _decl("or_mlp_loss_grad", F64, [P, P, P, U32, P])
 _decl("or_mlp_forward_bf16", None, [P, P, U32, P])
@@ -494,3 +500,56 @@ def f7_score(pts, idx, t, mode, min_sep_sq0=0.0, min_sep_sq1=0.0, nthreads=0, wi
     _lib.or_f7_score(_p(pts), pts.shape[0], _p(idx), H, t, mode, min_sep_sq0, min_sep_sq1, _p(counts), _p(models),
                      nthreads)
     return counts, models
+
+
+# ---------------- DEGENSAC finisher ----------------
+def fit_ls_f(pts, sel):
+    """least-squares F on pts[sel] (None when degenerate)"""
+    pts = _c(pts, np.float32)
+    sel = _c(sel, np.uint32)
+    F = np.zeros(9, np.float32)
+    return F.reshape(3, 3) if _lib.or_fit_ls_f(_p(pts), _p(sel), sel.shape[0], _p(F)) else None
+
+
+def fit_ls_h(pts, sel):
+    pts = _c(pts, np.float32)
+    sel = _c(sel, np.uint32)
+    H = np.zeros(9, np.float32)
+    return H.reshape(3, 3) if _lib.or_fit_ls_h(_p(pts), _p(sel), sel.shape[0], _p(H)) else None
+
+
+def h_from_f(F, pts3):
+    F = _c(F, np.float32).reshape(9)
+    p = _c(pts3, np.float32).reshape(3, 4)
+    H = np.zeros(9, np.float32)
+    return H.reshape(3, 3) if _lib.or_h_from_f(_p(F), _p(p), _p(H)) else None
+
+
+def f_from_h(H, pa, pb):
+    H = _c(H, np.float32).reshape(9)
+    a = _c(pa, np.float32).reshape(4)
+    b = _c(pb, np.float32).reshape(4)
+    F = np.zeros(9, np.float32)
+    return F.reshape(3, 3) if _lib.or_f_from_h(_p(H), _p(a), _p(b), _p(F)) else None
+
+
+def h_inlier_f32(H, pts, t):
+    H = _c(H, np.float32).reshape(9)
+    pts = _c(pts, np.float32).reshape(-1, 4)
+    return np.array([_lib.or_h_inlier_f32(_p(H), _p(pts[i]), t) for i in range(pts.shape[0])], bool)
+
+
+def degensac_refine(pts, idx, counts, t_f, t_h, mode, n_cand, n_plane, seed):
+    """rsfm_degensac_refine for one pair: dict of F, H, result, mask_f, mask_h"""
+    pts = _c(pts, np.float32)
+    idx = _c(idx, np.uint32).reshape(-1, 7)
+    counts = _c(counts, np.uint32)
+    n = pts.shape[0]
+    F = np.zeros(9, np.float32)
+    H = np.zeros(9, np.float32)
+    res = np.zeros(4, np.uint32)
+    mf = np.zeros(max(n, 1), np.uint8)
+    mh = np.zeros(max(n, 1), np.uint8)
+    _lib.or_degensac_refine(_p(pts), n, _p(idx), _p(counts), idx.shape[0], t_f, t_h, mode, n_cand, n_plane, seed,
+                            _p(F), _p(H), _p(res), _p(mf), _p(mh))
+    return {"F": F, "H": H, "result": res, "mask_f": mf[:n], "mask_h": mh[:n]}

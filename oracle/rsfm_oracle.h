@@ -174,6 +174,21 @@ int  or_f7_fit_f64(const double* pts7, double* Fs);
 void or_f7_score(const float* pts, uint32_t n, const uint32_t* idx, uint32_t H, float t, int mode, float min_sep_sq0,
                  float min_sep_sq1, uint32_t* counts, float* models, int nthreads);
 
+/* ---------------- DEGENSAC finisher (degensac.c) ---------------- */
+/* least-squares F (m >= 8) / H (m >= 4) on pts[sel[0..m)], unit Frobenius */
+int  or_fit_ls_f(const float* pts, const uint32_t* sel, uint32_t m, float F[9]);
+int  or_fit_ls_h(const float* pts, const uint32_t* sel, uint32_t m, float H[9]);
+/* H compatible with F through three correspondences (3 x 4) */
+int  or_h_from_f(const float F[9], const float* pts3, float H[9]);
+/* F = [e']x H through two correspondences */
+int  or_f_from_h(const float H[9], const float* pa, const float* pb, float F[9]);
+/* checkProjectiveTransformation for one correspondence */
+int  or_h_inlier_f32(const float H[9], const float* p, float t);
+/* rsfm_degensac_refine for one pair */
+void or_degensac_refine(const float* pts, uint32_t n, const uint32_t* idx, const uint32_t* counts, uint32_t nhyp,
+                        float t_f, float t_h, int mode, uint32_t n_cand, uint32_t n_plane, uint64_t seed,
+                        float F_out[9], float H_out[9], uint32_t result[4], uint8_t* mask_f, uint8_t* mask_h);
+
 /* ---------------- learned proposal MLP (mlp.c) ---------------- */
 /* 8 -> 64 -> 64 -> 64 -> 1, ReLU, sigmoid. params: W1[64x8] b1[64] W2[64x64]
  * b2[64] W3[64x64] b3[64] W4[64] b4[1] (float64 master copy). */

@@ -64,6 +64,22 @@ class RansacTask(ctypes.Structure):
     ]
 
 
+class DegensacTask(ctypes.Structure):
+    _fields_ = [
+        ("pts", c_void_p),
+        ("nbPts", c_u32),
+        ("nbPtsDev", c_void_p),
+        ("sampleIdx", c_void_p),
+        ("counts", c_void_p),
+        ("nbHyp", c_u32),
+        ("F", c_void_p),
+        ("H", c_void_p),
+        ("result", c_void_p),
+        ("inlierMaskF", c_void_p),
+        ("inlierMaskH", c_void_p),
+    ]
+
+
 class FilterTask(ctypes.Structure):
     _fields_ = [
         ("fwd", c_void_p),
@@ -125,6 +141,9 @@ _SIGS = {
     ),
     "rsfm_ransac_f8": (c_int, [c_void_p, c_size, c_float, c_int, c_float, c_float, c_void_p]),
     "rsfm_ransac_f7": (c_int, [c_void_p, c_size, c_float, c_int, c_float, c_float, c_void_p]),
+    "rsfm_degensac_refine": (
+        c_int, [c_void_p, c_size, c_float, c_float, c_int, c_u32, c_u32, ctypes.c_uint64, c_void_p]
+    ),
     "rsfm_ransac_sample_indices": (c_int, [c_void_p, c_u32, c_u32, c_void_p, ctypes.c_uint64, c_void_p]),
     "rsfm_filter_matches_batch": (c_int, [c_void_p, c_size, c_u32, c_u32, c_void_p]),
     "rsfm_ransac_sample_indices_batch": (c_int, [c_void_p, c_size, c_void_p]),

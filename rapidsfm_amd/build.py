@@ -26,6 +26,7 @@ SOURCES = [
     "rmf.hip",
     "mlp.hip",
     "pipeline.hip",
+    "degensac.hip",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -1,0 +1,625 @@
+// degensac.hip - DEGENSAC finisher: local optimisation, the dominant-plane
+// degeneracy test and the plane-and-parallax F search, one workgroup per pair.
+//
+// Reference path: findEpipolarityDegensac (degensac.cpp:102-240). The
+// reference draws 7-point samples one at a time and, each time a sample's
+// best F (after checkEpipolarityAndOptimise, degensac.cpp:65-78) beats the
+// best so far, runs the degeneracy test on it (degensac.cpp:151-187), and for
+// a sample with >= 5 points on one homography refines that H
+// (checkHomographyAndOptimise, degensac.cpp:87-100) and searches F among the
+// off-plane points by 2-point plane-and-parallax RANSAC
+// (degensac.cpp:195-228).
+//
+// Here the 7-point samples are scored in bulk first (rsfm_ransac_f7), and this
+// kernel replays the reference's per-new-best logic over the nbCandidates
+// highest-scoring samples in descending order (count, then lowest index):
+//  - each of a sample's (up to three) F is locally optimised: inlier count
+//    n0; with n0 >= 8, a least-squares refit on at most 128 inliers (evenly
+//    spaced by rank among the inliers, where the reference takes a seeded
+//    Bernoulli subsample, degensac.cpp:33-55) and its count n1; the refit
+//    replaces F when n1 > n0;
+//  - a sample whose best optimised count beats the best so far becomes the
+//    best and gets the degeneracy test: H from F and each of the five sample
+//    triplets {0,1,2} {3,4,5} {0,1,6} {3,4,6} {2,5,6} (the table of the
+//    DEGENSAC paper, also degensac.cpp:152-165), scored on the other four
+//    sample points; 3 + most agreeing >= 5 marks the sample planar;
+//  - a planar sample's H is optimised the same way (>= 4 inliers, refit on at
+//    most 128) and kept when it has the most H inliers so far; the points off
+//    that H feed nbPlaneSamples 2-point samples (splitmix64 stream, ranks into
+//    the off-plane set), each giving F = [e']x H; the one with most inliers
+//    among the off-plane points (lowest sample on ties) is optimised on all
+//    points and replaces the best F when it has more inliers.
+// The reference's adaptive iteration bound (degensac.cpp:230) is replaced by
+// the caller's fixed hypothesis count.
+//
+// GPU layout: every per-point pass is spread over the 256 lanes (ballot
+// inlier bits into LDS bitmaps, integer counts), the plane search scores 256
+// F per round like the hypothesis scorer (one wavefront per 64 F, lanes over
+// LDS point tiles), and the small dense solves run on one lane in FP64
+// (hgeom.hpp).
+#include "internal.hpp"
+#include "fgeom.hpp"
+#include "hgeom.hpp"
+#include <algorithm>
+
+namespace rsfm_amd {
+namespace degensac {
+
+constexpr int kThreads = 256;
+constexpr uint32_t kMaxPts = 65536;
+constexpr uint32_t kMaxWords = kMaxPts / 32;
+constexpr int kMaxSel = 128;  // maxNbSamplesForFHOpt, degensac.cpp:31
+constexpr uint32_t kMaxCand = 64;
+constexpr uint32_t kMaxPlaneSamples = 4096;
+constexpr int kTile = 1024;
+constexpr int kMaxTasks = 16;
+
+struct DevTask {
+    const float4* pts;
+    const uint32_t* nDev;
+    uint32_t nHost;
+    const uint32_t* idx;
+    const uint32_t* counts;
+    uint32_t H;
+    float* F;
+    float* Hm;
+    uint32_t* result;
+    uint8_t* maskF;
+    uint8_t* maskH;
+};
+typedef ByValue<DevTask, kMaxTasks> Batch;
+
+struct Params {
+    float tF2, tH2;
+    uint32_t nCand, nPlane;
+    uint64_t seed;
+};
+
+__device__ __constant__ int kTri[5][3] = {{0, 1, 2}, {3, 4, 5}, {0, 1, 6}, {3, 4, 6}, {2, 5, 6}};
+__device__ __constant__ int kRem[5][4] = {{3, 4, 5, 6}, {0, 1, 2, 6}, {2, 3, 4, 5}, {0, 1, 2, 5}, {0, 1, 3, 4}};
+
+struct FitScratch {
+    double pt[kMaxSel][4];
+    double d0[kMaxSel], d1[kMaxSel];
+    double row[2 * kMaxSel][9];
+    double N[9][9];
+    hgeom::Norm nm;
+};
+struct PlaneScratch {
+    float Fh[kThreads][9];
+    uint32_t cnt[kThreads];
+    uint32_t ok[kThreads];
+    float4 tile[kTile];
+};
+
+struct Smem {
+    uint32_t bits[kMaxWords];     // inlier bits of the last counted model
+    uint32_t planar[kMaxWords];   // inlier bits of the current homography
+    uint32_t prefix[kMaxWords + 1];
+    uint32_t scan[kThreads];
+    uint64_t key[kThreads];
+    union {
+        FitScratch fit;
+        PlaneScratch plane;
+    } u;
+    uint32_t sel[kMaxSel];
+    float4 sample[7];
+    float models[3][9];
+    float fitted[9];
+    float curH[9];
+    float fh[9];
+    float bestF[9];
+    float bestH[9];
+    uint32_t acc;
+    int flag;
+    int nm;
+    uint32_t nbPlanar;
+};
+
+__device__ inline uint32_t task_n(const DevTask& T) { return T.nDev ? min(*T.nDev, T.nHost) : T.nHost; }
+
+__device__ inline uint64_t splitmix(uint64_t& s)
+{
+    s += 0x9E3779B97F4A7C15ull;
+    uint64_t z = s;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+__device__ inline void copy9(float* dst, const float* src)
+{
+    if (threadIdx.x < 9) dst[threadIdx.x] = src[threadIdx.x];
+}
+
+// model kinds: 0 / 1 = F with the two-sided line / Sampson test, 2 = homography
+template <int kKind>
+__device__ inline bool is_inlier(const float m[9], float4 p, float t2)
+{
+    if constexpr (kKind == 2) return hgeom::inlier_h(m, p, t2);
+    else return fgeom::inlier<kKind>(m, p, t2);
+}
+
+// inlier bits of model M (LDS) over the n points into dst; returns the count
+template <int kKind>
+__device__ uint32_t count_pass(Smem& s, const float4* pts, uint32_t n, const float* M, float t2, uint32_t* dst)
+{
+    float m[9];
+#pragma unroll
+    for (int q = 0; q < 9; q++) m[q] = M[q];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (threadIdx.x == 0) s.acc = 0;
+    __syncthreads();
+    uint32_t c = 0;
+    for (uint32_t base = 0; base < n; base += kThreads) {
+        const uint32_t i = base + threadIdx.x;
+        const bool in = i < n && is_inlier<kKind>(m, pts[i], t2);
+        const uint64_t b = __ballot(in);
+        if (lane == 0) {
+            const uint32_t w = (base >> 5) + 2 * wave;
+            dst[w] = (uint32_t)b;
+            dst[w + 1] = (uint32_t)(b >> 32);
+            c += (uint32_t)__popcll(b);
+        }
+    }
+    if (lane == 0) atomicAdd(&s.acc, c);
+    __syncthreads();
+    const uint32_t r = s.acc;
+    __syncthreads();
+    return r;
+}
+
+// word w of the set selected from src: bits below n, optionally complemented
+__device__ inline uint32_t word_of(const uint32_t* src, bool invert, uint32_t w, uint32_t n)
+{
+    uint32_t v = invert ? ~src[w] : src[w];
+    const uint32_t lo = w * 32;
+    if (lo + 32 > n) v &= (n > lo) ? ((1u << (n - lo)) - 1u) : 0u;
+    return v;
+}
+
+// exclusive prefix of the set's word popcounts into s.prefix; returns the total
+__device__ uint32_t prefix_scan(Smem& s, const uint32_t* src, bool invert, uint32_t n)
+{
+    const uint32_t nWords = (n + 31) / 32;
+    constexpr uint32_t kPer = kMaxWords / kThreads;
+    const uint32_t w0 = threadIdx.x * kPer;
+    uint32_t local = 0;
+    for (uint32_t k = 0; k < kPer; k++)
+        if (w0 + k < nWords) local += (uint32_t)__popc(word_of(src, invert, w0 + k, n));
+    s.scan[threadIdx.x] = local;
+    __syncthreads();
+    for (int off = 1; off < kThreads; off <<= 1) {
+        const uint32_t v = threadIdx.x >= (uint32_t)off ? s.scan[threadIdx.x - off] : 0u;
+        __syncthreads();
+        s.scan[threadIdx.x] += v;
+        __syncthreads();
+    }
+    uint32_t run = s.scan[threadIdx.x] - local;
+    for (uint32_t k = 0; k < kPer; k++)
+        if (w0 + k < nWords) {
+            s.prefix[w0 + k] = run;
+            run += (uint32_t)__popc(word_of(src, invert, w0 + k, n));
+        }
+    if (threadIdx.x == kThreads - 1) s.prefix[nWords] = s.scan[kThreads - 1];
+    __syncthreads();
+    const uint32_t total = s.prefix[nWords];
+    __syncthreads();
+    return total;
+}
+
+// at most kMaxSel members of src (count of them = total), evenly spaced by
+// rank: rank r is taken when r = floor(j total / kMaxSel) for some j, and
+// lands in s.sel[j]; all of them when total <= kMaxSel. Returns the number.
+__device__ uint32_t select_evenly(Smem& s, const uint32_t* src, uint32_t n, uint32_t total)
+{
+    prefix_scan(s, src, false, n);
+    const uint32_t nWords = (n + 31) / 32;
+    const bool all = total <= (uint32_t)kMaxSel;
+    for (uint32_t w = threadIdx.x; w < nWords; w += kThreads) {
+        uint32_t x = word_of(src, false, w, n);
+        uint32_t r = s.prefix[w];
+        while (x) {
+            const uint32_t b = (uint32_t)__ffs(x) - 1u;
+            x &= x - 1u;
+            if (all) s.sel[r] = w * 32 + b;
+            else {
+                const uint32_t j = (r * (uint32_t)kMaxSel + total - 1u) / total;
+                if (j < (uint32_t)kMaxSel && (j * total) / (uint32_t)kMaxSel == r) s.sel[j] = w * 32 + b;
+            }
+            r++;
+        }
+    }
+    __syncthreads();
+    return all ? total : (uint32_t)kMaxSel;
+}
+
+// index of the member of rank r (r < total) of the set scanned into s.prefix
+__device__ inline uint32_t select_rank(const Smem& s, const uint32_t* src, bool invert, uint32_t n, uint32_t r)
+{
+    const uint32_t nWords = (n + 31) / 32;
+    uint32_t lo = 0, hi = nWords;  // largest w with prefix[w] <= r
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (s.prefix[mid] <= r) lo = mid;
+        else hi = mid;
+    }
+    uint32_t x = word_of(src, invert, lo, n);
+    for (uint32_t k = s.prefix[lo]; k < r; k++) x &= x - 1u;
+    return min(lo * 32 + (uint32_t)__ffs(x) - 1u, n - 1u);  // the clamp never binds for r < total
+}
+
+// least-squares refit on the m >= 4 (H) / >= 8 (F) points in s.sel:
+// s.fitted and s.flag
+template <bool kH>
+__device__ void fit_ls(Smem& s, const float4* pts, uint32_t m)
+{
+    FitScratch& f = s.u.fit;
+    const uint32_t t = threadIdx.x;
+    if (t < m) {
+        const float4 p = pts[min(s.sel[t], kMaxPts - 1u)];
+        f.pt[t][0] = p.x;
+        f.pt[t][1] = p.y;
+        f.pt[t][2] = p.z;
+        f.pt[t][3] = p.w;
+    }
+    __syncthreads();
+    const double dm = (double)m;
+    if (t == 0) {
+        double a[4] = {0.0, 0.0, 0.0, 0.0};
+        for (uint32_t j = 0; j < m; j++)
+#pragma unroll
+            for (int k = 0; k < 4; k++) a[k] = a[k] + f.pt[j][k];
+#pragma unroll
+        for (int k = 0; k < 4; k++) f.nm.c[k] = a[k] / dm;
+    }
+    __syncthreads();
+    if (t < m) {
+        const double dx0 = f.pt[t][0] - f.nm.c[0], dy0 = f.pt[t][1] - f.nm.c[1];
+        const double dx1 = f.pt[t][2] - f.nm.c[2], dy1 = f.pt[t][3] - f.nm.c[3];
+        f.d0[t] = sqrt(dx0 * dx0 + dy0 * dy0);
+        f.d1[t] = sqrt(dx1 * dx1 + dy1 * dy1);
+    }
+    __syncthreads();
+    if (t == 0) {
+        double a0 = 0.0, a1 = 0.0;
+        for (uint32_t j = 0; j < m; j++) {
+            a0 = a0 + f.d0[j];
+            a1 = a1 + f.d1[j];
+        }
+        const double n0 = a0 / dm, n1 = a1 / dm;
+        s.flag = (n0 > 0.0) && (n1 > 0.0);
+        f.nm.s0 = hgeom::kSqrt2 / n0;
+        f.nm.s1 = hgeom::kSqrt2 / n1;
+    }
+    __syncthreads();
+    if (!s.flag) return;
+    if (t < m) {
+        const double x0 = (f.pt[t][0] - f.nm.c[0]) * f.nm.s0, y0 = (f.pt[t][1] - f.nm.c[1]) * f.nm.s0;
+        const double x1 = (f.pt[t][2] - f.nm.c[2]) * f.nm.s1, y1 = (f.pt[t][3] - f.nm.c[3]) * f.nm.s1;
+        if constexpr (kH) {
+            double* r1 = f.row[2 * t];
+            double* r2 = f.row[2 * t + 1];
+            r1[0] = -x0;
+            r1[1] = -y0;
+            r1[2] = -1.0;
+            r1[3] = 0.0;
+            r1[4] = 0.0;
+            r1[5] = 0.0;
+            r1[6] = x0 * x1;
+            r1[7] = y0 * x1;
+            r1[8] = x1;
+            r2[0] = 0.0;
+            r2[1] = 0.0;
+            r2[2] = 0.0;
+            r2[3] = -x0;
+            r2[4] = -y0;
+            r2[5] = -1.0;
+            r2[6] = x0 * y1;
+            r2[7] = y0 * y1;
+            r2[8] = y1;
+        } else {
+            double* r = f.row[t];
+            r[0] = x1 * x0;
+            r[1] = y1 * x0;
+            r[2] = x0;
+            r[3] = x1 * y0;
+            r[4] = y1 * y0;
+            r[5] = y0;
+            r[6] = x1;
+            r[7] = y1;
+            r[8] = 1.0;
+        }
+    }
+    __syncthreads();
+    if (t < 45) {
+        // (a, b), a <= b, in row-major order of the upper triangle
+        uint32_t a = 0, e = t;
+        while (e >= 9 - a) {
+            e -= 9 - a;
+            a++;
+        }
+        const uint32_t b = a + e;
+        const uint32_t rows = kH ? 2 * m : m;
+        double acc = 0.0;
+        for (uint32_t r = 0; r < rows; r++) acc = acc + f.row[r][a] * f.row[r][b];
+        f.N[a][b] = acc;
+        f.N[b][a] = acc;
+    }
+    __syncthreads();
+    if (t == 0) {
+        double v[9];
+        bool ok = hgeom::null_vector9(f.N, v);
+        if (ok) ok = kH ? hgeom::finish_h(v, f.nm, s.fitted) : hgeom::finish_f(v, f.nm, s.fitted);
+        s.flag = ok;
+    }
+    __syncthreads();
+}
+
+// checkEpipolarityAndOptimise (degensac.cpp:65-78) on the F in M (LDS);
+// M is replaced by the refit when that has more inliers. Returns the count.
+template <int kMode>
+__device__ uint32_t optimise_f(Smem& s, const float4* pts, uint32_t n, float* M, float t2)
+{
+    const uint32_t n0 = count_pass<kMode>(s, pts, n, M, t2, s.bits);
+    if (n0 < 8) return n0;
+    const uint32_t m = select_evenly(s, s.bits, n, n0);
+    fit_ls<false>(s, pts, m);
+    if (!s.flag) return n0;
+    const uint32_t n1 = count_pass<kMode>(s, pts, n, s.fitted, t2, s.bits);
+    if (n1 <= n0) return n0;
+    copy9(M, s.fitted);
+    __syncthreads();
+    return n1;
+}
+
+// checkHomographyAndOptimise (degensac.cpp:87-100) on s.curH; its inlier
+// bits end in s.planar. Returns the count.
+__device__ uint32_t optimise_h(Smem& s, const float4* pts, uint32_t n, float t2)
+{
+    const uint32_t n0 = count_pass<2>(s, pts, n, s.curH, t2, s.planar);
+    if (n0 < 4) return n0;
+    const uint32_t m = select_evenly(s, s.planar, n, n0);
+    fit_ls<true>(s, pts, m);
+    if (!s.flag) return n0;
+    const uint32_t n1 = count_pass<2>(s, pts, n, s.fitted, t2, s.bits);
+    if (n1 <= n0) return n0;
+    copy9(s.curH, s.fitted);
+    for (uint32_t w = threadIdx.x; w < kMaxWords; w += kThreads) s.planar[w] = s.bits[w];
+    __syncthreads();
+    return n1;
+}
+
+__device__ uint64_t block_max(Smem& s, uint64_t v)
+{
+    s.key[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = kThreads / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) s.key[threadIdx.x] = max(s.key[threadIdx.x], s.key[threadIdx.x + off]);
+        __syncthreads();
+    }
+    const uint64_t r = s.key[0];
+    __syncthreads();
+    return r;
+}
+
+// 2-point sample sidx of candidate ordinal cand: ranks into the off-plane set
+__device__ inline bool plane_sample(const Smem& s, const float4* pts, uint32_t n, uint32_t mOff, uint64_t seed,
+                                   uint32_t cand, uint32_t sidx, float F[9])
+{
+    uint64_t st = seed ^ ((uint64_t)(cand * kMaxPlaneSamples + sidx) * 0xD1B54A32D192ED03ull);
+    const uint32_t r0 = (uint32_t)(((splitmix(st) >> 32) * (uint64_t)mOff) >> 32);
+    uint32_t r1 = (uint32_t)(((splitmix(st) >> 32) * (uint64_t)(mOff - 1u)) >> 32);
+    if (r1 >= r0) r1++;
+    const uint32_t ia = select_rank(s, s.planar, true, n, r0);
+    const uint32_t ib = select_rank(s, s.planar, true, n, r1);
+    return hgeom::f_from_h(s.curH, pts[ia], pts[ib], F);
+}
+
+// plane-and-parallax search over the points off s.curH (mOff >= 2 of them):
+// best F in s.fh, false when no sample gave a usable F
+template <int kMode>
+__device__ bool plane_search(Smem& s, const float4* pts, uint32_t n, uint32_t mOff, const Params& p, uint32_t cand)
+{
+    PlaneScratch& ps = s.u.plane;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint64_t best = 0;
+    for (uint32_t chunk = 0; chunk < p.nPlane; chunk += kThreads) {
+        const uint32_t sidx = chunk + threadIdx.x;
+        float F[9];
+        const bool ok = sidx < p.nPlane && plane_sample(s, pts, n, mOff, p.seed, cand, sidx, F);
+#pragma unroll
+        for (int q = 0; q < 9; q++) ps.Fh[threadIdx.x][q] = ok ? F[q] : 0.f;
+        ps.ok[threadIdx.x] = ok;
+        ps.cnt[threadIdx.x] = 0;
+        for (uint32_t base = 0; base < n; base += kTile) {
+            const uint32_t tn = min((uint32_t)kTile, n - base);
+            __syncthreads();
+            for (uint32_t i = threadIdx.x; i < tn; i += kThreads) ps.tile[i] = pts[base + i];
+            __syncthreads();
+            for (int hh = 0; hh < 64; hh++) {
+                const int hyp = wave * 64 + hh;
+                if (!ps.ok[hyp]) continue;
+                float G[9];
+#pragma unroll
+                for (int q = 0; q < 9; q++) G[q] = ps.Fh[hyp][q];
+                uint32_t cnt = 0;
+                for (uint32_t q = lane; q < tn; q += 64) {
+                    const uint32_t i = base + q;
+                    const bool off = !((s.planar[i >> 5] >> (i & 31)) & 1u);
+                    const bool in = off && fgeom::inlier<kMode>(G, ps.tile[q], p.tF2);
+                    cnt += (uint32_t)__popcll(__ballot(in));
+                }
+                if (lane == 0) ps.cnt[hyp] += cnt;
+            }
+        }
+        __syncthreads();
+        const uint64_t key = ps.ok[threadIdx.x] ? (((uint64_t)ps.cnt[threadIdx.x] << 32) | (uint32_t)~sidx) : 0ull;
+        const uint64_t k = block_max(s, key);
+        best = k > best ? k : best;
+    }
+    if (best == 0) return false;
+    if (threadIdx.x == 0) {
+        float F[9];
+        plane_sample(s, pts, n, mOff, p.seed, cand, ~(uint32_t)best, F);
+#pragma unroll
+        for (int q = 0; q < 9; q++) s.fh[q] = F[q];
+    }
+    __syncthreads();
+    return true;
+}
+
+template <int kMode>
+__global__ __launch_bounds__(kThreads) void degensac_kernel(const Batch batch, const Params p)
+{
+    __shared__ Smem s;
+    const DevTask& T = batch[blockIdx.x];
+    const uint32_t n = task_n(T);
+    const float4* pts = T.pts;
+    if (threadIdx.x < 9) {
+        s.bestF[threadIdx.x] = 0.f;
+        s.bestH[threadIdx.x] = 0.f;
+    }
+    uint32_t bestCntF = 0, bestCntH = 0, source = 0, bestHyp = ~0u;
+    uint64_t prevKey = ~0ull;
+    for (uint32_t cand = 0; n >= 7 && cand < p.nCand; cand++) {
+        // next sample: largest (count, lowest index) below the previous one
+        uint64_t key = 0;
+        for (uint32_t h = threadIdx.x; h < T.H; h += kThreads) {
+            const uint32_t c = T.counts[h];
+            const uint64_t k = ((uint64_t)c << 32) | (uint32_t)~h;
+            if (c > 0 && k < prevKey && k > key) key = k;
+        }
+        key = block_max(s, key);
+        if (key == 0) break;
+        prevKey = key;
+        const uint32_t h = ~(uint32_t)key;
+        bool inRange = true;
+        for (int i = 0; i < 7; i++) inRange = inRange && T.idx[(size_t)h * 7 + i] < n;
+        if (threadIdx.x < 7) s.sample[threadIdx.x] = pts[inRange ? T.idx[(size_t)h * 7 + threadIdx.x] : 0];
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            float4 q[7];
+            for (int i = 0; i < 7; i++) q[i] = s.sample[i];
+            float Fs[3][9];
+            const int nm = inRange ? fgeom::fit7(q, Fs) : 0;
+            for (int m = 0; m < nm; m++)
+                for (int k = 0; k < 9; k++) s.models[m][k] = Fs[m][k];
+            s.nm = nm;
+        }
+        __syncthreads();
+        const int nm = s.nm;
+        uint32_t cntF = 0;
+        int bm = -1;
+        for (int m = 0; m < nm; m++) {
+            const uint32_t c = optimise_f<kMode>(s, pts, n, s.models[m], p.tF2);
+            if (c > cntF) {
+                cntF = c;
+                bm = m;
+            }
+        }
+        if (bm < 0 || cntF <= bestCntF) continue;
+        bestCntF = cntF;
+        source = 0;
+        bestHyp = h;
+        copy9(s.bestF, s.models[bm]);
+        __syncthreads();
+        // degeneracy test of the sample under its best F
+        if (threadIdx.x == 0) {
+            uint32_t nbPlanar = 3;
+            for (int i = 0; i < 5; i++) {
+                const float4 tri[3] = {s.sample[kTri[i][0]], s.sample[kTri[i][1]], s.sample[kTri[i][2]]};
+                float Hc[9];
+                if (!hgeom::h_from_f(s.bestF, tri, Hc)) continue;
+                uint32_t c = 3;
+                for (int k = 0; k < 4; k++) c += hgeom::inlier_h(Hc, s.sample[kRem[i][k]], p.tH2) ? 1u : 0u;
+                if (c > nbPlanar) {
+                    nbPlanar = c;
+                    for (int q = 0; q < 9; q++) s.curH[q] = Hc[q];
+                }
+            }
+            s.nbPlanar = nbPlanar;
+        }
+        __syncthreads();
+        if (s.nbPlanar < 5) continue;
+        const uint32_t cntH = optimise_h(s, pts, n, p.tH2);
+        if (cntH > bestCntH) {
+            bestCntH = cntH;
+            copy9(s.bestH, s.curH);
+            __syncthreads();
+        }
+        const uint32_t mOff = prefix_scan(s, s.planar, true, n);
+        if (mOff < 2) continue;
+        if (!plane_search<kMode>(s, pts, n, mOff, p, cand)) continue;
+        const uint32_t cFh = optimise_f<kMode>(s, pts, n, s.fh, p.tF2);
+        if (cFh > bestCntF) {
+            bestCntF = cFh;
+            source = 1;
+            copy9(s.bestF, s.fh);
+            __syncthreads();
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 9) {
+        T.F[threadIdx.x] = s.bestF[threadIdx.x];
+        T.Hm[threadIdx.x] = s.bestH[threadIdx.x];
+    }
+    if (threadIdx.x == 0) {
+        T.result[0] = bestCntF;
+        T.result[1] = bestCntH;
+        T.result[2] = source;
+        T.result[3] = bestHyp;
+    }
+    float F[9], Hh[9];
+#pragma unroll
+    for (int q = 0; q < 9; q++) {
+        F[q] = s.bestF[q];
+        Hh[q] = s.bestH[q];
+    }
+    for (uint32_t i = threadIdx.x; i < T.nHost; i += kThreads) {
+        const bool live = i < n;
+        if (T.maskF) T.maskF[i] = (live && bestCntF > 0) ? (uint8_t)fgeom::inlier<kMode>(F, pts[i], p.tF2) : 0;
+        if (T.maskH) T.maskH[i] = (live && bestCntH > 0) ? (uint8_t)hgeom::inlier_h(Hh, pts[i], p.tH2) : 0;
+    }
+}
+
+}  // namespace degensac
+}  // namespace rsfm_amd
+
+using namespace rsfm_amd;
+
+extern "C" int rsfm_degensac_refine(const rsfm_degensac_task* tasks, size_t nbTasks, float thresholdF,
+                                    float thresholdH, int mode, uint32_t nbCandidates, uint32_t nbPlaneSamples,
+                                    uint64_t seed, hipStream_t stream)
+{
+    using namespace degensac;
+    RSFM_RETURN_IF(nbTasks > 0 && tasks == nullptr, RSFM_ERR_INVALID, "degensac: tasks is null");
+    RSFM_RETURN_IF(mode != 0 && mode != 1, RSFM_ERR_INVALID, "degensac: mode must be 0 (two-sided line) or 1 (Sampson)");
+    RSFM_RETURN_IF(!(thresholdF > 0.f) || !(thresholdH > 0.f), RSFM_ERR_INVALID, "degensac: thresholds must be > 0");
+    RSFM_RETURN_IF(nbCandidates < 1 || nbCandidates > kMaxCand, RSFM_ERR_INVALID,
+                   "degensac: nbCandidates must be in [1, 64]");
+    RSFM_RETURN_IF(nbPlaneSamples < 1 || nbPlaneSamples > kMaxPlaneSamples, RSFM_ERR_INVALID,
+                   "degensac: nbPlaneSamples must be in [1, 4096]");
+    const Params prm{thresholdF * thresholdF, thresholdH * thresholdH, nbCandidates, nbPlaneSamples, seed};
+    for (size_t b = 0; b < nbTasks; b += kMaxTasks) {
+        const size_t n = std::min<size_t>(kMaxTasks, nbTasks - b);
+        Batch batch{};
+        for (size_t i = 0; i < n; i++) {
+            const rsfm_degensac_task& t = tasks[b + i];
+            RSFM_RETURN_IF(t.nbPts > kMaxPts, RSFM_ERR_UNSUPPORTED, "degensac: at most 65536 correspondences per pair");
+            RSFM_RETURN_IF(t.nbPts > 0 && t.pts == nullptr, RSFM_ERR_INVALID, "degensac: pts is null");
+            RSFM_RETURN_IF(reinterpret_cast<uintptr_t>(t.pts) % 16, RSFM_ERR_INVALID,
+                           "degensac: pts must be 16-byte aligned");
+            RSFM_RETURN_IF(t.nbHyp > 0 && (t.sampleIdx == nullptr || t.counts == nullptr), RSFM_ERR_INVALID,
+                           "degensac: sampleIdx/counts must be set when nbHyp > 0");
+            RSFM_RETURN_IF(t.F == nullptr || t.H == nullptr || t.result == nullptr, RSFM_ERR_INVALID,
+                           "degensac: F, H and result are required");
+            batch.t[i] = DevTask{reinterpret_cast<const float4*>(t.pts), t.nbPtsDev, t.nbPts, t.sampleIdx, t.counts,
+                                 t.nbHyp, t.F, t.H, t.result, t.inlierMaskF, t.inlierMaskH};
+        }
+        if (mode == 0) degensac_kernel<0><<<(uint32_t)n, kThreads, 0, stream>>>(batch, prm);
+        else degensac_kernel<1><<<(uint32_t)n, kThreads, 0, stream>>>(batch, prm);
+        RSFM_CHECK(check_launch("degensac refine"));
+    }
+    return RSFM_OK;
+}

@@ -8,7 +8,7 @@ import ctypes
 
 import torch
 
-from ._abi import BfTask, BfTop2Task, RansacTask, check, lib
+from ._abi import BfTask, BfTop2Task, DegensacTask, RansacTask, check, lib
 
 
 def _stream(stream):
@@ -101,6 +101,42 @@ def ransac_f7(problems, threshold, mode=1, min_sep_sq=(0.0, 0.0), want_models=Fa
     """7-point hypotheses: like ransac_f8 with sampleIdx of shape (H, 7); each
     hypothesis reports its best of up to three F."""
     return _ransac(lib.rsfm_ransac_f7, "rsfm_ransac_f7", problems, threshold, mode, min_sep_sq, want_models, stream)
+
+
+def degensac_refine(problems, threshold_f, threshold_h, mode=0, nb_candidates=8, nb_plane_samples=256, seed=0,
+                    want_masks=True, stream=None):
+    """DEGENSAC finisher over 7-point samples already scored by ransac_f7.
+
+    problems: list of (pts, sampleIdx (H, 7), counts (H,)[, nDev]) CUDA tensors.
+    Returns a list of dicts with F (3, 3), H (3, 3), result (4,) = [F inliers,
+    H inliers, 1 if F came from plane and parallax, best sample], and the inlier
+    masks mask_f / mask_h (n,) when asked.
+    """
+    tasks = (DegensacTask * max(1, len(problems)))()
+    outs = []
+    for i, prob in enumerate(problems):
+        pts, idx, counts = prob[0], prob[1], prob[2]
+        ndev = prob[3] if len(prob) > 3 else None
+        if pts.dtype != torch.float32 or pts.dim() != 2 or pts.shape[1] != 4 or not pts.is_contiguous():
+            raise ValueError("pts must be contiguous float32 (n, 4)")
+        if idx.dim() != 2 or idx.shape[1] != 7 or not idx.is_contiguous() or counts.shape[0] != idx.shape[0]:
+            raise ValueError("sampleIdx must be contiguous (H, 7) with counts (H,)")
+        n, dev = pts.shape[0], pts.device
+        o = {
+            "F": torch.empty((3, 3), dtype=torch.float32, device=dev),
+            "H": torch.empty((3, 3), dtype=torch.float32, device=dev),
+            "result": torch.empty(4, dtype=torch.int32, device=dev),
+            "mask_f": torch.empty(n, dtype=torch.uint8, device=dev) if want_masks else None,
+            "mask_h": torch.empty(n, dtype=torch.uint8, device=dev) if want_masks else None,
+        }
+        tasks[i] = DegensacTask(_ptr(pts), n, _ptr(ndev) if ndev is not None else None, _ptr(idx), _ptr(counts),
+                                idx.shape[0], _ptr(o["F"]), _ptr(o["H"]), _ptr(o["result"]),
+                                _ptr(o["mask_f"]) if want_masks else None, _ptr(o["mask_h"]) if want_masks else None)
+        outs.append(o)
+    check(lib.rsfm_degensac_refine(ctypes.cast(tasks, ctypes.c_void_p), len(problems), float(threshold_f),
+                                   float(threshold_h), int(mode), int(nb_candidates), int(nb_plane_samples),
+                                   int(seed) & 0xFFFFFFFFFFFFFFFF, _stream(stream)), "rsfm_degensac_refine")
+    return outs
 
 
 def ransac_f8(problems, threshold, mode=1, min_sep_sq=(0.0, 0.0), want_models=False, stream=None):
