@@ -33,7 +33,7 @@ def test_f7_bit_exact(orc, dev, mode, t, seed, n, n_out, H):
     assert best[0] == hb and best[1] == ref_c[hb]
     if ref_c[hb] > 0:
         F = ref_m[hb]
-        mask = np.array([orc.f_inlier_f32(F, q, t, mode) for q in pts], np.uint8)
+        mask = orc.f_inlier_f32(F, pts, t, mode).astype(np.uint8)
         assert np.array_equal(o["mask"].cpu().numpy(), mask)
         assert int(mask.sum()) == ref_c[hb]
 
