@@ -23,7 +23,7 @@ namespace ransac {
 
 constexpr int kThreads = 256;
 constexpr int kTile = 2048;
-constexpr int kMaxTasks = 16;
+constexpr int kMaxTasks = 32;
 
 typedef ByValue<DevTask, kMaxTasks> Batch;
 
@@ -62,57 +62,47 @@ __device__ inline bool fit_hypothesis(const DevTask& T, uint32_t n, uint32_t h, 
     return fgeom::fit8(p, F);
 }
 
+// Scoring: one lane per hypothesis, one wavefront per workgroup. Each lane
+// fits its hypothesis in registers and then walks every correspondence in
+// index order. The correspondence address is the same in all lanes, so the
+// points arrive through scalar loads (nothing is stored before the loop) and
+// the loop body is VALU work only: no LDS, no cross-lane reduction.
+constexpr int kScoreThreads = 64;
+
 template <int kMode, class Src>
-__global__ __launch_bounds__(kThreads) void score_kernel(const Src batch, float t2, float s0, float s1)
+__global__ __launch_bounds__(kScoreThreads) void score_kernel(const Src batch, float t2, float s0, float s1)
 {
     const Wg3 wg = xcd_local_block();  // all hypothesis blocks of a pair share one L2
     const DevTask& T = batch[wg.y];
-    const uint32_t hBase = wg.x * kThreads;
-    if (hBase >= T.H) return;
+    if (wg.x * kScoreThreads >= T.H) return;
+    const uint32_t h = wg.x * kScoreThreads + threadIdx.x;
     const uint32_t n = task_n(T);
-    __shared__ float sF[kThreads][9];
-    __shared__ uint32_t sValid[kThreads];
-    __shared__ uint32_t sCnt[kThreads];
-    __shared__ float4 sP[kTile];
-
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    {
-        float F[9];
-        const uint32_t h = hBase + tid;
-        const bool ok = fit_hypothesis(T, n, h, s0, s1, F);
-#pragma unroll
-        for (int q = 0; q < 9; q++) sF[tid][q] = ok ? F[q] : 0.f;
-        sValid[tid] = ok;
-        sCnt[tid] = 0;
-        if (T.models && h < T.H) {
+    float F[9];
+    const bool ok = fit_hypothesis(T, n, h, s0, s1, F);
+    uint32_t cnt = 0;
+    if (ok) {
+        const float4* __restrict__ P = T.pts;
+        auto test4 = [&](const float4& a, const float4& b, const float4& c, const float4& d) -> uint32_t {
+            return (fgeom::inlier<kMode>(F, a, t2) ? 1u : 0u) + (fgeom::inlier<kMode>(F, b, t2) ? 1u : 0u) +
+                   (fgeom::inlier<kMode>(F, c, t2) ? 1u : 0u) + (fgeom::inlier<kMode>(F, d, t2) ? 1u : 0u);
+        };
+        uint32_t i = 0;
+        // eight points per trip: two 64-byte scalar loads, then 8 x 64 tests;
+        // the other wavefronts of the SIMD cover the load latency
+        for (; i + 8 <= n; i += 8) {
+            const float4 p0 = P[i], p1 = P[i + 1], p2 = P[i + 2], p3 = P[i + 3];
+            const float4 p4 = P[i + 4], p5 = P[i + 5], p6 = P[i + 6], p7 = P[i + 7];
+            cnt += test4(p0, p1, p2, p3) + test4(p4, p5, p6, p7);
+        }
+        for (; i < n; i++) cnt += fgeom::inlier<kMode>(F, P[i], t2) ? 1u : 0u;
+    }
+    if (h < T.H) {
+        T.counts[h] = ok ? cnt : 0u;
+        if (T.models) {
 #pragma unroll
             for (int q = 0; q < 9; q++) T.models[(size_t)h * 9 + q] = ok ? F[q] : 0.f;
         }
     }
-    for (uint32_t base = 0; base < n; base += kTile) {
-        const uint32_t tn = min((uint32_t)kTile, n - base);
-        __syncthreads();
-        for (uint32_t i = tid; i < tn; i += kThreads) sP[i] = T.pts[base + i];
-        __syncthreads();
-        for (int hh = 0; hh < 64; hh++) {
-            const int hyp = wave * 64 + hh;
-            if (!sValid[hyp]) continue;
-            float F[9];
-#pragma unroll
-            for (int q = 0; q < 9; q++) F[q] = sF[hyp][q];
-            uint32_t cnt = 0;
-            for (uint32_t p = lane; p < tn; p += 64) {
-                const bool in = fgeom::inlier<kMode>(F, sP[p], t2);
-                cnt += (uint32_t)__popcll(__ballot(in));
-            }
-            // the loop bound is lane-dependent only in the last partial round;
-            // inactive lanes vote 0, so cnt is the same in every lane
-            if (lane == 0) sCnt[hyp] += cnt;
-        }
-    }
-    __syncthreads();
-    const uint32_t h = hBase + tid;
-    if (h < T.H) T.counts[h] = sValid[tid] ? sCnt[tid] : 0u;
 }
 
 // best hypothesis (largest count, lowest index among equals) and its inlier mask
@@ -526,9 +516,9 @@ int launch_score_table(const DevTask* tasks, uint32_t nTasks, uint32_t maxH, flo
     const float t2 = threshold * threshold;
     const InTable<DevTask> src{tasks};
     if (maxH > 0) {
-        const dim3 grid(div_up(maxH, kThreads), nTasks);
-        if (mode == 0) score_kernel<0><<<grid, kThreads, 0, stream>>>(src, t2, minSepSq0, minSepSq1);
-        else score_kernel<1><<<grid, kThreads, 0, stream>>>(src, t2, minSepSq0, minSepSq1);
+        const dim3 grid(div_up(maxH, kScoreThreads), nTasks);
+        if (mode == 0) score_kernel<0><<<grid, kScoreThreads, 0, stream>>>(src, t2, minSepSq0, minSepSq1);
+        else score_kernel<1><<<grid, kScoreThreads, 0, stream>>>(src, t2, minSepSq0, minSepSq1);
         RSFM_CHECK(check_launch("ransac score table"));
     }
     if (mode == 0) best_kernel<0><<<nTasks, kThreads, 0, stream>>>(src, t2, minSepSq0, minSepSq1);
@@ -564,9 +554,9 @@ extern "C" int rsfm_ransac_f8(const rsfm_ransac_task* tasks, size_t nbTasks, flo
             maxH = std::max(maxH, t.nbHyp);
         }
         if (maxH > 0) {
-            const dim3 grid(div_up(maxH, kThreads), (uint32_t)n);
-            if (mode == 0) score_kernel<0><<<grid, kThreads, 0, stream>>>(batch, t2, minSepSq0, minSepSq1);
-            else score_kernel<1><<<grid, kThreads, 0, stream>>>(batch, t2, minSepSq0, minSepSq1);
+            const dim3 grid(div_up(maxH, kScoreThreads), (uint32_t)n);
+            if (mode == 0) score_kernel<0><<<grid, kScoreThreads, 0, stream>>>(batch, t2, minSepSq0, minSepSq1);
+            else score_kernel<1><<<grid, kScoreThreads, 0, stream>>>(batch, t2, minSepSq0, minSepSq1);
             RSFM_CHECK(check_launch("ransac score"));
         }
         bool anyBest = false;
