@@ -299,38 +299,97 @@ __device__ inline v4i gather16(const DevImage& im, uint32_t s, uint32_t beg, uin
     return x;
 }
 
+// Work distribution: bucket sizes are far from uniform (with the reference's
+// projection construction the largest of 256 buckets holds ~15% of an image),
+// so a (task, scheme) is split by 32 x 32 tiles, not by bucket. gridDim.x
+// wavefronts share one (task, scheme): each computes the per-bucket tile
+// prefix (ceil(q/32) * ceil(t/32) per bucket) in LDS and takes an even slice
+// of the scheme's tile range, tiles ordered (bucket, query tile, train tile),
+// reusing the gathered query tile while it stays the same.
 __global__ __launch_bounds__(64) void match_kernel(const MatchBatch batch, uint32_t nSchemes)
 {
-    const uint32_t b = blockIdx.x, s = blockIdx.y, task = blockIdx.z;
+    const uint32_t w = blockIdx.x, s = blockIdx.y, task = blockIdx.z, nW = gridDim.x;
     const DevImage& Q = batch.q[task];
     const DevImage& T = batch.t;
     uint64_t* resQ = batch.resQ[task];
     uint64_t* resT = batch.resT[task];
     if (s >= nSchemes || Q.n == 0 || T.n == 0) return;
-    uint32_t qBeg, qCnt, tBeg, tCnt;
-    bucket_range(Q, s, b, qBeg, qCnt);
-    bucket_range(T, s, b, tBeg, tCnt);
-    if (qCnt == 0 || tCnt == 0) return;
+    __shared__ uint32_t sPre[kBuckets + 1];
+    __shared__ uint32_t sNT[kBuckets];
     __shared__ uint32_t sD[32][33];
     __shared__ uint32_t sQid[32], sTid[32];
     const int lane = threadIdx.x, l32 = lane & 31, hi = lane >> 5;
-    for (uint32_t q0 = 0; q0 < qCnt; q0 += 32) {
-        // query rows: u = 127 - a, NA' = |a - 128|^2
-        v4i aop[4];
-        bool qv;
-        uint32_t qid;
-        int32_t nap = 0;
+    {
+        constexpr int kPer = kBuckets / 64;
+        uint32_t cnt[kPer], sum = 0;
 #pragma unroll
-        for (int kc = 0; kc < 4; kc++) {
-            aop[kc] = gather16(Q, s, qBeg, q0 + l32, qCnt, kc * 32 + hi * 16, 0x7F7F7F7F, qv, qid);
-#pragma unroll
-            for (int q = 0; q < 4; q++) {
-                const int xa = aop[kc][q] ^ (int)0xFFFFFFFFu;  // (a ^ 0x7F) ^ 0xFF = a ^ 0x80
-                nap = qv ? __builtin_amdgcn_sdot4(xa, xa, nap, false) : 0;
-            }
+        for (int j = 0; j < kPer; j++) {
+            uint32_t qb, qc, tb, tc;
+            const uint32_t b = lane * kPer + j;
+            bucket_range(Q, s, b, qb, qc);
+            bucket_range(T, s, b, tb, tc);
+            const uint32_t nt = (tc + 31) / 32;
+            sNT[b] = nt;
+            cnt[j] = ((qc + 31) / 32) * nt;
+            sum += cnt[j];
         }
-        const int32_t na = nap + __shfl_xor(nap, 32);
-        for (uint32_t t0 = 0; t0 < tCnt; t0 += 32) {
+        // inclusive scan of the lane sums
+        uint32_t inc = sum;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t v = __shfl_up(inc, off);
+            if (lane >= off) inc += v;
+        }
+        uint32_t run = inc - sum;
+#pragma unroll
+        for (int j = 0; j < kPer; j++) {
+            sPre[lane * kPer + j] = run;
+            run += cnt[j];
+        }
+        if (lane == 63) sPre[kBuckets] = inc;
+    }
+    __syncthreads();
+    const uint32_t total = sPre[kBuckets];
+    const uint32_t kBeg = (uint32_t)(((uint64_t)w * total) / nW);
+    const uint32_t kEnd = (uint32_t)(((uint64_t)(w + 1) * total) / nW);
+    uint32_t curB = ~0u, curQt = ~0u;
+    v4i aop[4];
+    bool qv = false;
+    uint32_t qid = 0;
+    int32_t na = 0;
+    for (uint32_t k = kBeg; k < kEnd; k++) {
+        // bucket of tile k: the largest b with sPre[b] <= k
+        uint32_t lo = 0, hiB = kBuckets;
+        while (hiB - lo > 1) {
+            const uint32_t mid = (lo + hiB) >> 1;
+            if (sPre[mid] <= k) lo = mid;
+            else hiB = mid;
+        }
+        const uint32_t b = lo;
+        const uint32_t local = k - sPre[b];
+        const uint32_t nTT = sNT[b];
+        const uint32_t qt = local / nTT, tt = local - qt * nTT;
+        uint32_t qBeg, qCnt, tBeg, tCnt;
+        bucket_range(Q, s, b, qBeg, qCnt);
+        bucket_range(T, s, b, tBeg, tCnt);
+        if (b != curB || qt != curQt) {
+            // query rows: u = 127 - a, NA' = |a - 128|^2
+            int32_t nap = 0;
+#pragma unroll
+            for (int kc = 0; kc < 4; kc++) {
+                aop[kc] = gather16(Q, s, qBeg, qt * 32 + l32, qCnt, kc * 32 + hi * 16, 0x7F7F7F7F, qv, qid);
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const int xa = aop[kc][q] ^ (int)0xFFFFFFFFu;  // (a ^ 0x7F) ^ 0xFF = a ^ 0x80
+                    nap = qv ? __builtin_amdgcn_sdot4(xa, xa, nap, false) : 0;
+                }
+            }
+            na = nap + __shfl_xor(nap, 32);
+            curB = b;
+            curQt = qt;
+        }
+        const uint32_t t0 = tt * 32;
+        {
             v4i bop[4];
             bool tv;
             uint32_t tid;
@@ -535,7 +594,9 @@ extern "C" int rsfm_cashash_find_best_match(const rsfm_cashash_image* queries, c
         if (maxN == 0) continue;
         match_init_kernel<<<dim3(div_up(maxN, 256), 1, n), 256, 0, stream>>>(batch);
         RSFM_CHECK(check_launch("find_best_match init"));
-        match_kernel<<<dim3(kBuckets, nbSchemes, n), 64, 0, stream>>>(batch, nbSchemes);
+        // wavefronts per (task, scheme): about 4096 in flight over the launch
+        const uint32_t perScheme = std::min<uint32_t>(512u, std::max<uint32_t>(16u, div_up(4096u, n * nbSchemes)));
+        match_kernel<<<dim3(perScheme, nbSchemes, n), 64, 0, stream>>>(batch, nbSchemes);
         RSFM_CHECK(check_launch("find_best_match"));
     }
     return RSFM_OK;

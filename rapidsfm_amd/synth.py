@@ -126,3 +126,15 @@ def make_pair(seed, n_a, n_b, copy_frac, geo_frac, device="cuda", sigma=0.05):
         geo[g_slots] = True
     return dict(descA=desc_a, descB=desc_b, kpA=kp_a.astype(np.float32), kpB=kp_b.astype(np.float32), src=src,
                 geo=geo, F=view.F)
+
+
+def cashash_weights(seed, n_channels=192, device="cuda"):
+    """Gaussian random projections for cascade hashing (seeded): int8 weights
+    [n_channels][128] = round(clamp(32 N(0,1), -128, 127)) and int32 bias_base[c]
+    = round(sum_k w[c][k] r_k), r_k = clamp(16 N(0,1), -64, 64)."""
+    gen = torch.Generator()
+    gen.manual_seed(int(seed) & 0x7FFFFFFFFFFFFFFF)
+    w = torch.round((32.0 * torch.randn((n_channels, 128), generator=gen, dtype=torch.float64)).clamp(-128, 127))
+    r = (16.0 * torch.randn((n_channels, 128), generator=gen, dtype=torch.float64)).clamp(-64, 64)
+    base = torch.round((w * r).sum(dim=1))
+    return w.to(torch.int8).to(device), base.to(torch.int32).to(device)
