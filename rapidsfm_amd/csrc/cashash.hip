@@ -357,6 +357,15 @@ __global__ __launch_bounds__(64) void match_kernel(const MatchBatch batch, uint3
     bool qv = false;
     uint32_t qid = 0;
     int32_t na = 0;
+    // query-row minimum over the consecutive tiles of the current query block
+    // (lanes 0..31), merged into resQ when the block changes and at the end
+    uint64_t rowBest = kNoMatch;
+    uint32_t rowQid = ~0u;
+    auto flush_row = [&]() {
+        if (hi == 0 && rowQid != ~0u && rowBest != kNoMatch)
+            atomicMin(reinterpret_cast<unsigned long long*>(resQ + rowQid), (unsigned long long)rowBest);
+        rowBest = kNoMatch;
+    };
     for (uint32_t k = kBeg; k < kEnd; k++) {
         // bucket of tile k: the largest b with sPre[b] <= k
         uint32_t lo = 0, hiB = kBuckets;
@@ -373,6 +382,7 @@ __global__ __launch_bounds__(64) void match_kernel(const MatchBatch batch, uint3
         bucket_range(Q, s, b, qBeg, qCnt);
         bucket_range(T, s, b, tBeg, tCnt);
         if (b != curB || qt != curQt) {
+            flush_row();
             // query rows: u = 127 - a, NA' = |a - 128|^2
             int32_t nap = 0;
 #pragma unroll
@@ -421,16 +431,15 @@ __global__ __launch_bounds__(64) void match_kernel(const MatchBatch batch, uint3
             }
             __syncthreads();
             if (hi == 0) {
-                // lane l32 < 32: query row l32 -> best train column
-                const uint32_t gq = sQid[l32];
-                if (gq != ~0u) {
-                    uint64_t best = kNoMatch;
+                // lane l32 < 32: query row l32 -> best train column of this tile,
+                // folded into the running minimum of the query block
+                rowQid = sQid[l32];
+                if (rowQid != ~0u) {
                     for (int c = 0; c < 32; c++) {
                         const uint32_t gt = sTid[c];
                         if (gt == ~0u) continue;
-                        best = min(best, make_key(sD[l32][c], gt));
+                        rowBest = min(rowBest, make_key(sD[l32][c], gt));
                     }
-                    if (best != kNoMatch) atomicMin(reinterpret_cast<unsigned long long*>(resQ + gq), (unsigned long long)best);
                 }
             } else if (resT) {
                 // lane 32 + c: train column c -> best query row
@@ -447,6 +456,7 @@ __global__ __launch_bounds__(64) void match_kernel(const MatchBatch batch, uint3
             }
         }
     }
+    flush_row();
 }
 
 __global__ void match_init_kernel(const MatchBatch batch)
@@ -594,8 +604,8 @@ extern "C" int rsfm_cashash_find_best_match(const rsfm_cashash_image* queries, c
         if (maxN == 0) continue;
         match_init_kernel<<<dim3(div_up(maxN, 256), 1, n), 256, 0, stream>>>(batch);
         RSFM_CHECK(check_launch("find_best_match init"));
-        // wavefronts per (task, scheme): about 4096 in flight over the launch
-        const uint32_t perScheme = std::min<uint32_t>(512u, std::max<uint32_t>(16u, div_up(4096u, n * nbSchemes)));
+        // wavefronts per (task, scheme): about 16384 over the launch
+        const uint32_t perScheme = std::min<uint32_t>(2048u, std::max<uint32_t>(16u, div_up(16384u, n * nbSchemes)));
         match_kernel<<<dim3(perScheme, nbSchemes, n), 64, 0, stream>>>(batch, nbSchemes);
         RSFM_CHECK(check_launch("find_best_match"));
     }
