@@ -183,66 +183,58 @@ __device__ inline int fit_hypothesis7(const DevTask& T, uint32_t n, uint32_t h, 
     return fgeom::fit7(p, Fs);
 }
 
+// one lane per 7-point sample, as score_kernel: up to three F in registers,
+// each point tested against all three (a missing model is all zeros and
+// never counts: 0 < 0 is false in both tests)
 template <int kMode, class Src>
-__global__ __launch_bounds__(kThreads) void score7_kernel(const Src batch, float t2, float s0, float s1)
+__global__ __launch_bounds__(kScoreThreads) void score7_kernel(const Src batch, float t2, float s0, float s1)
 {
     const Wg3 wg = xcd_local_block();
     const DevTask& T = batch[wg.y];
-    const uint32_t hBase = wg.x * kThreads;
-    if (hBase >= T.H) return;
+    if (wg.x * kScoreThreads >= T.H) return;
+    const uint32_t h = wg.x * kScoreThreads + threadIdx.x;
     const uint32_t n = task_n(T);
-    __shared__ float sF[kThreads][kMaxF7][9];
-    __shared__ uint32_t sNM[kThreads];
-    __shared__ uint32_t sCnt[kThreads][kMaxF7];
-    __shared__ float4 sP[kTile];
-
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    {
-        float Fs[kMaxF7][9];
-        const int nm = fit_hypothesis7(T, n, hBase + tid, s0, s1, Fs);
+    float Fw[kMaxF7][9];  // written by the solver with a running index
+    const int nm = fit_hypothesis7(T, n, h, s0, s1, Fw);
+    float Fs[kMaxF7][9];  // register copy for the scoring loop
 #pragma unroll
-        for (int m = 0; m < kMaxF7; m++) {
+    for (int m = 0; m < kMaxF7; m++)
 #pragma unroll
-            for (int q = 0; q < 9; q++) sF[tid][m][q] = m < nm ? Fs[m][q] : 0.f;
-            sCnt[tid][m] = 0;
+        for (int q = 0; q < 9; q++) Fs[m][q] = m < nm ? Fw[m][q] : 0.f;
+    uint32_t c[kMaxF7] = {0u, 0u, 0u};
+    if (nm > 0) {
+        const float4* __restrict__ P = T.pts;
+        auto test = [&](const float4& p) {
+#pragma unroll
+            for (int m = 0; m < kMaxF7; m++) c[m] += fgeom::inlier<kMode>(Fs[m], p, t2) ? 1u : 0u;
+        };
+        uint32_t i = 0;
+        for (; i + 4 <= n; i += 4) {
+            const float4 p0 = P[i], p1 = P[i + 1], p2 = P[i + 2], p3 = P[i + 3];
+            test(p0);
+            test(p1);
+            test(p2);
+            test(p3);
         }
-        sNM[tid] = (uint32_t)nm;
+        for (; i < n; i++) test(P[i]);
     }
-    for (uint32_t base = 0; base < n; base += kTile) {
-        const uint32_t tn = min((uint32_t)kTile, n - base);
-        __syncthreads();
-        for (uint32_t i = tid; i < tn; i += kThreads) sP[i] = T.pts[base + i];
-        __syncthreads();
-        for (int hh = 0; hh < 64; hh++) {
-            const int hyp = wave * 64 + hh;
-            const uint32_t nm = sNM[hyp];
-            for (uint32_t m = 0; m < nm; m++) {
-                float F[9];
-#pragma unroll
-                for (int q = 0; q < 9; q++) F[q] = sF[hyp][m][q];
-                uint32_t cnt = 0;
-                for (uint32_t p = lane; p < tn; p += 64) {
-                    const bool in = fgeom::inlier<kMode>(F, sP[p], t2);
-                    cnt += (uint32_t)__popcll(__ballot(in));
-                }
-                if (lane == 0) sCnt[hyp][m] += cnt;
-            }
-        }
-    }
-    __syncthreads();
-    const uint32_t h = hBase + tid;
     if (h >= T.H) return;
-    const uint32_t nm = sNM[tid];
-    uint32_t best = 0, bm = 0;
-    for (uint32_t m = 0; m < nm; m++)
-        if (m == 0 || sCnt[tid][m] > best) {
-            best = sCnt[tid][m];
+    uint32_t best = 0;
+    int bm = 0;
+    for (int m = 0; m < nm; m++)
+        if (m == 0 || c[m] > best) {
+            best = c[m];
             bm = m;
         }
     T.counts[h] = nm ? best : 0u;
     if (T.models) {
 #pragma unroll
-        for (int q = 0; q < 9; q++) T.models[(size_t)h * 9 + q] = nm ? sF[tid][bm][q] : 0.f;
+        for (int q = 0; q < 9; q++) {
+            float v = Fs[0][q];
+            if (bm == 1) v = Fs[1][q];
+            if (bm == 2) v = Fs[2][q];
+            T.models[(size_t)h * 9 + q] = nm ? v : 0.f;
+        }
     }
 }
 
@@ -652,9 +644,9 @@ extern "C" int rsfm_ransac_f7(const rsfm_ransac_task* tasks, size_t nbTasks, flo
             maxH = std::max(maxH, t.nbHyp);
         }
         if (maxH > 0) {
-            const dim3 grid(div_up(maxH, kThreads), (uint32_t)n);
-            if (mode == 0) score7_kernel<0><<<grid, kThreads, 0, stream>>>(batch, t2, minSepSq0, minSepSq1);
-            else score7_kernel<1><<<grid, kThreads, 0, stream>>>(batch, t2, minSepSq0, minSepSq1);
+            const dim3 grid(div_up(maxH, kScoreThreads), (uint32_t)n);
+            if (mode == 0) score7_kernel<0><<<grid, kScoreThreads, 0, stream>>>(batch, t2, minSepSq0, minSepSq1);
+            else score7_kernel<1><<<grid, kScoreThreads, 0, stream>>>(batch, t2, minSepSq0, minSepSq1);
             RSFM_CHECK(check_launch("ransac7 score"));
         }
         bool anyBest = false;
