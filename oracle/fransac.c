@@ -24,8 +24,10 @@
  * Pinned in tests by: noise-free two-view geometry (the fit must recover the
  * closed-form F = K^-T [t]x R K^-1 up to scale), numpy SVD null vectors, and
  * hand-solved epipolar distances for rectified pairs.
- * The inlier tests follow geometry.cpp:66-83 expression by expression; the
- * Sampson mode is new (e^2 < t^2 (l0x^2+l0y^2+l1x^2+l1y^2)).
+ * The two-sided inlier test (mode 0) follows geometry.cpp:66-83 expression by
+ * expression without contraction; the Sampson mode (1) is this library's own
+ * test, e^2 < t^2 (l0x^2+l0y^2+l1x^2+l1y^2), written with fmaf exactly where
+ * the GPU kernel fuses (one rounding per fmaf on both sides).
  */
 #include "rsfm_oracle.h"
 #include <math.h>
@@ -173,19 +175,29 @@ int or_f8_fit_f32(const float* pts8, float F[9])
 int or_f_inlier_f32(const float F[9], const float* p, float t, int mode)
 {
     const float x0 = p[0], y0 = p[1], x1 = p[2], y1 = p[3];
-    /* epipole0 = x1^T F, epipole1 = F x0 (geometry.cpp:66-75) */
-    const float l0a = x1 * F[0] + (y1 * F[3] + F[6]);
-    const float l0b = x1 * F[1] + (y1 * F[4] + F[7]);
-    const float l0c = x1 * F[2] + (y1 * F[5] + F[8]);
-    const float l1a = x0 * F[0] + (y0 * F[1] + F[2]);
-    const float l1b = x0 * F[3] + (y0 * F[4] + F[5]);
-    const float e = l0a * x0 + (l0b * y0 + l0c);
-    const float e2 = e * e;
     const float t2 = t * t;
-    const float n0 = l0a * l0a + l0b * l0b;
-    const float n1 = l1a * l1a + l1b * l1b;
-    if (mode == 0) return (e2 < t2 * n0) && (e2 < t2 * n1);
-    return e2 < t2 * (n0 + n1);
+    if (mode == 0) {
+        /* epipole0 = x1^T F, epipole1 = F x0 (geometry.cpp:66-75), no contraction */
+        const float l0a = x1 * F[0] + (y1 * F[3] + F[6]);
+        const float l0b = x1 * F[1] + (y1 * F[4] + F[7]);
+        const float l0c = x1 * F[2] + (y1 * F[5] + F[8]);
+        const float l1a = x0 * F[0] + (y0 * F[1] + F[2]);
+        const float l1b = x0 * F[3] + (y0 * F[4] + F[5]);
+        const float e = l0a * x0 + (l0b * y0 + l0c);
+        const float e2 = e * e;
+        const float n0 = l0a * l0a + l0b * l0b;
+        const float n1 = l1a * l1a + l1b * l1b;
+        return (e2 < t2 * n0) && (e2 < t2 * n1);
+    }
+    /* Sampson (own test): the GPU's fused multiply-adds, one rounding each */
+    const float l0a = fmaf(x1, F[0], fmaf(y1, F[3], F[6]));
+    const float l0b = fmaf(x1, F[1], fmaf(y1, F[4], F[7]));
+    const float l0c = fmaf(x1, F[2], fmaf(y1, F[5], F[8]));
+    const float l1a = fmaf(x0, F[0], fmaf(y0, F[1], F[2]));
+    const float l1b = fmaf(x0, F[3], fmaf(y0, F[4], F[5]));
+    const float e = fmaf(l0a, x0, fmaf(l0b, y0, l0c));
+    const float n = fmaf(l0a, l0a, fmaf(l0b, l0b, fmaf(l1a, l1a, l1b * l1b)));
+    return e * e < t2 * n;
 }
 
 /* ---------------- FP64 oracle fit ---------------- */

@@ -439,24 +439,37 @@ __device__ inline int fit7(const float4 pts[7], float Fs[3][9])
     return out;
 }
 
-// checkEpipolarTransformation (legacy/geometry.cpp:55-86), expression by
-// expression; kMode 0 = both point-to-line distances below t (reference),
-// kMode 1 = Sampson distance below t
+// Inlier tests. kMode 0 = checkEpipolarTransformation (legacy/geometry.cpp:
+// 55-86), expression by expression without fused multiply-add: both
+// point-to-line distances below t. kMode 1 = Sampson distance below t, this
+// library's own test (no reference counterpart), written with fused
+// multiply-adds (fmaf: one rounding each, identical on CPU and GPU):
+// e = x1^T F x0, e^2 < t^2 (|F^T x1|_12^2 + |F x0|_12^2).
 template <int kMode>
 __device__ inline bool inlier(const float F[9], float4 p, float t2)
 {
     const float x0 = p.x, y0 = p.y, x1 = p.z, y1 = p.w;
-    const float l0a = x1 * F[0] + (y1 * F[3] + F[6]);
-    const float l0b = x1 * F[1] + (y1 * F[4] + F[7]);
-    const float l0c = x1 * F[2] + (y1 * F[5] + F[8]);
-    const float l1a = x0 * F[0] + (y0 * F[1] + F[2]);
-    const float l1b = x0 * F[3] + (y0 * F[4] + F[5]);
-    const float e = l0a * x0 + (l0b * y0 + l0c);
-    const float e2 = e * e;
-    const float n0 = l0a * l0a + l0b * l0b;
-    const float n1 = l1a * l1a + l1b * l1b;
-    if constexpr (kMode == 0) return (e2 < t2 * n0) && (e2 < t2 * n1);
-    else return e2 < t2 * (n0 + n1);
+    if constexpr (kMode == 0) {
+        const float l0a = x1 * F[0] + (y1 * F[3] + F[6]);
+        const float l0b = x1 * F[1] + (y1 * F[4] + F[7]);
+        const float l0c = x1 * F[2] + (y1 * F[5] + F[8]);
+        const float l1a = x0 * F[0] + (y0 * F[1] + F[2]);
+        const float l1b = x0 * F[3] + (y0 * F[4] + F[5]);
+        const float e = l0a * x0 + (l0b * y0 + l0c);
+        const float e2 = e * e;
+        const float n0 = l0a * l0a + l0b * l0b;
+        const float n1 = l1a * l1a + l1b * l1b;
+        return (e2 < t2 * n0) && (e2 < t2 * n1);
+    } else {
+        const float l0a = fmaf(x1, F[0], fmaf(y1, F[3], F[6]));
+        const float l0b = fmaf(x1, F[1], fmaf(y1, F[4], F[7]));
+        const float l0c = fmaf(x1, F[2], fmaf(y1, F[5], F[8]));
+        const float l1a = fmaf(x0, F[0], fmaf(y0, F[1], F[2]));
+        const float l1b = fmaf(x0, F[3], fmaf(y0, F[4], F[5]));
+        const float e = fmaf(l0a, x0, fmaf(l0b, y0, l0c));
+        const float n = fmaf(l0a, l0a, fmaf(l0b, l0b, fmaf(l1a, l1a, l1b * l1b)));
+        return e * e < t2 * n;
+    }
 }
 
 }  // namespace fgeom
