@@ -81,18 +81,35 @@ __global__ __launch_bounds__(kScoreThreads) void score_kernel(const Src batch, f
     const bool ok = fit_hypothesis(T, n, h, s0, s1, F);
     uint32_t cnt = 0;
     if (ok) {
-        const float4* __restrict__ P = T.pts;
+        // The point index is hidden from the uniformity analysis so the loads
+        // take the vector memory path (in-order vmcnt): the next eight points
+        // are in flight while the current eight are tested. All 64 lanes read
+        // the same address, one cache line per wave.
+        uint32_t base = 0;
+        asm volatile("" : "+v"(base));
+        const float4* __restrict__ P = T.pts + base;
         auto test4 = [&](const float4& a, const float4& b, const float4& c, const float4& d) -> uint32_t {
             return (fgeom::inlier<kMode>(F, a, t2) ? 1u : 0u) + (fgeom::inlier<kMode>(F, b, t2) ? 1u : 0u) +
                    (fgeom::inlier<kMode>(F, c, t2) ? 1u : 0u) + (fgeom::inlier<kMode>(F, d, t2) ? 1u : 0u);
         };
         uint32_t i = 0;
-        // eight points per trip: two 64-byte scalar loads, then 8 x 64 tests;
-        // the other wavefronts of the SIMD cover the load latency
-        for (; i + 8 <= n; i += 8) {
-            const float4 p0 = P[i], p1 = P[i + 1], p2 = P[i + 2], p3 = P[i + 3];
-            const float4 p4 = P[i + 4], p5 = P[i + 5], p6 = P[i + 6], p7 = P[i + 7];
-            cnt += test4(p0, p1, p2, p3) + test4(p4, p5, p6, p7);
+        if (n >= 8) {
+            float4 c0 = P[0], c1 = P[1], c2 = P[2], c3 = P[3], c4 = P[4], c5 = P[5], c6 = P[6], c7 = P[7];
+            for (; i + 16 <= n; i += 8) {
+                const float4 n0 = P[i + 8], n1 = P[i + 9], n2 = P[i + 10], n3 = P[i + 11];
+                const float4 n4 = P[i + 12], n5 = P[i + 13], n6 = P[i + 14], n7 = P[i + 15];
+                cnt += test4(c0, c1, c2, c3) + test4(c4, c5, c6, c7);
+                c0 = n0;
+                c1 = n1;
+                c2 = n2;
+                c3 = n3;
+                c4 = n4;
+                c5 = n5;
+                c6 = n6;
+                c7 = n7;
+            }
+            cnt += test4(c0, c1, c2, c3) + test4(c4, c5, c6, c7);
+            i += 8;
         }
         for (; i < n; i++) cnt += fgeom::inlier<kMode>(F, P[i], t2) ? 1u : 0u;
     }
