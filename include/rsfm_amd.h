@@ -370,8 +370,10 @@ typedef struct rsfm_pipeline_config {
     uint64_t seed;             /* pair p samples with seed + p */
     float minSepSq0, minSepSq1;  /* sample sanity gate, 0 = off */
     uint32_t segCols;          /* matcher tile width (multiple of 64, <= 8192); 0 = autotune */
-    uint32_t wgPerXcd;         /* persistent matcher workgroups per XCD; 0 = autotune */
+    uint32_t wgPerXcd;         /* persistent matcher workgroups per XCD, RSFM_PIPELINE_WG_PER_TILE
+                                  (one workgroup per tile, no persistent loop); 0 = autotune */
 } rsfm_pipeline_config;
+#define RSFM_PIPELINE_WG_PER_TILE 0xFFFFFFFFu
 typedef struct rsfm_pipeline_tuning_entry {
     uint32_t segCols, wgPerXcd;
     float matchMs;             /* fill + match + merge, best of three */

@@ -428,6 +428,22 @@ __global__ __launch_bounds__(512) void match_tiles_kernel(const DevTask* __restr
     }
 }
 
+// The same tile table, one workgroup per tile: workgroup b takes element b / 8
+// of XCD list b % 8 (the XCD the hardware dispatches it to). The body is
+// match_tile alone, without the persistent loop's extra live registers.
+template <int kBits, bool kTop2>
+__global__ __launch_bounds__(512) void match_tile_grid_kernel(const DevTask* __restrict__ tasks,
+                                                              const TileDesc* __restrict__ tiles,
+                                                              const uint32_t* __restrict__ xcdOff)
+{
+    const uint32_t xcd = blockIdx.x % kNumXcd;
+    const uint32_t i = xcdOff[xcd] + blockIdx.x / kNumXcd;
+    if (i >= xcdOff[xcd + 1]) return;
+    const TileDesc d = tiles[i];
+    const DevTask T = tasks[d.task];
+    match_tile<kBits, kTop2>(T, d.rowBase, d.colBeg, d.seg);
+}
+
 template <class Src>
 __global__ void init_kernel(const Src batch, bool initA)
 {
@@ -553,6 +569,14 @@ int launch_match_tiles_top2(const DevTask* tasks, const TileDesc* tiles, const u
     RSFM_RETURN_IF(grid == 0 || grid % kNumXcd != 0, RSFM_ERR_INVALID, "match tiles: grid must be a multiple of 8");
     match_tiles_kernel<8, true><<<grid, 512, 0, stream>>>(tasks, tiles, xcdOff);
     return check_launch("bf match tiles");
+}
+
+int launch_match_tile_grid_top2(const DevTask* tasks, const TileDesc* tiles, const uint32_t* xcdOff, uint32_t maxList,
+                                hipStream_t stream)
+{
+    if (maxList == 0) return RSFM_OK;
+    match_tile_grid_kernel<8, true><<<maxList * kNumXcd, 512, 0, stream>>>(tasks, tiles, xcdOff);
+    return check_launch("bf match tile grid");
 }
 
 int launch_merge_top2_table(const DevTask* tasks, uint32_t nTasks, uint32_t maxA, hipStream_t stream)

@@ -46,6 +46,9 @@ struct TileDesc {
 // persistent top-2 matcher over a tile table split into kNumXcd lists
 int launch_match_tiles_top2(const DevTask* tasks, const TileDesc* tiles, const uint32_t* xcdOff, uint32_t grid,
                             hipStream_t stream);
+// one workgroup per tile of the same table; maxList = longest XCD list
+int launch_match_tile_grid_top2(const DevTask* tasks, const TileDesc* tiles, const uint32_t* xcdOff, uint32_t maxList,
+                                hipStream_t stream);
 // merge of the segment partials of the listed tasks (all with nSeg > 1)
 int launch_merge_top2_table(const DevTask* tasks, uint32_t nTasks, uint32_t maxA, hipStream_t stream);
 }  // namespace bf

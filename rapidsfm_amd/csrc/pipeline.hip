@@ -67,7 +67,7 @@ struct rsfm_pipeline {
     uint64_t* dPart = nullptr;
     size_t partCap = 0;
     // chosen tiling
-    uint32_t segCols = 0, wgPerXcd = 0, nTiles = 0;
+    uint32_t segCols = 0, wgPerXcd = 0, nTiles = 0, maxList = 0;
     float tunedMs = 0.f;
     std::vector<rsfm_pipeline_tuning_entry> tuning;
     hipGraph_t graph = nullptr;
@@ -157,7 +157,12 @@ int enqueue_match(rsfm_pipeline* pl, hipStream_t s)
                                                                                                          kNoMatch);
         RSFM_CHECK(check_launch("pipeline fill"));
     }
-    if (pl->nTiles) RSFM_CHECK(bf::launch_match_tiles_top2(pl->dTasks, pl->dTiles, pl->dXcdOff, pl->wgPerXcd * kNumXcd, s));
+    if (pl->nTiles) {
+        if (pl->wgPerXcd == RSFM_PIPELINE_WG_PER_TILE)
+            RSFM_CHECK(bf::launch_match_tile_grid_top2(pl->dTasks, pl->dTiles, pl->dXcdOff, pl->maxList, s));
+        else
+            RSFM_CHECK(bf::launch_match_tiles_top2(pl->dTasks, pl->dTiles, pl->dXcdOff, pl->wgPerXcd * kNumXcd, s));
+    }
     RSFM_CHECK(bf::launch_merge_top2_table(pl->dMerge, pl->nMerge, pl->maxAMerge, s));
     return RSFM_OK;
 }
@@ -183,6 +188,8 @@ int apply_plan(rsfm_pipeline* pl, uint32_t segCols)
     pl->maxAMerge = maxA;
     pl->segCols = segCols;
     pl->nTiles = (uint32_t)p.tiles.size();
+    pl->maxList = 0;
+    for (uint32_t x = 0; x < kNumXcd; x++) pl->maxList = std::max(pl->maxList, p.xcdOff[x + 1] - p.xcdOff[x]);
     return RSFM_OK;
 }
 
@@ -196,7 +203,7 @@ int tune(rsfm_pipeline* pl)
     if (pl->cfg.wgPerXcd)
         wgs.push_back(pl->cfg.wgPerXcd);
     else
-        wgs = {32, 64};
+        wgs = {32, 64, RSFM_PIPELINE_WG_PER_TILE};
     const bool timed = segs.size() * wgs.size() > 1;
     hipEvent_t e0, e1;
     if (timed) {
@@ -282,7 +289,8 @@ extern "C" int rsfm_pipeline_create(const rsfm_pipeline_pair* pairs, size_t nbPa
     RSFM_RETURN_IF(cfg->ratioNum == 0 || cfg->ratioDen == 0, RSFM_ERR_INVALID, "pipeline: ratio must be > 0");
     RSFM_RETURN_IF(cfg->segCols % bf::kBN != 0 || cfg->segCols > bf::kMaxSegCols, RSFM_ERR_INVALID,
                    "pipeline: segCols must be 0 or a multiple of 64 up to 8192");
-    RSFM_RETURN_IF(cfg->wgPerXcd > 1024, RSFM_ERR_INVALID, "pipeline: wgPerXcd must be <= 1024");
+    RSFM_RETURN_IF(cfg->wgPerXcd > 1024 && cfg->wgPerXcd != RSFM_PIPELINE_WG_PER_TILE, RSFM_ERR_INVALID,
+                   "pipeline: wgPerXcd must be <= 1024 or RSFM_PIPELINE_WG_PER_TILE");
     for (size_t i = 0; i < nbPairs; i++) {
         const auto& x = pairs[i];
         RSFM_RETURN_IF(x.nA > 0 && (x.descA == nullptr || x.bestA == nullptr || x.secondA == nullptr || x.kpA == nullptr ||

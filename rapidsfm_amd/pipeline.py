@@ -10,6 +10,10 @@ from ._abi import check, lib
 c_u32, c_f32, c_vp = ctypes.c_uint32, ctypes.c_float, ctypes.c_void_p
 
 
+# wg_per_xcd value: one matcher workgroup per tile instead of a persistent grid
+WG_PER_TILE = 0xFFFFFFFF
+
+
 class PipelinePair(ctypes.Structure):
     _fields_ = [("nA", c_u32), ("nB", c_u32), ("descA", c_vp), ("descB", c_vp), ("kpA", c_vp), ("kpB", c_vp),
                 ("bestA", c_vp), ("secondA", c_vp), ("bestB", c_vp), ("matches", c_vp), ("count", c_vp),
