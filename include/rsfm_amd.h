@@ -270,6 +270,18 @@ typedef struct rsfm_ransac_task {
 int rsfm_ransac_f8(const rsfm_ransac_task* tasks, size_t nbTasks, float threshold, int mode, float minSepSq0,
                    float minSepSq1, hipStream_t stream);
 
+/* 4-point homography hypotheses: replaces the per-hypothesis body of
+ * ransac<H,4> in findHomographyRansacWithInsufficientTiePts<0>
+ * (ImagePair.cpp:349-378): sanity gate (squared separations, as for
+ * rsfm_ransac_f8), FP32 4-point DLT fit with Hartley normalisation
+ * (findHomography, legacy/geometry.hpp:139-209; null vector by Householder QR)
+ * and the inlier count of checkProjectiveTransformation
+ * (legacy/geometry.cpp:41-53) with threshold t pixels. sampleIdx is
+ * [nbHyp][4]; models are row-major H (x1 ~ H x0), unit Frobenius; best and
+ * inlierMask as for rsfm_ransac_f8. */
+int rsfm_ransac_h4(const rsfm_ransac_task* tasks, size_t nbTasks, float threshold, float minSepSq0, float minSepSq1,
+                   hipStream_t stream);
+
 /* 7-point hypotheses (the sampling model of findEpipolarityDegensac,
  * degensac.cpp:113-150, and findEpipolarityWith7Points,
  * legacy/geometry.cpp:221-307): sampleIdx is [nbHyp][7]; each sample gives up

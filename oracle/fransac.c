@@ -70,6 +70,43 @@ static void jacobi3_f32(float S[3][3], float V[3][3])
         }
 }
 
+/* null vector of the 8 x 9 system whose transpose is M (9 x 8): Householder
+ * QR of M, f = Q e9; the GPU's fgeom::null_vector_9x8 step for step */
+static void null_vector_9x8_f32(float M[9][8], float f[9])
+{
+    /* Householder QR of M; reflector k stored in vk[k][k..8], beta[k] */
+    float vk[8][9];
+    float beta[8];
+    for (int k = 0; k < 8; k++) {
+        float nrm2 = 0.f;
+        for (int r = k; r < 9; r++) nrm2 = nrm2 + M[r][k] * M[r][k];
+        const float nrm = sqrtf(nrm2);
+        for (int r = 0; r < 9; r++) vk[k][r] = 0.f;
+        if (nrm == 0.f) { beta[k] = 0.f; continue; }
+        const float alpha = (M[k][k] > 0.f) ? -nrm : nrm;
+        vk[k][k] = M[k][k] - alpha;
+        for (int r = k + 1; r < 9; r++) vk[k][r] = M[r][k];
+        float vtv = 0.f;
+        for (int r = k; r < 9; r++) vtv = vtv + vk[k][r] * vk[k][r];
+        if (vtv == 0.f) { beta[k] = 0.f; continue; }
+        beta[k] = 2.f / vtv;
+        for (int j = k + 1; j < 8; j++) {
+            float w = 0.f;
+            for (int r = k; r < 9; r++) w = w + vk[k][r] * M[r][j];
+            const float bw = beta[k] * w;
+            for (int r = k; r < 9; r++) M[r][j] = M[r][j] - bw * vk[k][r];
+        }
+    }
+    for (int r = 0; r < 9; r++) f[r] = r == 8 ? 1.f : 0.f;
+    for (int k = 7; k >= 0; k--) {
+        if (beta[k] == 0.f) continue;
+        float w = 0.f;
+        for (int r = k; r < 9; r++) w = w + vk[k][r] * f[r];
+        const float bw = beta[k] * w;
+        for (int r = k; r < 9; r++) f[r] = f[r] - bw * vk[k][r];
+    }
+}
+
 int or_f8_fit_f32(const float* pts8, float F[9])
 {
     /* Hartley normalisation */
@@ -98,37 +135,8 @@ int or_f8_fit_f32(const float* pts8, float F[9])
         M[3][i] = x1 * y0; M[4][i] = y1 * y0; M[5][i] = y0;
         M[6][i] = x1;      M[7][i] = y1;      M[8][i] = 1.f;
     }
-    /* Householder QR of M; reflector k stored in vk[k][k..8], beta[k] */
-    float vk[8][9];
-    float beta[8];
-    for (int k = 0; k < 8; k++) {
-        float nrm2 = 0.f;
-        for (int r = k; r < 9; r++) nrm2 = nrm2 + M[r][k] * M[r][k];
-        const float nrm = sqrtf(nrm2);
-        for (int r = 0; r < 9; r++) vk[k][r] = 0.f;
-        if (nrm == 0.f) { beta[k] = 0.f; continue; }
-        const float alpha = (M[k][k] > 0.f) ? -nrm : nrm;
-        vk[k][k] = M[k][k] - alpha;
-        for (int r = k + 1; r < 9; r++) vk[k][r] = M[r][k];
-        float vtv = 0.f;
-        for (int r = k; r < 9; r++) vtv = vtv + vk[k][r] * vk[k][r];
-        if (vtv == 0.f) { beta[k] = 0.f; continue; }
-        beta[k] = 2.f / vtv;
-        for (int j = k + 1; j < 8; j++) {
-            float w = 0.f;
-            for (int r = k; r < 9; r++) w = w + vk[k][r] * M[r][j];
-            const float bw = beta[k] * w;
-            for (int r = k; r < 9; r++) M[r][j] = M[r][j] - bw * vk[k][r];
-        }
-    }
-    float f[9] = {0, 0, 0, 0, 0, 0, 0, 0, 1.f};
-    for (int k = 7; k >= 0; k--) {
-        if (beta[k] == 0.f) continue;
-        float w = 0.f;
-        for (int r = k; r < 9; r++) w = w + vk[k][r] * f[r];
-        const float bw = beta[k] * w;
-        for (int r = k; r < 9; r++) f[r] = f[r] - bw * vk[k][r];
-    }
+    float f[9];
+    null_vector_9x8_f32(M, f);
     /* F^(r,c) = f[3c + r] (column-major) */
     float Fh[3][3];
     for (int r = 0; r < 3; r++)
@@ -452,4 +460,102 @@ uint32_t or_ransac_loop_hash(uint32_t n, float confidence, uint32_t max_iter, ui
 {
     hash_ctx c = {modulus};
     return ransac_loop(n, confidence, max_iter, seed, hash_score_fn, &c, best_idx, draws_out);
+}
+
+
+/* ---------------- 4-point homography hypotheses ---------------- */
+
+/* findHomography (legacy/geometry.hpp:139-209) on 4 correspondences, the GPU's
+ * fgeom::fit_h4 step for step: Hartley normalisation as in or_f8_fit_f32, DLT
+ * rows [-x,-y,-1,0,0,0,x x',y x',x'] / [0,0,0,-x,-y,-1,x y',y y',y'], the same
+ * Householder null vector, H = T1^-1 H^ T0 (row-major), unit Frobenius norm. */
+int or_h4_fit_f32(const float* pts4, float H[9])
+{
+    float c[4] = {0.f, 0.f, 0.f, 0.f};
+    for (int i = 0; i < 4; i++)
+        for (int k = 0; k < 4; k++) c[k] = c[k] + pts4[i * 4 + k];
+    for (int k = 0; k < 4; k++) c[k] = c[k] / 4.f;
+    float n0 = 0.f, n1 = 0.f;
+    float d[4][4];
+    for (int i = 0; i < 4; i++) {
+        for (int k = 0; k < 4; k++) d[i][k] = pts4[i * 4 + k] - c[k];
+        n0 = n0 + sqrtf(d[i][0] * d[i][0] + d[i][1] * d[i][1]);
+        n1 = n1 + sqrtf(d[i][2] * d[i][2] + d[i][3] * d[i][3]);
+    }
+    n0 = n0 / 4.f;
+    n1 = n1 / 4.f;
+    if (!(n0 > 0.f) || !(n1 > 0.f)) return 0;
+    const float sqrt2 = 1.41421354f;
+    const float s0 = sqrt2 / n0, s1 = sqrt2 / n1;
+    float M[9][8];
+    for (int i = 0; i < 4; i++) {
+        const float x = d[i][0] * s0, y = d[i][1] * s0;
+        const float xp = d[i][2] * s1, yp = d[i][3] * s1;
+        const int a = 2 * i, b = 2 * i + 1;
+        M[0][a] = -x;  M[1][a] = -y;  M[2][a] = -1.f;
+        M[3][a] = 0.f; M[4][a] = 0.f; M[5][a] = 0.f;
+        M[6][a] = x * xp; M[7][a] = y * xp; M[8][a] = xp;
+        M[0][b] = 0.f; M[1][b] = 0.f; M[2][b] = 0.f;
+        M[3][b] = -x;  M[4][b] = -y;  M[5][b] = -1.f;
+        M[6][b] = x * yp; M[7][b] = y * yp; M[8][b] = yp;
+    }
+    float h[9];
+    null_vector_9x8_f32(M, h);
+    const float a0 = -(s0 * c[0]), b0 = -(s0 * c[1]);
+    float G[3][3], R[3][3];
+    for (int r = 0; r < 3; r++) {
+        G[r][0] = h[3 * r + 0] * s0;
+        G[r][1] = h[3 * r + 1] * s0;
+        G[r][2] = (h[3 * r + 0] * a0 + h[3 * r + 1] * b0) + h[3 * r + 2];
+    }
+    const float inv1 = 1.f / s1;
+    for (int cc = 0; cc < 3; cc++) {
+        R[0][cc] = G[0][cc] * inv1 + c[2] * G[2][cc];
+        R[1][cc] = G[1][cc] * inv1 + c[3] * G[2][cc];
+        R[2][cc] = G[2][cc];
+    }
+    float fro = 0.f;
+    for (int r = 0; r < 3; r++)
+        for (int cc = 0; cc < 3; cc++) fro = fro + R[r][cc] * R[r][cc];
+    fro = sqrtf(fro);
+    if (!(fro > 0.f) || !isfinite(fro)) return 0;
+    const float inv = 1.f / fro;
+    for (int r = 0; r < 3; r++)
+        for (int cc = 0; cc < 3; cc++) H[r * 3 + cc] = R[r][cc] * inv;
+    return 1;
+}
+
+/* findHomographyRansacWithInsufficientTiePts<0>'s per-hypothesis body
+ * (ImagePair.cpp:365-378) over given [H][4] samples: sanity gate, 4-point fit,
+ * checkProjectiveTransformation count */
+void or_h4_score(const float* pts, uint32_t n, const uint32_t* idx, uint32_t H, float t, float min_sep_sq0,
+                 float min_sep_sq1, uint32_t* counts, float* models, int nthreads)
+{
+    (void)nthreads;
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads > 0 ? nthreads : 1)
+    for (int64_t h = 0; h < (int64_t)H; h++) {
+        const uint32_t* id = idx + (size_t)h * 4;
+        int ok = n >= 4;
+        for (int i = 0; i < 4; i++) ok &= id[i] < n;
+        if (ok && (min_sep_sq0 > 0.f || min_sep_sq1 > 0.f))
+            for (int i = 0; i < 4 && ok; i++)
+                for (int j = 0; j < i; j++) {
+                    const float* a = pts + (size_t)id[i] * 4;
+                    const float* b = pts + (size_t)id[j] * 4;
+                    const float dx0 = a[0] - b[0], dy0 = a[1] - b[1], dx1 = a[2] - b[2], dy1 = a[3] - b[3];
+                    if (dx0 * dx0 + dy0 * dy0 < min_sep_sq0 || dx1 * dx1 + dy1 * dy1 < min_sep_sq1) ok = 0;
+                }
+        float Hm[9] = {0};
+        if (ok) {
+            float p4[16];
+            for (int i = 0; i < 4; i++) memcpy(p4 + 4 * i, pts + (size_t)id[i] * 4, 16);
+            ok = or_h4_fit_f32(p4, Hm);
+        }
+        uint32_t cnt = 0;
+        if (ok)
+            for (uint32_t k = 0; k < n; k++) cnt += (uint32_t)or_h_inlier_f32(Hm, pts + (size_t)k * 4, t);
+        counts[h] = ok ? cnt : 0u;
+        if (models)
+            for (int q = 0; q < 9; q++) models[(size_t)h * 9 + q] = ok ? Hm[q] : 0.f;
+    }
 }

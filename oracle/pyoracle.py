@@ -87,6 +87,8 @@ _decl("or_sample_weighted", None, [P, U32, U32, U64, P])
 _decl("or_f7_fit_f32", I32, [P, P])
 _decl("or_f7_fit_f64", I32, [P, P])
 _decl("or_f7_score", None, [P, U32, P, U32, F32, I32, F32, F32, P, P, I32])
+_decl("or_h4_fit_f32", I32, [P, P])
+_decl("or_h4_score", None, [P, U32, P, U32, F32, F32, F32, P, P, I32])
 _decl("or_fit_ls_f", I32, [P, P, U32, P])
 _decl("or_fit_ls_h", I32, [P, P, U32, P])
 _decl("or_h_from_f", I32, [P, P, P])
@@ -553,3 +555,20 @@ def degensac_refine(pts, idx, counts, t_f, t_h, mode, n_cand, n_plane, seed):
     _lib.or_degensac_refine(_p(pts), n, _p(idx), _p(counts), idx.shape[0], t_f, t_h, mode, n_cand, n_plane, seed,
                             _p(F), _p(H), _p(res), _p(mf), _p(mh))
     return {"F": F, "H": H, "result": res, "mask_f": mf[:n], "mask_h": mh[:n]}
+
+
+# ---------------- 4-point homography hypotheses ----------------
+def h4_fit_f32(p4):
+    p = _c(p4, np.float32).reshape(4, 4)
+    H = np.zeros(9, np.float32)
+    return H.reshape(3, 3) if _lib.or_h4_fit_f32(_p(p), _p(H)) else None
+
+
+def h4_score(pts, idx, t, min_sep_sq0=0.0, min_sep_sq1=0.0, nthreads=0, with_models=True):
+    pts = _c(pts, np.float32).reshape(-1, 4)
+    idx = _c(idx, np.uint32).reshape(-1, 4)
+    H = idx.shape[0]
+    counts = np.zeros(H, np.uint32)
+    models = np.zeros((H, 9), np.float32) if with_models else None
+    _lib.or_h4_score(_p(pts), pts.shape[0], _p(idx), H, t, min_sep_sq0, min_sep_sq1, _p(counts), _p(models), nthreads)
+    return counts, models

@@ -174,6 +174,11 @@ int  or_f7_fit_f64(const double* pts7, double* Fs);
 void or_f7_score(const float* pts, uint32_t n, const uint32_t* idx, uint32_t H, float t, int mode, float min_sep_sq0,
                  float min_sep_sq1, uint32_t* counts, float* models, int nthreads);
 
+/* ---------------- 4-point homography hypotheses (fransac.c) ---------------- */
+int  or_h4_fit_f32(const float* pts4 /*4 x 4*/, float H[9]);
+void or_h4_score(const float* pts, uint32_t n, const uint32_t* idx, uint32_t H, float t, float min_sep_sq0,
+                 float min_sep_sq1, uint32_t* counts, float* models, int nthreads);
+
 /* ---------------- DEGENSAC finisher (degensac.c) ---------------- */
 /* least-squares F (m >= 8) / H (m >= 4) on pts[sel[0..m)], unit Frobenius */
 int  or_fit_ls_f(const float* pts, const uint32_t* sel, uint32_t m, float F[9]);

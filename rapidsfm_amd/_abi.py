@@ -141,6 +141,7 @@ _SIGS = {
     ),
     "rsfm_ransac_f8": (c_int, [c_void_p, c_size, c_float, c_int, c_float, c_float, c_void_p]),
     "rsfm_ransac_f7": (c_int, [c_void_p, c_size, c_float, c_int, c_float, c_float, c_void_p]),
+    "rsfm_ransac_h4": (c_int, [c_void_p, c_size, c_float, c_float, c_float, c_void_p]),
     "rsfm_degensac_refine": (
         c_int, [c_void_p, c_size, c_float, c_float, c_int, c_u32, c_u32, ctypes.c_uint64, c_void_p]
     ),

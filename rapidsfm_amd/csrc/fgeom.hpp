@@ -64,6 +64,57 @@ __device__ inline void jacobi3(float S[3][3], float V[3][3])
     }
 }
 
+// null vector of the 8 x 9 system whose transpose is M (9 x 8): Householder QR
+// of M, f = Q e9 (the last column of Q). M is overwritten.
+__device__ inline void null_vector_9x8(float M[9][8], float f[9])
+{
+    float vkk[8], beta[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        float nrm2 = 0.f;
+#pragma unroll
+        for (int r = k; r < 9; r++) nrm2 = nrm2 + M[r][k] * M[r][k];
+        const float nrm = sqrtf(nrm2);
+        vkk[k] = 0.f;
+        beta[k] = 0.f;
+        if (nrm == 0.f) {
+#pragma unroll
+            for (int r = k + 1; r < 9; r++) M[r][k] = 0.f;
+            continue;
+        }
+        const float alpha = (M[k][k] > 0.f) ? -nrm : nrm;
+        vkk[k] = M[k][k] - alpha;
+        float vtv = vkk[k] * vkk[k];
+#pragma unroll
+        for (int r = k + 1; r < 9; r++) vtv = vtv + M[r][k] * M[r][k];
+        if (vtv == 0.f) continue;
+        beta[k] = 2.f / vtv;
+#pragma unroll
+        for (int j = k + 1; j < 8; j++) {
+            float w = vkk[k] * M[k][j];
+#pragma unroll
+            for (int r = k + 1; r < 9; r++) w = w + M[r][k] * M[r][j];
+            const float bw = beta[k] * w;
+            M[k][j] = M[k][j] - bw * vkk[k];
+#pragma unroll
+            for (int r = k + 1; r < 9; r++) M[r][j] = M[r][j] - bw * M[r][k];
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < 9; r++) f[r] = r == 8 ? 1.f : 0.f;
+#pragma unroll
+    for (int k = 7; k >= 0; k--) {
+        if (beta[k] == 0.f) continue;
+        float w = vkk[k] * f[k];
+#pragma unroll
+        for (int r = k + 1; r < 9; r++) w = w + M[r][k] * f[r];
+        const float bw = beta[k] * w;
+        f[k] = f[k] - bw * vkk[k];
+#pragma unroll
+        for (int r = k + 1; r < 9; r++) f[r] = f[r] - bw * M[r][k];
+    }
+}
+
 // pts[i] = (x0, y0, x1, y1) of sample i; F row-major; returns false when degenerate
 __device__ inline bool fit8(const float4 pts[8], float F[9])
 {
@@ -106,50 +157,8 @@ __device__ inline bool fit8(const float4 pts[8], float F[9])
         M[7][i] = y1;
         M[8][i] = 1.f;
     }
-    float vkk[8], beta[8];
-#pragma unroll
-    for (int k = 0; k < 8; k++) {
-        float nrm2 = 0.f;
-#pragma unroll
-        for (int r = k; r < 9; r++) nrm2 = nrm2 + M[r][k] * M[r][k];
-        const float nrm = sqrtf(nrm2);
-        vkk[k] = 0.f;
-        beta[k] = 0.f;
-        if (nrm == 0.f) {
-#pragma unroll
-            for (int r = k + 1; r < 9; r++) M[r][k] = 0.f;
-            continue;
-        }
-        const float alpha = (M[k][k] > 0.f) ? -nrm : nrm;
-        vkk[k] = M[k][k] - alpha;
-        float vtv = vkk[k] * vkk[k];
-#pragma unroll
-        for (int r = k + 1; r < 9; r++) vtv = vtv + M[r][k] * M[r][k];
-        if (vtv == 0.f) continue;
-        beta[k] = 2.f / vtv;
-#pragma unroll
-        for (int j = k + 1; j < 8; j++) {
-            float w = vkk[k] * M[k][j];
-#pragma unroll
-            for (int r = k + 1; r < 9; r++) w = w + M[r][k] * M[r][j];
-            const float bw = beta[k] * w;
-            M[k][j] = M[k][j] - bw * vkk[k];
-#pragma unroll
-            for (int r = k + 1; r < 9; r++) M[r][j] = M[r][j] - bw * M[r][k];
-        }
-    }
-    float f[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 1.f};
-#pragma unroll
-    for (int k = 7; k >= 0; k--) {
-        if (beta[k] == 0.f) continue;
-        float w = vkk[k] * f[k];
-#pragma unroll
-        for (int r = k + 1; r < 9; r++) w = w + M[r][k] * f[r];
-        const float bw = beta[k] * w;
-        f[k] = f[k] - bw * vkk[k];
-#pragma unroll
-        for (int r = k + 1; r < 9; r++) f[r] = f[r] - bw * M[r][k];
-    }
+    float f[9];
+    null_vector_9x8(M, f);
     float Fh[3][3];
 #pragma unroll
     for (int r = 0; r < 3; r++)
@@ -199,6 +208,93 @@ __device__ inline bool fit8(const float4 pts[8], float F[9])
     for (int r = 0; r < 3; r++)
 #pragma unroll
         for (int cc = 0; cc < 3; cc++) F[r * 3 + cc] = R[r][cc] * inv;
+    return true;
+}
+
+// ---- 4-point homography (findHomography, legacy/geometry.hpp:139-209, the
+// hypothesis fit of findHomographyRansacWithInsufficientTiePts<0>,
+// ImagePair.cpp:349-378): Hartley normalisation as in fit8, the DLT rows
+// [-x, -y, -1, 0, 0, 0, x x', y x', x'] and [0, 0, 0, -x, -y, -1, x y', y y', y']
+// of each correspondence, the null vector of the 8 x 9 system (Householder QR
+// of its transpose), row-major H^, H = T1^-1 H^ T0, unit Frobenius norm.
+__device__ inline bool fit_h4(const float4 pts[4], float H[9])
+{
+    float c[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        c[0] = c[0] + pts[i].x;
+        c[1] = c[1] + pts[i].y;
+        c[2] = c[2] + pts[i].z;
+        c[3] = c[3] + pts[i].w;
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) c[k] = c[k] / 4.f;
+    float n0 = 0.f, n1 = 0.f;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const float d0 = pts[i].x - c[0], d1 = pts[i].y - c[1], d2 = pts[i].z - c[2], d3 = pts[i].w - c[3];
+        n0 = n0 + sqrtf(d0 * d0 + d1 * d1);
+        n1 = n1 + sqrtf(d2 * d2 + d3 * d3);
+    }
+    n0 = n0 / 4.f;
+    n1 = n1 / 4.f;
+    if (!(n0 > 0.f) || !(n1 > 0.f)) return false;
+    const float sqrt2 = 1.41421354f;
+    const float s0 = sqrt2 / n0, s1 = sqrt2 / n1;
+    float M[9][8];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const float x = (pts[i].x - c[0]) * s0, y = (pts[i].y - c[1]) * s0;
+        const float xp = (pts[i].z - c[2]) * s1, yp = (pts[i].w - c[3]) * s1;
+        const int a = 2 * i, b = 2 * i + 1;
+        M[0][a] = -x;
+        M[1][a] = -y;
+        M[2][a] = -1.f;
+        M[3][a] = 0.f;
+        M[4][a] = 0.f;
+        M[5][a] = 0.f;
+        M[6][a] = x * xp;
+        M[7][a] = y * xp;
+        M[8][a] = xp;
+        M[0][b] = 0.f;
+        M[1][b] = 0.f;
+        M[2][b] = 0.f;
+        M[3][b] = -x;
+        M[4][b] = -y;
+        M[5][b] = -1.f;
+        M[6][b] = x * yp;
+        M[7][b] = y * yp;
+        M[8][b] = yp;
+    }
+    float h[9];
+    null_vector_9x8(M, h);
+    const float a0 = -(s0 * c[0]), b0 = -(s0 * c[1]);
+    float G[3][3], R[3][3];
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+        G[r][0] = h[3 * r + 0] * s0;
+        G[r][1] = h[3 * r + 1] * s0;
+        G[r][2] = (h[3 * r + 0] * a0 + h[3 * r + 1] * b0) + h[3 * r + 2];
+    }
+    const float inv1 = 1.f / s1;
+#pragma unroll
+    for (int cc = 0; cc < 3; cc++) {
+        R[0][cc] = G[0][cc] * inv1 + c[2] * G[2][cc];
+        R[1][cc] = G[1][cc] * inv1 + c[3] * G[2][cc];
+        R[2][cc] = G[2][cc];
+    }
+    float fro = 0.f;
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int cc = 0; cc < 3; cc++) fro = fro + R[r][cc] * R[r][cc];
+    fro = sqrtf(fro);
+    if (!(fro > 0.f) || !isfinite(fro)) return false;
+    const float invf = 1.f / fro;
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int cc = 0; cc < 3; cc++) H[r * 3 + cc] = R[r][cc] * invf;
     return true;
 }
 

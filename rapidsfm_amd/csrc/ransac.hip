@@ -16,6 +16,7 @@
 // ballot + popcount on the scalar unit.
 #include "internal.hpp"
 #include "fgeom.hpp"
+#include "hgeom.hpp"
 #include <algorithm>
 
 namespace rsfm_amd {
@@ -29,14 +30,28 @@ typedef ByValue<DevTask, kMaxTasks> Batch;
 
 __device__ inline uint32_t task_n(const DevTask& T) { return T.nDev ? min(*T.nDev, T.nHost) : T.nHost; }
 
-// sanity gate of ImagePair.cpp:167-177: any two samples closer than the
-// threshold in either image reject the hypothesis
-__device__ inline bool sane(const float4 p[8], float s0, float s1)
+// Model kinds of the hypothesis kernels: 0 / 1 = fundamental matrix from 8
+// points with the two-sided / Sampson test, 2 = homography from 4 points with
+// checkProjectiveTransformation's test.
+template <int kMode>
+constexpr int sample_size() { return kMode == 2 ? 4 : 8; }
+
+template <int kMode>
+__device__ inline bool model_inlier(const float M[9], float4 p, float t2)
+{
+    if constexpr (kMode == 2) return hgeom::inlier_h(M, p, t2);
+    else return fgeom::inlier<kMode>(M, p, t2);
+}
+
+// sanity gate of ImagePair.cpp:167-177 (and :366-374 for H): any two samples
+// closer than the threshold in either image reject the hypothesis
+template <int kN>
+__device__ inline bool sane(const float4 p[kN], float s0, float s1)
 {
     if (s0 <= 0.f && s1 <= 0.f) return true;
     bool ok = true;
 #pragma unroll
-    for (int i = 0; i < 8; i++)
+    for (int i = 0; i < kN; i++)
 #pragma unroll
         for (int j = 0; j < i; j++) {
             const float dx0 = p[i].x - p[j].x, dy0 = p[i].y - p[j].y;
@@ -47,26 +62,28 @@ __device__ inline bool sane(const float4 p[8], float s0, float s1)
 }
 
 // fit of hypothesis h; false when out of range, insane or degenerate
-__device__ inline bool fit_hypothesis(const DevTask& T, uint32_t n, uint32_t h, float s0, float s1, float F[9])
+template <int kMode>
+__device__ inline bool fit_hypothesis(const DevTask& T, uint32_t n, uint32_t h, float s0, float s1, float M[9])
 {
-    if (h >= T.H || n < 8) return false;
-    float4 p[8];
+    constexpr int kN = sample_size<kMode>();
+    if (h >= T.H || n < (uint32_t)kN) return false;
+    float4 p[kN];
     bool inRange = true;
 #pragma unroll
-    for (int i = 0; i < 8; i++) {
-        const uint32_t k = T.idx[(size_t)h * 8 + i];
+    for (int i = 0; i < kN; i++) {
+        const uint32_t k = T.idx[(size_t)h * kN + i];
         inRange = inRange && k < n;
         p[i] = T.pts[k < n ? k : 0];
     }
-    if (!inRange || !sane(p, s0, s1)) return false;
-    return fgeom::fit8(p, F);
+    if (!inRange || !sane<kN>(p, s0, s1)) return false;
+    if constexpr (kMode == 2) return fgeom::fit_h4(p, M);
+    else return fgeom::fit8(p, M);
 }
 
 // Scoring: one lane per hypothesis, one wavefront per workgroup. Each lane
 // fits its hypothesis in registers and then walks every correspondence in
-// index order. The correspondence address is the same in all lanes, so the
-// points arrive through scalar loads (nothing is stored before the loop) and
-// the loop body is VALU work only: no LDS, no cross-lane reduction.
+// index order; all lanes read the same correspondence, so the loop body is
+// VALU work only: no LDS, no cross-lane reduction.
 constexpr int kScoreThreads = 64;
 
 template <int kMode, class Src>
@@ -78,7 +95,7 @@ __global__ __launch_bounds__(kScoreThreads) void score_kernel(const Src batch, f
     const uint32_t h = wg.x * kScoreThreads + threadIdx.x;
     const uint32_t n = task_n(T);
     float F[9];
-    const bool ok = fit_hypothesis(T, n, h, s0, s1, F);
+    const bool ok = fit_hypothesis<kMode>(T, n, h, s0, s1, F);
     uint32_t cnt = 0;
     if (ok) {
         // The point index is hidden from the uniformity analysis so the loads
@@ -89,8 +106,8 @@ __global__ __launch_bounds__(kScoreThreads) void score_kernel(const Src batch, f
         asm volatile("" : "+v"(base));
         const float4* __restrict__ P = T.pts + base;
         auto test4 = [&](const float4& a, const float4& b, const float4& c, const float4& d) -> uint32_t {
-            return (fgeom::inlier<kMode>(F, a, t2) ? 1u : 0u) + (fgeom::inlier<kMode>(F, b, t2) ? 1u : 0u) +
-                   (fgeom::inlier<kMode>(F, c, t2) ? 1u : 0u) + (fgeom::inlier<kMode>(F, d, t2) ? 1u : 0u);
+            return (model_inlier<kMode>(F, a, t2) ? 1u : 0u) + (model_inlier<kMode>(F, b, t2) ? 1u : 0u) +
+                   (model_inlier<kMode>(F, c, t2) ? 1u : 0u) + (model_inlier<kMode>(F, d, t2) ? 1u : 0u);
         };
         uint32_t i = 0;
         if (n >= 8) {
@@ -111,7 +128,7 @@ __global__ __launch_bounds__(kScoreThreads) void score_kernel(const Src batch, f
             cnt += test4(c0, c1, c2, c3) + test4(c4, c5, c6, c7);
             i += 8;
         }
-        for (; i < n; i++) cnt += fgeom::inlier<kMode>(F, P[i], t2) ? 1u : 0u;
+        for (; i < n; i++) cnt += model_inlier<kMode>(F, P[i], t2) ? 1u : 0u;
     }
     if (h < T.H) {
         T.counts[h] = ok ? cnt : 0u;
@@ -147,7 +164,7 @@ __global__ __launch_bounds__(kThreads) void best_kernel(const Src batch, float t
     const uint32_t bestC = (uint32_t)(bk >> 32);
     if (threadIdx.x == 0) {
         float F[9];
-        const bool ok = T.H > 0 && bestC > 0 && fit_hypothesis(T, n, bestH, s0, s1, F);
+        const bool ok = T.H > 0 && bestC > 0 && fit_hypothesis<kMode>(T, n, bestH, s0, s1, F);
         for (int q = 0; q < 9; q++) sF[q] = ok ? F[q] : 0.f;
         sOk = ok;
         if (T.best) {
@@ -160,7 +177,7 @@ __global__ __launch_bounds__(kThreads) void best_kernel(const Src batch, float t
     float F[9];
     for (int q = 0; q < 9; q++) F[q] = sF[q];
     for (uint32_t i = threadIdx.x; i < T.nHost; i += kThreads)
-        T.mask[i] = (sOk && i < n) ? (uint8_t)fgeom::inlier<kMode>(F, T.pts[i], t2) : 0;
+        T.mask[i] = (sOk && i < n) ? (uint8_t)model_inlier<kMode>(F, T.pts[i], t2) : 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -540,13 +557,15 @@ int launch_score_table(const DevTask* tasks, uint32_t nTasks, uint32_t maxH, flo
 
 using namespace rsfm_amd;
 
-extern "C" int rsfm_ransac_f8(const rsfm_ransac_task* tasks, size_t nbTasks, float threshold, int mode,
-                              float minSepSq0, float minSepSq1, hipStream_t stream)
+namespace rsfm_amd {
+namespace ransac {
+// score + best over batches of at most kMaxTasks tasks; kind = model kind
+// (0 / 1 fundamental matrix, 2 homography)
+static int run_hypotheses(const rsfm_ransac_task* tasks, size_t nbTasks, float threshold, int kind, float minSepSq0,
+                          float minSepSq1, hipStream_t stream, const char* name)
 {
-    using namespace ransac;
-    RSFM_RETURN_IF(nbTasks > 0 && tasks == nullptr, RSFM_ERR_INVALID, "ransac: tasks is null");
-    RSFM_RETURN_IF(mode != 0 && mode != 1, RSFM_ERR_INVALID, "ransac: mode must be 0 (two-sided line) or 1 (Sampson)");
-    RSFM_RETURN_IF(!(threshold > 0.f), RSFM_ERR_INVALID, "ransac: threshold must be > 0");
+    RSFM_RETURN_IF(nbTasks > 0 && tasks == nullptr, RSFM_ERR_INVALID, std::string(name) + ": tasks is null");
+    RSFM_RETURN_IF(!(threshold > 0.f), RSFM_ERR_INVALID, std::string(name) + ": threshold must be > 0");
     const float t2 = threshold * threshold;
     for (size_t b = 0; b < nbTasks; b += kMaxTasks) {
         const size_t n = std::min<size_t>(kMaxTasks, nbTasks - b);
@@ -555,28 +574,46 @@ extern "C" int rsfm_ransac_f8(const rsfm_ransac_task* tasks, size_t nbTasks, flo
         for (size_t i = 0; i < n; i++) {
             const rsfm_ransac_task& t = tasks[b + i];
             RSFM_RETURN_IF(t.nbHyp > 0 && (t.sampleIdx == nullptr || t.counts == nullptr), RSFM_ERR_INVALID,
-                           "ransac: sampleIdx/counts must be set when nbHyp > 0");
-            RSFM_RETURN_IF(t.nbPts > 0 && t.pts == nullptr, RSFM_ERR_INVALID, "ransac: pts is null");
-            RSFM_RETURN_IF(reinterpret_cast<uintptr_t>(t.pts) % 16, RSFM_ERR_INVALID, "ransac: pts must be 16-byte aligned");
+                           std::string(name) + ": sampleIdx/counts must be set when nbHyp > 0");
+            RSFM_RETURN_IF(t.nbPts > 0 && t.pts == nullptr, RSFM_ERR_INVALID, std::string(name) + ": pts is null");
+            RSFM_RETURN_IF(reinterpret_cast<uintptr_t>(t.pts) % 16, RSFM_ERR_INVALID,
+                           std::string(name) + ": pts must be 16-byte aligned");
             batch.t[i] = DevTask{reinterpret_cast<const float4*>(t.pts), t.nbPtsDev, t.nbPts, t.nbHyp, t.sampleIdx,
                                  t.counts, t.models, t.best, t.inlierMask};
             maxH = std::max(maxH, t.nbHyp);
         }
         if (maxH > 0) {
             const dim3 grid(div_up(maxH, kScoreThreads), (uint32_t)n);
-            if (mode == 0) score_kernel<0><<<grid, kScoreThreads, 0, stream>>>(batch, t2, minSepSq0, minSepSq1);
-            else score_kernel<1><<<grid, kScoreThreads, 0, stream>>>(batch, t2, minSepSq0, minSepSq1);
-            RSFM_CHECK(check_launch("ransac score"));
+            if (kind == 0) score_kernel<0><<<grid, kScoreThreads, 0, stream>>>(batch, t2, minSepSq0, minSepSq1);
+            else if (kind == 1) score_kernel<1><<<grid, kScoreThreads, 0, stream>>>(batch, t2, minSepSq0, minSepSq1);
+            else score_kernel<2><<<grid, kScoreThreads, 0, stream>>>(batch, t2, minSepSq0, minSepSq1);
+            RSFM_CHECK(check_launch(name));
         }
         bool anyBest = false;
         for (size_t i = 0; i < n; i++) anyBest |= batch.t[i].best != nullptr || batch.t[i].mask != nullptr;
         if (anyBest) {
-            if (mode == 0) best_kernel<0><<<(uint32_t)n, kThreads, 0, stream>>>(batch, t2, minSepSq0, minSepSq1);
-            else best_kernel<1><<<(uint32_t)n, kThreads, 0, stream>>>(batch, t2, minSepSq0, minSepSq1);
-            RSFM_CHECK(check_launch("ransac best"));
+            if (kind == 0) best_kernel<0><<<(uint32_t)n, kThreads, 0, stream>>>(batch, t2, minSepSq0, minSepSq1);
+            else if (kind == 1) best_kernel<1><<<(uint32_t)n, kThreads, 0, stream>>>(batch, t2, minSepSq0, minSepSq1);
+            else best_kernel<2><<<(uint32_t)n, kThreads, 0, stream>>>(batch, t2, minSepSq0, minSepSq1);
+            RSFM_CHECK(check_launch(name));
         }
     }
     return RSFM_OK;
+}
+}  // namespace ransac
+}  // namespace rsfm_amd
+
+extern "C" int rsfm_ransac_f8(const rsfm_ransac_task* tasks, size_t nbTasks, float threshold, int mode,
+                              float minSepSq0, float minSepSq1, hipStream_t stream)
+{
+    RSFM_RETURN_IF(mode != 0 && mode != 1, RSFM_ERR_INVALID, "ransac: mode must be 0 (two-sided line) or 1 (Sampson)");
+    return ransac::run_hypotheses(tasks, nbTasks, threshold, mode, minSepSq0, minSepSq1, stream, "ransac");
+}
+
+extern "C" int rsfm_ransac_h4(const rsfm_ransac_task* tasks, size_t nbTasks, float threshold, float minSepSq0,
+                              float minSepSq1, hipStream_t stream)
+{
+    return ransac::run_hypotheses(tasks, nbTasks, threshold, 2, minSepSq0, minSepSq1, stream, "ransac_h4");
 }
 
 extern "C" int rsfm_ransac_sample_indices(uint32_t* sampleIdx, uint32_t nbHyp, uint32_t nbPts,

@@ -139,6 +139,15 @@ def degensac_refine(problems, threshold_f, threshold_h, mode=0, nb_candidates=8,
     return outs
 
 
+def ransac_h4(problems, threshold, min_sep_sq=(0.0, 0.0), want_models=False, stream=None):
+    """4-point homography hypotheses: problems of (pts, sampleIdx (H, 4)[, nDev]);
+    the inlier test is checkProjectiveTransformation's with threshold t pixels."""
+    tasks, outs = _ransac_tasks(problems, want_models)
+    check(lib.rsfm_ransac_h4(ctypes.cast(tasks, ctypes.c_void_p), len(problems), float(threshold),
+                             float(min_sep_sq[0]), float(min_sep_sq[1]), _stream(stream)), "rsfm_ransac_h4")
+    return outs
+
+
 def ransac_f8(problems, threshold, mode=1, min_sep_sq=(0.0, 0.0), want_models=False, stream=None):
     """Score 8-point hypotheses for a list of (pts, sampleIdx[, nDev]) problems.
 
@@ -150,7 +159,7 @@ def ransac_f8(problems, threshold, mode=1, min_sep_sq=(0.0, 0.0), want_models=Fa
     return _ransac(lib.rsfm_ransac_f8, "rsfm_ransac_f8", problems, threshold, mode, min_sep_sq, want_models, stream)
 
 
-def _ransac(fn, name, problems, threshold, mode, min_sep_sq, want_models, stream):
+def _ransac_tasks(problems, want_models):
     tasks = (RansacTask * max(1, len(problems)))()
     outs = []
     for i, prob in enumerate(problems):
@@ -169,6 +178,11 @@ def _ransac(fn, name, problems, threshold, mode, min_sep_sq, want_models, stream
         tasks[i] = RansacTask(_ptr(pts), n, _ptr(ndev) if ndev is not None else None, _ptr(idx), H, _ptr(o["counts"]),
                               _ptr(o["models"]) if want_models else None, _ptr(o["best"]), _ptr(o["mask"]))
         outs.append(o)
+    return tasks, outs
+
+
+def _ransac(fn, name, problems, threshold, mode, min_sep_sq, want_models, stream):
+    tasks, outs = _ransac_tasks(problems, want_models)
     check(fn(ctypes.cast(tasks, ctypes.c_void_p), len(problems), float(threshold), int(mode), float(min_sep_sq[0]),
              float(min_sep_sq[1]), _stream(stream)), name)
     return outs
