@@ -346,19 +346,63 @@ public:
         });
     }
 
+    // With crossCheck the filter runs on the device (rsfm_filter_matches_batch):
+    // only the match counts and the kept matches come back to the host, where
+    // the reference copies both full best-match arrays and filters on the CPU
+    // (bruteForceMatcher.cpp:354-369). Without crossCheck, removeMatchConflicts
+    // runs on the host as in the reference.
     std::future<std::vector<std::vector<ValidMatch>>> matchAndFilter(const std::vector<Img>& queries, const Img& train,
                                                                     bool crossCheck, hipStream_t userStream)
     {
-        auto f = std::make_shared<std::future<Result>>(match(queries, train, true, userStream));
-        const uint32_t nT = train.nbDesc;
-        return std::async(std::launch::deferred, [f, crossCheck, nT]() {
-            Result r = f->get();
-            std::vector<std::vector<ValidMatch>> out(r.size());
-            for (size_t i = 0; i < r.size(); i++) {
-                const auto& fwd = r[i].first;
-                out[i] = crossCheck ? crossCheckMatches(fwd.data(), uint32_t(fwd.size()), r[i].second.data(), nT)
-                                    : removeMatchConflicts(fwd.data(), uint32_t(fwd.size()), nT);
+        if (!crossCheck) {
+            auto f = std::make_shared<std::future<Result>>(match(queries, train, true, userStream));
+            const uint32_t nT = train.nbDesc;
+            return std::async(std::launch::deferred, [f, nT]() {
+                Result r = f->get();
+                std::vector<std::vector<ValidMatch>> out(r.size());
+                for (size_t i = 0; i < r.size(); i++)
+                    out[i] = removeMatchConflicts(r[i].first.data(), uint32_t(r[i].first.size()), nT);
+                return out;
+            });
+        }
+        std::lock_guard<std::mutex> lk(mLock);
+        join(userStream, mStream);
+        auto dev = std::make_shared<std::vector<std::pair<DeviceBuffer<BestMatch>, DeviceBuffer<BestMatch>>>>(
+            matchImpl(queries, train));
+        const size_t nq = queries.size();
+        auto kept = std::make_shared<std::vector<DeviceBuffer<ValidMatch>>>();
+        auto counts = std::make_shared<DeviceBuffer<uint32_t>>(std::max<size_t>(nq, 1));
+        std::vector<rsfm_filter_task> tasks(nq);
+        for (size_t i = 0; i < nq; i++) {
+            kept->emplace_back(std::max<uint32_t>(queries[i].nbDesc, 1));
+            tasks[i] = rsfm_filter_task{reinterpret_cast<const rsfm_best_match*>((*dev)[i].first.ptr), nullptr,
+                                        queries[i].nbDesc, reinterpret_cast<const rsfm_best_match*>((*dev)[i].second.ptr),
+                                        train.nbDesc, reinterpret_cast<rsfm_valid_match*>((*kept)[i].ptr),
+                                        counts->ptr + i, nullptr, nullptr, nullptr};
+        }
+        if (nq) rsfmCheck(rsfm_filter_matches_batch(tasks.data(), nq, 4, 5, mStream), "rsfm_filter_matches_batch");
+        auto hostCounts = std::make_shared<std::vector<uint32_t>>(nq);
+        if (nq)
+            hipCheck(hipMemcpyAsync(hostCounts->data(), counts->ptr, nq * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                    mStream),
+                     "D2H counts");
+        hipEvent_t done;
+        hipCheck(hipEventCreateWithFlags(&done, hipEventDisableTiming), "event");
+        hipCheck(hipEventRecord(done, mStream), "record");
+        join(mStream, userStream);
+        const hipStream_t st = mStream;
+        return std::async(std::launch::deferred, [done, dev, kept, counts, hostCounts, st]() {
+            hipCheck(hipEventSynchronize(done), "wait");
+            (void)hipEventDestroy(done);
+            std::vector<std::vector<ValidMatch>> out(hostCounts->size());
+            for (size_t i = 0; i < out.size(); i++) {
+                out[i].resize((*hostCounts)[i]);
+                if (!out[i].empty())
+                    hipCheck(hipMemcpyAsync(out[i].data(), (*kept)[i].ptr, out[i].size() * sizeof(ValidMatch),
+                                            hipMemcpyDeviceToHost, st),
+                             "D2H kept matches");
             }
+            hipCheck(hipStreamSynchronize(st), "sync");
             return out;
         });
     }

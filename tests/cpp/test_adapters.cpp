@@ -133,8 +133,15 @@ int main()
         auto cc = crossCheckMatches(ra.data(), nA, rb.data(), nB);
         bool ok = filt.size() == 1 && filt[0].size() == cc.size();
         for (size_t k = 0; ok && k < cc.size(); k++)
-            ok &= filt[0][k].idxQuery == cc[k].idxQuery && filt[0][k].idxTrain == cc[k].idxTrain;
-        report(ok, "BruteForceMatcher::matchAndFilter");
+            ok &= filt[0][k].idxQuery == cc[k].idxQuery && filt[0][k].idxTrain == cc[k].idxTrain &&
+                  filt[0][k].distance == cc[k].distance;
+        report(ok, "BruteForceMatcher::matchAndFilter (device cross-check)");
+        auto filt2 = m.matchAndFilter({{(const D*)dA, nA, 0}, {(const D*)dA2, 333, 1}}, {(const D*)dB, nB, 2}, false, s).get();
+        auto rc = removeMatchConflicts(ra.data(), nA, nB);
+        bool ok2 = filt2.size() == 2 && filt2[0].size() == rc.size();
+        for (size_t k = 0; ok2 && k < rc.size(); k++)
+            ok2 &= filt2[0][k].idxQuery == rc[k].idxQuery && filt2[0][k].idxTrain == rc[k].idxTrain;
+        report(ok2, "BruteForceMatcher::matchAndFilter (conflicts)");
         hipCheck(hipFree(dA2), "free");
     }
     // 4-bit method: L2 on the high nibbles
