@@ -328,6 +328,36 @@ int rsfm_degensac_refine(const rsfm_degensac_task* tasks, size_t nbTasks, float 
                          int mode, uint32_t nbCandidates, uint32_t nbPlaneSamples, uint64_t seed,
                          hipStream_t stream);
 
+/* ---------------------------------------------------------------------------
+ * Relative pose from F (the step after verification)
+ * ------------------------------------------------------------------------ */
+
+/* PinHoleCamera (Types.h:231-251): pixel = f * normalised + c */
+typedef struct rsfm_camera {
+    float fx, fy, cx, cy;
+} rsfm_camera;
+typedef struct rsfm_pose_task {
+    const float* F;             /* [9] row-major, x1^T F x0 = 0 (device) */
+    rsfm_camera cam0, cam1;
+    const rsfm_pt_pair* pts;    /* nbPts correspondences, 16-byte aligned */
+    uint32_t nbPts;
+    const uint32_t* nbPtsDev;   /* optional device count (capped at nbPts) */
+    const uint8_t* inlierMask;  /* optional [nbPts]: only points with a nonzero mask count */
+    float* poses;               /* [4][12]: R row-major then t, x_cam1 = R x_cam0 + t */
+    uint32_t* result;           /* [10]: points in front per candidate [4], candidates by
+                                   decreasing count [4], number kept, points considered */
+} rsfm_pose_task;
+/* replaces decomposeEpipolarity (legacy/geometry.hpp:268-346) followed by
+ * disambiguateSolutions / isInFrontOfCam (ImagePair.cpp:760-786): E =
+ * K1^T F K0 and its four (R, t) in the reference's order (U W^T V^T, U W V^T)
+ * x (+U.col(2), -U.col(2)), each scored by the number of correspondences whose
+ * triangulated point (legacy/geometry.cpp:146-206) is in front of both
+ * cameras; the kept candidates are the leading ones in count order with at
+ * least round(n * frontRatio) and at least 2 points in front (the reference's
+ * disambiguityFrontRatio, 0.95 by default, Config.h:175). FP64 decomposition,
+ * FP32 triangulation; one workgroup per task. */
+int rsfm_pose_from_f(const rsfm_pose_task* tasks, size_t nbTasks, float frontRatio, hipStream_t stream);
+
 /* deterministic device sampler (no reference counterpart: the reference's
  * engine is random_device-seeded, ransac.hpp:117): hypothesis h draws 8
  * distinct indices from splitmix64 seeded with seed ^ h * 0xD1B54A32D192ED03,

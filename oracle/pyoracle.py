@@ -87,6 +87,9 @@ _decl("or_sample_weighted", None, [P, U32, U32, U64, P])
 _decl("or_f7_fit_f32", I32, [P, P])
 _decl("or_f7_fit_f64", I32, [P, P])
 _decl("or_f7_score", None, [P, U32, P, U32, F32, I32, F32, F32, P, P, I32])
+_decl("or_pose_decompose", I32, [P, P, P, P])
+_decl("or_pose_triangulate", I32, [P, P, P, P, P])
+_decl("or_pose_from_f", None, [P, P, P, P, U32, P, F32, P, P])
 _decl("or_h4_fit_f32", I32, [P, P])
 _decl("or_h4_score", None, [P, U32, P, U32, F32, F32, F32, P, P, I32])
 _decl("or_fit_ls_f", I32, [P, P, U32, P])
@@ -572,3 +575,40 @@ def h4_score(pts, idx, t, min_sep_sq0=0.0, min_sep_sq1=0.0, nthreads=0, with_mod
     models = np.zeros((H, 9), np.float32) if with_models else None
     _lib.or_h4_score(_p(pts), pts.shape[0], _p(idx), H, t, min_sep_sq0, min_sep_sq1, _p(counts), _p(models), nthreads)
     return counts, models
+
+
+# ---------------- relative pose from F ----------------
+def pose_decompose(F, cam0, cam1):
+    """four (R, t) candidates as (4, 12) float32 [R row-major, t], or None"""
+    F = _c(F, np.float32).reshape(9)
+    c0 = _c(cam0, np.float32).reshape(4)
+    c1 = _c(cam1, np.float32).reshape(4)
+    P = np.zeros(48, np.float32)
+    return P.reshape(4, 12) if _lib.or_pose_decompose(_p(F), _p(c0), _p(c1), _p(P)) else None
+
+
+def pose_triangulate(P, p, cam0, cam1):
+    P = _c(P, np.float32).reshape(12)
+    p = _c(p, np.float32).reshape(4)
+    c0 = np.asarray(cam0, np.float64)
+    c1 = np.asarray(cam1, np.float64)
+    inv = []
+    for c in (c0, c1):
+        fx = np.float32(1.0) / np.float32(c[0])
+        fy = np.float32(1.0) / np.float32(c[1])
+        inv.append(np.array([fx, fy, -np.float32(c[2]) * fx, -np.float32(c[3]) * fy], np.float32))
+    X = np.zeros(3, np.float32)
+    front = _lib.or_pose_triangulate(_p(P), _p(p), _p(inv[0]), _p(inv[1]), _p(X))
+    return X, bool(front)
+
+
+def pose_from_f(F, cam0, cam1, pts, mask=None, ratio=0.95):
+    F = _c(F, np.float32).reshape(9)
+    c0 = _c(cam0, np.float32).reshape(4)
+    c1 = _c(cam1, np.float32).reshape(4)
+    pts = _c(pts, np.float32).reshape(-1, 4)
+    m = _c(mask, np.uint8) if mask is not None else None
+    poses = np.zeros(48, np.float32)
+    res = np.zeros(10, np.uint32)
+    _lib.or_pose_from_f(_p(F), _p(c0), _p(c1), _p(pts), pts.shape[0], _p(m), ratio, _p(poses), _p(res))
+    return poses.reshape(4, 12), res

@@ -179,6 +179,13 @@ int  or_h4_fit_f32(const float* pts4 /*4 x 4*/, float H[9]);
 void or_h4_score(const float* pts, uint32_t n, const uint32_t* idx, uint32_t H, float t, float min_sep_sq0,
                  float min_sep_sq1, uint32_t* counts, float* models, int nthreads);
 
+/* ---------------- relative pose from F (pose.c) ---------------- */
+/* cameras as {fx, fy, cx, cy}; poses [4][12] = R row-major, t */
+int  or_pose_decompose(const float F[9], const float c0[4], const float c1[4], float poses[48]);
+int  or_pose_triangulate(const float P[12], const float* p, const float inv0[4], const float inv1[4], float X[3]);
+void or_pose_from_f(const float F[9], const float c0[4], const float c1[4], const float* pts, uint32_t n,
+                    const uint8_t* mask, float ratio, float poses[48], uint32_t result[10]);
+
 /* ---------------- DEGENSAC finisher (degensac.c) ---------------- */
 /* least-squares F (m >= 8) / H (m >= 4) on pts[sel[0..m)], unit Frobenius */
 int  or_fit_ls_f(const float* pts, const uint32_t* sel, uint32_t m, float F[9]);

@@ -27,6 +27,7 @@ SOURCES = [
     "mlp.hip",
     "pipeline.hip",
     "degensac.hip",
+    "pose.hip",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -1,0 +1,285 @@
+// pose.hip - relative pose from a fundamental matrix: the four (R, t)
+// decompositions and the cheirality count that disambiguates them, one
+// workgroup per image pair.
+//
+// Reference path: decomposeEpipolarity / decomposeEssentialMatrix
+// (legacy/geometry.hpp:268-346: E = K1^T F K0, SVD, R = U W^T V^T or U W V^T,
+// t = +-U.col(2), candidates in the order (R0,t0) (R0,-t0) (R1,t0) (R1,-t0)),
+// isInFrontOfCam (ImagePair.cpp:760-766) over triangulate
+// (legacy/geometry.cpp:146-206) and disambiguateSolutions
+// (ImagePair.cpp:768-786: keep candidates with at least round(n * ratio) and
+// at least 2 points in front of both cameras, most first).
+//
+// Own numerics: E and its SVD in FP64 on one lane (V from a cyclic Jacobi on
+// E^T E, columns ordered by decreasing eigenvalue, u_k = E v_k / |E v_k| for
+// the two largest, u3 = u1 x u2 and v3 = v1 x v2 so det U = det V = +1, which
+// is the sign convention the reference enforces); R and t rounded to FP32;
+// the triangulation and depth tests in FP32 over the inliers, 256 lanes, with
+// the 3x3 solve by the adjugate (A^-1 = adj(A) / det A). The CPU oracle
+// repeats the same operations.
+#include "internal.hpp"
+#include "hgeom.hpp"
+#include <algorithm>
+
+namespace rsfm_amd {
+namespace pose {
+
+constexpr int kThreads = 256;
+constexpr int kMaxTasks = 32;
+
+struct DevTask {
+    const float* F;
+    float cam0[4], cam1[4];  // fx, fy, cx, cy
+    const float4* pts;
+    const uint32_t* nDev;
+    uint32_t nHost;
+    const uint8_t* mask;
+    float* poses;
+    uint32_t* result;
+};
+typedef ByValue<DevTask, kMaxTasks> Batch;
+
+__device__ inline uint32_t task_n(const DevTask& T) { return T.nDev ? min(*T.nDev, T.nHost) : T.nHost; }
+
+// the four candidates, R row-major then t, FP32; false when E is degenerate
+__device__ inline bool decompose(const float Fin[9], const float c0[4], const float c1[4], float out[4][12])
+{
+    double F[3][3];
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) F[r][c] = (double)Fin[r * 3 + c];
+    const double fx0 = c0[0], fy0 = c0[1], cx0 = c0[2], cy0 = c0[3];
+    const double fx1 = c1[0], fy1 = c1[1], cx1 = c1[2], cy1 = c1[3];
+    double G[3][3], E[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        G[a][0] = F[a][0] * fx0;
+        G[a][1] = F[a][1] * fy0;
+        G[a][2] = (F[a][0] * cx0 + F[a][1] * cy0) + F[a][2];
+    }
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        E[0][c] = fx1 * G[0][c];
+        E[1][c] = fy1 * G[1][c];
+        E[2][c] = (cx1 * G[0][c] + cy1 * G[1][c]) + G[2][c];
+    }
+    double S[3][3], V[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) S[i][j] = (E[0][i] * E[0][j] + E[1][i] * E[1][j]) + E[2][i] * E[2][j];
+    hgeom::jacobi3d(S, V);
+    // columns by decreasing eigenvalue (first index on ties)
+    int i0 = 0;
+    if (S[1][1] > S[i0][i0]) i0 = 1;
+    if (S[2][2] > S[i0][i0]) i0 = 2;
+    int i1 = i0 == 0 ? 1 : 0;
+    for (int k = 0; k < 3; k++)
+        if (k != i0 && k != i1 && S[k][k] > S[i1][i1]) i1 = k;
+    const double v1[3] = {V[0][i0], V[1][i0], V[2][i0]};
+    const double v2[3] = {V[0][i1], V[1][i1], V[2][i1]};
+    double u1[3], u2[3], u3[3], v3[3];
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+        u1[r] = (E[r][0] * v1[0] + E[r][1] * v1[1]) + E[r][2] * v1[2];
+        u2[r] = (E[r][0] * v2[0] + E[r][1] * v2[1]) + E[r][2] * v2[2];
+    }
+    const double n1 = sqrt(hgeom::dot3(u1, u1)), n2 = sqrt(hgeom::dot3(u2, u2));
+    if (!(n1 > 0.0) || !(n2 > 0.0) || !isfinite(n1) || !isfinite(n2)) return false;
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+        u1[r] = u1[r] / n1;
+        u2[r] = u2[r] / n2;
+    }
+    hgeom::cross3(u1, u2, u3);
+    hgeom::cross3(v1, v2, v3);
+    // R0 = U W^T V^T = -u2 v1^T + u1 v2^T + u3 v3^T, R1 = U W V^T = u2 v1^T - u1 v2^T + u3 v3^T
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const double a = u1[r] * v2[c] - u2[r] * v1[c];
+            const double b = u3[r] * v3[c];
+            const float R0 = (float)(a + b), R1 = (float)(b - a);
+            out[0][r * 3 + c] = R0;
+            out[1][r * 3 + c] = R0;
+            out[2][r * 3 + c] = R1;
+            out[3][r * 3 + c] = R1;
+        }
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+        const float t = (float)u3[r];
+        out[0][9 + r] = t;
+        out[1][9 + r] = -t;
+        out[2][9 + r] = t;
+        out[3][9 + r] = -t;
+    }
+    return true;
+}
+
+// isInFrontOfCam for one correspondence under pose P (R row-major, t):
+// triangulate (legacy/geometry.cpp:158-203) then depth > 0 in both cameras
+__device__ inline bool in_front(const float P[12], float4 p, const float inv0[4], const float inv1[4])
+{
+    const float r0[3] = {inv0[0] * p.x + inv0[2], inv0[1] * p.y + inv0[3], 1.f};
+    const float q1[3] = {inv1[0] * p.z + inv1[2], inv1[1] * p.w + inv1[3], 1.f};
+    // ray1 = R^T q1, C1 = -R^T t
+    float r1[3], C1[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        r1[k] = (P[0 * 3 + k] * q1[0] + P[1 * 3 + k] * q1[1]) + P[2 * 3 + k] * q1[2];
+        C1[k] = -((P[0 * 3 + k] * P[9] + P[1 * 3 + k] * P[10]) + P[2 * 3 + k] * P[11]);
+    }
+    const float l0 = sqrtf((r0[0] * r0[0] + r0[1] * r0[1]) + r0[2] * r0[2]);
+    const float l1 = sqrtf((r1[0] * r1[0] + r1[1] * r1[1]) + r1[2] * r1[2]);
+    float a[3], b[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        a[k] = r0[k] / l0;
+        b[k] = r1[k] / l1;
+    }
+    float A[3][3], M1[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            const float d = i == j ? 1.f : 0.f;
+            M1[i][j] = b[i] * b[j] - d;
+            A[i][j] = (a[i] * a[j] - d) + M1[i][j];
+        }
+    float rhs[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) rhs[i] = (M1[i][0] * C1[0] + M1[i][1] * C1[1]) + M1[i][2] * C1[2];
+    // adjugate solve
+    float adj[3][3];
+    adj[0][0] = A[1][1] * A[2][2] - A[1][2] * A[2][1];
+    adj[0][1] = A[0][2] * A[2][1] - A[0][1] * A[2][2];
+    adj[0][2] = A[0][1] * A[1][2] - A[0][2] * A[1][1];
+    adj[1][0] = A[1][2] * A[2][0] - A[1][0] * A[2][2];
+    adj[1][1] = A[0][0] * A[2][2] - A[0][2] * A[2][0];
+    adj[1][2] = A[0][2] * A[1][0] - A[0][0] * A[1][2];
+    adj[2][0] = A[1][0] * A[2][1] - A[1][1] * A[2][0];
+    adj[2][1] = A[0][1] * A[2][0] - A[0][0] * A[2][1];
+    adj[2][2] = A[0][0] * A[1][1] - A[0][1] * A[1][0];
+    const float det = (A[0][0] * adj[0][0] + A[0][1] * adj[1][0]) + A[0][2] * adj[2][0];
+    const float inv = 1.f / det;
+    float X[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) X[i] = ((adj[i][0] * rhs[0] + adj[i][1] * rhs[1]) + adj[i][2] * rhs[2]) * inv;
+    const float z1 = ((P[6] * X[0] + P[7] * X[1]) + P[8] * X[2]) + P[11];
+    return X[2] > 0.f && z1 > 0.f;
+}
+
+__global__ __launch_bounds__(kThreads) void pose_kernel(const Batch batch, float ratio)
+{
+    const DevTask& T = batch[blockIdx.x];
+    const uint32_t n = task_n(T);
+    __shared__ float sP[4][12];
+    __shared__ int sOk;
+    __shared__ uint32_t sCnt[4], sIn;
+    if (threadIdx.x == 0) {
+        float F[9];
+        for (int q = 0; q < 9; q++) F[q] = T.F[q];
+        float P[4][12];
+        const bool ok = decompose(F, T.cam0, T.cam1, P);
+        for (int k = 0; k < 4; k++)
+            for (int q = 0; q < 12; q++) sP[k][q] = ok ? P[k][q] : 0.f;
+        sOk = ok;
+        for (int k = 0; k < 4; k++) sCnt[k] = 0;
+        sIn = 0;
+    }
+    __syncthreads();
+    // inverse cameras: f' = 1 / f, c' = -c f' (PinHoleCameraTemplate::inverse, Types.h:236-241)
+    float inv0[4], inv1[4];
+    inv0[0] = 1.f / T.cam0[0];
+    inv0[1] = 1.f / T.cam0[1];
+    inv0[2] = -T.cam0[2] * inv0[0];
+    inv0[3] = -T.cam0[3] * inv0[1];
+    inv1[0] = 1.f / T.cam1[0];
+    inv1[1] = 1.f / T.cam1[1];
+    inv1[2] = -T.cam1[2] * inv1[0];
+    inv1[3] = -T.cam1[3] * inv1[1];
+    uint32_t c[4] = {0u, 0u, 0u, 0u}, in = 0;
+    if (sOk) {
+        float P[4][12];
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+#pragma unroll
+            for (int q = 0; q < 12; q++) P[k][q] = sP[k][q];
+        for (uint32_t i = threadIdx.x; i < n; i += kThreads) {
+            if (T.mask && !T.mask[i]) continue;
+            const float4 p = T.pts[i];
+            in++;
+#pragma unroll
+            for (int k = 0; k < 4; k++) c[k] += in_front(P[k], p, inv0, inv1) ? 1u : 0u;
+        }
+    }
+    for (int k = 0; k < 4; k++)
+        if (c[k]) atomicAdd(&sCnt[k], c[k]);
+    if (in) atomicAdd(&sIn, in);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t order[4] = {0, 1, 2, 3};
+        // most points in front first, lower candidate index on ties
+        for (int a = 1; a < 4; a++)
+            for (int b = a; b > 0 && sCnt[order[b]] > sCnt[order[b - 1]]; b--) {
+                const uint32_t t = order[b];
+                order[b] = order[b - 1];
+                order[b - 1] = t;
+            }
+        const uint32_t minCnt = (uint32_t)roundf((float)sIn * ratio);
+        uint32_t kept = 0;
+        while (kept < 4 && sOk && sCnt[order[kept]] >= minCnt && sCnt[order[kept]] >= 2) kept++;
+        for (int k = 0; k < 4; k++) {
+            T.result[k] = sCnt[k];
+            T.result[4 + k] = order[k];
+        }
+        T.result[8] = kept;
+        T.result[9] = sIn;
+    }
+    if (threadIdx.x < 48) T.poses[threadIdx.x] = sP[threadIdx.x / 12][threadIdx.x % 12];
+}
+
+}  // namespace pose
+}  // namespace rsfm_amd
+
+using namespace rsfm_amd;
+
+extern "C" int rsfm_pose_from_f(const rsfm_pose_task* tasks, size_t nbTasks, float frontRatio, hipStream_t stream)
+{
+    using namespace pose;
+    RSFM_RETURN_IF(nbTasks > 0 && tasks == nullptr, RSFM_ERR_INVALID, "pose: tasks is null");
+    RSFM_RETURN_IF(!(frontRatio >= 0.f) || frontRatio > 1.f, RSFM_ERR_INVALID, "pose: frontRatio must be in [0, 1]");
+    for (size_t b = 0; b < nbTasks; b += kMaxTasks) {
+        const size_t n = std::min<size_t>(kMaxTasks, nbTasks - b);
+        Batch batch{};
+        for (size_t i = 0; i < n; i++) {
+            const rsfm_pose_task& t = tasks[b + i];
+            RSFM_RETURN_IF(t.F == nullptr || t.poses == nullptr || t.result == nullptr, RSFM_ERR_INVALID,
+                           "pose: F, poses and result are required");
+            RSFM_RETURN_IF(t.nbPts > 0 && t.pts == nullptr, RSFM_ERR_INVALID, "pose: pts is null");
+            RSFM_RETURN_IF(reinterpret_cast<uintptr_t>(t.pts) % 16, RSFM_ERR_INVALID, "pose: pts must be 16-byte aligned");
+            RSFM_RETURN_IF(!(t.cam0.fx != 0.f) || !(t.cam0.fy != 0.f) || !(t.cam1.fx != 0.f) || !(t.cam1.fy != 0.f),
+                           RSFM_ERR_INVALID, "pose: focal lengths must be nonzero");
+            DevTask d{};
+            d.F = t.F;
+            const float c0[4] = {t.cam0.fx, t.cam0.fy, t.cam0.cx, t.cam0.cy};
+            const float c1[4] = {t.cam1.fx, t.cam1.fy, t.cam1.cx, t.cam1.cy};
+            for (int k = 0; k < 4; k++) {
+                d.cam0[k] = c0[k];
+                d.cam1[k] = c1[k];
+            }
+            d.pts = reinterpret_cast<const float4*>(t.pts);
+            d.nDev = t.nbPtsDev;
+            d.nHost = t.nbPts;
+            d.mask = t.inlierMask;
+            d.poses = t.poses;
+            d.result = t.result;
+            batch.t[i] = d;
+        }
+        pose_kernel<<<(uint32_t)n, kThreads, 0, stream>>>(batch, frontRatio);
+        RSFM_CHECK(check_launch("pose from F"));
+    }
+    return RSFM_OK;
+}

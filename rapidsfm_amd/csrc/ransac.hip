@@ -23,7 +23,6 @@ namespace rsfm_amd {
 namespace ransac {
 
 constexpr int kThreads = 256;
-constexpr int kTile = 2048;
 constexpr int kMaxTasks = 32;
 
 typedef ByValue<DevTask, kMaxTasks> Batch;

@@ -8,7 +8,7 @@ import ctypes
 
 import torch
 
-from ._abi import BfTask, BfTop2Task, DegensacTask, RansacTask, check, lib
+from ._abi import BfTask, BfTop2Task, Camera, DegensacTask, PoseTask, RansacTask, check, lib
 
 
 def _stream(stream):
@@ -136,6 +136,30 @@ def degensac_refine(problems, threshold_f, threshold_h, mode=0, nb_candidates=8,
     check(lib.rsfm_degensac_refine(ctypes.cast(tasks, ctypes.c_void_p), len(problems), float(threshold_f),
                                    float(threshold_h), int(mode), int(nb_candidates), int(nb_plane_samples),
                                    int(seed) & 0xFFFFFFFFFFFFFFFF, _stream(stream)), "rsfm_degensac_refine")
+    return outs
+
+
+def pose_from_f(problems, front_ratio=0.95, stream=None):
+    """Relative pose from F for a list of (F (3, 3) float32, cam0, cam1, pts (n, 4)[, mask (n,) u8[, nDev]])
+    with cameras (fx, fy, cx, cy). Returns dicts with poses (4, 12) = [R row-major, t] in the reference's
+    candidate order and result (10,) = front counts [4], order by count [4], kept, points considered."""
+    tasks = (PoseTask * max(1, len(problems)))()
+    outs = []
+    for i, prob in enumerate(problems):
+        F, c0, c1, pts = prob[:4]
+        mask = prob[4] if len(prob) > 4 else None
+        ndev = prob[5] if len(prob) > 5 else None
+        if pts.dtype != torch.float32 or pts.dim() != 2 or pts.shape[1] != 4 or not pts.is_contiguous():
+            raise ValueError("pts must be contiguous float32 (n, 4)")
+        Fd = F.to(torch.float32).contiguous()
+        o = {"poses": torch.empty((4, 12), dtype=torch.float32, device=pts.device),
+             "result": torch.empty(10, dtype=torch.int32, device=pts.device), "_F": Fd}
+        tasks[i] = PoseTask(_ptr(Fd), Camera(*[float(v) for v in c0]), Camera(*[float(v) for v in c1]), _ptr(pts),
+                            pts.shape[0], _ptr(ndev) if ndev is not None else None,
+                            _ptr(mask) if mask is not None else None, _ptr(o["poses"]), _ptr(o["result"]))
+        outs.append(o)
+    check(lib.rsfm_pose_from_f(ctypes.cast(tasks, ctypes.c_void_p), len(problems), float(front_ratio), _stream(stream)),
+          "rsfm_pose_from_f")
     return outs
 
 
