@@ -358,6 +358,20 @@ typedef struct rsfm_pose_task {
  * FP32 triangulation; one workgroup per task. */
 int rsfm_pose_from_f(const rsfm_pose_task* tasks, size_t nbTasks, float frontRatio, hipStream_t stream);
 
+/* RealCamera (Types.h:282-296): pinhole plus distortion {k1, k2, p1, p2, k3} */
+typedef struct rsfm_real_camera {
+    rsfm_camera pinHole;
+    float distortion[5];
+} rsfm_real_camera;
+/* replaces undistortInPlace (distortion.cpp:251-272): pixel -> normalised
+ * coordinates, nbIters Gauss-Newton steps of undistort (distortion.cpp:134-210;
+ * the reference uses 2) on the k1, k2, p1, p2, k3 model, back to pixels; px and
+ * py (device, nbPts each) are overwritten. Unlike the reference driver, which
+ * discards undistort's result (distortion.cpp:238), the undistorted points are
+ * written. All-zero distortion leaves the points untouched. */
+int rsfm_undistort_in_place(float* px, float* py, size_t nbPts, const rsfm_real_camera* camera, uint32_t nbIters,
+                            hipStream_t stream);
+
 /* deterministic device sampler (no reference counterpart: the reference's
  * engine is random_device-seeded, ransac.hpp:117): hypothesis h draws 8
  * distinct indices from splitmix64 seeded with seed ^ h * 0xD1B54A32D192ED03,

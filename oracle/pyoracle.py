@@ -90,6 +90,8 @@ _decl("or_f7_score", None, [P, U32, P, U32, F32, I32, F32, F32, P, P, I32])
 _decl("or_pose_decompose", I32, [P, P, P, P])
 _decl("or_pose_triangulate", I32, [P, P, P, P, P])
 _decl("or_pose_from_f", None, [P, P, P, P, U32, P, F32, P, P])
+_decl("or_distort_f32", None, [P, F32, F32, P])
+_decl("or_undistort_in_place", None, [P, P, U32, P, P, U32])
 _decl("or_h4_fit_f32", I32, [P, P])
 _decl("or_h4_score", None, [P, U32, P, U32, F32, F32, F32, P, P, I32])
 _decl("or_fit_ls_f", I32, [P, P, U32, P])
@@ -612,3 +614,21 @@ def pose_from_f(F, cam0, cam1, pts, mask=None, ratio=0.95):
     res = np.zeros(10, np.uint32)
     _lib.or_pose_from_f(_p(F), _p(c0), _p(c1), _p(pts), pts.shape[0], _p(m), ratio, _p(poses), _p(res))
     return poses.reshape(4, 12), res
+
+
+# ---------------- keypoint undistortion ----------------
+def distort_f32(k, x0, x1):
+    k = _c(k, np.float32).reshape(5)
+    out = np.zeros(2, np.float32)
+    _lib.or_distort_f32(_p(k), x0, x1, _p(out))
+    return out
+
+
+def undistort(px, py, cam, k, iters=2):
+    """undistorted copies of the pixel arrays px, py"""
+    x = _c(px, np.float32).copy()
+    y = _c(py, np.float32).copy()
+    c = _c(cam, np.float32).reshape(4)
+    kk = _c(k, np.float32).reshape(5)
+    _lib.or_undistort_in_place(_p(x), _p(y), x.shape[0], _p(c), _p(kk), iters)
+    return x, y

@@ -186,6 +186,11 @@ int  or_pose_triangulate(const float P[12], const float* p, const float inv0[4],
 void or_pose_from_f(const float F[9], const float c0[4], const float c1[4], const float* pts, uint32_t n,
                     const uint8_t* mask, float ratio, float poses[48], uint32_t result[10]);
 
+/* ---------------- keypoint undistortion (undistort.c) ---------------- */
+/* distortion model k = {k1, k2, p1, p2, k3} applied to normalised (x0, x1) */
+void or_distort_f32(const float k[5], float x0, float x1, float out[2]);
+void or_undistort_in_place(float* px, float* py, uint32_t n, const float cam[4], const float k[5], uint32_t iters);
+
 /* ---------------- DEGENSAC finisher (degensac.c) ---------------- */
 /* least-squares F (m >= 8) / H (m >= 4) on pts[sel[0..m)], unit Frobenius */
 int  or_fit_ls_f(const float* pts, const uint32_t* sel, uint32_t m, float F[9]);

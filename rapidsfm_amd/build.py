@@ -28,6 +28,7 @@ SOURCES = [
     "pipeline.hip",
     "degensac.hip",
     "pose.hip",
+    "undistort.hip",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

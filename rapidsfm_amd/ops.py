@@ -8,7 +8,7 @@ import ctypes
 
 import torch
 
-from ._abi import BfTask, BfTop2Task, Camera, DegensacTask, PoseTask, RansacTask, check, lib
+from ._abi import BfTask, BfTop2Task, Camera, DegensacTask, PoseTask, RansacTask, RealCamera, check, lib
 
 
 def _stream(stream):
@@ -137,6 +137,18 @@ def degensac_refine(problems, threshold_f, threshold_h, mode=0, nb_candidates=8,
                                    float(threshold_h), int(mode), int(nb_candidates), int(nb_plane_samples),
                                    int(seed) & 0xFFFFFFFFFFFFFFFF, _stream(stream)), "rsfm_degensac_refine")
     return outs
+
+
+def undistort_in_place(px, py, cam, distortion, n_iters=2, stream=None):
+    """Undistort pixel coordinates in place (px, py: contiguous float32 CUDA tensors);
+    cam = (fx, fy, cx, cy), distortion = (k1, k2, p1, p2, k3)."""
+    if px.dtype != torch.float32 or py.dtype != torch.float32 or px.shape != py.shape:
+        raise ValueError("px, py must be float32 tensors of one shape")
+    if px.numel() and not (px.is_cuda and py.is_cuda and px.is_contiguous() and py.is_contiguous()):
+        raise ValueError("px, py must be contiguous device tensors")
+    rc = RealCamera(Camera(*[float(v) for v in cam]), (ctypes.c_float * 5)(*[float(v) for v in distortion]))
+    check(lib.rsfm_undistort_in_place(_ptr(px), _ptr(py), px.numel(), ctypes.byref(rc), int(n_iters), _stream(stream)),
+          "rsfm_undistort_in_place")
 
 
 def pose_from_f(problems, front_ratio=0.95, stream=None):
