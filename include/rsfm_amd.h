@@ -358,6 +358,31 @@ typedef struct rsfm_pose_task {
  * FP32 triangulation; one workgroup per task. */
 int rsfm_pose_from_f(const rsfm_pose_task* tasks, size_t nbTasks, float frontRatio, hipStream_t stream);
 
+typedef struct rsfm_pose_h_task {
+    const float* H;             /* [9] row-major pixel homography, x1 ~ H x0 (device) */
+    rsfm_camera cam0, cam1;
+    const rsfm_pt_pair* pts;    /* nbPts correspondences, 16-byte aligned */
+    uint32_t nbPts;
+    const uint32_t* nbPtsDev;   /* optional device count (capped at nbPts) */
+    const uint8_t* inlierMask;  /* optional [nbPts]: only points with a nonzero mask count */
+    float* poses;               /* [8][12]: the valid candidates first, R row-major then t (|t| = 1) */
+    float* normals;             /* optional [8][3]: plane normal N of each candidate, camera 0 frame */
+    uint32_t* result;           /* [19]: points in front per candidate [8], valid candidates by
+                                   decreasing count [8], number kept, points considered,
+                                   number of valid candidates */
+} rsfm_pose_h_task;
+/* replaces decomposeHomography (legacy/geometry.hpp:364-447) and the
+ * transform to (R, T/d normalised) in solveByHomography /
+ * solveByDegensacForBoth (ImagePair.cpp:805-816, 846-856) followed by
+ * disambiguateSolutions (ImagePair.cpp:768-786): H normalised by the cameras
+ * and its middle singular value, the four (R, N, T/d) of H and the four of -H,
+ * those with N . T/d < 1 kept in that order, each scored by the points in
+ * front of both cameras, kept = the leading ones with at least
+ * round(n * frontRatio) and at least 2. The reference's minNbInliers gate
+ * before the decomposition stays with the caller. FP64 decomposition, FP32
+ * triangulation; one workgroup per task. */
+int rsfm_pose_from_h(const rsfm_pose_h_task* tasks, size_t nbTasks, float frontRatio, hipStream_t stream);
+
 /* RealCamera (Types.h:282-296): pinhole plus distortion {k1, k2, p1, p2, k3} */
 typedef struct rsfm_real_camera {
     rsfm_camera pinHole;

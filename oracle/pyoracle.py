@@ -90,6 +90,8 @@ _decl("or_f7_score", None, [P, U32, P, U32, F32, I32, F32, F32, P, P, I32])
 _decl("or_pose_decompose", I32, [P, P, P, P])
 _decl("or_pose_triangulate", I32, [P, P, P, P, P])
 _decl("or_pose_from_f", None, [P, P, P, P, U32, P, F32, P, P])
+_decl("or_pose_decompose_h", I32, [P, P, P, P, P])
+_decl("or_pose_from_h", None, [P, P, P, P, U32, P, F32, P, P, P])
 _decl("or_distort_f32", None, [P, F32, F32, P])
 _decl("or_undistort_in_place", None, [P, P, U32, P, P, U32])
 _decl("or_h4_fit_f32", I32, [P, P])
@@ -632,3 +634,30 @@ def undistort(px, py, cam, k, iters=2):
     kk = _c(k, np.float32).reshape(5)
     _lib.or_undistort_in_place(_p(x), _p(y), x.shape[0], _p(c), _p(kk), iters)
     return x, y
+
+
+# ---------------- relative pose from H ----------------
+def pose_decompose_h(H, cam0, cam1):
+    """(poses [n][12], normals [n][3]) of the valid homography candidates"""
+    Hm = _c(H, np.float32).reshape(9)
+    c0 = _c(cam0, np.float32).reshape(4)
+    c1 = _c(cam1, np.float32).reshape(4)
+    P = np.zeros(96, np.float32)
+    N = np.zeros(24, np.float32)
+    n = _lib.or_pose_decompose_h(_p(Hm), _p(c0), _p(c1), _p(P), _p(N))
+    return P.reshape(8, 12)[:n], N.reshape(8, 3)[:n]
+
+
+def pose_from_h(H, cam0, cam1, pts, mask=None, ratio=0.95):
+    """(poses [8][12], normals [8][3], result [19]) as rsfm_pose_from_h writes them"""
+    Hm = _c(H, np.float32).reshape(9)
+    c0 = _c(cam0, np.float32).reshape(4)
+    c1 = _c(cam1, np.float32).reshape(4)
+    pts = _c(pts, np.float32).reshape(-1, 4)
+    m = None if mask is None else _c(mask, np.uint8)
+    poses = np.zeros(96, np.float32)
+    normals = np.zeros(24, np.float32)
+    res = np.zeros(19, np.uint32)
+    _lib.or_pose_from_h(_p(Hm), _p(c0), _p(c1), _p(pts), pts.shape[0], _p(m), ratio, _p(poses), _p(normals),
+                        _p(res))
+    return poses.reshape(8, 12), normals.reshape(8, 3), res

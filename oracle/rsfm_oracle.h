@@ -185,6 +185,10 @@ int  or_pose_decompose(const float F[9], const float c0[4], const float c1[4], f
 int  or_pose_triangulate(const float P[12], const float* p, const float inv0[4], const float inv1[4], float X[3]);
 void or_pose_from_f(const float F[9], const float c0[4], const float c1[4], const float* pts, uint32_t n,
                     const uint8_t* mask, float ratio, float poses[48], uint32_t result[10]);
+/* homography branch: up to 8 valid candidates compacted, poses [8][12], normals [8][3] */
+int  or_pose_decompose_h(const float H[9], const float c0[4], const float c1[4], float poses[96], float normals[24]);
+void or_pose_from_h(const float H[9], const float c0[4], const float c1[4], const float* pts, uint32_t n,
+                    const uint8_t* mask, float ratio, float poses[96], float normals[24], uint32_t result[19]);
 
 /* ---------------- keypoint undistortion (undistort.c) ---------------- */
 /* distortion model k = {k1, k2, p1, p2, k3} applied to normalised (x0, x1) */

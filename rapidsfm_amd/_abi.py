@@ -84,6 +84,21 @@ class Camera(ctypes.Structure):
     _fields_ = [("fx", c_float), ("fy", c_float), ("cx", c_float), ("cy", c_float)]
 
 
+class PoseHTask(ctypes.Structure):
+    _fields_ = [
+        ("H", c_void_p),
+        ("cam0", Camera),
+        ("cam1", Camera),
+        ("pts", c_void_p),
+        ("nbPts", c_u32),
+        ("nbPtsDev", c_void_p),
+        ("inlierMask", c_void_p),
+        ("poses", c_void_p),
+        ("normals", c_void_p),
+        ("result", c_void_p),
+    ]
+
+
 class RealCamera(ctypes.Structure):
     _fields_ = [("pinHole", Camera), ("distortion", c_float * 5)]
 
@@ -165,6 +180,7 @@ _SIGS = {
     "rsfm_ransac_f7": (c_int, [c_void_p, c_size, c_float, c_int, c_float, c_float, c_void_p]),
     "rsfm_ransac_h4": (c_int, [c_void_p, c_size, c_float, c_float, c_float, c_void_p]),
     "rsfm_pose_from_f": (c_int, [c_void_p, c_size, c_float, c_void_p]),
+    "rsfm_pose_from_h": (c_int, [c_void_p, c_size, c_float, c_void_p]),
     "rsfm_undistort_in_place": (c_int, [c_void_p, c_void_p, c_size, c_void_p, c_u32, c_void_p]),
     "rsfm_degensac_refine": (
         c_int, [c_void_p, c_size, c_float, c_float, c_int, c_u32, c_u32, ctypes.c_uint64, c_void_p]

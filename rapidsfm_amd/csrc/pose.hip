@@ -1,5 +1,5 @@
-// pose.hip - relative pose from a fundamental matrix: the four (R, t)
-// decompositions and the cheirality count that disambiguates them, one
+// pose.hip - relative pose from a fundamental matrix or a homography: the
+// (R, t) decompositions and the cheirality count that disambiguates them, one
 // workgroup per image pair.
 //
 // Reference path: decomposeEpipolarity / decomposeEssentialMatrix
@@ -9,11 +9,19 @@
 // (legacy/geometry.cpp:146-206) and disambiguateSolutions
 // (ImagePair.cpp:768-786: keep candidates with at least round(n * ratio) and
 // at least 2 points in front of both cameras, most first).
+// Homography branch: decomposeHomography (legacy/geometry.hpp:348-447, the
+// SVD method of Ma, Soatto, Kosecka and Sastry: H = K1^-1 G K0 scaled by its
+// middle singular value, V of H^T H with det V = +1, u1,2 = a v1 +- b v3,
+// R = W U^T, N = v2 x u, T/d = (H - R) N, the four (R, N, T/d) of H and the
+// four of -H, kept where N . T/d < 1), t = T/d normalised
+// (ImagePair.cpp:810-816), then the same disambiguation.
 //
 // Own numerics: E and its SVD in FP64 on one lane (V from a cyclic Jacobi on
 // E^T E, columns ordered by decreasing eigenvalue, u_k = E v_k / |E v_k| for
 // the two largest, u3 = u1 x u2 and v3 = v1 x v2 so det U = det V = +1, which
-// is the sign convention the reference enforces); R and t rounded to FP32;
+// is the sign convention the reference enforces; for H the eigenvalues of
+// H^T H divided by the middle one stand for the singular values of the
+// normalised matrix); R, t and N rounded to FP32;
 // the triangulation and depth tests in FP32 over the inliers, 256 lanes, with
 // the 3x3 solve by the adjugate (A^-1 = adj(A) / det A). The CPU oracle
 // repeats the same operations.
@@ -28,13 +36,14 @@ constexpr int kThreads = 256;
 constexpr int kMaxTasks = 32;
 
 struct DevTask {
-    const float* F;
+    const float* M;  // F or H
     float cam0[4], cam1[4];  // fx, fy, cx, cy
     const float4* pts;
     const uint32_t* nDev;
     uint32_t nHost;
     const uint8_t* mask;
     float* poses;
+    float* normals;  // H only, optional
     uint32_t* result;
 };
 typedef ByValue<DevTask, kMaxTasks> Batch;
@@ -118,6 +127,116 @@ __device__ inline bool decompose(const float Fin[9], const float c0[4], const fl
     return true;
 }
 
+// the valid homography candidates (N . T/d < 1) in the reference's order,
+// compacted: R row-major then t = T/d / |T/d|, and N; returns their number
+__device__ inline int decompose_h(const float Hin[9], const float c0[4], const float c1[4], float out[8][12],
+                                  float nrm[8][3])
+{
+    double G[3][3], Hn[3][3];
+    const double fx0 = c0[0], fy0 = c0[1], cx0 = c0[2], cy0 = c0[3];
+    const double fx1 = c1[0], fy1 = c1[1], cx1 = c1[2], cy1 = c1[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const double h0 = Hin[a * 3 + 0], h1 = Hin[a * 3 + 1], h2 = Hin[a * 3 + 2];
+        G[a][0] = h0 * fx0;
+        G[a][1] = h1 * fy0;
+        G[a][2] = (h0 * cx0 + h1 * cy0) + h2;
+    }
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        Hn[0][c] = (G[0][c] - cx1 * G[2][c]) / fx1;
+        Hn[1][c] = (G[1][c] - cy1 * G[2][c]) / fy1;
+        Hn[2][c] = G[2][c];
+    }
+    double S[3][3], V[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) S[i][j] = (Hn[0][i] * Hn[0][j] + Hn[1][i] * Hn[1][j]) + Hn[2][i] * Hn[2][j];
+    hgeom::jacobi3d(S, V);
+    int i0 = 0;
+    if (S[1][1] > S[i0][i0]) i0 = 1;
+    if (S[2][2] > S[i0][i0]) i0 = 2;
+    int i1 = i0 == 0 ? 1 : 0;
+    for (int k = 0; k < 3; k++)
+        if (k != i0 && k != i1 && S[k][k] > S[i1][i1]) i1 = k;
+    const int i2 = 3 - i0 - i1;
+    const double lam1 = S[i1][i1];
+    if (!(lam1 > 0.0) || !isfinite(S[i0][i0])) return 0;
+    const double sc = sqrt(lam1);
+    const double s0 = S[i0][i0] / lam1, s2 = S[i2][i2] / lam1;
+    double v1[3] = {V[0][i0], V[1][i0], V[2][i0]};
+    double v2[3] = {V[0][i1], V[1][i1], V[2][i1]};
+    double v3[3] = {V[0][i2], V[1][i2], V[2][i2]};
+    double x[3];
+    hgeom::cross3(v2, v3, x);
+    if (hgeom::dot3(v1, x) < 0.0) {
+#pragma unroll
+        for (int r = 0; r < 3; r++) {
+            v1[r] = -v1[r];
+            v2[r] = -v2[r];
+            v3[r] = -v3[r];
+        }
+    }
+    const double dd = s0 - s2;
+    if (!(dd > 0.0)) return 0;
+    const double cc = 1.0 / sqrt(dd);
+    const double a = sqrt(1.0 - s2) * cc, b = sqrt(s0 - 1.0) * cc;
+    double u[2][3], nv[2][3];
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+        u[0][r] = a * v1[r] + b * v3[r];
+        u[1][r] = a * v1[r] - b * v3[r];
+    }
+    hgeom::cross3(v2, u[0], nv[0]);
+    hgeom::cross3(v2, u[1], nv[1]);
+    int nOut = 0;
+    for (int sgn = 0; sgn < 2; sgn++) {
+        double Hs[3][3];
+#pragma unroll
+        for (int r = 0; r < 3; r++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) Hs[r][c] = sgn ? -(Hn[r][c] / sc) : Hn[r][c] / sc;
+        double w0[3];
+#pragma unroll
+        for (int r = 0; r < 3; r++) w0[r] = (Hs[r][0] * v2[0] + Hs[r][1] * v2[1]) + Hs[r][2] * v2[2];
+        double R[2][3][3], T[2][3];
+#pragma unroll
+        for (int k = 0; k < 2; k++) {
+            double w1[3], w2[3];
+#pragma unroll
+            for (int r = 0; r < 3; r++) w1[r] = (Hs[r][0] * u[k][0] + Hs[r][1] * u[k][1]) + Hs[r][2] * u[k][2];
+            hgeom::cross3(w0, w1, w2);
+#pragma unroll
+            for (int r = 0; r < 3; r++)
+#pragma unroll
+                for (int c = 0; c < 3; c++) R[k][r][c] = (w0[r] * v2[c] + w1[r] * u[k][c]) + w2[r] * nv[k][c];
+#pragma unroll
+            for (int r = 0; r < 3; r++)
+                T[k][r] = ((Hs[r][0] - R[k][r][0]) * nv[k][0] + (Hs[r][1] - R[k][r][1]) * nv[k][1]) +
+                          (Hs[r][2] - R[k][r][2]) * nv[k][2];
+        }
+        // (R1, N1, T1), (R2, N2, T2), (R1, -N1, -T1), (R2, -N2, -T2); -N . -T = N . T
+        for (int q = 0; q < 4; q++) {
+            const int k = q & 1;
+            if (!(hgeom::dot3(nv[k], T[k]) < 1.0)) continue;
+            const double tn = sqrt(hgeom::dot3(T[k], T[k]));
+#pragma unroll
+            for (int r = 0; r < 3; r++)
+#pragma unroll
+                for (int c = 0; c < 3; c++) out[nOut][r * 3 + c] = (float)R[k][r][c];
+#pragma unroll
+            for (int r = 0; r < 3; r++) {
+                const double t = tn > 0.0 ? T[k][r] / tn : T[k][r];
+                out[nOut][9 + r] = (float)(q < 2 ? t : -t);
+                nrm[nOut][r] = (float)(q < 2 ? nv[k][r] : -nv[k][r]);
+            }
+            nOut++;
+        }
+    }
+    return nOut;
+}
+
 // isInFrontOfCam for one correspondence under pose P (R row-major, t):
 // triangulate (legacy/geometry.cpp:158-203) then depth > 0 in both cameras
 __device__ inline bool in_front(const float P[12], float4 p, const float inv0[4], const float inv1[4])
@@ -171,22 +290,33 @@ __device__ inline bool in_front(const float P[12], float4 p, const float inv0[4]
     return X[2] > 0.f && z1 > 0.f;
 }
 
+// kKind 0: F (4 candidates, result [10]); 1: H (up to 8, result [19])
+template <int kKind>
 __global__ __launch_bounds__(kThreads) void pose_kernel(const Batch batch, float ratio)
 {
+    constexpr int kC = kKind ? 8 : 4;
     const DevTask& T = batch[blockIdx.x];
     const uint32_t n = task_n(T);
-    __shared__ float sP[4][12];
-    __shared__ int sOk;
-    __shared__ uint32_t sCnt[4], sIn;
+    __shared__ float sP[kC][12], sN[kC][3];
+    __shared__ int sNC;
+    __shared__ uint32_t sCnt[kC], sIn;
     if (threadIdx.x == 0) {
-        float F[9];
-        for (int q = 0; q < 9; q++) F[q] = T.F[q];
-        float P[4][12];
-        const bool ok = decompose(F, T.cam0, T.cam1, P);
-        for (int k = 0; k < 4; k++)
-            for (int q = 0; q < 12; q++) sP[k][q] = ok ? P[k][q] : 0.f;
-        sOk = ok;
-        for (int k = 0; k < 4; k++) sCnt[k] = 0;
+        float M[9];
+        for (int q = 0; q < 9; q++) M[q] = T.M[q];
+        float P[kC][12], Nv[kC][3];
+        int nc;
+        if constexpr (kKind == 0) {
+            nc = decompose(M, T.cam0, T.cam1, P) ? 4 : 0;
+        } else {
+            nc = decompose_h(M, T.cam0, T.cam1, P, Nv);
+        }
+        for (int k = 0; k < kC; k++) {
+            for (int q = 0; q < 12; q++) sP[k][q] = k < nc ? P[k][q] : 0.f;
+            if constexpr (kKind == 1)
+                for (int q = 0; q < 3; q++) sN[k][q] = k < nc ? Nv[k][q] : 0.f;
+            sCnt[k] = 0;
+        }
+        sNC = nc;
         sIn = 0;
     }
     __syncthreads();
@@ -200,11 +330,14 @@ __global__ __launch_bounds__(kThreads) void pose_kernel(const Batch batch, float
     inv1[1] = 1.f / T.cam1[1];
     inv1[2] = -T.cam1[2] * inv1[0];
     inv1[3] = -T.cam1[3] * inv1[1];
-    uint32_t c[4] = {0u, 0u, 0u, 0u}, in = 0;
-    if (sOk) {
-        float P[4][12];
+    const int nc = sNC;
+    uint32_t c[kC], in = 0;
 #pragma unroll
-        for (int k = 0; k < 4; k++)
+    for (int k = 0; k < kC; k++) c[k] = 0u;
+    if (nc > 0) {
+        float P[kC][12];
+#pragma unroll
+        for (int k = 0; k < kC; k++)
 #pragma unroll
             for (int q = 0; q < 12; q++) P[k][q] = sP[k][q];
         for (uint32_t i = threadIdx.x; i < n; i += kThreads) {
@@ -212,17 +345,19 @@ __global__ __launch_bounds__(kThreads) void pose_kernel(const Batch batch, float
             const float4 p = T.pts[i];
             in++;
 #pragma unroll
-            for (int k = 0; k < 4; k++) c[k] += in_front(P[k], p, inv0, inv1) ? 1u : 0u;
+            for (int k = 0; k < kC; k++)
+                if (k < nc) c[k] += in_front(P[k], p, inv0, inv1) ? 1u : 0u;
         }
     }
-    for (int k = 0; k < 4; k++)
+    for (int k = 0; k < kC; k++)
         if (c[k]) atomicAdd(&sCnt[k], c[k]);
     if (in) atomicAdd(&sIn, in);
     __syncthreads();
     if (threadIdx.x == 0) {
-        uint32_t order[4] = {0, 1, 2, 3};
-        // most points in front first, lower candidate index on ties
-        for (int a = 1; a < 4; a++)
+        uint32_t order[kC];
+        for (int k = 0; k < kC; k++) order[k] = k;
+        // most points in front first, lower candidate index on ties; only the nc valid ones
+        for (int a = 1; a < nc; a++)
             for (int b = a; b > 0 && sCnt[order[b]] > sCnt[order[b - 1]]; b--) {
                 const uint32_t t = order[b];
                 order[b] = order[b - 1];
@@ -230,15 +365,18 @@ __global__ __launch_bounds__(kThreads) void pose_kernel(const Batch batch, float
             }
         const uint32_t minCnt = (uint32_t)roundf((float)sIn * ratio);
         uint32_t kept = 0;
-        while (kept < 4 && sOk && sCnt[order[kept]] >= minCnt && sCnt[order[kept]] >= 2) kept++;
-        for (int k = 0; k < 4; k++) {
+        while ((int)kept < nc && sCnt[order[kept]] >= minCnt && sCnt[order[kept]] >= 2) kept++;
+        for (int k = 0; k < kC; k++) {
             T.result[k] = sCnt[k];
-            T.result[4 + k] = order[k];
+            T.result[kC + k] = order[k];
         }
-        T.result[8] = kept;
-        T.result[9] = sIn;
+        T.result[2 * kC] = kept;
+        T.result[2 * kC + 1] = sIn;
+        if constexpr (kKind == 1) T.result[2 * kC + 2] = (uint32_t)nc;
     }
-    if (threadIdx.x < 48) T.poses[threadIdx.x] = sP[threadIdx.x / 12][threadIdx.x % 12];
+    if (threadIdx.x < kC * 12) T.poses[threadIdx.x] = sP[threadIdx.x / 12][threadIdx.x % 12];
+    if constexpr (kKind == 1)
+        if (T.normals && threadIdx.x < kC * 3) T.normals[threadIdx.x] = sN[threadIdx.x / 3][threadIdx.x % 3];
 }
 
 }  // namespace pose
@@ -246,7 +384,9 @@ __global__ __launch_bounds__(kThreads) void pose_kernel(const Batch batch, float
 
 using namespace rsfm_amd;
 
-extern "C" int rsfm_pose_from_f(const rsfm_pose_task* tasks, size_t nbTasks, float frontRatio, hipStream_t stream)
+namespace {
+template <int kKind, class Task>
+int launch_pose(const Task* tasks, size_t nbTasks, float frontRatio, hipStream_t stream, const char* what)
 {
     using namespace pose;
     RSFM_RETURN_IF(nbTasks > 0 && tasks == nullptr, RSFM_ERR_INVALID, "pose: tasks is null");
@@ -255,15 +395,20 @@ extern "C" int rsfm_pose_from_f(const rsfm_pose_task* tasks, size_t nbTasks, flo
         const size_t n = std::min<size_t>(kMaxTasks, nbTasks - b);
         Batch batch{};
         for (size_t i = 0; i < n; i++) {
-            const rsfm_pose_task& t = tasks[b + i];
-            RSFM_RETURN_IF(t.F == nullptr || t.poses == nullptr || t.result == nullptr, RSFM_ERR_INVALID,
-                           "pose: F, poses and result are required");
+            const Task& t = tasks[b + i];
+            DevTask d{};
+            if constexpr (kKind == 0) {
+                d.M = t.F;
+            } else {
+                d.M = t.H;
+                d.normals = t.normals;
+            }
+            RSFM_RETURN_IF(d.M == nullptr || t.poses == nullptr || t.result == nullptr, RSFM_ERR_INVALID,
+                           "pose: the model, poses and result are required");
             RSFM_RETURN_IF(t.nbPts > 0 && t.pts == nullptr, RSFM_ERR_INVALID, "pose: pts is null");
             RSFM_RETURN_IF(reinterpret_cast<uintptr_t>(t.pts) % 16, RSFM_ERR_INVALID, "pose: pts must be 16-byte aligned");
             RSFM_RETURN_IF(!(t.cam0.fx != 0.f) || !(t.cam0.fy != 0.f) || !(t.cam1.fx != 0.f) || !(t.cam1.fy != 0.f),
                            RSFM_ERR_INVALID, "pose: focal lengths must be nonzero");
-            DevTask d{};
-            d.F = t.F;
             const float c0[4] = {t.cam0.fx, t.cam0.fy, t.cam0.cx, t.cam0.cy};
             const float c1[4] = {t.cam1.fx, t.cam1.fy, t.cam1.cx, t.cam1.cy};
             for (int k = 0; k < 4; k++) {
@@ -278,8 +423,19 @@ extern "C" int rsfm_pose_from_f(const rsfm_pose_task* tasks, size_t nbTasks, flo
             d.result = t.result;
             batch.t[i] = d;
         }
-        pose_kernel<<<(uint32_t)n, kThreads, 0, stream>>>(batch, frontRatio);
-        RSFM_CHECK(check_launch("pose from F"));
+        pose_kernel<kKind><<<(uint32_t)n, kThreads, 0, stream>>>(batch, frontRatio);
+        RSFM_CHECK(check_launch(what));
     }
     return RSFM_OK;
+}
+}  // namespace
+
+extern "C" int rsfm_pose_from_f(const rsfm_pose_task* tasks, size_t nbTasks, float frontRatio, hipStream_t stream)
+{
+    return launch_pose<0>(tasks, nbTasks, frontRatio, stream, "pose from F");
+}
+
+extern "C" int rsfm_pose_from_h(const rsfm_pose_h_task* tasks, size_t nbTasks, float frontRatio, hipStream_t stream)
+{
+    return launch_pose<1>(tasks, nbTasks, frontRatio, stream, "pose from H");
 }

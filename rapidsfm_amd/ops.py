@@ -8,7 +8,7 @@ import ctypes
 
 import torch
 
-from ._abi import BfTask, BfTop2Task, Camera, DegensacTask, PoseTask, RansacTask, RealCamera, check, lib
+from ._abi import BfTask, BfTop2Task, Camera, DegensacTask, PoseHTask, PoseTask, RansacTask, RealCamera, check, lib
 
 
 def _stream(stream):
@@ -172,6 +172,33 @@ def pose_from_f(problems, front_ratio=0.95, stream=None):
         outs.append(o)
     check(lib.rsfm_pose_from_f(ctypes.cast(tasks, ctypes.c_void_p), len(problems), float(front_ratio), _stream(stream)),
           "rsfm_pose_from_f")
+    return outs
+
+
+def pose_from_h(problems, front_ratio=0.95, stream=None):
+    """Relative pose from a pixel homography for a list of (H (3, 3) float32, cam0, cam1, pts (n, 4)[, mask
+    (n,) u8[, nDev]]). Returns dicts with poses (8, 12) (the valid candidates first, in the reference's
+    order), normals (8, 3) and result (19,) = front counts [8], order by count [8], kept, points
+    considered, number of valid candidates."""
+    tasks = (PoseHTask * max(1, len(problems)))()
+    outs = []
+    for i, prob in enumerate(problems):
+        H, c0, c1, pts = prob[:4]
+        mask = prob[4] if len(prob) > 4 else None
+        ndev = prob[5] if len(prob) > 5 else None
+        if pts.dtype != torch.float32 or pts.dim() != 2 or pts.shape[1] != 4 or not pts.is_contiguous():
+            raise ValueError("pts must be contiguous float32 (n, 4)")
+        Hd = H.to(torch.float32).contiguous()
+        o = {"poses": torch.empty((8, 12), dtype=torch.float32, device=pts.device),
+             "normals": torch.empty((8, 3), dtype=torch.float32, device=pts.device),
+             "result": torch.empty(19, dtype=torch.int32, device=pts.device), "_H": Hd}
+        tasks[i] = PoseHTask(_ptr(Hd), Camera(*[float(v) for v in c0]), Camera(*[float(v) for v in c1]), _ptr(pts),
+                             pts.shape[0], _ptr(ndev) if ndev is not None else None,
+                             _ptr(mask) if mask is not None else None, _ptr(o["poses"]), _ptr(o["normals"]),
+                             _ptr(o["result"]))
+        outs.append(o)
+    check(lib.rsfm_pose_from_h(ctypes.cast(tasks, ctypes.c_void_p), len(problems), float(front_ratio), _stream(stream)),
+          "rsfm_pose_from_h")
     return outs
 
 

@@ -109,3 +109,44 @@ def posed_scene(seed, n, n_out=0, noise=0.0, width=1600.0, height=1200.0, focal=
     F = np.linalg.inv(K1).T @ _skew(t) @ R @ np.linalg.inv(K0)
     return (np.vstack([pts, out]).astype(np.float32), cam0.astype(np.float32), cam1.astype(np.float32), R, t,
             np.array(Xs), F / np.linalg.norm(F))
+
+
+def posed_plane_scene(seed, n, n_out=0, noise=0.0, width=1600.0, height=1200.0, focal=1200.0, focal1=None):
+    """Plane-only two-view scene with its pose: returns float32 rows (x0, y0, x1, y1)
+    (plane points first, then n_out outliers), cameras as (fx, fy, cx, cy), the true
+    R, t (x_cam1 = R x_cam0 + t, |t| = 1), the plane (normal nrm, |nrm| = 1, and
+    distance d: nrm . X = d in camera 0) and the pixel homography (unit Frobenius)."""
+    rng = np.random.default_rng(seed)
+    f1 = focal if focal1 is None else focal1
+    cam0 = np.array([focal, focal, width / 2, height / 2])
+    cam1 = np.array([f1, f1, width / 2 - 12.0, height / 2 + 9.0])
+    K0 = np.array([[cam0[0], 0, cam0[2]], [0, cam0[1], cam0[3]], [0, 0, 1.0]])
+    K1 = np.array([[cam1[0], 0, cam1[2]], [0, cam1[1], cam1[3]], [0, 0, 1.0]])
+    R = _rot(rng, 10.0)
+    t = rng.normal(size=3)
+    t[2] *= 0.3
+    t /= np.linalg.norm(t)
+    nrm = np.array([rng.uniform(-0.4, 0.4), rng.uniform(-0.4, 0.4), 1.0])
+    nrm /= np.linalg.norm(nrm)
+    d = rng.uniform(5.0, 9.0)
+    rows = []
+    while len(rows) < n:
+        u = np.array([rng.uniform(0, width), rng.uniform(0, height), 1.0])
+        ray = np.linalg.inv(K0) @ u
+        X = ray * (d / (ray @ nrm))
+        X1 = R @ X + t
+        if X[2] < 0.5 or X1[2] < 0.5:
+            continue
+        p1 = K1 @ X1
+        p1 = p1[:2] / p1[2]
+        if not (0 < p1[0] < width and 0 < p1[1] < height):
+            continue
+        rows.append([u[0], u[1], p1[0], p1[1]])
+    pts = np.array(rows)
+    if noise > 0:
+        pts = pts + rng.normal(scale=noise, size=pts.shape)
+    out = np.column_stack([rng.uniform(0, width, n_out), rng.uniform(0, height, n_out), rng.uniform(0, width, n_out),
+                           rng.uniform(0, height, n_out)])
+    H = K1 @ (R + np.outer(t, nrm) / d) @ np.linalg.inv(K0)
+    return (np.vstack([pts, out]).astype(np.float32), cam0.astype(np.float32), cam1.astype(np.float32), R, t, nrm, d,
+            H / np.linalg.norm(H))
