@@ -47,6 +47,7 @@ def parse():
                     help="also run one 256 MiB device copy (HBM-counter calibration for rocprofv3 --pmc)")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-pipeline leg")
     ap.add_argument("--batched-pairs", type=int, default=256)
+    ap.add_argument("--no-degensac", action="store_true", help="skip the batched DEGENSAC + pose run")
     ap.add_argument("--no-cashash", action="store_true", help="skip the cascade-hashing leg")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-proposal leg")
     ap.add_argument("--learned-train", type=int, default=10000, help="training pairs (1K-2K keypoints)")
@@ -73,7 +74,7 @@ def batched_leg(args, dev, log):
         kA = torch.from_numpy(np.ascontiguousarray(pr["kpA"][:, :2])).to(dev)
         kB = torch.from_numpy(np.ascontiguousarray(pr["kpB"][:, :2])).to(dev)
         pairs.append((pr["descA"], pr["descB"], kA, kB))
-        truth.append((pr["src"], pr["geo"]))
+        truth.append((pr["src"], pr["geo"], pr["R"]))
     t0 = time.perf_counter()
     pl = Pipeline(pairs, n_hyp=args.hyp, threshold=1.0, mode=1, seed=77)
     t_create = time.perf_counter() - t0
@@ -90,7 +91,7 @@ def batched_leg(args, dev, log):
     ms = e0.elapsed_time(e1) / reps
     # realised outlier rate of the kept matches (ground truth of the generator)
     kept = out = 0
-    for (src, geo), o in zip(truth, pl.out):
+    for (src, geo, _), o in zip(truth, pl.out):
         c = int(o["count"].item())
         m = o["matches"][:c].cpu().numpy()
         ok = (src[m[:, 0]] == m[:, 1]) & geo[m[:, 0]]
@@ -100,6 +101,7 @@ def batched_leg(args, dev, log):
     ops = sum(2.0 * a.shape[0] * b.shape[0] * 128 for a, b, _, _ in pairs)
     pl.close()
     log("batched: %.3f ms per batch of %d" % (ms, P))
+    dg = None if args.no_degensac else degensac_batch(args, pairs, truth, info, log)
     return {
         "workload": "%d pairs, keypoints U[1024,16384] per image, top2+ratio0.8+crosscheck, %d-hypothesis "
                     "F-RANSAC (Sampson 1px) per pair, one HIP graph launch" % (P, args.hyp),
@@ -110,6 +112,66 @@ def batched_leg(args, dev, log):
         "kept_matches": kept,
         "outlier_rate_kept": round(out / max(1, kept), 4),
         "tiling": info,
+        "create_s": round(t_create, 2),
+        "degensac": dg,
+    }
+
+
+def degensac_batch(args, pairs, truth, info, log):
+    """The same batch through the reference's default pair solver in one graph:
+    7-point samples, DEGENSAC finisher (F and H), relative pose from F and H.
+    Thresholds as the reference derives them for 4000 x 3000 images (two-sided
+    line test 0.008 x 3000 px, homography 0.005 x 3000 px, Config.h:172-173,
+    degensac.cpp:119-120); the matcher tiling is the one tuned above."""
+    import numpy as np
+    import torch
+
+    from rapidsfm_amd import synth
+    from rapidsfm_amd.pipeline import Pipeline
+
+    cam = (synth.FOCAL, synth.FOCAL, synth.WIDTH / 2, synth.HEIGHT / 2)
+    P = len(pairs)
+    dim = min(synth.WIDTH, synth.HEIGHT)
+    t0 = time.perf_counter()
+    pl = Pipeline(pairs, n_hyp=args.hyp, threshold=0.008 * dim, mode=0, seed=77, solver="degensac",
+                  threshold_h=0.005 * dim, n_candidates=8, n_plane_samples=256, front_ratio=0.95,
+                  cams=[(cam, cam)] * P, seg_cols=info["segCols"], wg_per_xcd=info["wgPerXcd"])
+    t_create = time.perf_counter() - t0
+    for _ in range(2):
+        pl.run()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = max(3, args.steps // 2)
+    e0.record()
+    for _ in range(reps):
+        pl.run()
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    # pose quality: rotation error of the leading kept F pose against the generator's pose
+    errs, kept_one = [], 0
+    for (_, _, R), o in zip(truth, pl.out):
+        res = o["poseF"].cpu().numpy()
+        if res[8] < 1:
+            errs.append(180.0)
+            continue
+        kept_one += int(res[8] == 1)
+        Re = o["posesF"][int(res[4])].cpu().numpy()[:9].reshape(3, 3).astype(np.float64)
+        c = np.clip((np.trace(Re @ R.T) - 1.0) / 2.0, -1.0, 1.0)
+        errs.append(float(np.degrees(np.arccos(c))))
+    errs = np.array(errs)
+    pl.close()
+    log("batched DEGENSAC + pose: %.3f ms per batch of %d" % (ms, P))
+    return {
+        "workload": "same batch: 7-point samples (%d per pair), DEGENSAC finisher (F and H, 8 candidates, "
+                    "256 plane-and-parallax samples), relative pose from F and H; one HIP graph launch" % args.hyp,
+        "value": round(P / (ms / 1e3), 1),
+        "unit": "pairs/s",
+        "ms_per_batch": round(ms, 4),
+        "pose_rot_err_deg_median": round(float(np.median(errs)), 4),
+        "pose_within_1deg": int((errs < 1.0).sum()),
+        "pose_unique": kept_one,
+        "pairs": P,
         "create_s": round(t_create, 2),
     }
 

@@ -413,6 +413,12 @@ typedef struct rsfm_sample_task {
     uint64_t seed;
 } rsfm_sample_task;
 int rsfm_ransac_sample_indices_batch(const rsfm_sample_task* tasks, size_t nbTasks, hipStream_t stream);
+/* the same draw with sampleSize (4..8) indices per hypothesis: sample h holds
+ * the first sampleSize distinct draws of hypothesis h's sequence, rows of
+ * sampleSize (7 for rsfm_ransac_f7, 4 for rsfm_ransac_h4); fewer points than
+ * sampleSize give ~0u entries */
+int rsfm_ransac_sample_indices_batch_k(const rsfm_sample_task* tasks, size_t nbTasks, uint32_t sampleSize,
+                                       hipStream_t stream);
 
 /* ---- batched pipeline (no reference counterpart: the reference matches each
  * candidate pair with BruteForceMatcher::match, bruteForceMatcher.cpp:163-443,
@@ -441,7 +447,22 @@ typedef struct rsfm_pipeline_pair {
     uint32_t* sampleIdx;        /* [nbHyp][8] */
     uint32_t* counts;           /* [nbHyp] inlier counts */
     uint32_t* best;             /* [2] best hypothesis, its count */
-    uint8_t* inlierMask;        /* optional [nA] */
+    uint8_t* inlierMask;        /* optional [nA]; with the DEGENSAC solver F's inliers */
+    /* DEGENSAC solver only (cfg.solver = RSFM_PIPELINE_SOLVER_DEGENSAC): sampleIdx is then
+     * [nbHyp][7], counts are rsfm_ransac_f7's and best its best sample; the finisher
+     * (rsfm_degensac_refine) writes these */
+    float* F;                   /* [9] */
+    float* H;                   /* [9] */
+    uint32_t* solverResult;     /* [4]: F inliers, H inliers, plane-and-parallax flag, sample */
+    uint8_t* inlierMaskH;       /* optional [nA] */
+    /* optional pose stage, DEGENSAC solver only: rsfm_pose_from_f over F's inliers when
+     * posesF is set, rsfm_pose_from_h over H's inliers when posesH is set */
+    rsfm_camera cam0, cam1;
+    float* posesF;              /* [4][12] */
+    uint32_t* poseResultF;      /* [10] */
+    float* posesH;              /* [8][12] */
+    float* normalsH;            /* optional [8][3] */
+    uint32_t* poseResultH;      /* [19] */
 } rsfm_pipeline_pair;
 typedef struct rsfm_pipeline_config {
     uint32_t nbHyp;            /* hypotheses per pair */
@@ -453,8 +474,21 @@ typedef struct rsfm_pipeline_config {
     uint32_t segCols;          /* matcher tile width (multiple of 64, <= 8192); 0 = autotune */
     uint32_t wgPerXcd;         /* persistent matcher workgroups per XCD, RSFM_PIPELINE_WG_PER_TILE
                                   (one workgroup per tile, no persistent loop); 0 = autotune */
+    uint32_t solver;           /* RSFM_PIPELINE_SOLVER_F8 or RSFM_PIPELINE_SOLVER_DEGENSAC */
+    float thresholdH;          /* DEGENSAC: homography threshold in pixels */
+    uint32_t nbCandidates;     /* DEGENSAC: best 7-point samples replayed */
+    uint32_t nbPlaneSamples;   /* DEGENSAC: plane-and-parallax samples per candidate */
+    float frontRatio;          /* pose stage: disambiguation ratio (the reference's 0.95) */
 } rsfm_pipeline_config;
 #define RSFM_PIPELINE_WG_PER_TILE 0xFFFFFFFFu
+/* solvers: 8-point F RANSAC (samples, scores, best and inlier mask), or the
+ * reference's default pair solver (Config.h:169, solveByDegensacForBoth,
+ * ImagePair.cpp:833-866): 7-point samples drawn by
+ * rsfm_ransac_sample_indices_batch_k, rsfm_ransac_f7, rsfm_degensac_refine
+ * (seed = cfg.seed) and the optional pose stage, with the pair's match count
+ * as the device point count */
+#define RSFM_PIPELINE_SOLVER_F8 0u
+#define RSFM_PIPELINE_SOLVER_DEGENSAC 1u
 typedef struct rsfm_pipeline_tuning_entry {
     uint32_t segCols, wgPerXcd;
     float matchMs;             /* fill + match + merge, best of three */

@@ -295,24 +295,29 @@ static uint64_t splitmix_next(uint64_t* s)
     return z ^ (z >> 31);
 }
 
-void or_sample_indices_hash(uint64_t seed, uint32_t n, uint32_t H, uint32_t* idx)
+void or_sample_indices_hash_k(uint64_t seed, uint32_t n, uint32_t k, uint32_t H, uint32_t* idx)
 {
     for (uint32_t h = 0; h < H; h++) {
-        uint32_t* out = idx + (size_t)h * 8;
-        if (n < 8) {
-            for (int i = 0; i < 8; i++) out[i] = ~0u;
+        uint32_t* out = idx + (size_t)h * k;
+        if (n < k) {
+            for (uint32_t i = 0; i < k; i++) out[i] = ~0u;
             continue;
         }
         uint64_t s = seed ^ ((uint64_t)h * 0xD1B54A32D192ED03ULL);
-        for (int i = 0; i < 8; i++) {
+        for (uint32_t i = 0; i < k; i++) {
             for (;;) {
                 const uint32_t c = (uint32_t)(splitmix_next(&s) % n);
                 int dup = 0;
-                for (int j = 0; j < i; j++) dup |= out[j] == c;
+                for (uint32_t j = 0; j < i; j++) dup |= out[j] == c;
                 if (!dup) { out[i] = c; break; }
             }
         }
     }
+}
+
+void or_sample_indices_hash(uint64_t seed, uint32_t n, uint32_t H, uint32_t* idx)
+{
+    or_sample_indices_hash_k(seed, n, 8, H, idx);
 }
 
 /* Weighted draw of 8 distinct indices per hypothesis (the learned-proposal

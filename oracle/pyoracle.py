@@ -51,6 +51,7 @@ _decl("or_lcg_step", I32, [ctypes.POINTER(ctypes.c_int32)])
 _decl("or_gen_sift_pair", None, [U64, U32, U32, U32, F64, P, P, P])
 _decl("or_gen_random_u8", None, [U64, U32, P])
 _decl("or_gen_two_view", None, [U64, U32, U32, F64, F64, F64, F64, P, P, P])
+_decl("or_sample_indices_hash_k", None, [U64, U32, U32, U32, P])
 _decl("or_sample_indices", None, [U64, U32, U32, U32, P])
 _decl("or_sqrnorm_u8", None, [P, U32, P])
 _decl("or_desc8b_to4b", None, [P, U32, P])
@@ -393,6 +394,12 @@ def sample_weighted(w, H, seed):
 def sample_indices_hash(seed, n, H):
     idx = np.zeros((H, 8), np.uint32)
     _lib.or_sample_indices_hash(seed, n, H, _p(idx))
+    return idx
+
+
+def sample_indices_hash_k(seed, n, k, H):
+    idx = np.zeros((H, k), np.uint32)
+    _lib.or_sample_indices_hash_k(seed, n, k, H, _p(idx))
     return idx
 
 

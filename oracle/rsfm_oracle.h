@@ -59,6 +59,8 @@ void or_gen_match_pair(uint64_t seed, uint32_t nA, uint32_t nB, uint32_t n_true,
 void or_sample_indices(uint64_t seed, uint32_t n, uint32_t k, uint32_t H, uint32_t* idx);
 /* restatement of the GPU's splitmix64 sampler (rsfm_ransac_sample_indices) */
 void or_sample_indices_hash(uint64_t seed, uint32_t n, uint32_t H, uint32_t* idx);
+/* the same with k (4..8) indices per sample, rows of k */
+void or_sample_indices_hash_k(uint64_t seed, uint32_t n, uint32_t k, uint32_t H, uint32_t* idx);
 /* learned-proposal weighted draw (gen.c) */
 void or_sample_weighted(const float* w, uint32_t n, uint32_t H, uint64_t seed, uint32_t* idx);
 

@@ -54,26 +54,7 @@ constexpr uint32_t kMaxPlaneSamples = 4096;
 constexpr int kTile = 1024;
 constexpr int kMaxTasks = 16;
 
-struct DevTask {
-    const float4* pts;
-    const uint32_t* nDev;
-    uint32_t nHost;
-    const uint32_t* idx;
-    const uint32_t* counts;
-    uint32_t H;
-    float* F;
-    float* Hm;
-    uint32_t* result;
-    uint8_t* maskF;
-    uint8_t* maskH;
-};
 typedef ByValue<DevTask, kMaxTasks> Batch;
-
-struct Params {
-    float tF2, tH2;
-    uint32_t nCand, nPlane;
-    uint64_t seed;
-};
 
 __device__ __constant__ int kTri[5][3] = {{0, 1, 2}, {3, 4, 5}, {0, 1, 6}, {3, 4, 6}, {2, 5, 6}};
 __device__ __constant__ int kRem[5][4] = {{3, 4, 5, 6}, {0, 1, 2, 6}, {2, 3, 4, 5}, {0, 1, 2, 5}, {0, 1, 3, 4}};
@@ -469,8 +450,8 @@ __device__ bool plane_search(Smem& s, const float4* pts, uint32_t n, uint32_t mO
     return true;
 }
 
-template <int kMode>
-__global__ __launch_bounds__(kThreads) void degensac_kernel(const Batch batch, const Params p)
+template <int kMode, class Src>
+__global__ __launch_bounds__(kThreads) void degensac_kernel(const Src batch, const Params p)
 {
     __shared__ Smem s;
     const DevTask& T = batch[blockIdx.x];
@@ -588,35 +569,60 @@ __global__ __launch_bounds__(kThreads) void degensac_kernel(const Batch batch, c
 
 using namespace rsfm_amd;
 
-extern "C" int rsfm_degensac_refine(const rsfm_degensac_task* tasks, size_t nbTasks, float thresholdF,
-                                    float thresholdH, int mode, uint32_t nbCandidates, uint32_t nbPlaneSamples,
-                                    uint64_t seed, hipStream_t stream)
+namespace rsfm_amd {
+namespace degensac {
+
+int make_params(float thresholdF, float thresholdH, int mode, uint32_t nbCandidates, uint32_t nbPlaneSamples,
+                uint64_t seed, Params& out)
 {
-    using namespace degensac;
-    RSFM_RETURN_IF(nbTasks > 0 && tasks == nullptr, RSFM_ERR_INVALID, "degensac: tasks is null");
     RSFM_RETURN_IF(mode != 0 && mode != 1, RSFM_ERR_INVALID, "degensac: mode must be 0 (two-sided line) or 1 (Sampson)");
     RSFM_RETURN_IF(!(thresholdF > 0.f) || !(thresholdH > 0.f), RSFM_ERR_INVALID, "degensac: thresholds must be > 0");
     RSFM_RETURN_IF(nbCandidates < 1 || nbCandidates > kMaxCand, RSFM_ERR_INVALID,
                    "degensac: nbCandidates must be in [1, 64]");
     RSFM_RETURN_IF(nbPlaneSamples < 1 || nbPlaneSamples > kMaxPlaneSamples, RSFM_ERR_INVALID,
                    "degensac: nbPlaneSamples must be in [1, 4096]");
-    const Params prm{thresholdF * thresholdF, thresholdH * thresholdH, nbCandidates, nbPlaneSamples, seed};
+    out = Params{thresholdF * thresholdF, thresholdH * thresholdH, nbCandidates, nbPlaneSamples, seed};
+    return RSFM_OK;
+}
+
+int make_task(const rsfm_degensac_task& t, DevTask& out)
+{
+    RSFM_RETURN_IF(t.nbPts > kMaxPts, RSFM_ERR_UNSUPPORTED, "degensac: at most 65536 correspondences per pair");
+    RSFM_RETURN_IF(t.nbPts > 0 && t.pts == nullptr, RSFM_ERR_INVALID, "degensac: pts is null");
+    RSFM_RETURN_IF(reinterpret_cast<uintptr_t>(t.pts) % 16, RSFM_ERR_INVALID, "degensac: pts must be 16-byte aligned");
+    RSFM_RETURN_IF(t.nbHyp > 0 && (t.sampleIdx == nullptr || t.counts == nullptr), RSFM_ERR_INVALID,
+                   "degensac: sampleIdx/counts must be set when nbHyp > 0");
+    RSFM_RETURN_IF(t.F == nullptr || t.H == nullptr || t.result == nullptr, RSFM_ERR_INVALID,
+                   "degensac: F, H and result are required");
+    out = DevTask{reinterpret_cast<const float4*>(t.pts), t.nbPtsDev, t.nbPts, t.sampleIdx, t.counts, t.nbHyp, t.F, t.H,
+                  t.result, t.inlierMaskF, t.inlierMaskH};
+    return RSFM_OK;
+}
+
+int launch_degensac_table(const DevTask* tasks, uint32_t nTasks, const Params& p, int mode, hipStream_t stream)
+{
+    if (nTasks == 0) return RSFM_OK;
+    const InTable<DevTask> src{tasks};
+    if (mode == 0) degensac_kernel<0><<<nTasks, kThreads, 0, stream>>>(src, p);
+    else degensac_kernel<1><<<nTasks, kThreads, 0, stream>>>(src, p);
+    return check_launch("degensac table");
+}
+
+}  // namespace degensac
+}  // namespace rsfm_amd
+
+extern "C" int rsfm_degensac_refine(const rsfm_degensac_task* tasks, size_t nbTasks, float thresholdF,
+                                    float thresholdH, int mode, uint32_t nbCandidates, uint32_t nbPlaneSamples,
+                                    uint64_t seed, hipStream_t stream)
+{
+    using namespace degensac;
+    RSFM_RETURN_IF(nbTasks > 0 && tasks == nullptr, RSFM_ERR_INVALID, "degensac: tasks is null");
+    Params prm;
+    RSFM_CHECK(make_params(thresholdF, thresholdH, mode, nbCandidates, nbPlaneSamples, seed, prm));
     for (size_t b = 0; b < nbTasks; b += kMaxTasks) {
         const size_t n = std::min<size_t>(kMaxTasks, nbTasks - b);
         Batch batch{};
-        for (size_t i = 0; i < n; i++) {
-            const rsfm_degensac_task& t = tasks[b + i];
-            RSFM_RETURN_IF(t.nbPts > kMaxPts, RSFM_ERR_UNSUPPORTED, "degensac: at most 65536 correspondences per pair");
-            RSFM_RETURN_IF(t.nbPts > 0 && t.pts == nullptr, RSFM_ERR_INVALID, "degensac: pts is null");
-            RSFM_RETURN_IF(reinterpret_cast<uintptr_t>(t.pts) % 16, RSFM_ERR_INVALID,
-                           "degensac: pts must be 16-byte aligned");
-            RSFM_RETURN_IF(t.nbHyp > 0 && (t.sampleIdx == nullptr || t.counts == nullptr), RSFM_ERR_INVALID,
-                           "degensac: sampleIdx/counts must be set when nbHyp > 0");
-            RSFM_RETURN_IF(t.F == nullptr || t.H == nullptr || t.result == nullptr, RSFM_ERR_INVALID,
-                           "degensac: F, H and result are required");
-            batch.t[i] = DevTask{reinterpret_cast<const float4*>(t.pts), t.nbPtsDev, t.nbPts, t.sampleIdx, t.counts,
-                                 t.nbHyp, t.F, t.H, t.result, t.inlierMaskF, t.inlierMaskH};
-        }
+        for (size_t i = 0; i < n; i++) RSFM_CHECK(make_task(tasks[b + i], batch.t[i]));
         if (mode == 0) degensac_kernel<0><<<(uint32_t)n, kThreads, 0, stream>>>(batch, prm);
         else degensac_kernel<1><<<(uint32_t)n, kThreads, 0, stream>>>(batch, prm);
         RSFM_CHECK(check_launch("degensac refine"));

@@ -87,10 +87,41 @@ struct SampleTask {
     uint32_t nHost;
     const uint32_t* nDev;
     uint64_t seed;
+    uint32_t k;  // indices per sample (4..8), the first k draws of the sequence
 };
 int launch_sample_table(const SampleTask* tasks, uint32_t nTasks, uint32_t maxH, hipStream_t stream);
 int launch_score_table(const DevTask* tasks, uint32_t nTasks, uint32_t maxH, float threshold, int mode,
                        float minSepSq0, float minSepSq1, hipStream_t stream);
+// 7-point scoring + best over a device task table (sampleIdx rows of 7)
+int launch_score7_table(const DevTask* tasks, uint32_t nTasks, uint32_t maxH, float threshold, int mode,
+                        float minSepSq0, float minSepSq1, hipStream_t stream);
 }  // namespace ransac
+
+namespace degensac {
+struct DevTask {
+    const float4* pts;
+    const uint32_t* nDev;
+    uint32_t nHost;
+    const uint32_t* idx;
+    const uint32_t* counts;
+    uint32_t H;
+    float* F;
+    float* Hm;
+    uint32_t* result;
+    uint8_t* maskF;
+    uint8_t* maskH;
+};
+struct Params {
+    float tF2, tH2;
+    uint32_t nCand, nPlane;
+    uint64_t seed;
+};
+// argument checks of rsfm_degensac_refine: the shared parameters, then one task
+int make_params(float thresholdF, float thresholdH, int mode, uint32_t nbCandidates, uint32_t nbPlaneSamples,
+                uint64_t seed, Params& out);
+int make_task(const rsfm_degensac_task& t, DevTask& out);
+// one workgroup per task of a device task table, all tasks in one launch
+int launch_degensac_table(const DevTask* tasks, uint32_t nTasks, const Params& p, int mode, hipStream_t stream);
+}  // namespace degensac
 
 }  // namespace rsfm_amd

@@ -23,6 +23,13 @@
 //    inlier mask) are captured once into a HIP graph on a private stream;
 //    rsfm_pipeline_run launches that graph. Buffers are cleared by a fill
 //    kernel (no memset nodes).
+//  * With the DEGENSAC solver the verification stages are the reference's
+//    default pair solver instead: 7-point samples, the 7-point scorer, the
+//    DEGENSAC finisher (F and H with their inlier masks) and, optionally, the
+//    relative pose from F and from H, captured into the same graph. The
+//    7-point scorer and the finisher read device task tables, so every pair
+//    runs in one launch (one finisher workgroup per pair); the pose entries
+//    take task batches by value (32 pairs per launch).
 #include "internal.hpp"
 #include <algorithm>
 #include <numeric>
@@ -70,6 +77,13 @@ struct rsfm_pipeline {
     uint32_t segCols = 0, wgPerXcd = 0, nTiles = 0, maxList = 0;
     float tunedMs = 0.f;
     std::vector<rsfm_pipeline_tuning_entry> tuning;
+    // DEGENSAC solver stages: device task tables for the 7-point scorer and the
+    // finisher (every pair in one launch), host task arrays for the pose entries
+    ransac::DevTask* dF7 = nullptr;
+    degensac::DevTask* dDg = nullptr;
+    degensac::Params dgParams{};
+    std::vector<rsfm_pose_task> poseF;
+    std::vector<rsfm_pose_h_task> poseH;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 };
@@ -256,9 +270,20 @@ int capture(rsfm_pipeline* pl)
     int rc = enqueue_match(pl, pl->stream);
     if (rc == RSFM_OK) rc = aux::launch_filter_table(pl->dFilter, P, c.ratioNum, c.ratioDen, pl->stream);
     if (rc == RSFM_OK && c.nbHyp > 0) rc = ransac::launch_sample_table(pl->dSample, P, c.nbHyp, pl->stream);
-    if (rc == RSFM_OK)
-        rc = ransac::launch_score_table(pl->dRansac, P, c.nbHyp, c.threshold, c.mode, c.minSepSq0, c.minSepSq1,
-                                        pl->stream);
+    if (c.solver == RSFM_PIPELINE_SOLVER_F8) {
+        if (rc == RSFM_OK)
+            rc = ransac::launch_score_table(pl->dRansac, P, c.nbHyp, c.threshold, c.mode, c.minSepSq0, c.minSepSq1,
+                                            pl->stream);
+    } else {
+        if (rc == RSFM_OK)
+            rc = ransac::launch_score7_table(pl->dF7, P, c.nbHyp, c.threshold, c.mode, c.minSepSq0, c.minSepSq1,
+                                             pl->stream);
+        if (rc == RSFM_OK) rc = degensac::launch_degensac_table(pl->dDg, P, pl->dgParams, c.mode, pl->stream);
+        if (rc == RSFM_OK && !pl->poseF.empty())
+            rc = rsfm_pose_from_f(pl->poseF.data(), pl->poseF.size(), c.frontRatio, pl->stream);
+        if (rc == RSFM_OK && !pl->poseH.empty())
+            rc = rsfm_pose_from_h(pl->poseH.data(), pl->poseH.size(), c.frontRatio, pl->stream);
+    }
     hipGraph_t g = nullptr;
     const int rc2 = check_hip(hipStreamEndCapture(pl->stream, &g), "end capture");
     RSFM_CHECK(rc);
@@ -291,6 +316,9 @@ extern "C" int rsfm_pipeline_create(const rsfm_pipeline_pair* pairs, size_t nbPa
                    "pipeline: segCols must be 0 or a multiple of 64 up to 8192");
     RSFM_RETURN_IF(cfg->wgPerXcd > 1024 && cfg->wgPerXcd != RSFM_PIPELINE_WG_PER_TILE, RSFM_ERR_INVALID,
                    "pipeline: wgPerXcd must be <= 1024 or RSFM_PIPELINE_WG_PER_TILE");
+    RSFM_RETURN_IF(cfg->solver != RSFM_PIPELINE_SOLVER_F8 && cfg->solver != RSFM_PIPELINE_SOLVER_DEGENSAC,
+                   RSFM_ERR_INVALID, "pipeline: unknown solver");
+    const bool degensac = cfg->solver == RSFM_PIPELINE_SOLVER_DEGENSAC;
     for (size_t i = 0; i < nbPairs; i++) {
         const auto& x = pairs[i];
         RSFM_RETURN_IF(x.nA > 0 && (x.descA == nullptr || x.bestA == nullptr || x.secondA == nullptr || x.kpA == nullptr ||
@@ -302,6 +330,13 @@ extern "C" int rsfm_pipeline_create(const rsfm_pipeline_pair* pairs, size_t nbPa
                        "pipeline pair " + std::to_string(i) + ": count/best missing");
         RSFM_RETURN_IF(cfg->nbHyp > 0 && (x.sampleIdx == nullptr || x.counts == nullptr), RSFM_ERR_INVALID,
                        "pipeline pair " + std::to_string(i) + ": sampleIdx/counts missing");
+        RSFM_RETURN_IF(degensac && (x.F == nullptr || x.H == nullptr || x.solverResult == nullptr), RSFM_ERR_INVALID,
+                       "pipeline pair " + std::to_string(i) + ": F/H/solverResult missing (DEGENSAC solver)");
+        RSFM_RETURN_IF(!degensac && (x.posesF != nullptr || x.posesH != nullptr), RSFM_ERR_INVALID,
+                       "pipeline pair " + std::to_string(i) + ": the pose stage needs the DEGENSAC solver");
+        RSFM_RETURN_IF((x.posesF != nullptr && x.poseResultF == nullptr) ||
+                           (x.posesH != nullptr && x.poseResultH == nullptr),
+                       RSFM_ERR_INVALID, "pipeline pair " + std::to_string(i) + ": pose result missing");
         RSFM_RETURN_IF((reinterpret_cast<uintptr_t>(x.descA) | reinterpret_cast<uintptr_t>(x.descB)) % 16 ||
                            (reinterpret_cast<uintptr_t>(x.kpA) | reinterpret_cast<uintptr_t>(x.kpB)) % 8 ||
                            reinterpret_cast<uintptr_t>(x.pts) % 16,
@@ -349,6 +384,8 @@ extern "C" int rsfm_pipeline_create(const rsfm_pipeline_pair* pairs, size_t nbPa
     const size_t oFilter = take(P * sizeof(aux::FilterTask));
     const size_t oSample = take(P * sizeof(ransac::SampleTask));
     const size_t oRansac = take(P * sizeof(ransac::DevTask));
+    const size_t oF7 = take(P * sizeof(ransac::DevTask));
+    const size_t oDg = take(P * sizeof(degensac::DevTask));
     const size_t oFill = take(std::max<size_t>(1, fill.size()) * sizeof(pipe::FillRange));
     const size_t oPart = take(std::max<size_t>(1, pl->partCap) * sizeof(uint64_t));
     rc = check_hip(hipMalloc(reinterpret_cast<void**>(&pl->arena), off), "pipeline workspace");
@@ -360,25 +397,48 @@ extern "C" int rsfm_pipeline_create(const rsfm_pipeline_pair* pairs, size_t nbPa
     pl->dFilter = reinterpret_cast<aux::FilterTask*>(pl->arena + oFilter);
     pl->dSample = reinterpret_cast<ransac::SampleTask*>(pl->arena + oSample);
     pl->dRansac = reinterpret_cast<ransac::DevTask*>(pl->arena + oRansac);
+    pl->dF7 = reinterpret_cast<ransac::DevTask*>(pl->arena + oF7);
+    pl->dDg = reinterpret_cast<degensac::DevTask*>(pl->arena + oDg);
     pl->dFill = reinterpret_cast<pipe::FillRange*>(pl->arena + oFill);
     pl->dPart = reinterpret_cast<uint64_t*>(pl->arena + oPart);
 
     std::vector<aux::FilterTask> ft(P);
     std::vector<ransac::SampleTask> st(P);
-    std::vector<ransac::DevTask> rt(P);
+    std::vector<ransac::DevTask> rt(P), f7;
+    std::vector<degensac::DevTask> dg;
+    if (degensac &&
+        (rc = degensac::make_params(cfg->threshold, cfg->thresholdH, cfg->mode, cfg->nbCandidates,
+                                    cfg->nbPlaneSamples, cfg->seed, pl->dgParams)) != RSFM_OK)
+        return fail_out(rc);
     for (uint32_t i = 0; i < P; i++) {
         const auto& x = pl->pairs[i];
         ft[i] = aux::FilterTask{reinterpret_cast<const uint64_t*>(x.bestA), reinterpret_cast<const uint64_t*>(x.secondA),
                                 x.nA, reinterpret_cast<const uint64_t*>(x.bestB), x.nB, x.matches, x.count,
                                 reinterpret_cast<const float2*>(x.kpA), reinterpret_cast<const float2*>(x.kpB),
                                 reinterpret_cast<float4*>(x.pts)};
-        st[i] = ransac::SampleTask{x.sampleIdx, cfg->nbHyp, x.nA, x.count, cfg->seed + i};
+        st[i] = ransac::SampleTask{x.sampleIdx, cfg->nbHyp, x.nA, x.count, cfg->seed + i, degensac ? 7u : 8u};
         rt[i] = ransac::DevTask{reinterpret_cast<const float4*>(x.pts), x.count, x.nA, cfg->nbHyp, x.sampleIdx,
                                 x.counts, nullptr, x.best, x.inlierMask};
+        if (!degensac) continue;
+        f7.push_back(ransac::DevTask{reinterpret_cast<const float4*>(x.pts), x.count, x.nA, cfg->nbHyp, x.sampleIdx,
+                                     x.counts, nullptr, x.best, nullptr});
+        dg.emplace_back();
+        if ((rc = degensac::make_task(rsfm_degensac_task{x.pts, x.nA, x.count, x.sampleIdx, x.counts, cfg->nbHyp, x.F,
+                                                         x.H, x.solverResult, x.inlierMask, x.inlierMaskH},
+                                      dg.back())) != RSFM_OK)
+            return fail_out(rc);
+        if (x.posesF)
+            pl->poseF.push_back(rsfm_pose_task{x.F, x.cam0, x.cam1, x.pts, x.nA, x.count, x.inlierMask, x.posesF,
+                                               x.poseResultF});
+        if (x.posesH)
+            pl->poseH.push_back(rsfm_pose_h_task{x.H, x.cam0, x.cam1, x.pts, x.nA, x.count, x.inlierMaskH, x.posesH,
+                                                 x.normalsH, x.poseResultH});
     }
     if ((rc = upload(pl->dFilter, ft.data(), P * sizeof(aux::FilterTask))) != RSFM_OK ||
         (rc = upload(pl->dSample, st.data(), P * sizeof(ransac::SampleTask))) != RSFM_OK ||
         (rc = upload(pl->dRansac, rt.data(), P * sizeof(ransac::DevTask))) != RSFM_OK ||
+        (rc = upload(pl->dF7, f7.data(), f7.size() * sizeof(ransac::DevTask))) != RSFM_OK ||
+        (rc = upload(pl->dDg, dg.data(), dg.size() * sizeof(degensac::DevTask))) != RSFM_OK ||
         (rc = upload(pl->dFill, fill.data(), fill.size() * sizeof(pipe::FillRange))) != RSFM_OK)
         return fail_out(rc);
     if ((rc = tune(pl)) != RSFM_OK) return fail_out(rc);

@@ -357,15 +357,18 @@ __global__ void sample_batch_kernel(const Src batch)
     const uint32_t h = blockIdx.x * blockDim.x + threadIdx.x;
     if (h >= T.H) return;
     const uint32_t n = T.nDev ? min(*T.nDev, T.nHost) : T.nHost;
-    uint32_t* out = T.idx + (size_t)h * 8;
-    if (n < 8) {
-        for (int i = 0; i < 8; i++) out[i] = ~0u;
+    const uint32_t k = T.k;
+    uint32_t* out = T.idx + (size_t)h * k;
+    if (n < k) {
+        for (uint32_t i = 0; i < k; i++) out[i] = ~0u;
         return;
     }
     uint64_t s = T.seed ^ ((uint64_t)h * 0xD1B54A32D192ED03ull);
     uint32_t v[8];
 #pragma unroll
     for (int i = 0; i < 8; i++) {
+        v[i] = ~0u;
+        if ((uint32_t)i >= k) continue;
         for (;;) {
             const uint32_t c = (uint32_t)(splitmix(s) % n);
             bool dup = false;
@@ -378,7 +381,8 @@ __global__ void sample_batch_kernel(const Src batch)
         }
     }
 #pragma unroll
-    for (int i = 0; i < 8; i++) out[i] = v[i];
+    for (int i = 0; i < 8; i++)
+        if ((uint32_t)i < k) out[i] = v[i];
 }
 
 // ---- weighted sampler for learned proposals (no reference counterpart; the
@@ -551,6 +555,24 @@ int launch_score_table(const DevTask* tasks, uint32_t nTasks, uint32_t maxH, flo
     return check_launch("ransac best table");
 }
 
+int launch_score7_table(const DevTask* tasks, uint32_t nTasks, uint32_t maxH, float threshold, int mode,
+                        float minSepSq0, float minSepSq1, hipStream_t stream)
+{
+    RSFM_RETURN_IF(mode != 0 && mode != 1, RSFM_ERR_INVALID, "ransac7: mode must be 0 or 1");
+    if (nTasks == 0) return RSFM_OK;
+    const float t2 = threshold * threshold;
+    const InTable<DevTask> src{tasks};
+    if (maxH > 0) {
+        const dim3 grid(div_up(maxH, kScoreThreads), nTasks);
+        if (mode == 0) score7_kernel<0><<<grid, kScoreThreads, 0, stream>>>(src, t2, minSepSq0, minSepSq1);
+        else score7_kernel<1><<<grid, kScoreThreads, 0, stream>>>(src, t2, minSepSq0, minSepSq1);
+        RSFM_CHECK(check_launch("ransac7 score table"));
+    }
+    if (mode == 0) best7_kernel<0><<<nTasks, kThreads, 0, stream>>>(src, t2, minSepSq0, minSepSq1);
+    else best7_kernel<1><<<nTasks, kThreads, 0, stream>>>(src, t2, minSepSq0, minSepSq1);
+    return check_launch("ransac7 best table");
+}
+
 }  // namespace ransac
 }  // namespace rsfm_amd
 
@@ -624,10 +646,12 @@ extern "C" int rsfm_ransac_sample_indices(uint32_t* sampleIdx, uint32_t nbHyp, u
     return check_launch("ransac sample_indices");
 }
 
-extern "C" int rsfm_ransac_sample_indices_batch(const rsfm_sample_task* tasks, size_t nbTasks, hipStream_t stream)
+namespace {
+int sample_batch(const rsfm_sample_task* tasks, size_t nbTasks, uint32_t k, hipStream_t stream)
 {
     using namespace ransac;
     RSFM_RETURN_IF(nbTasks > 0 && tasks == nullptr, RSFM_ERR_INVALID, "sample batch: tasks is null");
+    RSFM_RETURN_IF(k < 4 || k > 8, RSFM_ERR_INVALID, "sample batch: sampleSize must be in [4, 8]");
     for (size_t b = 0; b < nbTasks; b += kSampleMaxTasks) {
         const size_t n = std::min<size_t>(kSampleMaxTasks, nbTasks - b);
         SampleBatch batch{};
@@ -635,7 +659,7 @@ extern "C" int rsfm_ransac_sample_indices_batch(const rsfm_sample_task* tasks, s
         for (size_t i = 0; i < n; i++) {
             const rsfm_sample_task& t = tasks[b + i];
             RSFM_RETURN_IF(t.nbHyp > 0 && t.sampleIdx == nullptr, RSFM_ERR_INVALID, "sample batch: sampleIdx is null");
-            batch.t[i] = SampleTask{t.sampleIdx, t.nbHyp, t.nbPts, t.nbPtsDev, t.seed};
+            batch.t[i] = SampleTask{t.sampleIdx, t.nbHyp, t.nbPts, t.nbPtsDev, t.seed, k};
             maxH = std::max(maxH, t.nbHyp);
         }
         if (maxH == 0) continue;
@@ -643,6 +667,18 @@ extern "C" int rsfm_ransac_sample_indices_batch(const rsfm_sample_task* tasks, s
         RSFM_CHECK(check_launch("ransac sample_indices_batch"));
     }
     return RSFM_OK;
+}
+}  // namespace
+
+extern "C" int rsfm_ransac_sample_indices_batch(const rsfm_sample_task* tasks, size_t nbTasks, hipStream_t stream)
+{
+    return sample_batch(tasks, nbTasks, 8, stream);
+}
+
+extern "C" int rsfm_ransac_sample_indices_batch_k(const rsfm_sample_task* tasks, size_t nbTasks, uint32_t sampleSize,
+                                                  hipStream_t stream)
+{
+    return sample_batch(tasks, nbTasks, sampleSize, stream);
 }
 
 extern "C" int rsfm_ransac_sample_weighted_batch(const rsfm_weighted_sample_task* tasks, size_t nbTasks,

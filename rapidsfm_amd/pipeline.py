@@ -5,25 +5,30 @@ import ctypes
 
 import torch
 
-from ._abi import check, lib
+from ._abi import Camera, check, lib
 
 c_u32, c_f32, c_vp = ctypes.c_uint32, ctypes.c_float, ctypes.c_void_p
 
 
 # wg_per_xcd value: one matcher workgroup per tile instead of a persistent grid
 WG_PER_TILE = 0xFFFFFFFF
+SOLVERS = {"f8": 0, "degensac": 1}
 
 
 class PipelinePair(ctypes.Structure):
     _fields_ = [("nA", c_u32), ("nB", c_u32), ("descA", c_vp), ("descB", c_vp), ("kpA", c_vp), ("kpB", c_vp),
                 ("bestA", c_vp), ("secondA", c_vp), ("bestB", c_vp), ("matches", c_vp), ("count", c_vp),
-                ("pts", c_vp), ("sampleIdx", c_vp), ("counts", c_vp), ("best", c_vp), ("inlierMask", c_vp)]
+                ("pts", c_vp), ("sampleIdx", c_vp), ("counts", c_vp), ("best", c_vp), ("inlierMask", c_vp),
+                ("F", c_vp), ("H", c_vp), ("solverResult", c_vp), ("inlierMaskH", c_vp), ("cam0", Camera),
+                ("cam1", Camera), ("posesF", c_vp), ("poseResultF", c_vp), ("posesH", c_vp), ("normalsH", c_vp),
+                ("poseResultH", c_vp)]
 
 
 class PipelineConfig(ctypes.Structure):
     _fields_ = [("nbHyp", c_u32), ("threshold", c_f32), ("mode", ctypes.c_int), ("ratioNum", c_u32),
                 ("ratioDen", c_u32), ("seed", ctypes.c_uint64), ("minSepSq0", c_f32), ("minSepSq1", c_f32),
-                ("segCols", c_u32), ("wgPerXcd", c_u32)]
+                ("segCols", c_u32), ("wgPerXcd", c_u32), ("solver", c_u32), ("thresholdH", c_f32),
+                ("nbCandidates", c_u32), ("nbPlaneSamples", c_u32), ("frontRatio", c_f32)]
 
 
 class TuningEntry(ctypes.Structure):
@@ -51,11 +56,20 @@ class Pipeline:
     After run(), self.out[p] holds the device results of pair p: bestA, secA,
     bestB (int32 (n,2) BestMatch bit patterns), matches (nA,3), count (1,),
     pts (nA,4), idx (H,8), counts (H,), best (2,), mask (nA,).
+
+    solver="degensac" runs the DEGENSAC pair solver instead of 8-point F RANSAC:
+    idx is then (H,7), mask holds F's inliers, and F (3,3), H (3,3), result (4,),
+    maskH (nA,) are added; with cams (a list of (cam0, cam1) as (fx, fy, cx, cy)
+    per pair) also posesF (4,12), poseF (10,), posesH (8,12), normalsH (8,3),
+    poseH (19,).
     """
 
     def __init__(self, pairs, n_hyp=4096, threshold=1.0, mode=1, ratio=(4, 5), seed=77, min_sep_sq=(0.0, 0.0),
-                 seg_cols=0, wg_per_xcd=0):
+                 seg_cols=0, wg_per_xcd=0, solver="f8", threshold_h=3.0, n_candidates=8, n_plane_samples=256,
+                 front_ratio=0.95, cams=None):
         dev = pairs[0][0].device
+        dg = solver == "degensac"
+        k = 7 if dg else 8
         self.inputs = pairs
         self.out = []
         arr = (PipelinePair * len(pairs))()
@@ -68,18 +82,33 @@ class Pipeline:
                 "matches": torch.empty((max(1, nA), 3), dtype=torch.int32, device=dev),
                 "count": torch.zeros(1, dtype=torch.int32, device=dev),
                 "pts": torch.empty((max(1, nA), 4), dtype=torch.float32, device=dev),
-                "idx": torch.empty((max(1, n_hyp), 8), dtype=torch.int32, device=dev),
+                "idx": torch.empty((max(1, n_hyp), k), dtype=torch.int32, device=dev),
                 "counts": torch.empty(max(1, n_hyp), dtype=torch.int32, device=dev),
                 "best": torch.empty(2, dtype=torch.int32, device=dev),
                 "mask": torch.empty(max(1, nA), dtype=torch.uint8, device=dev),
             }
+            if dg:
+                o.update(F=torch.zeros((3, 3), dtype=torch.float32, device=dev),
+                         H=torch.zeros((3, 3), dtype=torch.float32, device=dev),
+                         result=torch.zeros(4, dtype=torch.int32, device=dev),
+                         maskH=torch.empty(max(1, nA), dtype=torch.uint8, device=dev))
+                if cams is not None:
+                    o.update(posesF=torch.zeros((4, 12), dtype=torch.float32, device=dev),
+                             poseF=torch.zeros(10, dtype=torch.int32, device=dev),
+                             posesH=torch.zeros((8, 12), dtype=torch.float32, device=dev),
+                             normalsH=torch.zeros((8, 3), dtype=torch.float32, device=dev),
+                             poseH=torch.zeros(19, dtype=torch.int32, device=dev))
             p = lambda t: t.data_ptr()  # noqa: E731
+            q = lambda key: o[key].data_ptr() if key in o else None  # noqa: E731
+            c0, c1 = cams[i] if (dg and cams is not None) else ((1.0, 1.0, 0.0, 0.0), (1.0, 1.0, 0.0, 0.0))
             arr[i] = PipelinePair(nA, nB, p(dA), p(dB), p(kA), p(kB), p(o["bestA"]), p(o["secA"]), p(o["bestB"]),
                                   p(o["matches"]), p(o["count"]), p(o["pts"]), p(o["idx"]), p(o["counts"]),
-                                  p(o["best"]), p(o["mask"]))
+                                  p(o["best"]), p(o["mask"]), q("F"), q("H"), q("result"), q("maskH"),
+                                  Camera(*[float(v) for v in c0]), Camera(*[float(v) for v in c1]), q("posesF"),
+                                  q("poseF"), q("posesH"), q("normalsH"), q("poseH"))
             self.out.append(o)
         cfg = PipelineConfig(n_hyp, threshold, mode, ratio[0], ratio[1], seed, min_sep_sq[0], min_sep_sq[1], seg_cols,
-                             wg_per_xcd)
+                             wg_per_xcd, SOLVERS[solver], threshold_h, n_candidates, n_plane_samples, front_ratio)
         torch.cuda.synchronize()  # the inputs must be complete: creation times candidate tilings on them
         self.handle = c_vp()
         check(lib.rsfm_pipeline_create(ctypes.cast(arr, c_vp), len(pairs), ctypes.byref(cfg), ctypes.byref(self.handle)),

@@ -125,7 +125,7 @@ def make_pair(seed, n_a, n_b, copy_frac, geo_frac, device="cuda", sigma=0.05):
         kp_a[g_slots, 3] = np.angle(np.exp(1j * (kp_b[g_src, 3] - rot + rng.normal(0, 0.1, len(g_slots)))))
         geo[g_slots] = True
     return dict(descA=desc_a, descB=desc_b, kpA=kp_a.astype(np.float32), kpB=kp_b.astype(np.float32), src=src,
-                geo=geo, F=view.F)
+                geo=geo, F=view.F, R=view.R, t=view.t)
 
 
 def cashash_weights(seed, n_channels=192, device="cuda"):

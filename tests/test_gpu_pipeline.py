@@ -33,10 +33,10 @@ def _batch(dev, sizes, seed=0):
     return pairs
 
 
-def _reference(pairs, H, seed):
-    """The same pass through the individual entry points."""
+def _matched(pairs):
+    """match + ratio test + cross-check + keypoint gather through the individual entry points"""
     from rapidsfm_amd import ops
-    from rapidsfm_amd._abi import FilterTask, SampleTask, check, lib
+    from rapidsfm_amd._abi import FilterTask, check, lib
 
     dev = pairs[0][0].device
     res = ops.bf_match_top2([(a, b) for a, b, _, _ in pairs])
@@ -55,6 +55,29 @@ def _reference(pairs, H, seed):
         outs.append(o)
     s = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
     check(lib.rsfm_filter_matches_batch(ctypes.cast(filt, ctypes.c_void_p), len(pairs), 4, 5, s), "filter")
+    return outs
+
+
+def _sample(pairs, outs, H, seed, k):
+    from rapidsfm_amd._abi import SampleTask, check, lib
+
+    dev = pairs[0][0].device
+    s = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    samp = (SampleTask * len(pairs))()
+    for i, o in enumerate(outs):
+        o["idx"] = torch.empty((H, k), dtype=torch.int32, device=dev)
+        samp[i] = SampleTask(o["idx"].data_ptr(), H, max(1, pairs[i][0].shape[0]), o["count"].data_ptr(), seed + i)
+    check(lib.rsfm_ransac_sample_indices_batch_k(ctypes.cast(samp, ctypes.c_void_p), len(pairs), k, s), "sample")
+
+
+def _reference(pairs, H, seed):
+    """The same pass through the individual entry points."""
+    from rapidsfm_amd import ops
+    from rapidsfm_amd._abi import SampleTask, check, lib
+
+    dev = pairs[0][0].device
+    outs = _matched(pairs)
+    s = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
     samp = (SampleTask * len(pairs))()
     for i, o in enumerate(outs):
         o["idx"] = torch.empty((H, 8), dtype=torch.int32, device=dev)
@@ -148,3 +171,106 @@ def test_pipeline_pair_against_oracle(orc, dev):
     counts, _ = orc.f8_score(pts, idx, 1.0, 1, with_models=False)
     assert np.array_equal(u32(o["counts"]), counts)
     pl.close()
+
+
+CAM = (2000.0, 2000.0, 2000.0, 1500.0)
+
+
+def _reference_degensac(pairs, H, seed, thr, thr_h, mode, ncand, nplane, front):
+    """The DEGENSAC solver chain through the individual entry points."""
+    from rapidsfm_amd import ops
+
+    outs = _matched(pairs)
+    _sample(pairs, outs, H, seed, 7)
+    r7 = ops.ransac_f7([(o["pts"], o["idx"], o["count"]) for o in outs], thr, mode=mode)
+    dg = ops.degensac_refine([(o["pts"], o["idx"], r["counts"], o["count"]) for o, r in zip(outs, r7)], thr, thr_h,
+                             mode=mode, nb_candidates=ncand, nb_plane_samples=nplane, seed=seed)
+    pf = ops.pose_from_f([(d["F"], CAM, CAM, o["pts"], d["mask_f"], o["count"]) for o, d in zip(outs, dg)], front)
+    ph = ops.pose_from_h([(d["H"], CAM, CAM, o["pts"], d["mask_h"], o["count"]) for o, d in zip(outs, dg)], front)
+    for o, r, d, a, b in zip(outs, r7, dg, pf, ph):
+        o.update(counts=r["counts"], best=r["best"], mask=d["mask_f"], maskH=d["mask_h"], F=d["F"], H=d["H"],
+                 result=d["result"], posesF=a["poses"], poseF=a["result"], posesH=b["poses"],
+                 normalsH=b["normals"], poseH=b["result"])
+    return outs
+
+
+def _compare_degensac(pl, ref, pairs):
+    for i, (o, r) in enumerate(zip(pl.out, ref)):
+        nA = pairs[i][0].shape[0]
+        c = int(r["count"].item())
+        assert int(o["count"].item()) == c, i
+        assert torch.equal(o["pts"][:c], r["pts"][:c]), i
+        for k in ("idx", "counts", "best", "F", "H", "result", "posesF", "poseF", "posesH", "normalsH", "poseH"):
+            assert torch.equal(o[k].view(torch.int32), r[k].view(torch.int32)), (i, k)
+        assert torch.equal(o["mask"][:nA], r["mask"][:nA]), i
+        assert torch.equal(o["maskH"][:nA], r["maskH"][:nA]), i
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_pipeline_degensac_matches_entry_points(dev, mode):
+    from rapidsfm_amd.pipeline import Pipeline
+
+    rng = np.random.default_rng(5 + mode)
+    sizes = SIZES[:8] + [(int(a), int(b)) for a, b in rng.integers(150, 3000, size=(28, 2))]
+    _say("pipeline DEGENSAC vs entry points", len(sizes), "mode", mode)
+    pairs = _batch(dev, sizes, seed=300 + 50 * mode)
+    H = 384
+    args = dict(thr=1.5, thr_h=3.0, mode=mode, ncand=6, nplane=128, front=0.95)
+    pl = Pipeline(pairs, n_hyp=H, seed=21, threshold=args["thr"], mode=mode, solver="degensac",
+                  threshold_h=args["thr_h"], n_candidates=args["ncand"], n_plane_samples=args["nplane"],
+                  front_ratio=args["front"], cams=[(CAM, CAM)] * len(pairs), seg_cols=2048, wg_per_xcd=32)
+    pl.run()
+    ref = _reference_degensac(pairs, H, 21, **args)
+    torch.cuda.synchronize()
+    _compare_degensac(pl, ref, pairs)
+    # geometry: on pairs with many matches the F pose keeps a candidate
+    good = [o for o in pl.out if int(o["result"][0].item()) >= 100]
+    assert good and sum(int(o["poseF"][8].item()) >= 1 for o in good) >= 0.8 * len(good)
+    for o in pl.out:
+        for t in o.values():
+            t.fill_(-7)
+    pl.run()
+    torch.cuda.synchronize()
+    _compare_degensac(pl, ref, pairs)
+    pl.close()
+
+
+def test_pipeline_degensac_sampler_matches_oracle(orc, dev):
+    from rapidsfm_amd.pipeline import Pipeline
+
+    _say("pipeline DEGENSAC samples vs oracle")
+    pairs = _batch(dev, [(1500, 1700)], seed=8)
+    pl = Pipeline(pairs, n_hyp=200, seed=4, solver="degensac", seg_cols=2048, wg_per_xcd=32)
+    pl.run()
+    torch.cuda.synchronize()
+    o = pl.out[0]
+    c = int(o["count"].item())
+    idx = orc.sample_indices_hash_k(4, c, 7, 200)
+    assert np.array_equal(o["idx"].cpu().numpy().view(np.uint32), idx)
+    pts = o["pts"][:c].cpu().numpy()
+    counts, _ = orc.f7_score(pts, idx, 1.0, 1, nthreads=8)
+    assert np.array_equal(o["counts"].cpu().numpy().view(np.uint32), counts)
+    pl.close()
+
+
+def test_pipeline_solver_arguments(dev):
+    from rapidsfm_amd.pipeline import Pipeline
+
+    _say("pipeline solver arguments")
+    pairs = _batch(dev, [(300, 300)], seed=9)
+    with pytest.raises(RuntimeError):  # pose stage without the DEGENSAC solver
+        from rapidsfm_amd import pipeline as P
+
+        arr = (P.PipelinePair * 1)()
+        cfg = P.PipelineConfig(64, 1.0, 1, 4, 5, 1, 0.0, 0.0, 2048, 32, 0, 3.0, 8, 256, 0.95)
+        o = [torch.empty(max(1, 300 * 4), dtype=torch.int32, device=dev) for _ in range(12)]
+        a, b, ka, kb = pairs[0]
+        p = [t.data_ptr() for t in o]
+        arr[0] = P.PipelinePair(300, 300, a.data_ptr(), b.data_ptr(), ka.data_ptr(), kb.data_ptr(), *p[:10],
+                                None, None, None, None, P.Camera(1, 1, 0, 0), P.Camera(1, 1, 0, 0), p[10], p[11],
+                                None, None, None)
+        h = ctypes.c_void_p()
+        P.check(P.lib.rsfm_pipeline_create(ctypes.cast(arr, ctypes.c_void_p), 1, ctypes.byref(cfg), ctypes.byref(h)),
+                "create")
+    with pytest.raises(KeyError):
+        Pipeline(pairs, n_hyp=64, solver="lo-ransac")

@@ -124,3 +124,24 @@ def test_batched_pairs_uneven(rs, orc, dev):
     outs = rs.ransac_f8(probs, 2.0, 0)
     for o, r in zip(outs, refs):
         assert np.array_equal(_u32(o["counts"]), r)
+
+
+@pytest.mark.parametrize("k", [4, 5, 7, 8])
+def test_device_sampler_k(orc, dev, k):
+    import ctypes
+
+    from rapidsfm_amd._abi import SampleTask, check, lib
+
+    print("[case] sampler k", k, flush=True)
+    ns = [3, k - 1, k, 50, 3000]
+    idx = [torch.full((333, k), -5, dtype=torch.int32, device=dev) for _ in ns]
+    tasks = (SampleTask * len(ns))()
+    for i, n in enumerate(ns):
+        tasks[i] = SampleTask(idx[i].data_ptr(), 333, n, None, 1000 + i)
+    check(lib.rsfm_ransac_sample_indices_batch_k(ctypes.cast(tasks, ctypes.c_void_p), len(ns), k,
+                                                 ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)), "sample k")
+    torch.cuda.synchronize()
+    for i, n in enumerate(ns):
+        assert np.array_equal(_u32(idx[i]), orc.sample_indices_hash_k(1000 + i, n, k, 333)), (k, n)
+    with pytest.raises(RuntimeError):
+        check(lib.rsfm_ransac_sample_indices_batch_k(ctypes.cast(tasks, ctypes.c_void_p), 1, 9, None), "sample k")
