@@ -79,62 +79,105 @@ __device__ inline bool fit_hypothesis(const DevTask& T, uint32_t n, uint32_t h, 
     else return fgeom::fit8(p, M);
 }
 
-// Scoring: one lane per hypothesis, one wavefront per workgroup. Each lane
-// fits its hypothesis in registers and then walks every correspondence in
-// index order; all lanes read the same correspondence, so the loop body is
-// VALU work only: no LDS, no cross-lane reduction.
+// Scoring: one lane per hypothesis; a workgroup holds 64 hypotheses and
+// kScoreWaves wavefronts. Wavefront 0 fits the 64 models in registers and
+// shares them through LDS; then each wavefront walks its own contiguous part
+// of the correspondences in index order (all lanes read the same
+// correspondence, so the loop body is VALU work only), and the partial counts
+// are summed through LDS. Splitting the points keeps enough wavefronts per
+// SIMD to hide load and dependency latency: one wavefront per 64 hypotheses
+// gives only two per SIMD for 32 pairs of 4096 hypotheses.
 constexpr int kScoreThreads = 64;
+#ifndef RSFM_SCORE_WAVES
+#define RSFM_SCORE_WAVES 4
+#endif
+constexpr int kScoreWaves = RSFM_SCORE_WAVES;
+constexpr int kScoreBlock = kScoreThreads * kScoreWaves;
+
+// points [beg, end) of wavefront w: contiguous parts starting on multiples of 8
+__device__ inline void wave_range(uint32_t n, uint32_t w, uint32_t& beg, uint32_t& end)
+{
+    const uint32_t part = ((n + kScoreWaves - 1) / kScoreWaves + 7u) & ~7u;
+    beg = min(n, w * part);
+    end = min(n, beg + part);
+}
+
+// number of inliers of model F among P[0, m): the next eight points are in
+// flight while the current eight are tested
+template <int kMode>
+__device__ inline uint32_t count_inliers(const float F[9], const float4* __restrict__ P, uint32_t m, float t2)
+{
+    auto test4 = [&](const float4& a, const float4& b, const float4& c, const float4& d) -> uint32_t {
+        return (model_inlier<kMode>(F, a, t2) ? 1u : 0u) + (model_inlier<kMode>(F, b, t2) ? 1u : 0u) +
+               (model_inlier<kMode>(F, c, t2) ? 1u : 0u) + (model_inlier<kMode>(F, d, t2) ? 1u : 0u);
+    };
+    uint32_t cnt = 0, i = 0;
+    if (m >= 8) {
+        float4 c0 = P[0], c1 = P[1], c2 = P[2], c3 = P[3], c4 = P[4], c5 = P[5], c6 = P[6], c7 = P[7];
+        for (; i + 16 <= m; i += 8) {
+            const float4 n0 = P[i + 8], n1 = P[i + 9], n2 = P[i + 10], n3 = P[i + 11];
+            const float4 n4 = P[i + 12], n5 = P[i + 13], n6 = P[i + 14], n7 = P[i + 15];
+            cnt += test4(c0, c1, c2, c3) + test4(c4, c5, c6, c7);
+            c0 = n0;
+            c1 = n1;
+            c2 = n2;
+            c3 = n3;
+            c4 = n4;
+            c5 = n5;
+            c6 = n6;
+            c7 = n7;
+        }
+        cnt += test4(c0, c1, c2, c3) + test4(c4, c5, c6, c7);
+        i += 8;
+    }
+    for (; i < m; i++) cnt += model_inlier<kMode>(F, P[i], t2) ? 1u : 0u;
+    return cnt;
+}
 
 template <int kMode, class Src>
-__global__ __launch_bounds__(kScoreThreads) void score_kernel(const Src batch, float t2, float s0, float s1)
+__global__ __launch_bounds__(kScoreBlock) void score_kernel(const Src batch, float t2, float s0, float s1)
 {
     const Wg3 wg = xcd_local_block();  // all hypothesis blocks of a pair share one L2
     const DevTask& T = batch[wg.y];
     if (wg.x * kScoreThreads >= T.H) return;
-    const uint32_t h = wg.x * kScoreThreads + threadIdx.x;
+    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint32_t h = wg.x * kScoreThreads + lane;
     const uint32_t n = task_n(T);
+    __shared__ float sF[9][kScoreThreads];
+    __shared__ uint32_t sOk[kScoreThreads];
+    __shared__ uint32_t sCnt[kScoreWaves][kScoreThreads];
     float F[9];
-    const bool ok = fit_hypothesis<kMode>(T, n, h, s0, s1, F);
+    if (w == 0) {
+        const bool ok = fit_hypothesis<kMode>(T, n, h, s0, s1, F);
+#pragma unroll
+        for (int q = 0; q < 9; q++) sF[q][lane] = ok ? F[q] : 0.f;
+        sOk[lane] = ok ? 1u : 0u;
+    }
+    __syncthreads();
+    const bool ok = sOk[lane] != 0;
+#pragma unroll
+    for (int q = 0; q < 9; q++) F[q] = sF[q][lane];
     uint32_t cnt = 0;
     if (ok) {
-        // The point index is hidden from the uniformity analysis so the loads
-        // take the vector memory path (in-order vmcnt): the next eight points
-        // are in flight while the current eight are tested. All 64 lanes read
-        // the same address, one cache line per wave.
-        uint32_t base = 0;
+        uint32_t beg, end;
+        wave_range(n, w, beg, end);
+        // the offset is hidden from the uniformity analysis so the loads take
+        // the vector memory path (in-order vmcnt, so the prefetch overlaps);
+        // all 64 lanes read the same address, one cache line per wavefront
+        uint32_t base = beg;
         asm volatile("" : "+v"(base));
-        const float4* __restrict__ P = T.pts + base;
-        auto test4 = [&](const float4& a, const float4& b, const float4& c, const float4& d) -> uint32_t {
-            return (model_inlier<kMode>(F, a, t2) ? 1u : 0u) + (model_inlier<kMode>(F, b, t2) ? 1u : 0u) +
-                   (model_inlier<kMode>(F, c, t2) ? 1u : 0u) + (model_inlier<kMode>(F, d, t2) ? 1u : 0u);
-        };
-        uint32_t i = 0;
-        if (n >= 8) {
-            float4 c0 = P[0], c1 = P[1], c2 = P[2], c3 = P[3], c4 = P[4], c5 = P[5], c6 = P[6], c7 = P[7];
-            for (; i + 16 <= n; i += 8) {
-                const float4 n0 = P[i + 8], n1 = P[i + 9], n2 = P[i + 10], n3 = P[i + 11];
-                const float4 n4 = P[i + 12], n5 = P[i + 13], n6 = P[i + 14], n7 = P[i + 15];
-                cnt += test4(c0, c1, c2, c3) + test4(c4, c5, c6, c7);
-                c0 = n0;
-                c1 = n1;
-                c2 = n2;
-                c3 = n3;
-                c4 = n4;
-                c5 = n5;
-                c6 = n6;
-                c7 = n7;
-            }
-            cnt += test4(c0, c1, c2, c3) + test4(c4, c5, c6, c7);
-            i += 8;
-        }
-        for (; i < n; i++) cnt += model_inlier<kMode>(F, P[i], t2) ? 1u : 0u;
+        cnt = count_inliers<kMode>(F, T.pts + base, end - beg, t2);
     }
-    if (h < T.H) {
-        T.counts[h] = ok ? cnt : 0u;
-        if (T.models) {
+    sCnt[w][lane] = cnt;
+    __syncthreads();
+    if (w != 0 || h >= T.H) return;
+    uint32_t total = 0;
 #pragma unroll
-            for (int q = 0; q < 9; q++) T.models[(size_t)h * 9 + q] = ok ? F[q] : 0.f;
-        }
+    for (int k = 0; k < kScoreWaves; k++) total += sCnt[k][lane];
+    T.counts[h] = ok ? total : 0u;
+    if (T.models) {
+#pragma unroll
+        for (int q = 0; q < 9; q++) T.models[(size_t)h * 9 + q] = F[q];
     }
 }
 
@@ -216,42 +259,69 @@ __device__ inline int fit_hypothesis7(const DevTask& T, uint32_t n, uint32_t h, 
     return fgeom::fit7(p, Fs);
 }
 
-// one lane per 7-point sample, as score_kernel: up to three F in registers,
-// each point tested against all three (a missing model is all zeros and
-// never counts: 0 < 0 is false in both tests)
+// one lane per 7-point sample, workgroups as score_kernel: wavefront 0 solves
+// the samples and shares up to three F per lane through LDS; every point is
+// tested against all three (a missing model is all zeros and never counts:
+// 0 < 0 is false in both tests)
 template <int kMode, class Src>
-__global__ __launch_bounds__(kScoreThreads) void score7_kernel(const Src batch, float t2, float s0, float s1)
+__global__ __launch_bounds__(kScoreBlock) void score7_kernel(const Src batch, float t2, float s0, float s1)
 {
     const Wg3 wg = xcd_local_block();
     const DevTask& T = batch[wg.y];
     if (wg.x * kScoreThreads >= T.H) return;
-    const uint32_t h = wg.x * kScoreThreads + threadIdx.x;
+    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint32_t h = wg.x * kScoreThreads + lane;
     const uint32_t n = task_n(T);
-    float Fw[kMaxF7][9];  // written by the solver with a running index
-    const int nm = fit_hypothesis7(T, n, h, s0, s1, Fw);
+    __shared__ float sF[kMaxF7][9][kScoreThreads];
+    __shared__ int sNm[kScoreThreads];
+    __shared__ uint32_t sCnt[kScoreWaves][kMaxF7][kScoreThreads];
+    if (w == 0) {
+        float Fw[kMaxF7][9];  // written by the solver with a running index
+        const int nm = fit_hypothesis7(T, n, h, s0, s1, Fw);
+#pragma unroll
+        for (int m = 0; m < kMaxF7; m++)
+#pragma unroll
+            for (int q = 0; q < 9; q++) sF[m][q][lane] = m < nm ? Fw[m][q] : 0.f;
+        sNm[lane] = nm;
+    }
+    __syncthreads();
+    const int nm = sNm[lane];
     float Fs[kMaxF7][9];  // register copy for the scoring loop
 #pragma unroll
     for (int m = 0; m < kMaxF7; m++)
 #pragma unroll
-        for (int q = 0; q < 9; q++) Fs[m][q] = m < nm ? Fw[m][q] : 0.f;
+        for (int q = 0; q < 9; q++) Fs[m][q] = sF[m][q][lane];
     uint32_t c[kMaxF7] = {0u, 0u, 0u};
     if (nm > 0) {
-        const float4* __restrict__ P = T.pts;
+        uint32_t beg, end;
+        wave_range(n, w, beg, end);
+        const float4* __restrict__ P = T.pts + beg;
+        const uint32_t m = end - beg;
         auto test = [&](const float4& p) {
 #pragma unroll
-            for (int m = 0; m < kMaxF7; m++) c[m] += fgeom::inlier<kMode>(Fs[m], p, t2) ? 1u : 0u;
+            for (int k = 0; k < kMaxF7; k++) c[k] += fgeom::inlier<kMode>(Fs[k], p, t2) ? 1u : 0u;
         };
         uint32_t i = 0;
-        for (; i + 4 <= n; i += 4) {
+        for (; i + 4 <= m; i += 4) {
             const float4 p0 = P[i], p1 = P[i + 1], p2 = P[i + 2], p3 = P[i + 3];
             test(p0);
             test(p1);
             test(p2);
             test(p3);
         }
-        for (; i < n; i++) test(P[i]);
+        for (; i < m; i++) test(P[i]);
     }
-    if (h >= T.H) return;
+#pragma unroll
+    for (int k = 0; k < kMaxF7; k++) sCnt[w][k][lane] = c[k];
+    __syncthreads();
+    if (w != 0 || h >= T.H) return;
+#pragma unroll
+    for (int k = 0; k < kMaxF7; k++) {
+        uint32_t t = 0;
+#pragma unroll
+        for (int j = 0; j < kScoreWaves; j++) t += sCnt[j][k][lane];
+        c[k] = t;
+    }
     uint32_t best = 0;
     int bm = 0;
     for (int m = 0; m < nm; m++)
@@ -546,8 +616,8 @@ int launch_score_table(const DevTask* tasks, uint32_t nTasks, uint32_t maxH, flo
     const InTable<DevTask> src{tasks};
     if (maxH > 0) {
         const dim3 grid(div_up(maxH, kScoreThreads), nTasks);
-        if (mode == 0) score_kernel<0><<<grid, kScoreThreads, 0, stream>>>(src, t2, minSepSq0, minSepSq1);
-        else score_kernel<1><<<grid, kScoreThreads, 0, stream>>>(src, t2, minSepSq0, minSepSq1);
+        if (mode == 0) score_kernel<0><<<grid, kScoreBlock, 0, stream>>>(src, t2, minSepSq0, minSepSq1);
+        else score_kernel<1><<<grid, kScoreBlock, 0, stream>>>(src, t2, minSepSq0, minSepSq1);
         RSFM_CHECK(check_launch("ransac score table"));
     }
     if (mode == 0) best_kernel<0><<<nTasks, kThreads, 0, stream>>>(src, t2, minSepSq0, minSepSq1);
@@ -564,8 +634,8 @@ int launch_score7_table(const DevTask* tasks, uint32_t nTasks, uint32_t maxH, fl
     const InTable<DevTask> src{tasks};
     if (maxH > 0) {
         const dim3 grid(div_up(maxH, kScoreThreads), nTasks);
-        if (mode == 0) score7_kernel<0><<<grid, kScoreThreads, 0, stream>>>(src, t2, minSepSq0, minSepSq1);
-        else score7_kernel<1><<<grid, kScoreThreads, 0, stream>>>(src, t2, minSepSq0, minSepSq1);
+        if (mode == 0) score7_kernel<0><<<grid, kScoreBlock, 0, stream>>>(src, t2, minSepSq0, minSepSq1);
+        else score7_kernel<1><<<grid, kScoreBlock, 0, stream>>>(src, t2, minSepSq0, minSepSq1);
         RSFM_CHECK(check_launch("ransac7 score table"));
     }
     if (mode == 0) best7_kernel<0><<<nTasks, kThreads, 0, stream>>>(src, t2, minSepSq0, minSepSq1);
@@ -605,9 +675,9 @@ static int run_hypotheses(const rsfm_ransac_task* tasks, size_t nbTasks, float t
         }
         if (maxH > 0) {
             const dim3 grid(div_up(maxH, kScoreThreads), (uint32_t)n);
-            if (kind == 0) score_kernel<0><<<grid, kScoreThreads, 0, stream>>>(batch, t2, minSepSq0, minSepSq1);
-            else if (kind == 1) score_kernel<1><<<grid, kScoreThreads, 0, stream>>>(batch, t2, minSepSq0, minSepSq1);
-            else score_kernel<2><<<grid, kScoreThreads, 0, stream>>>(batch, t2, minSepSq0, minSepSq1);
+            if (kind == 0) score_kernel<0><<<grid, kScoreBlock, 0, stream>>>(batch, t2, minSepSq0, minSepSq1);
+            else if (kind == 1) score_kernel<1><<<grid, kScoreBlock, 0, stream>>>(batch, t2, minSepSq0, minSepSq1);
+            else score_kernel<2><<<grid, kScoreBlock, 0, stream>>>(batch, t2, minSepSq0, minSepSq1);
             RSFM_CHECK(check_launch(name));
         }
         bool anyBest = false;
@@ -734,8 +804,8 @@ extern "C" int rsfm_ransac_f7(const rsfm_ransac_task* tasks, size_t nbTasks, flo
         }
         if (maxH > 0) {
             const dim3 grid(div_up(maxH, kScoreThreads), (uint32_t)n);
-            if (mode == 0) score7_kernel<0><<<grid, kScoreThreads, 0, stream>>>(batch, t2, minSepSq0, minSepSq1);
-            else score7_kernel<1><<<grid, kScoreThreads, 0, stream>>>(batch, t2, minSepSq0, minSepSq1);
+            if (mode == 0) score7_kernel<0><<<grid, kScoreBlock, 0, stream>>>(batch, t2, minSepSq0, minSepSq1);
+            else score7_kernel<1><<<grid, kScoreBlock, 0, stream>>>(batch, t2, minSepSq0, minSepSq1);
             RSFM_CHECK(check_launch("ransac7 score"));
         }
         bool anyBest = false;
