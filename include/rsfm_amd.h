@@ -397,6 +397,18 @@ typedef struct rsfm_real_camera {
 int rsfm_undistort_in_place(float* px, float* py, size_t nbPts, const rsfm_real_camera* camera, uint32_t nbIters,
                             hipStream_t stream);
 
+/* the same undistortion for gathered keypoint pairs, as makePtPair does it
+ * (ImagePair.cpp:105-112: the first point of each pair with the first image's
+ * camera, the second with the second's); a side whose distortion is all zero
+ * is left untouched; many pairs per launch */
+typedef struct rsfm_undistort_task {
+    rsfm_pt_pair* pts;          /* nbPts keypoint pairs, overwritten, 16-byte aligned (device) */
+    uint32_t nbPts;
+    const uint32_t* nbPtsDev;   /* optional device count (capped at nbPts) */
+    rsfm_real_camera cam0, cam1;
+} rsfm_undistort_task;
+int rsfm_undistort_pt_pairs(const rsfm_undistort_task* tasks, size_t nbTasks, uint32_t nbIters, hipStream_t stream);
+
 /* deterministic device sampler (no reference counterpart: the reference's
  * engine is random_device-seeded, ransac.hpp:117): hypothesis h draws 8
  * distinct indices from splitmix64 seeded with seed ^ h * 0xD1B54A32D192ED03,
@@ -463,6 +475,11 @@ typedef struct rsfm_pipeline_pair {
     float* posesH;              /* [8][12] */
     float* normalsH;            /* optional [8][3] */
     uint32_t* poseResultH;      /* [19] */
+    /* optional host [10]: k1, k2, p1, p2, k3 of camera 0, then of camera 1 (read at
+     * creation); when set, the gathered keypoint pairs are undistorted in place with
+     * cam0/cam1 (rsfm_undistort_pt_pairs, 2 Gauss-Newton steps as the reference's
+     * makePtPair) before sampling */
+    const float* distortion;
 } rsfm_pipeline_pair;
 typedef struct rsfm_pipeline_config {
     uint32_t nbHyp;            /* hypotheses per pair */

@@ -103,6 +103,11 @@ class RealCamera(ctypes.Structure):
     _fields_ = [("pinHole", Camera), ("distortion", c_float * 5)]
 
 
+class UndistortTask(ctypes.Structure):
+    _fields_ = [("pts", c_void_p), ("nbPts", c_u32), ("nbPtsDev", c_void_p), ("cam0", RealCamera),
+                ("cam1", RealCamera)]
+
+
 class PoseTask(ctypes.Structure):
     _fields_ = [
         ("F", c_void_p),
@@ -182,6 +187,7 @@ _SIGS = {
     "rsfm_pose_from_f": (c_int, [c_void_p, c_size, c_float, c_void_p]),
     "rsfm_pose_from_h": (c_int, [c_void_p, c_size, c_float, c_void_p]),
     "rsfm_undistort_in_place": (c_int, [c_void_p, c_void_p, c_size, c_void_p, c_u32, c_void_p]),
+    "rsfm_undistort_pt_pairs": (c_int, [c_void_p, c_size, c_u32, c_void_p]),
     "rsfm_degensac_refine": (
         c_int, [c_void_p, c_size, c_float, c_float, c_int, c_u32, c_u32, ctypes.c_uint64, c_void_p]
     ),

@@ -97,6 +97,26 @@ int launch_score7_table(const DevTask* tasks, uint32_t nTasks, uint32_t maxH, fl
                         float minSepSq0, float minSepSq1, hipStream_t stream);
 }  // namespace ransac
 
+namespace undist {
+struct Params {
+    float fx, fy, cx, cy;      // pinhole
+    float ifx, ify, icx, icy;  // inverse pinhole: 1/f, -c/f
+    float k1, k2, p1, p2, k3;
+    uint32_t iters;
+};
+struct PairTask {
+    float4* pts;
+    const uint32_t* nDev;
+    uint32_t nHost;
+    int on0, on1;  // side has nonzero distortion
+    Params d0, d1;
+};
+// argument checks of rsfm_undistort_in_place; active = any nonzero distortion parameter
+int make_params(const rsfm_real_camera& camera, uint32_t nbIters, Params& d, bool& active);
+// one launch over a device table of keypoint-pair tasks
+int launch_pairs_table(const PairTask* tasks, uint32_t nTasks, uint32_t maxN, hipStream_t stream);
+}  // namespace undist
+
 namespace degensac {
 struct DevTask {
     const float4* pts;

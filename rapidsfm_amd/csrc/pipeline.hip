@@ -23,6 +23,9 @@
 //    inlier mask) are captured once into a HIP graph on a private stream;
 //    rsfm_pipeline_run launches that graph. Buffers are cleared by a fill
 //    kernel (no memset nodes).
+//  * Pairs that carry lens distortion have their gathered keypoint pairs
+//    undistorted in place right after the filter (one launch over a device
+//    table), as the reference's makePtPair does before solving.
 //  * With the DEGENSAC solver the verification stages are the reference's
 //    default pair solver instead: 7-point samples, the 7-point scorer, the
 //    DEGENSAC finisher (F and H with their inlier masks) and, optionally, the
@@ -82,6 +85,8 @@ struct rsfm_pipeline {
     ransac::DevTask* dF7 = nullptr;
     degensac::DevTask* dDg = nullptr;
     degensac::Params dgParams{};
+    undist::PairTask* dUnd = nullptr;  // pairs with distortion
+    uint32_t nUnd = 0, maxUnd = 0;
     std::vector<rsfm_pose_task> poseF;
     std::vector<rsfm_pose_h_task> poseH;
     hipGraph_t graph = nullptr;
@@ -269,6 +274,7 @@ int capture(rsfm_pipeline* pl)
     RSFM_CHECK(check_hip(hipStreamBeginCapture(pl->stream, hipStreamCaptureModeThreadLocal), "begin capture"));
     int rc = enqueue_match(pl, pl->stream);
     if (rc == RSFM_OK) rc = aux::launch_filter_table(pl->dFilter, P, c.ratioNum, c.ratioDen, pl->stream);
+    if (rc == RSFM_OK) rc = undist::launch_pairs_table(pl->dUnd, pl->nUnd, pl->maxUnd, pl->stream);
     if (rc == RSFM_OK && c.nbHyp > 0) rc = ransac::launch_sample_table(pl->dSample, P, c.nbHyp, pl->stream);
     if (c.solver == RSFM_PIPELINE_SOLVER_F8) {
         if (rc == RSFM_OK)
@@ -386,6 +392,7 @@ extern "C" int rsfm_pipeline_create(const rsfm_pipeline_pair* pairs, size_t nbPa
     const size_t oRansac = take(P * sizeof(ransac::DevTask));
     const size_t oF7 = take(P * sizeof(ransac::DevTask));
     const size_t oDg = take(P * sizeof(degensac::DevTask));
+    const size_t oUnd = take(P * sizeof(undist::PairTask));
     const size_t oFill = take(std::max<size_t>(1, fill.size()) * sizeof(pipe::FillRange));
     const size_t oPart = take(std::max<size_t>(1, pl->partCap) * sizeof(uint64_t));
     rc = check_hip(hipMalloc(reinterpret_cast<void**>(&pl->arena), off), "pipeline workspace");
@@ -399,6 +406,7 @@ extern "C" int rsfm_pipeline_create(const rsfm_pipeline_pair* pairs, size_t nbPa
     pl->dRansac = reinterpret_cast<ransac::DevTask*>(pl->arena + oRansac);
     pl->dF7 = reinterpret_cast<ransac::DevTask*>(pl->arena + oF7);
     pl->dDg = reinterpret_cast<degensac::DevTask*>(pl->arena + oDg);
+    pl->dUnd = reinterpret_cast<undist::PairTask*>(pl->arena + oUnd);
     pl->dFill = reinterpret_cast<pipe::FillRange*>(pl->arena + oFill);
     pl->dPart = reinterpret_cast<uint64_t*>(pl->arena + oPart);
 
@@ -406,6 +414,7 @@ extern "C" int rsfm_pipeline_create(const rsfm_pipeline_pair* pairs, size_t nbPa
     std::vector<ransac::SampleTask> st(P);
     std::vector<ransac::DevTask> rt(P), f7;
     std::vector<degensac::DevTask> dg;
+    std::vector<undist::PairTask> ud;
     if (degensac &&
         (rc = degensac::make_params(cfg->threshold, cfg->thresholdH, cfg->mode, cfg->nbCandidates,
                                     cfg->nbPlaneSamples, cfg->seed, pl->dgParams)) != RSFM_OK)
@@ -419,6 +428,26 @@ extern "C" int rsfm_pipeline_create(const rsfm_pipeline_pair* pairs, size_t nbPa
         st[i] = ransac::SampleTask{x.sampleIdx, cfg->nbHyp, x.nA, x.count, cfg->seed + i, degensac ? 7u : 8u};
         rt[i] = ransac::DevTask{reinterpret_cast<const float4*>(x.pts), x.count, x.nA, cfg->nbHyp, x.sampleIdx,
                                 x.counts, nullptr, x.best, x.inlierMask};
+        if (x.distortion) {
+            rsfm_real_camera c0{x.cam0, {}}, c1{x.cam1, {}};
+            for (int k = 0; k < 5; k++) {
+                c0.distortion[k] = x.distortion[k];
+                c1.distortion[k] = x.distortion[5 + k];
+            }
+            undist::PairTask u{};
+            bool a0 = false, a1 = false;
+            if ((rc = undist::make_params(c0, 2, u.d0, a0)) != RSFM_OK || (rc = undist::make_params(c1, 2, u.d1, a1)) != RSFM_OK)
+                return fail_out(rc);
+            if (a0 || a1) {
+                u.pts = reinterpret_cast<float4*>(x.pts);
+                u.nDev = x.count;
+                u.nHost = x.nA;
+                u.on0 = a0;
+                u.on1 = a1;
+                ud.push_back(u);
+                pl->maxUnd = std::max(pl->maxUnd, x.nA);
+            }
+        }
         if (!degensac) continue;
         f7.push_back(ransac::DevTask{reinterpret_cast<const float4*>(x.pts), x.count, x.nA, cfg->nbHyp, x.sampleIdx,
                                      x.counts, nullptr, x.best, nullptr});
@@ -439,8 +468,10 @@ extern "C" int rsfm_pipeline_create(const rsfm_pipeline_pair* pairs, size_t nbPa
         (rc = upload(pl->dRansac, rt.data(), P * sizeof(ransac::DevTask))) != RSFM_OK ||
         (rc = upload(pl->dF7, f7.data(), f7.size() * sizeof(ransac::DevTask))) != RSFM_OK ||
         (rc = upload(pl->dDg, dg.data(), dg.size() * sizeof(degensac::DevTask))) != RSFM_OK ||
+        (rc = upload(pl->dUnd, ud.data(), ud.size() * sizeof(undist::PairTask))) != RSFM_OK ||
         (rc = upload(pl->dFill, fill.data(), fill.size() * sizeof(pipe::FillRange))) != RSFM_OK)
         return fail_out(rc);
+    pl->nUnd = (uint32_t)ud.size();
     if ((rc = tune(pl)) != RSFM_OK) return fail_out(rc);
     if ((rc = capture(pl)) != RSFM_OK) return fail_out(rc);
     *out = pl;

@@ -8,7 +8,7 @@ import ctypes
 
 import torch
 
-from ._abi import BfTask, BfTop2Task, Camera, DegensacTask, PoseHTask, PoseTask, RansacTask, RealCamera, check, lib
+from ._abi import BfTask, BfTop2Task, Camera, DegensacTask, PoseHTask, PoseTask, RansacTask, RealCamera, UndistortTask, check, lib
 
 
 def _stream(stream):
@@ -149,6 +149,22 @@ def undistort_in_place(px, py, cam, distortion, n_iters=2, stream=None):
     rc = RealCamera(Camera(*[float(v) for v in cam]), (ctypes.c_float * 5)(*[float(v) for v in distortion]))
     check(lib.rsfm_undistort_in_place(_ptr(px), _ptr(py), px.numel(), ctypes.byref(rc), int(n_iters), _stream(stream)),
           "rsfm_undistort_in_place")
+
+
+def undistort_pt_pairs(problems, n_iters=2, stream=None):
+    """Undistort gathered keypoint pairs in place: problems of (pts float32 CUDA (n, 4), cam0, cam1, dist0,
+    dist1[, nDev]) with cameras (fx, fy, cx, cy) and distortions (k1, k2, p1, p2, k3)."""
+    tasks = (UndistortTask * max(1, len(problems)))()
+    for i, prob in enumerate(problems):
+        pts, c0, c1, k0, k1 = prob[:5]
+        ndev = prob[5] if len(prob) > 5 else None
+        if pts.dtype != torch.float32 or pts.dim() != 2 or pts.shape[1] != 4 or not pts.is_contiguous():
+            raise ValueError("pts must be contiguous float32 (n, 4)")
+        mk = lambda c, k: RealCamera(Camera(*[float(v) for v in c]), (ctypes.c_float * 5)(*[float(v) for v in k]))  # noqa: E731
+        tasks[i] = UndistortTask(_ptr(pts), pts.shape[0], _ptr(ndev) if ndev is not None else None, mk(c0, k0),
+                                 mk(c1, k1))
+    check(lib.rsfm_undistort_pt_pairs(ctypes.cast(tasks, ctypes.c_void_p), len(problems), int(n_iters),
+                                      _stream(stream)), "rsfm_undistort_pt_pairs")
 
 
 def pose_from_f(problems, front_ratio=0.95, stream=None):

@@ -21,7 +21,7 @@ class PipelinePair(ctypes.Structure):
                 ("pts", c_vp), ("sampleIdx", c_vp), ("counts", c_vp), ("best", c_vp), ("inlierMask", c_vp),
                 ("F", c_vp), ("H", c_vp), ("solverResult", c_vp), ("inlierMaskH", c_vp), ("cam0", Camera),
                 ("cam1", Camera), ("posesF", c_vp), ("poseResultF", c_vp), ("posesH", c_vp), ("normalsH", c_vp),
-                ("poseResultH", c_vp)]
+                ("poseResultH", c_vp), ("distortion", c_vp)]
 
 
 class PipelineConfig(ctypes.Structure):
@@ -62,17 +62,22 @@ class Pipeline:
     maskH (nA,) are added; with cams (a list of (cam0, cam1) as (fx, fy, cx, cy)
     per pair) also posesF (4,12), poseF (10,), posesH (8,12), normalsH (8,3),
     poseH (19,).
+
+    distortion: optional list (one entry per pair, None for none) of 10 floats,
+    k1, k2, p1, p2, k3 of camera 0 then camera 1; those pairs' gathered keypoint
+    pairs are undistorted (with cams, required then) before sampling.
     """
 
     def __init__(self, pairs, n_hyp=4096, threshold=1.0, mode=1, ratio=(4, 5), seed=77, min_sep_sq=(0.0, 0.0),
                  seg_cols=0, wg_per_xcd=0, solver="f8", threshold_h=3.0, n_candidates=8, n_plane_samples=256,
-                 front_ratio=0.95, cams=None):
+                 front_ratio=0.95, cams=None, distortion=None):
         dev = pairs[0][0].device
         dg = solver == "degensac"
         k = 7 if dg else 8
         self.inputs = pairs
         self.out = []
         arr = (PipelinePair * len(pairs))()
+        keep = []  # host distortion arrays, read by the creation call
         for i, (dA, dB, kA, kB) in enumerate(pairs):
             nA, nB = dA.shape[0], dB.shape[0]
             o = {
@@ -100,12 +105,19 @@ class Pipeline:
                              poseH=torch.zeros(19, dtype=torch.int32, device=dev))
             p = lambda t: t.data_ptr()  # noqa: E731
             q = lambda key: o[key].data_ptr() if key in o else None  # noqa: E731
-            c0, c1 = cams[i] if (dg and cams is not None) else ((1.0, 1.0, 0.0, 0.0), (1.0, 1.0, 0.0, 0.0))
+            c0, c1 = cams[i] if cams is not None else ((1.0, 1.0, 0.0, 0.0), (1.0, 1.0, 0.0, 0.0))
+            dist = None
+            if distortion is not None and distortion[i] is not None:
+                if cams is None:
+                    raise ValueError("distortion needs cams")
+                dist = (ctypes.c_float * 10)(*[float(v) for v in distortion[i]])
+                keep.append(dist)
             arr[i] = PipelinePair(nA, nB, p(dA), p(dB), p(kA), p(kB), p(o["bestA"]), p(o["secA"]), p(o["bestB"]),
                                   p(o["matches"]), p(o["count"]), p(o["pts"]), p(o["idx"]), p(o["counts"]),
                                   p(o["best"]), p(o["mask"]), q("F"), q("H"), q("result"), q("maskH"),
                                   Camera(*[float(v) for v in c0]), Camera(*[float(v) for v in c1]), q("posesF"),
-                                  q("poseF"), q("posesH"), q("normalsH"), q("poseH"))
+                                  q("poseF"), q("posesH"), q("normalsH"), q("poseH"),
+                                  ctypes.cast(dist, c_vp) if dist is not None else None)
             self.out.append(o)
         cfg = PipelineConfig(n_hyp, threshold, mode, ratio[0], ratio[1], seed, min_sep_sq[0], min_sep_sq[1], seg_cols,
                              wg_per_xcd, SOLVERS[solver], threshold_h, n_candidates, n_plane_samples, front_ratio)

@@ -21,6 +21,7 @@ PAIRS = [
     ("rsfm_pose_task", "_abi", "PoseTask"),
     ("rsfm_pose_h_task", "_abi", "PoseHTask"),
     ("rsfm_real_camera", "_abi", "RealCamera"),
+    ("rsfm_undistort_task", "_abi", "UndistortTask"),
     ("rsfm_filter_task", "_abi", "FilterTask"),
     ("rsfm_sample_task", "_abi", "SampleTask"),
     ("rsfm_weighted_sample_task", "_abi", "WSampleTask"),

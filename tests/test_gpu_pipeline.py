@@ -274,3 +274,45 @@ def test_pipeline_solver_arguments(dev):
                 "create")
     with pytest.raises(KeyError):
         Pipeline(pairs, n_hyp=64, solver="lo-ransac")
+
+
+def test_pipeline_undistorts_pairs(orc, dev):
+    from rapidsfm_amd import ops
+    from rapidsfm_amd.pipeline import Pipeline
+
+    _say("pipeline with lens distortion")
+    sizes = [(1200, 1300), (800, 900), (2000, 1500), (300, 0)]
+    pairs = _batch(dev, sizes, seed=61)
+    dist = [(-0.05, 0.01, 0.0, 0.0005, 0.0, 0.03, 0.0, 0.0002, 0.0, 0.001), None,
+            (0.0,) * 5 + (-0.02, 0.0, 0.0, 0.0, 0.0), (-0.05,) + (0.0,) * 9]
+    H = 256
+    pl = Pipeline(pairs, n_hyp=H, seed=5, cams=[(CAM, CAM)] * len(pairs), distortion=dist, seg_cols=2048,
+                  wg_per_xcd=32)
+    pl.run()
+    outs = _matched(pairs)
+    probs = []
+    for o, d in zip(outs, dist):
+        if d is not None:
+            probs.append((o["pts"], CAM, CAM, d[:5], d[5:], o["count"]))
+    ops.undistort_pt_pairs(probs, 2)
+    _sample(pairs, outs, H, 5, 8)
+    r = ops.ransac_f8([(o["pts"], o["idx"], o["count"]) for o in outs], 1.0, mode=1)
+    torch.cuda.synchronize()
+    for i, (o, ref, x) in enumerate(zip(pl.out, outs, r)):
+        c = int(ref["count"].item())
+        assert int(o["count"].item()) == c, i
+        assert torch.equal(o["pts"][:c], ref["pts"][:c]), i
+        assert torch.equal(o["counts"], x["counts"]) and torch.equal(o["best"], x["best"]), i
+    # pair 0 against the oracle: the gathered raw keypoints, undistorted on the CPU
+    dA, dB, kA, kB = (t.cpu().numpy() for t in pairs[0])
+    ra, sa, rb = orc.bf_match_top2_u8(dA, dB)
+    m = orc.filter_matches(ra, sa, rb, (4, 5))
+    raw = np.concatenate([kA[m[:, 0]], kB[m[:, 1]]], 1).astype(np.float32)
+    cam = np.array(CAM, np.float32)
+    e0 = orc.undistort(raw[:, 0], raw[:, 1], cam, np.array(dist[0][:5], np.float32), 2)
+    e1 = orc.undistort(raw[:, 2], raw[:, 3], cam, np.array(dist[0][5:], np.float32), 2)
+    want = np.column_stack([e0[0], e0[1], e1[0], e1[1]]).astype(np.float32)
+    got = pl.out[0]["pts"][:len(m)].cpu().numpy()
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+    assert np.abs(got - raw).max() > 1.0
+    pl.close()
