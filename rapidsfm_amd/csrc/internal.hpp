@@ -117,6 +117,26 @@ int make_params(const rsfm_real_camera& camera, uint32_t nbIters, Params& d, boo
 int launch_pairs_table(const PairTask* tasks, uint32_t nTasks, uint32_t maxN, hipStream_t stream);
 }  // namespace undist
 
+namespace pose {
+struct DevTask {
+    const float* M;          // F or H
+    float cam0[4], cam1[4];  // fx, fy, cx, cy
+    const float4* pts;
+    const uint32_t* nDev;
+    uint32_t nHost;
+    const uint8_t* mask;
+    float* poses;
+    float* normals;  // H only, optional
+    uint32_t* result;
+};
+// argument checks of rsfm_pose_from_f / rsfm_pose_from_h for one task
+int make_task(const rsfm_pose_task& t, DevTask& d);
+int make_task(const rsfm_pose_h_task& t, DevTask& d);
+// one workgroup per task of a device table; kind 0 = F, 1 = H
+int check_ratio(float frontRatio);
+int launch_pose_table(const DevTask* tasks, uint32_t nTasks, int kind, float frontRatio, hipStream_t stream);
+}  // namespace pose
+
 namespace degensac {
 struct DevTask {
     const float4* pts;

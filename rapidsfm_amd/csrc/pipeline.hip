@@ -30,9 +30,9 @@
 //    default pair solver instead: 7-point samples, the 7-point scorer, the
 //    DEGENSAC finisher (F and H with their inlier masks) and, optionally, the
 //    relative pose from F and from H, captured into the same graph. The
-//    7-point scorer and the finisher read device task tables, so every pair
-//    runs in one launch (one finisher workgroup per pair); the pose entries
-//    take task batches by value (32 pairs per launch).
+//    7-point scorer, the finisher and the pose stages read device task
+//    tables, so each covers every pair in one launch (one finisher or pose
+//    workgroup per pair).
 #include "internal.hpp"
 #include <algorithm>
 #include <numeric>
@@ -87,8 +87,9 @@ struct rsfm_pipeline {
     degensac::Params dgParams{};
     undist::PairTask* dUnd = nullptr;  // pairs with distortion
     uint32_t nUnd = 0, maxUnd = 0;
-    std::vector<rsfm_pose_task> poseF;
-    std::vector<rsfm_pose_h_task> poseH;
+    pose::DevTask* dPoseF = nullptr;
+    pose::DevTask* dPoseH = nullptr;
+    uint32_t nPoseF = 0, nPoseH = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 };
@@ -285,10 +286,10 @@ int capture(rsfm_pipeline* pl)
             rc = ransac::launch_score7_table(pl->dF7, P, c.nbHyp, c.threshold, c.mode, c.minSepSq0, c.minSepSq1,
                                              pl->stream);
         if (rc == RSFM_OK) rc = degensac::launch_degensac_table(pl->dDg, P, pl->dgParams, c.mode, pl->stream);
-        if (rc == RSFM_OK && !pl->poseF.empty())
-            rc = rsfm_pose_from_f(pl->poseF.data(), pl->poseF.size(), c.frontRatio, pl->stream);
-        if (rc == RSFM_OK && !pl->poseH.empty())
-            rc = rsfm_pose_from_h(pl->poseH.data(), pl->poseH.size(), c.frontRatio, pl->stream);
+        if (rc == RSFM_OK && pl->nPoseF)
+            rc = pose::launch_pose_table(pl->dPoseF, pl->nPoseF, 0, c.frontRatio, pl->stream);
+        if (rc == RSFM_OK && pl->nPoseH)
+            rc = pose::launch_pose_table(pl->dPoseH, pl->nPoseH, 1, c.frontRatio, pl->stream);
     }
     hipGraph_t g = nullptr;
     const int rc2 = check_hip(hipStreamEndCapture(pl->stream, &g), "end capture");
@@ -393,6 +394,8 @@ extern "C" int rsfm_pipeline_create(const rsfm_pipeline_pair* pairs, size_t nbPa
     const size_t oF7 = take(P * sizeof(ransac::DevTask));
     const size_t oDg = take(P * sizeof(degensac::DevTask));
     const size_t oUnd = take(P * sizeof(undist::PairTask));
+    const size_t oPoseF = take(P * sizeof(pose::DevTask));
+    const size_t oPoseH = take(P * sizeof(pose::DevTask));
     const size_t oFill = take(std::max<size_t>(1, fill.size()) * sizeof(pipe::FillRange));
     const size_t oPart = take(std::max<size_t>(1, pl->partCap) * sizeof(uint64_t));
     rc = check_hip(hipMalloc(reinterpret_cast<void**>(&pl->arena), off), "pipeline workspace");
@@ -407,6 +410,8 @@ extern "C" int rsfm_pipeline_create(const rsfm_pipeline_pair* pairs, size_t nbPa
     pl->dF7 = reinterpret_cast<ransac::DevTask*>(pl->arena + oF7);
     pl->dDg = reinterpret_cast<degensac::DevTask*>(pl->arena + oDg);
     pl->dUnd = reinterpret_cast<undist::PairTask*>(pl->arena + oUnd);
+    pl->dPoseF = reinterpret_cast<pose::DevTask*>(pl->arena + oPoseF);
+    pl->dPoseH = reinterpret_cast<pose::DevTask*>(pl->arena + oPoseH);
     pl->dFill = reinterpret_cast<pipe::FillRange*>(pl->arena + oFill);
     pl->dPart = reinterpret_cast<uint64_t*>(pl->arena + oPart);
 
@@ -415,6 +420,7 @@ extern "C" int rsfm_pipeline_create(const rsfm_pipeline_pair* pairs, size_t nbPa
     std::vector<ransac::DevTask> rt(P), f7;
     std::vector<degensac::DevTask> dg;
     std::vector<undist::PairTask> ud;
+    std::vector<pose::DevTask> pf, ph;
     if (degensac &&
         (rc = degensac::make_params(cfg->threshold, cfg->thresholdH, cfg->mode, cfg->nbCandidates,
                                     cfg->nbPlaneSamples, cfg->seed, pl->dgParams)) != RSFM_OK)
@@ -456,12 +462,20 @@ extern "C" int rsfm_pipeline_create(const rsfm_pipeline_pair* pairs, size_t nbPa
                                                          x.H, x.solverResult, x.inlierMask, x.inlierMaskH},
                                       dg.back())) != RSFM_OK)
             return fail_out(rc);
-        if (x.posesF)
-            pl->poseF.push_back(rsfm_pose_task{x.F, x.cam0, x.cam1, x.pts, x.nA, x.count, x.inlierMask, x.posesF,
-                                               x.poseResultF});
-        if (x.posesH)
-            pl->poseH.push_back(rsfm_pose_h_task{x.H, x.cam0, x.cam1, x.pts, x.nA, x.count, x.inlierMaskH, x.posesH,
-                                                 x.normalsH, x.poseResultH});
+        if (x.posesF) {
+            pf.emplace_back();
+            if ((rc = pose::make_task(rsfm_pose_task{x.F, x.cam0, x.cam1, x.pts, x.nA, x.count, x.inlierMask,
+                                                     x.posesF, x.poseResultF},
+                                      pf.back())) != RSFM_OK)
+                return fail_out(rc);
+        }
+        if (x.posesH) {
+            ph.emplace_back();
+            if ((rc = pose::make_task(rsfm_pose_h_task{x.H, x.cam0, x.cam1, x.pts, x.nA, x.count, x.inlierMaskH,
+                                                       x.posesH, x.normalsH, x.poseResultH},
+                                      ph.back())) != RSFM_OK)
+                return fail_out(rc);
+        }
     }
     if ((rc = upload(pl->dFilter, ft.data(), P * sizeof(aux::FilterTask))) != RSFM_OK ||
         (rc = upload(pl->dSample, st.data(), P * sizeof(ransac::SampleTask))) != RSFM_OK ||
@@ -469,9 +483,14 @@ extern "C" int rsfm_pipeline_create(const rsfm_pipeline_pair* pairs, size_t nbPa
         (rc = upload(pl->dF7, f7.data(), f7.size() * sizeof(ransac::DevTask))) != RSFM_OK ||
         (rc = upload(pl->dDg, dg.data(), dg.size() * sizeof(degensac::DevTask))) != RSFM_OK ||
         (rc = upload(pl->dUnd, ud.data(), ud.size() * sizeof(undist::PairTask))) != RSFM_OK ||
+        (rc = upload(pl->dPoseF, pf.data(), pf.size() * sizeof(pose::DevTask))) != RSFM_OK ||
+        (rc = upload(pl->dPoseH, ph.data(), ph.size() * sizeof(pose::DevTask))) != RSFM_OK ||
         (rc = upload(pl->dFill, fill.data(), fill.size() * sizeof(pipe::FillRange))) != RSFM_OK)
         return fail_out(rc);
     pl->nUnd = (uint32_t)ud.size();
+    pl->nPoseF = (uint32_t)pf.size();
+    pl->nPoseH = (uint32_t)ph.size();
+    if (degensac && (rc = pose::check_ratio(cfg->frontRatio)) != RSFM_OK) return fail_out(rc);
     if ((rc = tune(pl)) != RSFM_OK) return fail_out(rc);
     if ((rc = capture(pl)) != RSFM_OK) return fail_out(rc);
     *out = pl;

@@ -35,17 +35,6 @@ namespace pose {
 constexpr int kThreads = 256;
 constexpr int kMaxTasks = 32;
 
-struct DevTask {
-    const float* M;  // F or H
-    float cam0[4], cam1[4];  // fx, fy, cx, cy
-    const float4* pts;
-    const uint32_t* nDev;
-    uint32_t nHost;
-    const uint8_t* mask;
-    float* poses;
-    float* normals;  // H only, optional
-    uint32_t* result;
-};
 typedef ByValue<DevTask, kMaxTasks> Batch;
 
 __device__ inline uint32_t task_n(const DevTask& T) { return T.nDev ? min(*T.nDev, T.nHost) : T.nHost; }
@@ -291,8 +280,8 @@ __device__ inline bool in_front(const float P[12], float4 p, const float inv0[4]
 }
 
 // kKind 0: F (4 candidates, result [10]); 1: H (up to 8, result [19])
-template <int kKind>
-__global__ __launch_bounds__(kThreads) void pose_kernel(const Batch batch, float ratio)
+template <int kKind, class Src>
+__global__ __launch_bounds__(kThreads) void pose_kernel(const Src batch, float ratio)
 {
     constexpr int kC = kKind ? 8 : 4;
     const DevTask& T = batch[blockIdx.x];
@@ -384,45 +373,73 @@ __global__ __launch_bounds__(kThreads) void pose_kernel(const Batch batch, float
 
 using namespace rsfm_amd;
 
+namespace rsfm_amd {
+namespace pose {
+
+template <int kKind, class Task>
+int make_task_t(const Task& t, DevTask& d)
+{
+    d = DevTask{};
+    if constexpr (kKind == 0) {
+        d.M = t.F;
+    } else {
+        d.M = t.H;
+        d.normals = t.normals;
+    }
+    RSFM_RETURN_IF(d.M == nullptr || t.poses == nullptr || t.result == nullptr, RSFM_ERR_INVALID,
+                   "pose: the model, poses and result are required");
+    RSFM_RETURN_IF(t.nbPts > 0 && t.pts == nullptr, RSFM_ERR_INVALID, "pose: pts is null");
+    RSFM_RETURN_IF(reinterpret_cast<uintptr_t>(t.pts) % 16, RSFM_ERR_INVALID, "pose: pts must be 16-byte aligned");
+    RSFM_RETURN_IF(!(t.cam0.fx != 0.f) || !(t.cam0.fy != 0.f) || !(t.cam1.fx != 0.f) || !(t.cam1.fy != 0.f),
+                   RSFM_ERR_INVALID, "pose: focal lengths must be nonzero");
+    const float c0[4] = {t.cam0.fx, t.cam0.fy, t.cam0.cx, t.cam0.cy};
+    const float c1[4] = {t.cam1.fx, t.cam1.fy, t.cam1.cx, t.cam1.cy};
+    for (int k = 0; k < 4; k++) {
+        d.cam0[k] = c0[k];
+        d.cam1[k] = c1[k];
+    }
+    d.pts = reinterpret_cast<const float4*>(t.pts);
+    d.nDev = t.nbPtsDev;
+    d.nHost = t.nbPts;
+    d.mask = t.inlierMask;
+    d.poses = t.poses;
+    d.result = t.result;
+    return RSFM_OK;
+}
+
+int make_task(const rsfm_pose_task& t, DevTask& d) { return make_task_t<0>(t, d); }
+int make_task(const rsfm_pose_h_task& t, DevTask& d) { return make_task_t<1>(t, d); }
+
+int check_ratio(float frontRatio)
+{
+    RSFM_RETURN_IF(!(frontRatio >= 0.f) || frontRatio > 1.f, RSFM_ERR_INVALID, "pose: frontRatio must be in [0, 1]");
+    return RSFM_OK;
+}
+
+int launch_pose_table(const DevTask* tasks, uint32_t nTasks, int kind, float frontRatio, hipStream_t stream)
+{
+    RSFM_CHECK(check_ratio(frontRatio));
+    if (nTasks == 0) return RSFM_OK;
+    const InTable<DevTask> src{tasks};
+    if (kind == 0) pose_kernel<0><<<nTasks, kThreads, 0, stream>>>(src, frontRatio);
+    else pose_kernel<1><<<nTasks, kThreads, 0, stream>>>(src, frontRatio);
+    return check_launch("pose table");
+}
+
+}  // namespace pose
+}  // namespace rsfm_amd
+
 namespace {
 template <int kKind, class Task>
 int launch_pose(const Task* tasks, size_t nbTasks, float frontRatio, hipStream_t stream, const char* what)
 {
     using namespace pose;
     RSFM_RETURN_IF(nbTasks > 0 && tasks == nullptr, RSFM_ERR_INVALID, "pose: tasks is null");
-    RSFM_RETURN_IF(!(frontRatio >= 0.f) || frontRatio > 1.f, RSFM_ERR_INVALID, "pose: frontRatio must be in [0, 1]");
+    RSFM_CHECK(check_ratio(frontRatio));
     for (size_t b = 0; b < nbTasks; b += kMaxTasks) {
         const size_t n = std::min<size_t>(kMaxTasks, nbTasks - b);
         Batch batch{};
-        for (size_t i = 0; i < n; i++) {
-            const Task& t = tasks[b + i];
-            DevTask d{};
-            if constexpr (kKind == 0) {
-                d.M = t.F;
-            } else {
-                d.M = t.H;
-                d.normals = t.normals;
-            }
-            RSFM_RETURN_IF(d.M == nullptr || t.poses == nullptr || t.result == nullptr, RSFM_ERR_INVALID,
-                           "pose: the model, poses and result are required");
-            RSFM_RETURN_IF(t.nbPts > 0 && t.pts == nullptr, RSFM_ERR_INVALID, "pose: pts is null");
-            RSFM_RETURN_IF(reinterpret_cast<uintptr_t>(t.pts) % 16, RSFM_ERR_INVALID, "pose: pts must be 16-byte aligned");
-            RSFM_RETURN_IF(!(t.cam0.fx != 0.f) || !(t.cam0.fy != 0.f) || !(t.cam1.fx != 0.f) || !(t.cam1.fy != 0.f),
-                           RSFM_ERR_INVALID, "pose: focal lengths must be nonzero");
-            const float c0[4] = {t.cam0.fx, t.cam0.fy, t.cam0.cx, t.cam0.cy};
-            const float c1[4] = {t.cam1.fx, t.cam1.fy, t.cam1.cx, t.cam1.cy};
-            for (int k = 0; k < 4; k++) {
-                d.cam0[k] = c0[k];
-                d.cam1[k] = c1[k];
-            }
-            d.pts = reinterpret_cast<const float4*>(t.pts);
-            d.nDev = t.nbPtsDev;
-            d.nHost = t.nbPts;
-            d.mask = t.inlierMask;
-            d.poses = t.poses;
-            d.result = t.result;
-            batch.t[i] = d;
-        }
+        for (size_t i = 0; i < n; i++) RSFM_CHECK(make_task(tasks[b + i], batch.t[i]));
         pose_kernel<kKind><<<(uint32_t)n, kThreads, 0, stream>>>(batch, frontRatio);
         RSFM_CHECK(check_launch(what));
     }
