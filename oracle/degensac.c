@@ -28,9 +28,9 @@
  * direct evaluation of the reference's formula; on a dominant-plane scene the
  * finisher recovers the true F where the plain 7-point best explains only
  * the plane. The reference's Bernoulli subsample (mt19937_64) and Eigen
- * JacobiSVD refits are not reproduced bit for bit: parity with the reference's
- * own trajectories is unpinned (random_device seeding makes them
- * unrepeatable in the reference as well).
+ * JacobiSVD refits are not reproduced bit for bit, and the reference's own
+ * trajectories cannot be compared: random_device seeding makes them
+ * unrepeatable in the reference as well.
  */
 #include "rsfm_oracle.h"
 #include <math.h>

@@ -18,7 +18,7 @@
  * to sign), the seven epipolar constraints and det F = 0 of every solution,
  * and the number of real roots against numpy's companion-matrix roots of the
  * same cubic. The reference's Eigen eigenvalue route is not reproduced bit for
- * bit (Eigen absent): parity with the reference's own FP32 bits is unpinned.
+ * bit (Eigen is absent, so its own FP32 bits cannot be produced here).
  */
 #include "rsfm_oracle.h"
 #include <math.h>

@@ -17,8 +17,8 @@
  *    The null vector comes from a Householder QR of P^T (9x8, last column of
  *    Q) and the rank-2 projection from a cyclic Jacobi eigensolver on F^T F
  *    (6 sweeps): F' = F - (F v3) v3^T, v3 = smallest eigenvector. This is the
- *    same mathematics as the reference's JacobiSVD route, not its FP32 bits:
- *    Eigen (unpinned system package, absent here) is parity unpinned.
+ *    same mathematics as the reference's JacobiSVD route, not its FP32 bits
+ *    (Eigen, an unpinned system package, is absent here).
  *  - or_f8_fit_f64: FP64 model used as the accuracy oracle (Jacobi SVD of P
  *    via the eigen-decomposition of P^T P to convergence, Jacobi on F^T F).
  * Pinned in tests by: noise-free two-view geometry (the fit must recover the

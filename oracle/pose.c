@@ -13,7 +13,7 @@
  * and has every point in front, the four rotations are orthonormal with det +1,
  * [t]x R is proportional to K1^T F K0; the triangulated points of noise-free
  * correspondences are the true 3D points. The reference's Eigen FP32 SVD is not
- * reproduced bit for bit (parity unpinned).
+ * reproduced bit for bit (Eigen is absent here).
  *
  * Homography branch: decomposeHomography (legacy/geometry.hpp:364-447) and
  * solveByHomography's transform (ImagePair.cpp:805-816), repeating

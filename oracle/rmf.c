@@ -21,8 +21,9 @@
  *  diff_r = ((M_r0*lx + M_r1*ly) + t_r) - right_r ; sq = diff_0^2 + diff_1^2.
  * Pinning: the LCG against glibc rand_r; Gauss elimination against numpy's
  * solver; binning and votes on hand-built grids with known answers
- * (tests/test_oracle_rmf.py). Against the reference's clock()-seeded, kmat-based
- * kernel the exact votes are parity unpinned (cuda-utils kmat is absent).
+ * (tests/test_oracle_rmf.py). The reference's kernel seeds its LCG from
+ * clock() and uses cuda-utils kmat (absent), so its exact votes cannot be
+ * reproduced.
  */
 #include "rsfm_oracle.h"
 #include <math.h>

@@ -9,7 +9,8 @@
  * expression (FP32, no contraction, Cramer 2x2 solve). Pinned in tests by
  * distortion of the result reproducing the input, an FP64 Newton solve run to
  * convergence, and the identity for zero distortion. The reference is built
- * with -ffast-math (distortion.cpp:17-19): its FP32 bits are parity unpinned.
+ * with -ffast-math (distortion.cpp:17-19), so its own FP32 bits are not
+ * reproducible.
  */
 #include "rsfm_oracle.h"
 #include <math.h>
