@@ -10,10 +10,10 @@
 //    w = b - 128 (columns) so one v_mfma_i32_32x32x32_i8 chain gives
 //    dot = u.w and  L2^2 = NA' + NB' + 2 dot  with NA' = |a-128|^2 and
 //    NB' = |b-127|^2 - 128 (both in [-128, 2^21]).
-//  * A workgroup keeps 256 query rows resident in registers (kRB 32-row MFMA
-//    blocks per wave, K = 128 = 4 MFMAs each) and streams the train
-//    descriptors of its column segment through LDS in 64-column chunks
-//    (double buffered, XOR-swizzled 16-B slots).
+//  * A workgroup keeps 256 query rows resident in LDS (one 32-row MFMA block
+//    per wave, K = 128 = 4 MFMAs each) and streams the train descriptors of
+//    its column segment through LDS in 64-column chunks (a ring of four,
+//    XOR-swizzled 16-B slots); two workgroups share a CU.
 //  * Epilogue per accumulator: one v_lshl_add makes the row key
 //    ((NB'_j + 2dot) << 8 | column-block tag), one makes the column key
 //    ((NA'_i + 2dot) << 8 | local row); v_min3 folds top-1, v_med3 + v_min
@@ -100,13 +100,18 @@ __device__ inline int swz(int c) { return ((c >> 1) & 7) ^ ((c & 1) << 2); }
 // the 8 waves; a stride of 72 puts those 64 words in distinct banks
 constexpr int kColStride = 72;
 
-// One workgroup = 8 waves x 32 query rows. The column segment streams through
-// LDS in 64-column chunks. The loop is software pipelined: in iteration c the
-// wave issues the 8 MFMAs of chunk c into one accumulator pair while the VALU
-// folds the keys of chunk c-1 from the other pair, so matrix and vector work of
-// one wave overlap; the body has no branches, so the scheduler can interleave
-// them. Column partials of chunk c-1 are written to LDS in iteration c and
-// merged to global memory by all waves (8 columns each) in iteration c+1.
+// One workgroup = 8 waves x 32 query rows. The query tile (operands and
+// column-key offsets) sits in LDS and the column segment streams through an
+// LDS ring in 64-column chunks. Per chunk a wave issues its 8 MFMAs (K-step
+// major over the two 32-column blocks, operands read from LDS per K-step) into
+// one accumulator pair, then folds the keys of that chunk. With one
+// accumulator pair and streamed operands a wave fits in 128 VGPRs, so two
+// workgroups (four waves per SIMD) share a CU and the waves overlap each
+// other's matrix and vector work; the earlier layout software-pipelined two
+// accumulator pairs inside each wave at 210 VGPRs and two waves per SIMD, and
+// ran 10% slower on top-2. Column partials of chunk c are written to LDS in
+// iteration c and merged to global memory by all waves (8 columns each) after
+// the next barrier.
 template <int kBits, bool kTop2>
 __device__ __forceinline__ void match_tile(const DevTask& T, uint32_t rowBase, uint32_t colBeg, uint32_t seg)
 {
@@ -117,24 +122,22 @@ __device__ __forceinline__ void match_tile(const DevTask& T, uint32_t rowBase, u
     const uint32_t colEnd = min(colBeg + T.segCols, T.nB);
     const uint32_t nChunks = (colEnd - colBeg + kBN - 1) / kBN;
 
-    // ring of staged chunks; the row-tail transpose reuses its memory
     constexpr int kRing = 4, kColSlots = 4, kPSlots = 8;
     constexpr int kTailStride = 33;
     constexpr int kUnionBytes = (kRing * kChunkBytes > kBM * kTailStride * 4) ? kRing * kChunkBytes : kBM * kTailStride * 4;
     __shared__ __attribute__((aligned(16))) uint8_t sU[kUnionBytes];
-    __shared__ int32_t sP[kPSlots][kBN];  // per column: NB' << 8 | column block (kNone past the end)
+    __shared__ __attribute__((aligned(16))) uint8_t sA[kBM * 128];
+    __shared__ int32_t sP[kPSlots][kBN];
     __shared__ int32_t sCol[kColSlots][kWaves][kColStride];
-    __shared__ int32_t sNA[kBM];
+    __shared__ __attribute__((aligned(16))) int32_t sQ[kBM];  // NA' << 8 | local row, kNone past nA
 
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
     const int l32 = lane & 31, hi = lane >> 5;
+    const int rloc = wave * 32 + l32;
 
-    // ---- query rows: operands u = 127 - a, offsets NA' = |a - 128|^2
-    v4i aop[4];
-    int32_t Q[16];
+    // ---- query rows: operands u = 127 - a into LDS, offsets NA' = |a - 128|^2
     {
-        const int rloc = wave * 32 + l32;
         const uint32_t row = rowBase + rloc;
         const bool valid = row < T.nA;
         int32_t nap = 0;
@@ -149,17 +152,10 @@ __device__ __forceinline__ void match_tile(const DevTask& T, uint32_t rowBase, u
                 nap = __builtin_amdgcn_sdot4(xa, xa, nap, false);
                 u[q] = valid ? (x[q] ^ 0x7F7F7F7F) : 0;
             }
-            aop[kc] = u;
+            *reinterpret_cast<v4i*>(sA + rloc * 128 + (((2 * kc + hi) ^ swz(rloc)) << 4)) = u;
         }
         const int32_t na = nap + __shfl_xor(nap, 32);
-        if (hi == 0) sNA[rloc] = na;
-#pragma unroll
-        for (int v = 0; v < 16; v++) {
-            const int r = 8 * (v >> 2) + 4 * hi + (v & 3);
-            const int32_t nav = __shfl(na, r);
-            const int rl = wave * 32 + r;
-            Q[v] = rowBase + rl < T.nA ? (nav * 256 + rl) : kNone;
-        }
+        if (hi == 0) sQ[rloc] = valid ? (na * 256 + rloc) : kNone;
     }
 
     int32_t m1[16], m2[16];
@@ -169,14 +165,8 @@ __device__ __forceinline__ void match_tile(const DevTask& T, uint32_t rowBase, u
         m2[v] = kNone;
     }
 
-    // ---- staging: thread -> 16 bytes (slot sp) of chunk column sc. Chunks live
-    // in a ring of kRing LDS slots; chunk c is staged while chunk c-2 is being
-    // processed and the workgroup synchronises after every odd chunk. Between
-    // the staging of a slot and its next read, and between its read and its
-    // next staging, there is then always one barrier.
     const int sc = tid >> 3, sp = tid & 7;
     auto stage_load = [&](uint32_t c) -> v4i {
-        // past the segment end: any valid column (the value is masked at the store)
         const uint32_t gcol = min(colBeg + c * kBN + sc, colEnd - 1);
         return load16<kBits>(T.B + (size_t)gcol * kDescBytes, sp * 16);
     };
@@ -191,14 +181,11 @@ __device__ __forceinline__ void match_tile(const DevTask& T, uint32_t rowBase, u
             acc = __builtin_amdgcn_sdot4(w[q], 0x02020202, acc, false);
         }
         *reinterpret_cast<v4i*>(sU + (c % kRing) * kChunkBytes + sc * 128 + ((sp ^ swz(sc)) << 4)) = w;
-        // sum over the 8 threads of the column: (w^2 + 2w) over 128 values = |b - 127|^2 - 128 = NB'
         acc += dpp_xor1(acc);
         acc += dpp_xor2(acc);
         acc += dpp_mirror8(acc);
-        // row-key offset NB' << 8 | column block; columns past the end never win
-        sP[c % kPSlots][sc] = valid ? (acc * 256 + (int32_t)(c * 2 + (sc >> 5))) : kNone;  // 8 threads, same value
+        sP[c % kPSlots][sc] = valid ? (acc * 256 + (int32_t)(c * 2 + (sc >> 5))) : kNone;
     };
-    // merged column minima of chunk k -> one 64-bit atomicMin per column (8 columns per wave)
     auto flush = [&](uint32_t k) {
         const int col = wave * 8 + (lane >> 3);
         int32_t x = sCol[k % kColSlots][lane & 7][col];
@@ -212,119 +199,86 @@ __device__ __forceinline__ void match_tile(const DevTask& T, uint32_t rowBase, u
             atomicMin(reinterpret_cast<unsigned long long*>(T.bestB + gcol), (unsigned long long)make_key(dist, row));
         }
     };
-    // keys of row v of one chunk: the row keys (NB'_j + 2 dot) << 8 | column
-    // block fold into the row state, the column keys (NA'_i + 2 dot) << 8 |
-    // local row into the running column minima cm0, cm1 of the lane
-    auto fold_row = [&](int v, const v16i (&acc)[2], const int32_t (&P)[2], int32_t& cm0, int32_t& cm1) {
-        const int32_t a0 = acc[0][v], a1 = acc[1][v];
-        const int32_t r0 = shl9_add(a0, P[0]);
-        const int32_t r1 = shl9_add(a1, P[1]);
-        if constexpr (kTop2)
-            top2_fold2(m1[v], m2[v], r0, r1);
-        else
-            m1[v] = imin(m1[v], imin(r0, r1));
-        cm0 = imin(cm0, shl9_add_asm(a0, Q[v]));
-        cm1 = imin(cm1, shl9_add_asm(a1, Q[v]));
-    };
-    // lanes l and l + 32 hold the same column: exchange the upper half of
-    // block 0 with the lower half of block 1, then lane l holds column l
-    auto put_cols = [&](int32_t cm0, int32_t cm1, uint32_t k) {
-        const auto sw = __builtin_amdgcn_permlane32_swap((uint32_t)cm0, (uint32_t)cm1, false, false);
-        sCol[k % kColSlots][wave][lane] = imin((int32_t)sw[0], (int32_t)sw[1]);
-    };
-    auto fold_keys = [&](const v16i (&acc)[2], const int32_t (&P)[2], uint32_t k) {
-        int32_t cm0 = kNone, cm1 = kNone;
-#pragma unroll
-        for (int v = 0; v < 16; v++) fold_row(v, acc, P, cm0, cm1);
-        put_cols(cm0, cm1, k);
-    };
     const bool doCols = T.bestB != nullptr;
+    const uint8_t* aRow = sA + rloc * 128;
+    const int aSw = swz(rloc);
 
-    // chunk c: MFMAs of chunk c -> accN (row offsets Pn); keys of chunk c-1 from
-    // accP (row offsets Pp); stages chunk c+2 from xs and loads chunk c+4 into xs
-    auto body = [&](uint32_t c, v16i (&accN)[2], int32_t (&Pn)[2], const v16i (&accP)[2], const int32_t (&Pp)[2],
-                    v4i& xs, bool sync) {
+    // chunk c: stage c+2 (loaded one chunk earlier), MFMAs of chunk c (K-step major over both column
+    // blocks), then the keys of chunk c
+    auto body = [&](uint32_t c, v4i& xs, bool sync) {
         stage_store(c + 2, xs);
         const uint8_t* sb = sU + (c % kRing) * kChunkBytes;
-        v4i bop[2][4];
+        v16i acc[2];
 #pragma unroll
-        for (int cb = 0; cb < 2; cb++)
+        for (int kc = 0; kc < 4; kc++) {
+            const int s = 2 * kc + hi;
+            const v4i a = *reinterpret_cast<const v4i*>(aRow + ((s ^ aSw) << 4));
 #pragma unroll
-            for (int kc = 0; kc < 4; kc++) {
+            for (int cb = 0; cb < 2; cb++) {
                 const int cl = cb * 32 + l32;
-                const int s = 2 * kc + hi;
-                bop[cb][kc] = *reinterpret_cast<const v4i*>(sb + cl * 128 + ((s ^ swz(cl)) << 4));
+                const v4i b = *reinterpret_cast<const v4i*>(sb + cl * 128 + ((s ^ swz(cl)) << 4));
+                const v16i zero = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+                acc[cb] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b, kc == 0 ? zero : acc[cb], 0, 0, 0);
             }
-        xs = stage_load(c + 4);
-        // the keys of rows 2i, 2i+1 of the previous chunk, then MFMA i: the
-        // scheduling fences keep that order, so each wave overlaps its own
-        // vector and matrix work, the first MFMA finds its LDS operand loaded
-        // and dependent MFMAs are ~18 VALU apart
-        int32_t cm0 = kNone, cm1 = kNone;
-        __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-        for (int i = 0; i < 8; i++) {
-            const int cb = i >> 2, kc = i & 3;
-            fold_row(2 * i, accP, Pp, cm0, cm1);
-            fold_row(2 * i + 1, accP, Pp, cm0, cm1);
-            __builtin_amdgcn_sched_barrier(0);
-            const v16i zero = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-            accN[cb] = __builtin_amdgcn_mfma_i32_32x32x32_i8(aop[kc], bop[cb][kc], kc == 0 ? zero : accN[cb], 0, 0, 0);
             __builtin_amdgcn_sched_barrier(0);
         }
-        put_cols(cm0, cm1, c - 1);
+        xs = stage_load(c + 3);
+        int32_t P[2];
 #pragma unroll
-        for (int cb = 0; cb < 2; cb++) Pn[cb] = sP[c % kPSlots][cb * 32 + l32];
-        // keep this chunk's row folding here (the compiler would otherwise
-        // sink it past the barrier into the next chunk)
+        for (int cb = 0; cb < 2; cb++) P[cb] = sP[c % kPSlots][cb * 32 + l32];
+        int32_t cm0 = kNone, cm1 = kNone;
 #pragma unroll
-        for (int v = 0; v < 16; v++) asm volatile("" : "+v"(m1[v]), "+v"(m2[v]));
+        for (int g = 0; g < 4; g++) {
+            const v4i q4 = *reinterpret_cast<const v4i*>(sQ + wave * 32 + 8 * g + 4 * hi);
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int v = 4 * g + j;
+                const int32_t a0 = acc[0][v], a1 = acc[1][v];
+                const int32_t r0 = shl9_add(a0, P[0]);
+                const int32_t r1 = shl9_add(a1, P[1]);
+                if constexpr (kTop2)
+                    top2_fold2(m1[v], m2[v], r0, r1);
+                else
+                    m1[v] = imin(m1[v], imin(r0, r1));
+                cm0 = imin(cm0, shl9_add_asm(a0, q4[j]));
+                cm1 = imin(cm1, shl9_add_asm(a1, q4[j]));
+            }
+        }
+        const auto sw = __builtin_amdgcn_permlane32_swap((uint32_t)cm0, (uint32_t)cm1, false, false);
+        sCol[c % kColSlots][wave][lane] = imin((int32_t)sw[0], (int32_t)sw[1]);
         if (sync) __syncthreads();
     };
-    // column partials of chunks k-3 and k-2 are complete after the barrier that
-    // precedes the even chunk k
-    auto flush_before = [&](uint32_t k) {
-        if (!doCols) return;
-        if (k >= 3) flush(k - 3);
-        if (k >= 2) flush(k - 2);
-    };
 
-    v16i accA[2], accB[2];
-    int32_t PA[2], PB[2];
-#pragma unroll
-    for (int cb = 0; cb < 2; cb++) {
-        accB[cb] = v16i{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-        PB[cb] = kNone;  // the keys of "chunk -1" never win
-    }
-    v4i xs0 = stage_load(0), xs1 = stage_load(1);
-    stage_store(0, xs0);
-    stage_store(1, xs1);
-    xs0 = stage_load(2);
-    xs1 = stage_load(3);
+    v4i xs = stage_load(0);
+    stage_store(0, xs);
+    xs = stage_load(1);
+    stage_store(1, xs);
+    xs = stage_load(2);
     __syncthreads();
 
-    // two chunks per trip: the accumulator pairs swap roles without copies and
-    // the barrier follows the odd chunk
+    // two chunks per trip, barrier after the odd one; the column partials of a
+    // trip's chunks are complete after its barrier and flushed at the next trip
     uint32_t c = 0;
     for (; c + 2 <= nChunks; c += 2) {
-        flush_before(c);
-        body(c, accA, PA, accB, PB, xs0, false);
-        body(c + 1, accB, PB, accA, PA, xs1, true);
+        if (doCols && c >= 2) {
+            flush(c - 2);
+            flush(c - 1);
+        }
+        body(c, xs, false);
+        body(c + 1, xs, true);
     }
-    uint32_t lastEven;  // the last even chunk
+    uint32_t firstUnflushed = c >= 2 ? c - 2 : 0;
     if (c < nChunks) {
-        flush_before(c);
-        body(c, accA, PA, accB, PB, xs0, false);
-        fold_keys(accA, PA, nChunks - 1);
-        lastEven = c;
-    } else {
-        fold_keys(accB, PB, nChunks - 1);
-        lastEven = c - 2;
+        if (doCols && c >= 2) {
+            flush(c - 2);
+            flush(c - 1);
+        }
+        body(c, xs, false);
+        firstUnflushed = c;
     }
     __syncthreads();
-    // chunks from lastEven - 1 on are not flushed yet
     if (doCols)
-        for (uint32_t k = lastEven >= 1 ? lastEven - 1 : 0; k < nChunks; k++) flush(k);
+        for (uint32_t k = firstUnflushed; k < nChunks; k++) flush(k);
 
     // ---- row results: transpose the per-lane (row, column-class) bests through LDS
     int32_t* L = reinterpret_cast<int32_t*>(sU);
@@ -335,7 +289,6 @@ __device__ __forceinline__ void match_tile(const DevTask& T, uint32_t rowBase, u
         L[rl * kTailStride + l32] = m1[v];
     }
     __syncthreads();
-    // thread tid owns local row tid
     int32_t b1 = kNone, b2 = kNone;
     int c1 = 0, c2 = 0;
     if (tid < kBM) {
@@ -358,7 +311,6 @@ __device__ __forceinline__ void match_tile(const DevTask& T, uint32_t rowBase, u
         }
         __syncthreads();
         if (tid < kBM) {
-            // second best = min(second best over classes, second best inside the winning class)
             const int32_t x = L[tid * kTailStride + c1];
             if (x < b2 || (x == b2 && c1 < c2)) {
                 b2 = x;
@@ -369,7 +321,7 @@ __device__ __forceinline__ void match_tile(const DevTask& T, uint32_t rowBase, u
     if (tid < kBM) {
         const uint32_t row = rowBase + tid;
         if (row < T.nA) {
-            const int32_t na = sNA[tid];
+            const int32_t na = sQ[tid] >> 8;
             auto to_key = [&](int32_t m, int cls) -> uint64_t {
                 if (m == kNone) return kNoMatch;
                 return make_key((uint32_t)((m >> 8) + na), colBeg + (uint32_t)(m & 0xFF) * 32u + (uint32_t)cls);
@@ -396,9 +348,11 @@ __device__ __forceinline__ void match_tile(const DevTask& T, uint32_t rowBase, u
     }
 }
 
-// grid launch: x = row block, y = segment, z = task (XCD-local order)
+// grid launch: x = row block, y = segment, z = task (XCD-local order).
+// __launch_bounds__(512, 4): 512 threads, at least four waves per SIMD, i.e.
+// at most 128 VGPRs, so two workgroups fit a CU.
 template <int kBits, bool kTop2>
-__global__ __launch_bounds__(512) void match_kernel(const TaskBatch batch)
+__global__ __launch_bounds__(512, 4) void match_kernel(const TaskBatch batch)
 {
     const Wg3 wg = xcd_local_block();
     const DevTask& T = batch.t[wg.z];
@@ -414,7 +368,7 @@ __global__ __launch_bounds__(512) void match_kernel(const TaskBatch batch)
 // XCD, so all tiles of one task share one L2. gridDim.x is a multiple of
 // kNumXcd.
 template <int kBits, bool kTop2>
-__global__ __launch_bounds__(512) void match_tiles_kernel(const DevTask* __restrict__ tasks,
+__global__ __launch_bounds__(512, 4) void match_tiles_kernel(const DevTask* __restrict__ tasks,
                                                           const TileDesc* __restrict__ tiles,
                                                           const uint32_t* __restrict__ xcdOff)
 {
@@ -432,7 +386,7 @@ __global__ __launch_bounds__(512) void match_tiles_kernel(const DevTask* __restr
 // of XCD list b % 8 (the XCD the hardware dispatches it to). The body is
 // match_tile alone, without the persistent loop's extra live registers.
 template <int kBits, bool kTop2>
-__global__ __launch_bounds__(512) void match_tile_grid_kernel(const DevTask* __restrict__ tasks,
+__global__ __launch_bounds__(512, 4) void match_tile_grid_kernel(const DevTask* __restrict__ tasks,
                                                               const TileDesc* __restrict__ tiles,
                                                               const uint32_t* __restrict__ xcdOff)
 {
