@@ -115,8 +115,9 @@ __device__ inline uint32_t count_inliers(const float F[9], const float4* __restr
     if (m >= 8) {
         float4 c0 = P[0], c1 = P[1], c2 = P[2], c3 = P[3], c4 = P[4], c5 = P[5], c6 = P[6], c7 = P[7];
         for (; i + 16 <= m; i += 8) {
-            const float4 n0 = P[i + 8], n1 = P[i + 9], n2 = P[i + 10], n3 = P[i + 11];
-            const float4 n4 = P[i + 12], n5 = P[i + 13], n6 = P[i + 14], n7 = P[i + 15];
+            const float4* __restrict__ Q = P + i + 8;  // one address per trip, immediate offsets
+            const float4 n0 = Q[0], n1 = Q[1], n2 = Q[2], n3 = Q[3];
+            const float4 n4 = Q[4], n5 = Q[5], n6 = Q[6], n7 = Q[7];
             cnt += test4(c0, c1, c2, c3) + test4(c4, c5, c6, c7);
             c0 = n0;
             c1 = n1;
