@@ -289,8 +289,8 @@ def learned_leg(args, dev, log):
 def main():
     args = parse()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path (every pair is independent; "
-              "the sharding design is described in DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU only: there is no multi-GPU path (image pairs are independent and "
+              "would shard across GPUs without communication). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
