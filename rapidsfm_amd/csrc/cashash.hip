@@ -40,7 +40,7 @@ __global__ void zero_u64_kernel(uint64_t* p, uint32_t n)
 }
 
 constexpr int kAccThreads = 256;
-constexpr int kAccDescPerBlock = 512;
+constexpr int kAccDescPerBlock = 64;  // 128 workgroups for 8K descriptors
 
 __global__ __launch_bounds__(kAccThreads) void accumulate_kernel(uint64_t* __restrict__ acc, uint64_t* __restrict__ sqrAcc,
                                                                  const uint8_t* __restrict__ desc, uint32_t n)
@@ -178,7 +178,7 @@ __global__ void zero_u32_kernel(uint32_t* p, uint32_t n)
 
 // ---------------------------------------------------------------------------
 constexpr int kHistThreads = 256;
-constexpr int kHistPerBlock = 2048;
+constexpr int kHistPerBlock = 512;
 
 __global__ __launch_bounds__(kHistThreads) void histogram_kernel(uint32_t* __restrict__ hist, const uint8_t* __restrict__ codes,
                                                                  uint32_t n)
@@ -197,41 +197,53 @@ __global__ __launch_bounds__(kHistThreads) void histogram_kernel(uint32_t* __res
         if (sH[i]) atomicAdd(&hist[i], sH[i]);
 }
 
-// exclusive (lower) and inclusive (upper) prefix per scheme, largest bucket
-__global__ __launch_bounds__(256) void bounds_kernel(uint32_t* __restrict__ lower, uint32_t* __restrict__ upper,
-                                                     uint32_t* __restrict__ maxBucket, const uint32_t* __restrict__ hist)
+// exclusive (lower) and inclusive (upper) prefix per scheme, largest bucket:
+// wave s scans scheme s (four buckets per lane, then a wave prefix over the
+// lane totals), so the eight scans run side by side
+constexpr int kBoundsThreads = 64 * kSchemes;
+
+__global__ __launch_bounds__(kBoundsThreads) void bounds_kernel(uint32_t* __restrict__ lower, uint32_t* __restrict__ upper,
+                                                                uint32_t* __restrict__ maxBucket,
+                                                                const uint32_t* __restrict__ hist)
 {
-    __shared__ uint32_t sScan[256];
-    __shared__ uint32_t sMax[256];
-    uint32_t mx = 0;
-    for (int s = 0; s < 8; s++) {
-        const uint32_t h = hist[s * 256 + threadIdx.x];
-        mx = max(mx, h);
-        sScan[threadIdx.x] = h;
-        __syncthreads();
-        for (int off = 1; off < 256; off <<= 1) {
-            const uint32_t v = threadIdx.x >= (uint32_t)off ? sScan[threadIdx.x - off] : 0u;
-            __syncthreads();
-            sScan[threadIdx.x] += v;
-            __syncthreads();
-        }
-        upper[s * 256 + threadIdx.x] = sScan[threadIdx.x];
-        lower[s * 256 + threadIdx.x] = sScan[threadIdx.x] - h;
-        __syncthreads();
+    __shared__ uint32_t sMax[kSchemes];
+    const uint32_t s = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint32_t* hs = hist + s * 256 + lane * 4;
+    const uint4 h = make_uint4(hs[0], hs[1], hs[2], hs[3]);
+    const uint32_t p0 = h.x, p1 = p0 + h.y, p2 = p1 + h.z, p3 = p2 + h.w;
+    uint32_t scan = p3;  // inclusive prefix of the lane totals
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t v = __shfl_up(scan, off);
+        if (lane >= (uint32_t)off) scan += v;
     }
-    sMax[threadIdx.x] = mx;
+    const uint32_t base = scan - p3;
+    uint32_t* up = upper + s * 256 + lane * 4;
+    uint32_t* lo = lower + s * 256 + lane * 4;
+    up[0] = base + p0;
+    up[1] = base + p1;
+    up[2] = base + p2;
+    up[3] = base + p3;
+    lo[0] = base;
+    lo[1] = base + p0;
+    lo[2] = base + p1;
+    lo[3] = base + p2;
+    uint32_t mx = max(max(h.x, h.y), max(h.z, h.w));
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) mx = max(mx, (uint32_t)__shfl_xor(mx, off));
+    if (lane == 0) sMax[s] = mx;
     __syncthreads();
-    for (int off = 128; off > 0; off >>= 1) {
-        if (threadIdx.x < (uint32_t)off) sMax[threadIdx.x] = max(sMax[threadIdx.x], sMax[threadIdx.x + off]);
-        __syncthreads();
+    if (threadIdx.x == 0 && maxBucket) {
+        uint32_t m = 0;
+        for (uint32_t k = 0; k < kSchemes; k++) m = max(m, sMax[k]);
+        *maxBucket = m;
     }
-    if (threadIdx.x == 0 && maxBucket) *maxBucket = sMax[0];
 }
 
 // two-pass scatter: block histogram, one global slot reservation per
 // (scheme, bucket), then LDS slot counters
 constexpr int kBuildThreads = 256;
-constexpr int kBuildPerBlock = 1024;
+constexpr int kBuildPerBlock = 256;
 
 __global__ __launch_bounds__(kBuildThreads) void buckets_kernel(const uint8_t* __restrict__ codes, uint32_t n,
                                                                uint32_t* __restrict__ alloc, uint32_t* __restrict__ buckets)
@@ -559,7 +571,7 @@ extern "C" int rsfm_cashash_compute_bucket_bound(uint32_t* lower, uint32_t* uppe
 {
     RSFM_RETURN_IF(lower == nullptr || upper == nullptr || histogram == nullptr, RSFM_ERR_INVALID,
                    "compute_bucket_bound: null pointer");
-    cashash::bounds_kernel<<<1, 256, 0, stream>>>(lower, upper, maxBucketSize, histogram);
+    cashash::bounds_kernel<<<1, cashash::kBoundsThreads, 0, stream>>>(lower, upper, maxBucketSize, histogram);
     return check_launch("compute_bucket_bound");
 }
 
