@@ -205,8 +205,14 @@ __device__ __forceinline__ void match_tile(const DevTask& T, uint32_t rowBase, u
 
     // chunk c: stage c+2 (loaded one chunk earlier), MFMAs of chunk c (K-step major over both column
     // blocks), then the keys of chunk c
-    auto body = [&](uint32_t c, v4i& xs, bool sync) {
+    auto body = [&](uint32_t c, v4i& xs, bool sync, bool flushPrev) {
         stage_store(c + 2, xs);
+        // the previous trip's column partials: flushed after the staging store,
+        // so that store's wait on the staged load does not also wait on these atomics
+        if (flushPrev) {
+            flush(c - 2);
+            flush(c - 1);
+        }
         const uint8_t* sb = sU + (c % kRing) * kChunkBytes;
         v16i acc[2];
 #pragma unroll
@@ -260,20 +266,12 @@ __device__ __forceinline__ void match_tile(const DevTask& T, uint32_t rowBase, u
     // trip's chunks are complete after its barrier and flushed at the next trip
     uint32_t c = 0;
     for (; c + 2 <= nChunks; c += 2) {
-        if (doCols && c >= 2) {
-            flush(c - 2);
-            flush(c - 1);
-        }
-        body(c, xs, false);
-        body(c + 1, xs, true);
+        body(c, xs, false, doCols && c >= 2);
+        body(c + 1, xs, true, false);
     }
     uint32_t firstUnflushed = c >= 2 ? c - 2 : 0;
     if (c < nChunks) {
-        if (doCols && c >= 2) {
-            flush(c - 2);
-            flush(c - 1);
-        }
-        body(c, xs, false);
+        body(c, xs, false, doCols && c >= 2);
         firstUnflushed = c;
     }
     __syncthreads();
