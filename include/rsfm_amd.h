@@ -104,7 +104,9 @@ int rsfm_sqrnorm_u4(uint32_t* sqrNorm, const void* desc4, uint32_t nbDesc, hipSt
 int rsfm_bf_match_u4(const rsfm_bf_task* tasks, size_t nbTasks, hipStream_t stream);
 
 /* top-2 query side + top-1 train side (new). Needs a workspace of
- * rsfm_bf_match_top2_workspace_size() bytes, 256-B aligned. */
+ * rsfm_bf_match_top2_workspace_size() bytes, 256-B aligned: the staging form
+ * of each task's train descriptors (132 B per descriptor, nbDescB rounded up
+ * to 64), written by the call, and the segment partials of long train sets. */
 int rsfm_bf_match_top2_workspace_size(const rsfm_bf_top2_task* tasks, size_t nbTasks, size_t* bytes);
 int rsfm_bf_match_top2(const rsfm_bf_top2_task* tasks, size_t nbTasks, void* workspace,
                        size_t workspaceBytes, hipStream_t stream);

@@ -14,6 +14,11 @@
 //    per wave, K = 128 = 4 MFMAs each) and streams the train descriptors of
 //    its column segment through LDS in 64-column chunks (a ring of four,
 //    XOR-swizzled 16-B slots); two workgroups share a CU.
+//  * Top-2: the call first writes the train side in its staging form (w and
+//    NB' << 8 per column, padded to whole chunks) into the workspace, and the
+//    tile copies it into the LDS ring with direct global-to-LDS loads, so
+//    staging costs neither registers nor VALU in the main loop. Top-1 (no
+//    workspace) loads, transforms and stores each chunk itself.
 //  * Epilogue per accumulator: one v_lshl_add makes the row key
 //    ((NB'_j + 2dot) << 8 | column-block tag), one makes the column key
 //    ((NA'_i + 2dot) << 8 | local row); v_min3 folds top-1, v_med3 + v_min
@@ -96,6 +101,24 @@ __device__ inline int32_t dpp_mirror8(int32_t x) { return __builtin_amdgcn_updat
 // bank-conflict free
 __device__ inline int swz(int c) { return ((c >> 1) & 7) ^ ((c & 1) << 2); }
 
+// Direct global -> LDS copies (gfx950 global_load_lds): every lane moves
+// `size` bytes from its own global address to lds + size * lane; lds must be
+// the same for the whole wave. Completion is counted by vmcnt.
+__device__ inline void dma16(const void* src, void* lds)
+{
+    __builtin_amdgcn_global_load_lds(
+        reinterpret_cast<const __attribute__((address_space(1))) void*>(reinterpret_cast<uintptr_t>(src)),
+        reinterpret_cast<__attribute__((address_space(3))) void*>(reinterpret_cast<uintptr_t>(lds)), 16, 0, 0);
+}
+__device__ inline void dma4(const void* src, void* lds)
+{
+    __builtin_amdgcn_global_load_lds(
+        reinterpret_cast<const __attribute__((address_space(1))) void*>(reinterpret_cast<uintptr_t>(src)),
+        reinterpret_cast<__attribute__((address_space(3))) void*>(reinterpret_cast<uintptr_t>(lds)), 4, 0, 0);
+}
+// s_waitcnt vmcnt(0) (gfx9 encoding: expcnt and lgkmcnt left at their maxima)
+__device__ inline void wait_vm0() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
 // sCol row stride in words: the flush reads 8 consecutive columns from each of
 // the 8 waves; a stride of 72 puts those 64 words in distinct banks
 constexpr int kColStride = 72;
@@ -112,7 +135,7 @@ constexpr int kColStride = 72;
 // ran 10% slower on top-2. Column partials of chunk c are written to LDS in
 // iteration c and merged to global memory by all waves (8 columns each) after
 // the next barrier.
-template <int kBits, bool kTop2>
+template <int kBits, bool kTop2, bool kPrep>
 __device__ __forceinline__ void match_tile(const DevTask& T, uint32_t rowBase, uint32_t colBeg, uint32_t seg)
 {
     constexpr int kWaves = 8;
@@ -132,7 +155,7 @@ __device__ __forceinline__ void match_tile(const DevTask& T, uint32_t rowBase, u
     __shared__ __attribute__((aligned(16))) int32_t sQ[kBM];  // NA' << 8 | local row, kNone past nA
 
     const int tid = threadIdx.x;
-    const int lane = tid & 63, wave = tid >> 6;
+    const int lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int l32 = lane & 31, hi = lane >> 5;
     const int rloc = wave * 32 + l32;
 
@@ -186,6 +209,15 @@ __device__ __forceinline__ void match_tile(const DevTask& T, uint32_t rowBase, u
         acc += dpp_mirror8(acc);
         sP[c % kPSlots][sc] = valid ? (acc * 256 + (int32_t)(c * 2 + (sc >> 5))) : kNone;
     };
+    // prepared train data: lane L of wave w fills column w*8 + L/8, slot L%8 of
+    // the chunk's ring slot, i.e. it reads source slot (L%8) ^ swz(column);
+    // wave 0 also copies the chunk's 64 column offsets (NB' << 8, kNone past nB)
+    auto stage_dma = [&](uint32_t c) {
+        const int col = wave * 8 + (lane >> 3);
+        const uint8_t* src = T.W + (size_t)(colBeg + c * kBN + col) * 128 + (((lane & 7) ^ swz(col)) << 4);
+        dma16(src, sU + (c % kRing) * kChunkBytes + wave * 1024);
+        if (wave == 0) dma4(T.NB + colBeg + c * kBN + lane, &sP[c % kPSlots][0]);
+    };
     auto flush = [&](uint32_t k) {
         const int col = wave * 8 + (lane >> 3);
         int32_t x = sCol[k % kColSlots][lane & 7][col];
@@ -205,8 +237,18 @@ __device__ __forceinline__ void match_tile(const DevTask& T, uint32_t rowBase, u
 
     // chunk c: stage c+2 (loaded one chunk earlier), MFMAs of chunk c (K-step major over both column
     // blocks), then the keys of chunk c
-    auto body = [&](uint32_t c, v4i& xs, bool sync, bool flushPrev) {
-        stage_store(c + 2, xs);
+    // with prepared data the trip's first chunk c copies chunks c+2 and c+3
+    // (their ring slots were last read before the previous trip's barrier) and
+    // the trip's barrier waits for both
+    auto body = [&](uint32_t c, v4i& xs, bool sync, bool flushPrev, bool issue) {
+        if constexpr (kPrep) {
+            if (issue) {
+                if (c + 2 < nChunks) stage_dma(c + 2);
+                if (c + 3 < nChunks) stage_dma(c + 3);
+            }
+        } else {
+            stage_store(c + 2, xs);
+        }
         // the previous trip's column partials: flushed after the staging store,
         // so that store's wait on the staged load does not also wait on these atomics
         if (flushPrev) {
@@ -228,10 +270,14 @@ __device__ __forceinline__ void match_tile(const DevTask& T, uint32_t rowBase, u
             }
             __builtin_amdgcn_sched_barrier(0);
         }
-        xs = stage_load(c + 3);
+        if constexpr (!kPrep) xs = stage_load(c + 3);
         int32_t P[2];
 #pragma unroll
-        for (int cb = 0; cb < 2; cb++) P[cb] = sP[c % kPSlots][cb * 32 + l32];
+        for (int cb = 0; cb < 2; cb++) {
+            P[cb] = sP[c % kPSlots][cb * 32 + l32];
+            // the prepared offsets carry no column-block tag (kNone keeps its value)
+            if constexpr (kPrep) P[cb] |= (int32_t)(c * 2 + cb);
+        }
         int32_t cm0 = kNone, cm1 = kNone;
 #pragma unroll
         for (int g = 0; g < 4; g++) {
@@ -252,26 +298,36 @@ __device__ __forceinline__ void match_tile(const DevTask& T, uint32_t rowBase, u
         }
         const auto sw = __builtin_amdgcn_permlane32_swap((uint32_t)cm0, (uint32_t)cm1, false, false);
         sCol[c % kColSlots][wave][lane] = imin((int32_t)sw[0], (int32_t)sw[1]);
-        if (sync) __syncthreads();
+        if (sync) {
+            if constexpr (kPrep) wait_vm0();
+            __syncthreads();
+        }
     };
 
-    v4i xs = stage_load(0);
-    stage_store(0, xs);
-    xs = stage_load(1);
-    stage_store(1, xs);
-    xs = stage_load(2);
+    v4i xs = {0, 0, 0, 0};
+    if constexpr (kPrep) {
+        stage_dma(0);
+        if (nChunks > 1) stage_dma(1);
+        wait_vm0();
+    } else {
+        xs = stage_load(0);
+        stage_store(0, xs);
+        xs = stage_load(1);
+        stage_store(1, xs);
+        xs = stage_load(2);
+    }
     __syncthreads();
 
     // two chunks per trip, barrier after the odd one; the column partials of a
     // trip's chunks are complete after its barrier and flushed at the next trip
     uint32_t c = 0;
     for (; c + 2 <= nChunks; c += 2) {
-        body(c, xs, false, doCols && c >= 2);
-        body(c + 1, xs, true, false);
+        body(c, xs, false, doCols && c >= 2, true);
+        body(c + 1, xs, true, false, false);
     }
     uint32_t firstUnflushed = c >= 2 ? c - 2 : 0;
     if (c < nChunks) {
-        body(c, xs, false, doCols && c >= 2);
+        body(c, xs, false, doCols && c >= 2, true);
         firstUnflushed = c;
     }
     __syncthreads();
@@ -349,7 +405,7 @@ __device__ __forceinline__ void match_tile(const DevTask& T, uint32_t rowBase, u
 // grid launch: x = row block, y = segment, z = task (XCD-local order).
 // __launch_bounds__(512, 4): 512 threads, at least four waves per SIMD, i.e.
 // at most 128 VGPRs, so two workgroups fit a CU.
-template <int kBits, bool kTop2>
+template <int kBits, bool kTop2, bool kPrep>
 __global__ __launch_bounds__(512, 4) void match_kernel(const TaskBatch batch)
 {
     const Wg3 wg = xcd_local_block();
@@ -357,7 +413,7 @@ __global__ __launch_bounds__(512, 4) void match_kernel(const TaskBatch batch)
     const uint32_t rowBase = wg.x * kBM;
     const uint32_t colBeg = wg.y * T.segCols;
     if (rowBase >= T.nA || colBeg >= T.nB) return;
-    match_tile<kBits, kTop2>(T, rowBase, colBeg, wg.y);
+    match_tile<kBits, kTop2, kPrep>(T, rowBase, colBeg, wg.y);
 }
 
 // Persistent launch over a tile table: the tiles are split into kNumXcd lists
@@ -376,7 +432,7 @@ __global__ __launch_bounds__(512, 4) void match_tiles_kernel(const DevTask* __re
         const TileDesc d = tiles[i];
         const DevTask T = tasks[d.task];
         __syncthreads();  // the previous tile's last LDS reads
-        match_tile<kBits, kTop2>(T, d.rowBase, d.colBeg, d.seg);
+        match_tile<kBits, kTop2, true>(T, d.rowBase, d.colBeg, d.seg);
     }
 }
 
@@ -393,7 +449,7 @@ __global__ __launch_bounds__(512, 4) void match_tile_grid_kernel(const DevTask* 
     if (i >= xcdOff[xcd + 1]) return;
     const TileDesc d = tiles[i];
     const DevTask T = tasks[d.task];
-    match_tile<kBits, kTop2>(T, d.rowBase, d.colBeg, d.seg);
+    match_tile<kBits, kTop2, true>(T, d.rowBase, d.colBeg, d.seg);
 }
 
 template <class Src>
@@ -403,6 +459,60 @@ __global__ void init_kernel(const Src batch, bool initA)
     const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
     if (initA && x < T.nA) T.bestA[x] = kNoMatch;
     if (T.bestB != nullptr && x < T.nB) T.bestB[x] = kNoMatch;
+}
+
+// staging form of B (DevTask::W, NB): eight lanes per column, one 16-B slot
+// each; a workgroup of 256 covers 32 columns and the padded column count is a
+// multiple of 64, so a workgroup is wholly inside or outside it
+template <class Src>
+__global__ __launch_bounds__(256) void prep_train_kernel(const Src batch)
+{
+    const DevTask& T = batch[blockIdx.z];
+    const uint32_t nPad = (T.nB + kBN - 1) / kBN * kBN;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t col = t >> 3, sp = t & 7;
+    if (T.W == nullptr || col >= nPad) return;
+    const bool valid = col < T.nB;
+    v4i x = {0, 0, 0, 0};
+    if (valid) x = *reinterpret_cast<const v4i*>(T.B + (size_t)col * 128 + sp * 16);
+    int32_t acc = 0;
+    v4i w;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        w[q] = x[q] ^ (int)0x80808080u;
+        if (!valid) w[q] = 0;
+        acc = __builtin_amdgcn_sdot4(w[q], w[q], acc, false);
+        acc = __builtin_amdgcn_sdot4(w[q], 0x02020202, acc, false);
+    }
+    *reinterpret_cast<v4i*>(T.W + (size_t)col * 128 + sp * 16) = w;
+    acc += dpp_xor1(acc);
+    acc += dpp_xor2(acc);
+    acc += dpp_mirror8(acc);
+    if (sp == 0) T.NB[col] = valid ? acc * 256 : kNone;
+}
+
+size_t prep_bytes(uint32_t nB)
+{
+    const size_t nPad = div_up(nB, (uint32_t)kBN) * (size_t)kBN;
+    return align_up(nPad * 128, 256) + align_up(nPad * 4, 256);
+}
+
+// W and NB of a task from its part of a workspace
+static void place_prep(DevTask& t, uint8_t*& ws)
+{
+    if (t.nB == 0) return;
+    const size_t nPad = div_up(t.nB, (uint32_t)kBN) * (size_t)kBN;
+    t.W = ws;
+    t.NB = reinterpret_cast<int32_t*>(ws + align_up(nPad * 128, 256));
+    ws += prep_bytes(t.nB);
+}
+
+int launch_prep_table(const DevTask* tasks, uint32_t nTasks, uint32_t maxB, hipStream_t stream)
+{
+    if (nTasks == 0 || maxB == 0) return RSFM_OK;
+    const uint32_t nPad = div_up(maxB, (uint32_t)kBN) * kBN;
+    prep_train_kernel<<<dim3(nPad * 8 / 256, 1, nTasks), 256, 0, stream>>>(TaskTable{tasks});
+    return check_launch("bf prep table");
 }
 
 template <class Src>
@@ -495,7 +605,7 @@ static int launch_top1(const rsfm_bf_task* tasks, size_t nbTasks, hipStream_t st
         init_kernel<<<dim3(initX, 1, (uint32_t)n), 256, 0, stream>>>(batch, true);
         RSFM_CHECK(check_launch("bf init"));
         if (p.gridX == 0 || p.maxB == 0) continue;
-        match_kernel<kBits, false><<<dim3(p.gridX, p.nSeg, (uint32_t)n), 512, 0, stream>>>(batch);
+        match_kernel<kBits, false, false><<<dim3(p.gridX, p.nSeg, (uint32_t)n), 512, 0, stream>>>(batch);
         RSFM_CHECK(check_launch("bf match"));
     }
     return RSFM_OK;
@@ -509,8 +619,10 @@ static size_t top2_workspace(const rsfm_bf_top2_task* tasks, size_t nbTasks)
         DevTask t[kMaxTasks];
         for (size_t i = 0; i < n; i++) t[i] = to_dev(tasks[b + i].base);
         const Plan p = make_plan(t, n);
-        if (p.nSeg <= 1) continue;
-        for (size_t i = 0; i < n; i++) bytes += align_up((size_t)p.nSeg * t[i].nA * 16, 256);
+        for (size_t i = 0; i < n; i++) {
+            if (p.nSeg > 1) bytes += align_up((size_t)p.nSeg * t[i].nA * 16, 256);
+            if (t[i].nA > 0) bytes += prep_bytes(t[i].nB);
+        }
     }
     return bytes;
 }
@@ -591,11 +703,12 @@ extern "C" int rsfm_bf_match_top2(const rsfm_bf_top2_task* tasks, size_t nbTasks
             batch.t[i].nSeg = p.nSeg;
             batch.t[i].segCols = p.segCols;
         }
-        if (p.nSeg > 1) {
-            for (size_t i = 0; i < n; i++) {
+        for (size_t i = 0; i < n; i++) {
+            if (p.nSeg > 1) {
                 batch.t[i].part = reinterpret_cast<uint64_t*>(ws);
                 ws += align_up((size_t)p.nSeg * batch.t[i].nA * 16, 256);
             }
+            if (batch.t[i].nA > 0) place_prep(batch.t[i], ws);
         }
         const uint32_t initX = div_up(std::max(p.maxA, p.maxB), 256);
         if (initX == 0) continue;
@@ -612,7 +725,9 @@ extern "C" int rsfm_bf_match_top2(const rsfm_bf_top2_task* tasks, size_t nbTasks
             RSFM_CHECK(check_launch("bf top2 init2"));
         }
         if (p.gridX == 0 || p.maxB == 0) continue;
-        match_kernel<8, true><<<dim3(p.gridX, p.nSeg, (uint32_t)n), 512, 0, stream>>>(batch);
+        prep_train_kernel<<<dim3(div_up(p.maxB, (uint32_t)kBN) * kBN * 8 / 256, 1, (uint32_t)n), 256, 0, stream>>>(batch);
+        RSFM_CHECK(check_launch("bf top2 prep"));
+        match_kernel<8, true, true><<<dim3(p.gridX, p.nSeg, (uint32_t)n), 512, 0, stream>>>(batch);
         RSFM_CHECK(check_launch("bf top2 match"));
         if (p.nSeg > 1) {
             merge_top2_kernel<<<dim3(div_up(p.maxA, 256), 1, (uint32_t)n), 256, 0, stream>>>(batch);

@@ -76,6 +76,8 @@ struct rsfm_pipeline {
     uint32_t nFill = 0, maxFill = 0;
     uint64_t* dPart = nullptr;
     size_t partCap = 0;
+    uint8_t* dPrep = nullptr;  // staging form of every pair's train descriptors
+    uint32_t maxPrepB = 0;
     // chosen tiling
     uint32_t segCols = 0, wgPerXcd = 0, nTiles = 0, maxList = 0;
     float tunedMs = 0.f;
@@ -110,9 +112,10 @@ struct Plan {
     std::vector<bf::TileDesc> tiles;
     std::vector<uint32_t> xcdOff;
     size_t partWords = 0;
+    size_t prepBytes = 0;
 };
 
-Plan make_plan(const std::vector<rsfm_pipeline_pair>& pairs, uint32_t segCols, uint64_t* part)
+Plan make_plan(const std::vector<rsfm_pipeline_pair>& pairs, uint32_t segCols, uint64_t* part, uint8_t* prep)
 {
     Plan pl;
     const size_t P = pairs.size();
@@ -133,6 +136,14 @@ Plan make_plan(const std::vector<rsfm_pipeline_pair>& pairs, uint32_t segCols, u
         if (t.nSeg > 1) {
             t.part = part ? part + pl.partWords : nullptr;
             pl.partWords += (size_t)t.nSeg * x.nA * 2;
+        }
+        if (x.nA > 0 && x.nB > 0) {
+            const size_t nPad = (size_t)div_up(x.nB, (uint32_t)bf::kBN) * bf::kBN;
+            if (prep) {
+                t.W = prep + pl.prepBytes;
+                t.NB = reinterpret_cast<int32_t*>(prep + pl.prepBytes + align_up(nPad * 128, 256));
+            }
+            pl.prepBytes += bf::prep_bytes(x.nB);
         }
     }
     // pairs to XCDs: longest processing time first
@@ -178,6 +189,7 @@ int enqueue_match(rsfm_pipeline* pl, hipStream_t s)
         RSFM_CHECK(check_launch("pipeline fill"));
     }
     if (pl->nTiles) {
+        RSFM_CHECK(bf::launch_prep_table(pl->dTasks, (uint32_t)pl->pairs.size(), pl->maxPrepB, s));
         if (pl->wgPerXcd == RSFM_PIPELINE_WG_PER_TILE)
             RSFM_CHECK(bf::launch_match_tile_grid_top2(pl->dTasks, pl->dTiles, pl->dXcdOff, pl->maxList, s));
         else
@@ -190,7 +202,7 @@ int enqueue_match(rsfm_pipeline* pl, hipStream_t s)
 // upload the plan for segment width segCols
 int apply_plan(rsfm_pipeline* pl, uint32_t segCols)
 {
-    Plan p = make_plan(pl->pairs, segCols, pl->dPart);
+    Plan p = make_plan(pl->pairs, segCols, pl->dPart, pl->dPrep);
     RSFM_RETURN_IF(p.partWords > pl->partCap, RSFM_ERR_INVALID, "pipeline: partial buffer too small");
     RSFM_RETURN_IF(p.tiles.size() > pl->tileCap, RSFM_ERR_INVALID, "pipeline: tile table too small");
     RSFM_CHECK(upload(pl->dTasks, p.tasks.data(), p.tasks.size() * sizeof(bf::DevTask)));
@@ -362,7 +374,7 @@ extern "C" int rsfm_pipeline_create(const rsfm_pipeline_pair* pairs, size_t nbPa
     // sizes: the finest candidate segment width bounds tiles and partials
     const uint32_t P = (uint32_t)nbPairs;
     const uint32_t minSeg = cfg->segCols ? cfg->segCols : 1024;
-    Plan worst = make_plan(pl->pairs, minSeg, nullptr);
+    Plan worst = make_plan(pl->pairs, minSeg, nullptr, nullptr);
     pl->tileCap = std::max<size_t>(1, worst.tiles.size());
     pl->partCap = worst.partWords;
     std::vector<pipe::FillRange> fill;
@@ -398,6 +410,7 @@ extern "C" int rsfm_pipeline_create(const rsfm_pipeline_pair* pairs, size_t nbPa
     const size_t oPoseH = take(P * sizeof(pose::DevTask));
     const size_t oFill = take(std::max<size_t>(1, fill.size()) * sizeof(pipe::FillRange));
     const size_t oPart = take(std::max<size_t>(1, pl->partCap) * sizeof(uint64_t));
+    const size_t oPrep = take(std::max<size_t>(1, worst.prepBytes));
     rc = check_hip(hipMalloc(reinterpret_cast<void**>(&pl->arena), off), "pipeline workspace");
     if (rc != RSFM_OK) return fail_out(rc);
     pl->dTasks = reinterpret_cast<bf::DevTask*>(pl->arena + oTasks);
@@ -414,6 +427,9 @@ extern "C" int rsfm_pipeline_create(const rsfm_pipeline_pair* pairs, size_t nbPa
     pl->dPoseH = reinterpret_cast<pose::DevTask*>(pl->arena + oPoseH);
     pl->dFill = reinterpret_cast<pipe::FillRange*>(pl->arena + oFill);
     pl->dPart = reinterpret_cast<uint64_t*>(pl->arena + oPart);
+    pl->dPrep = pl->arena + oPrep;
+    for (auto& x : pl->pairs)
+        if (x.nA > 0) pl->maxPrepB = std::max(pl->maxPrepB, x.nB);
 
     std::vector<aux::FilterTask> ft(P);
     std::vector<ransac::SampleTask> st(P);
