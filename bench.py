@@ -30,7 +30,7 @@ INT8_PEAK_TOPS = 5000.0  # MI355X dense int8 MFMA, 5.0 POPS
 # WRITE_SIZE (one pass each) at (keypoints, pairs per launch), each scaled by
 # the reading of a 256 MiB-in / 256 MiB-out calibration dispatch (FETCH x2.0,
 # WRITE x1.0); other shapes report null
-MEASURED_TRAFFIC = {(8192, 32): 138572 * 1024}
+MEASURED_TRAFFIC = {(8192, 32): 136566 * 1024}
 
 
 def parse():

@@ -463,13 +463,23 @@ __global__ void init_kernel(const Src batch, bool initA)
 
 // staging form of B (DevTask::W, NB): eight lanes per column, one 16-B slot
 // each; a workgroup of 256 covers 32 columns and the padded column count is a
-// multiple of 64, so a workgroup is wholly inside or outside it
+// multiple of 64, so a workgroup is wholly inside or outside it. With
+// initResults the same threads also set the train-side results to "no match"
+// (they are merged by atomicMin) and, for a task without train descriptors,
+// the query-side results (no tile writes them).
 template <class Src>
-__global__ __launch_bounds__(256) void prep_train_kernel(const Src batch)
+__global__ __launch_bounds__(256) void prep_train_kernel(const Src batch, bool initResults)
 {
     const DevTask& T = batch[blockIdx.z];
     const uint32_t nPad = (T.nB + kBN - 1) / kBN * kBN;
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (initResults) {
+        if (T.bestB != nullptr && t < T.nB) T.bestB[t] = kNoMatch;
+        if (T.nB == 0 && t < T.nA) {
+            T.bestA[t] = kNoMatch;
+            T.secondA[t] = kNoMatch;
+        }
+    }
     const uint32_t col = t >> 3, sp = t & 7;
     if (T.W == nullptr || col >= nPad) return;
     const bool valid = col < T.nB;
@@ -511,7 +521,7 @@ int launch_prep_table(const DevTask* tasks, uint32_t nTasks, uint32_t maxB, hipS
 {
     if (nTasks == 0 || maxB == 0) return RSFM_OK;
     const uint32_t nPad = div_up(maxB, (uint32_t)kBN) * kBN;
-    prep_train_kernel<<<dim3(nPad * 8 / 256, 1, nTasks), 256, 0, stream>>>(TaskTable{tasks});
+    prep_train_kernel<<<dim3(nPad * 8 / 256, 1, nTasks), 256, 0, stream>>>(TaskTable{tasks}, false);
     return check_launch("bf prep table");
 }
 
@@ -710,23 +720,13 @@ extern "C" int rsfm_bf_match_top2(const rsfm_bf_top2_task* tasks, size_t nbTasks
             }
             if (batch.t[i].nA > 0) place_prep(batch.t[i], ws);
         }
-        const uint32_t initX = div_up(std::max(p.maxA, p.maxB), 256);
-        if (initX == 0) continue;
-        // with no train descriptors the query side must still read "no match"
-        init_kernel<<<dim3(initX, 1, (uint32_t)n), 256, 0, stream>>>(batch, true);
-        RSFM_CHECK(check_launch("bf top2 init"));
-        if (p.maxA > 0) {
-            TaskBatch b2 = batch;
-            for (size_t i = 0; i < n; i++) {
-                b2.t[i].bestA = batch.t[i].secondA;
-                b2.t[i].bestB = nullptr;
-            }
-            init_kernel<<<dim3(div_up(p.maxA, 256), 1, (uint32_t)n), 256, 0, stream>>>(b2, true);
-            RSFM_CHECK(check_launch("bf top2 init2"));
-        }
-        if (p.gridX == 0 || p.maxB == 0) continue;
-        prep_train_kernel<<<dim3(div_up(p.maxB, (uint32_t)kBN) * kBN * 8 / 256, 1, (uint32_t)n), 256, 0, stream>>>(batch);
+        // one launch: results to "no match" where no tile writes them, and
+        // the staging form of every task's train descriptors
+        const uint32_t prepThreads = std::max(div_up(p.maxB, (uint32_t)kBN) * kBN * 8, std::max(p.maxA, p.maxB));
+        if (prepThreads == 0) continue;
+        prep_train_kernel<<<dim3(div_up(prepThreads, 256), 1, (uint32_t)n), 256, 0, stream>>>(batch, true);
         RSFM_CHECK(check_launch("bf top2 prep"));
+        if (p.gridX == 0 || p.maxB == 0) continue;
         match_kernel<8, true, true><<<dim3(p.gridX, p.nSeg, (uint32_t)n), 512, 0, stream>>>(batch);
         RSFM_CHECK(check_launch("bf top2 match"));
         if (p.nSeg > 1) {
