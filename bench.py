@@ -7,7 +7,8 @@ resident in HBM, all on one stream:
   2. rsfm_filter_matches_batch    ratio test 0.8 + cross-check + keypoint gather
   3. rsfm_ransac_sample_indices_batch   H seeded 8-point samples per pair
   4. rsfm_ransac_f8               FP32 8-point fits + Sampson scoring (1 px),
-                                  best model + inlier mask per pair
+                                  every model stored, best model + inlier mask
+                                  per pair
 Synthetic data (rapidsfm_amd.synth, drawn on the device): SIFT-like
 descriptors (30% noisy true matches) with keypoints from one two-view geometry
 per pair (30% of true matches geometric outliers).
@@ -323,6 +324,7 @@ def main():
             "count": torch.zeros(1, dtype=torch.int32, device=dev),
             "idx": torch.empty((H, 8), dtype=torch.int32, device=dev),
             "counts": torch.empty(H, dtype=torch.int32, device=dev),
+            "models": torch.empty((H, 9), dtype=torch.float32, device=dev),
             "best": torch.empty(2, dtype=torch.int32, device=dev),
             "mask": torch.empty(n, dtype=torch.uint8, device=dev),
         }
@@ -338,8 +340,10 @@ def main():
         filt[p] = FilterTask(ptr(b["bestA"]), ptr(b["secA"]), n, ptr(b["bestB"]), n, None, ptr(b["count"]),
                              ptr(b["kA"]), ptr(b["kB"]), ptr(b["pts"]))
         samp[p] = SampleTask(ptr(b["idx"]), H, n, ptr(b["count"]), 77 + p)
-        rtask[p] = RansacTask(ptr(b["pts"]), n, ptr(b["count"]), ptr(b["idx"]), H, ptr(b["counts"]), None,
-                              ptr(b["best"]), ptr(b["mask"]))
+        # the scorer stores every model: the caller reads the best F as
+        # models[best[0]], and the best-model kernel takes it from there
+        rtask[p] = RansacTask(ptr(b["pts"]), n, ptr(b["count"]), ptr(b["idx"]), H, ptr(b["counts"]),
+                              ptr(b["models"]), ptr(b["best"]), ptr(b["mask"]))
     ws_need = ctypes.c_size_t(0)
     check(lib.rsfm_bf_match_top2_workspace_size(ctypes.cast(top2, ctypes.c_void_p), P, ctypes.byref(ws_need)), "ws")
     ws = torch.empty(ws_need.value + 256, dtype=torch.uint8, device=dev)

@@ -146,62 +146,100 @@ __global__ void idp_init_kernel(const IdpBatch batch)
 // Cross-check + ratio-test filter with in-order compaction, one workgroup.
 constexpr int kFilterThreads = 1024;
 
-__device__ inline bool keep_match(const uint64_t* fwd, const uint64_t* second, const uint64_t* bwd, uint32_t nT,
-                                  uint32_t i, uint32_t num, uint32_t den)
-{
-    const uint64_t f = fwd[i];
-    const uint32_t t = (uint32_t)f;
-    if (t == ~0u || t >= nT) return false;
-    if (bwd != nullptr && (uint32_t)bwd[t] != i) return false;
-    if (second != nullptr) {
-        const uint64_t d1 = f >> 32, d2 = second[i] >> 32;
-        if (!((uint64_t)den * den * d1 < (uint64_t)num * num * d2)) return false;
-    }
-    return true;
-}
-
 constexpr int kFilterMaxTasks = 32;
 typedef ByValue<FilterTask, kFilterMaxTasks> FilterBatch;
 
-// one workgroup per task: flags, block-wide exclusive scan, ordered scatter of
-// the kept matches and (optionally) of their keypoint pairs
+// Queries are taken in rounds of kFilterThreads consecutive indices (thread t
+// takes index round * kFilterThreads + t, so every load is coalesced) and
+// kFilterRounds rounds at a time: all their loads and gathers are issued
+// before the first flag is needed. The kept flags of a round are ranked with
+// a wave ballot (v_mbcnt) plus the kept counts of the lower waves through LDS
+// (two barriers per round), so the output stays in ascending query order.
+constexpr int kFilterRounds = 8;
+constexpr int kFilterWaves = kFilterThreads / 64;
+
+__device__ void filter_body(const FilterTask& T, uint32_t num, uint32_t den)
+{
+    __shared__ uint32_t sWave[kFilterWaves];
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint32_t nQ = T.nQ;
+    const uint64_t nn = (uint64_t)num * num, dd = (uint64_t)den * den;
+    uint32_t base = 0;  // kept matches of the earlier rounds (the same in every thread)
+    for (uint32_t r0 = 0; r0 < nQ; r0 += kFilterRounds * kFilterThreads) {
+        uint64_t f[kFilterRounds];
+        uint32_t back[kFilterRounds];
+        uint64_t d2[kFilterRounds];
+#pragma unroll
+        for (int k = 0; k < kFilterRounds; k++) {
+            const uint32_t i = r0 + k * kFilterThreads + tid;
+            f[k] = i < nQ ? T.fwd[i] : kNoMatch;
+            d2[k] = (T.second != nullptr && i < nQ) ? T.second[i] : 0;
+        }
+#pragma unroll
+        for (int k = 0; k < kFilterRounds; k++) {
+            const uint32_t i = r0 + k * kFilterThreads + tid;
+            const uint32_t t = (uint32_t)f[k];
+            back[k] = (T.bwd != nullptr && t < T.nT) ? (uint32_t)T.bwd[t] : i;
+        }
+        uint32_t keepMask = 0;
+#pragma unroll
+        for (int k = 0; k < kFilterRounds; k++) {
+            const uint32_t i = r0 + k * kFilterThreads + tid;
+            const uint32_t t = (uint32_t)f[k];
+            bool keep = i < nQ && t != ~0u && t < T.nT && back[k] == i;
+            if (T.second != nullptr) keep = keep && dd * (f[k] >> 32) < nn * (d2[k] >> 32);
+            keepMask |= (uint32_t)keep << k;
+        }
+        float4 pt[kFilterRounds];
+        if (T.pts) {
+#pragma unroll
+            for (int k = 0; k < kFilterRounds; k++) {
+                if ((keepMask >> k) & 1) {
+                    const uint32_t i = r0 + k * kFilterThreads + tid;
+                    const float2 a = T.kpQ[i], b = T.kpT[(uint32_t)f[k]];
+                    pt[k] = make_float4(a.x, a.y, b.x, b.y);
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < kFilterRounds; k++) {
+            if (r0 + k * kFilterThreads >= nQ) break;  // the same in every thread
+            const bool keep = (keepMask >> k) & 1;
+            const uint64_t m = __ballot(keep);
+            const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+            if (lane == 0) sWave[wave] = (uint32_t)__popcll(m);
+            __syncthreads();
+            uint32_t pre = 0, tot = 0;
+#pragma unroll
+            for (uint32_t w = 0; w < kFilterWaves; w++) {
+                const uint32_t c = sWave[w];
+                pre += w < wave ? c : 0u;
+                tot += c;
+            }
+            __syncthreads();  // sWave is rewritten by the next round
+            if (keep) {
+                const uint32_t pos = base + pre + below;
+                const uint32_t i = r0 + k * kFilterThreads + tid;
+                if (T.out) {
+                    rsfm_valid_match mv;
+                    mv.idxQuery = i;
+                    mv.idxTrain = (uint32_t)f[k];
+                    mv.distance = (float)(uint32_t)(f[k] >> 32);
+                    T.out[pos] = mv;
+                }
+                if (T.pts) T.pts[pos] = pt[k];
+            }
+            base += tot;
+        }
+    }
+    if (tid == 0) *T.count = base;
+}
+
+// one workgroup per task
 template <class Src>
 __global__ __launch_bounds__(kFilterThreads) void filter_batch_kernel(const Src batch, uint32_t num, uint32_t den)
 {
-    const FilterTask& T = batch[blockIdx.x];
-    __shared__ uint32_t sScan[kFilterThreads];
-    const uint32_t nQ = T.nQ;
-    const uint32_t per = (nQ + kFilterThreads - 1) / kFilterThreads;
-    const uint32_t beg = min(nQ, threadIdx.x * per), end = min(nQ, beg + per);
-    uint32_t local = 0;
-    for (uint32_t i = beg; i < end; i++) local += keep_match(T.fwd, T.second, T.bwd, T.nT, i, num, den);
-    sScan[threadIdx.x] = local;
-    __syncthreads();
-    for (uint32_t off = 1; off < kFilterThreads; off <<= 1) {
-        const uint32_t v = threadIdx.x >= off ? sScan[threadIdx.x - off] : 0u;
-        __syncthreads();
-        sScan[threadIdx.x] += v;
-        __syncthreads();
-    }
-    uint32_t pos = sScan[threadIdx.x] - local;
-    for (uint32_t i = beg; i < end; i++) {
-        if (!keep_match(T.fwd, T.second, T.bwd, T.nT, i, num, den)) continue;
-        const uint64_t f = T.fwd[i];
-        const uint32_t t = (uint32_t)f;
-        if (T.out) {
-            rsfm_valid_match m;
-            m.idxQuery = i;
-            m.idxTrain = t;
-            m.distance = (float)(uint32_t)(f >> 32);
-            T.out[pos] = m;
-        }
-        if (T.pts) {
-            const float2 a = T.kpQ[i], b = T.kpT[t];
-            T.pts[pos] = make_float4(a.x, a.y, b.x, b.y);
-        }
-        pos++;
-    }
-    if (threadIdx.x == kFilterThreads - 1) *T.count = sScan[threadIdx.x];
+    filter_body(batch[blockIdx.x], num, den);
 }
 
 __global__ __launch_bounds__(kFilterThreads) void filter_kernel(const uint64_t* __restrict__ fwd,
@@ -211,30 +249,15 @@ __global__ __launch_bounds__(kFilterThreads) void filter_kernel(const uint64_t* 
                                                                 rsfm_valid_match* __restrict__ out,
                                                                 uint32_t* __restrict__ count)
 {
-    __shared__ uint32_t sScan[kFilterThreads];
-    const uint32_t per = (nQ + kFilterThreads - 1) / kFilterThreads;
-    const uint32_t beg = min(nQ, threadIdx.x * per), end = min(nQ, beg + per);
-    uint32_t local = 0;
-    for (uint32_t i = beg; i < end; i++) local += keep_match(fwd, second, bwd, nT, i, num, den);
-    sScan[threadIdx.x] = local;
-    __syncthreads();
-    for (uint32_t off = 1; off < kFilterThreads; off <<= 1) {
-        const uint32_t v = threadIdx.x >= off ? sScan[threadIdx.x - off] : 0u;
-        __syncthreads();
-        sScan[threadIdx.x] += v;
-        __syncthreads();
-    }
-    uint32_t pos = sScan[threadIdx.x] - local;
-    for (uint32_t i = beg; i < end; i++) {
-        if (!keep_match(fwd, second, bwd, nT, i, num, den)) continue;
-        const uint64_t f = fwd[i];
-        rsfm_valid_match m;
-        m.idxQuery = i;
-        m.idxTrain = (uint32_t)f;
-        m.distance = (float)(uint32_t)(f >> 32);
-        out[pos++] = m;
-    }
-    if (threadIdx.x == kFilterThreads - 1) *count = sScan[threadIdx.x];
+    FilterTask T{};
+    T.fwd = fwd;
+    T.second = second;
+    T.nQ = nQ;
+    T.bwd = bwd;
+    T.nT = nT;
+    T.out = out;
+    T.count = count;
+    filter_body(T, num, den);
 }
 
 int launch_filter_table(const FilterTask* tasks, uint32_t nTasks, uint32_t num, uint32_t den, hipStream_t stream)

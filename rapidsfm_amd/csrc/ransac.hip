@@ -207,7 +207,14 @@ __global__ __launch_bounds__(kThreads) void best_kernel(const Src batch, float t
     const uint32_t bestC = (uint32_t)(bk >> 32);
     if (threadIdx.x == 0) {
         float F[9];
-        const bool ok = T.H > 0 && bestC > 0 && fit_hypothesis<kMode>(T, n, bestH, s0, s1, F);
+        bool ok = T.H > 0 && bestC > 0;
+        if (T.models != nullptr) {
+            // the scorer stored every model; a nonzero count means its fit succeeded
+            if (ok)
+                for (int q = 0; q < 9; q++) F[q] = T.models[(size_t)bestH * 9 + q];
+        } else {
+            ok = ok && fit_hypothesis<kMode>(T, n, bestH, s0, s1, F);
+        }
         for (int q = 0; q < 9; q++) sF[q] = ok ? F[q] : 0.f;
         sOk = ok;
         if (T.best) {
