@@ -182,30 +182,38 @@ __global__ __launch_bounds__(kScoreBlock) void score_kernel(const Src batch, flo
     }
 }
 
-// best hypothesis (largest count, lowest index among equals) and its inlier mask
+// best hypothesis (largest count, lowest index among equals) and its inlier
+// mask: 16 wavefronts per task, the maximum key through wave shuffles and one
+// LDS round, the mask four points per thread and trip with the loads first
+constexpr int kBestThreads = 1024;
+
 template <int kMode, class Src>
-__global__ __launch_bounds__(kThreads) void best_kernel(const Src batch, float t2, float s0, float s1)
+__global__ __launch_bounds__(kBestThreads) void best_kernel(const Src batch, float t2, float s0, float s1)
 {
     const DevTask& T = batch[blockIdx.x];
     const uint32_t n = task_n(T);
-    __shared__ uint64_t sKey[kThreads];
+    constexpr int kWaves = kBestThreads / 64;
+    __shared__ uint64_t sKey[kWaves];
     __shared__ float sF[9];
     __shared__ uint32_t sOk;
+    const uint32_t tid = threadIdx.x;
     uint64_t key = 0;  // (count << 32) | ~h : max count, then lowest h
-    for (uint32_t h = threadIdx.x; h < T.H; h += kThreads) {
+    for (uint32_t h = tid; h < T.H; h += kBestThreads) {
         const uint64_t k = ((uint64_t)T.counts[h] << 32) | (uint32_t)~h;
         key = k > key ? k : key;
     }
-    sKey[threadIdx.x] = key;
-    __syncthreads();
-    for (int s = kThreads / 2; s > 0; s >>= 1) {
-        if (threadIdx.x < s) sKey[threadIdx.x] = max(sKey[threadIdx.x], sKey[threadIdx.x + s]);
-        __syncthreads();
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const uint64_t k = __shfl_xor(key, o);
+        key = k > key ? k : key;
     }
-    const uint64_t bk = sKey[0];
-    const uint32_t bestH = T.H ? ~(uint32_t)bk : 0u;
-    const uint32_t bestC = (uint32_t)(bk >> 32);
-    if (threadIdx.x == 0) {
+    if ((tid & 63) == 0) sKey[tid >> 6] = key;
+    __syncthreads();
+    if (tid == 0) {
+        uint64_t bk = 0;
+        for (int w = 0; w < kWaves; w++) bk = sKey[w] > bk ? sKey[w] : bk;
+        const uint32_t bestH = T.H ? ~(uint32_t)bk : 0u;
+        const uint32_t bestC = (uint32_t)(bk >> 32);
         float F[9];
         bool ok = T.H > 0 && bestC > 0;
         if (T.models != nullptr) {
@@ -226,8 +234,20 @@ __global__ __launch_bounds__(kThreads) void best_kernel(const Src batch, float t
     if (!T.mask) return;
     float F[9];
     for (int q = 0; q < 9; q++) F[q] = sF[q];
-    for (uint32_t i = threadIdx.x; i < T.nHost; i += kThreads)
-        T.mask[i] = (sOk && i < n) ? (uint8_t)model_inlier<kMode>(F, T.pts[i], t2) : 0;
+    const bool ok = sOk != 0;
+    for (uint32_t i0 = tid; i0 < T.nHost; i0 += 4 * kBestThreads) {
+        float4 p[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const uint32_t i = i0 + u * kBestThreads;
+            if (ok && i < n) p[u] = T.pts[i];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const uint32_t i = i0 + u * kBestThreads;
+            if (i < T.nHost) T.mask[i] = (ok && i < n) ? (uint8_t)model_inlier<kMode>(F, p[u], t2) : 0;
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -628,8 +648,8 @@ int launch_score_table(const DevTask* tasks, uint32_t nTasks, uint32_t maxH, flo
         else score_kernel<1><<<grid, kScoreBlock, 0, stream>>>(src, t2, minSepSq0, minSepSq1);
         RSFM_CHECK(check_launch("ransac score table"));
     }
-    if (mode == 0) best_kernel<0><<<nTasks, kThreads, 0, stream>>>(src, t2, minSepSq0, minSepSq1);
-    else best_kernel<1><<<nTasks, kThreads, 0, stream>>>(src, t2, minSepSq0, minSepSq1);
+    if (mode == 0) best_kernel<0><<<nTasks, kBestThreads, 0, stream>>>(src, t2, minSepSq0, minSepSq1);
+    else best_kernel<1><<<nTasks, kBestThreads, 0, stream>>>(src, t2, minSepSq0, minSepSq1);
     return check_launch("ransac best table");
 }
 
@@ -691,9 +711,9 @@ static int run_hypotheses(const rsfm_ransac_task* tasks, size_t nbTasks, float t
         bool anyBest = false;
         for (size_t i = 0; i < n; i++) anyBest |= batch.t[i].best != nullptr || batch.t[i].mask != nullptr;
         if (anyBest) {
-            if (kind == 0) best_kernel<0><<<(uint32_t)n, kThreads, 0, stream>>>(batch, t2, minSepSq0, minSepSq1);
-            else if (kind == 1) best_kernel<1><<<(uint32_t)n, kThreads, 0, stream>>>(batch, t2, minSepSq0, minSepSq1);
-            else best_kernel<2><<<(uint32_t)n, kThreads, 0, stream>>>(batch, t2, minSepSq0, minSepSq1);
+            if (kind == 0) best_kernel<0><<<(uint32_t)n, kBestThreads, 0, stream>>>(batch, t2, minSepSq0, minSepSq1);
+            else if (kind == 1) best_kernel<1><<<(uint32_t)n, kBestThreads, 0, stream>>>(batch, t2, minSepSq0, minSepSq1);
+            else best_kernel<2><<<(uint32_t)n, kBestThreads, 0, stream>>>(batch, t2, minSepSq0, minSepSq1);
             RSFM_CHECK(check_launch(name));
         }
     }
