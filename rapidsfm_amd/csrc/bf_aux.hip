@@ -151,20 +151,23 @@ typedef ByValue<FilterTask, kFilterMaxTasks> FilterBatch;
 
 // Queries are taken in rounds of kFilterThreads consecutive indices (thread t
 // takes index round * kFilterThreads + t, so every load is coalesced) and
-// kFilterRounds rounds at a time: all their loads and gathers are issued
-// before the first flag is needed. The kept flags of a round are ranked with
-// a wave ballot (v_mbcnt) plus the kept counts of the lower waves through LDS
-// (two barriers per round), so the output stays in ascending query order.
+// kFilterRounds rounds per trip: all their loads and gathers are issued
+// before the first flag is needed. A kept query's slot is its rank in its
+// wave's ballot (v_mbcnt) plus the kept counts of all (round, wave) pairs
+// before its own, so the output stays in ascending query order; three
+// barriers per trip.
 constexpr int kFilterRounds = 8;
 constexpr int kFilterWaves = kFilterThreads / 64;
 
 __device__ void filter_body(const FilterTask& T, uint32_t num, uint32_t den)
 {
-    __shared__ uint32_t sWave[kFilterWaves];
+    __shared__ uint32_t sCnt[kFilterRounds * kFilterWaves];
+    __shared__ uint32_t sPre[kFilterRounds * kFilterWaves];
+    __shared__ uint32_t sTot;
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t nQ = T.nQ;
     const uint64_t nn = (uint64_t)num * num, dd = (uint64_t)den * den;
-    uint32_t base = 0;  // kept matches of the earlier rounds (the same in every thread)
+    uint32_t base = 0;  // kept matches of the earlier trips (the same in every thread)
     for (uint32_t r0 = 0; r0 < nQ; r0 += kFilterRounds * kFilterThreads) {
         uint64_t f[kFilterRounds];
         uint32_t back[kFilterRounds];
@@ -201,36 +204,55 @@ __device__ void filter_body(const FilterTask& T, uint32_t num, uint32_t den)
                 }
             }
         }
+        // rank: kept counts per (round, wave) into LDS, one wave turns them
+        // into exclusive offsets in (round, wave) order, i.e. query order
+        uint32_t below[kFilterRounds];
 #pragma unroll
         for (int k = 0; k < kFilterRounds; k++) {
-            if (r0 + k * kFilterThreads >= nQ) break;  // the same in every thread
-            const bool keep = (keepMask >> k) & 1;
-            const uint64_t m = __ballot(keep);
-            const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-            if (lane == 0) sWave[wave] = (uint32_t)__popcll(m);
-            __syncthreads();
-            uint32_t pre = 0, tot = 0;
-#pragma unroll
-            for (uint32_t w = 0; w < kFilterWaves; w++) {
-                const uint32_t c = sWave[w];
-                pre += w < wave ? c : 0u;
-                tot += c;
-            }
-            __syncthreads();  // sWave is rewritten by the next round
-            if (keep) {
-                const uint32_t pos = base + pre + below;
-                const uint32_t i = r0 + k * kFilterThreads + tid;
-                if (T.out) {
-                    rsfm_valid_match mv;
-                    mv.idxQuery = i;
-                    mv.idxTrain = (uint32_t)f[k];
-                    mv.distance = (float)(uint32_t)(f[k] >> 32);
-                    T.out[pos] = mv;
-                }
-                if (T.pts) T.pts[pos] = pt[k];
-            }
-            base += tot;
+            const uint64_t m = __ballot((keepMask >> k) & 1);
+            below[k] = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+            if (lane == 0) sCnt[k * kFilterWaves + wave] = (uint32_t)__popcll(m);
         }
+        __syncthreads();
+        if (wave == 0) {
+            constexpr int kPerLane = kFilterRounds * kFilterWaves / 64;
+            uint32_t c[kPerLane], sum = 0;
+#pragma unroll
+            for (int j = 0; j < kPerLane; j++) {
+                c[j] = sCnt[lane * kPerLane + j];
+                sum += c[j];
+            }
+            uint32_t incl = sum;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const uint32_t v = __shfl_up(incl, o);
+                if (lane >= (uint32_t)o) incl += v;
+            }
+            uint32_t run = incl - sum;
+#pragma unroll
+            for (int j = 0; j < kPerLane; j++) {
+                sPre[lane * kPerLane + j] = run;
+                run += c[j];
+            }
+            if (lane == 63) sTot = incl;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kFilterRounds; k++) {
+            if (!((keepMask >> k) & 1)) continue;
+            const uint32_t pos = base + sPre[k * kFilterWaves + wave] + below[k];
+            const uint32_t i = r0 + k * kFilterThreads + tid;
+            if (T.out) {
+                rsfm_valid_match mv;
+                mv.idxQuery = i;
+                mv.idxTrain = (uint32_t)f[k];
+                mv.distance = (float)(uint32_t)(f[k] >> 32);
+                T.out[pos] = mv;
+            }
+            if (T.pts) T.pts[pos] = pt[k];
+        }
+        base += sTot;
+        __syncthreads();  // sCnt, sPre and sTot are rewritten by the next trip
     }
     if (tid == 0) *T.count = base;
 }
