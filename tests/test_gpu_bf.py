@@ -189,3 +189,36 @@ def test_8k_properties(rs, orc, dev):
     cols = rng.choice(8192, 96, replace=False)
     _, rbc = orc.bf_match_u8(query, train[cols])
     assert np.array_equal(b[cols], rbc)
+
+
+@pytest.mark.parametrize("nq", [1, 1025, 8192, 8193, 20000])
+def test_filter_trip_boundaries(rs, orc, dev, nq):
+    """The device filter takes 1024 queries per round and 8192 per trip; match
+    tables of these sizes (about half the forward matches mutual, 5% "no match")
+    must give the oracle's ordered output with and without each input."""
+    _say("filter: synthetic match tables, nQ", nq)
+    rng = np.random.default_rng(nq)
+    nt = 12000
+    dist1 = rng.integers(0, 1 << 20, nq).astype(np.uint64)
+    idx = rng.integers(0, nt, nq).astype(np.uint64)
+    idx[rng.random(nq) < 0.05] = 0xFFFFFFFF
+    fwd = (dist1 << np.uint64(32)) | idx
+    dist2 = dist1 + rng.integers(0, 1 << 19, nq).astype(np.uint64)
+    second = (dist2 << np.uint64(32)) | rng.integers(0, nt, nq).astype(np.uint64)
+    bq = rng.integers(0, nq, nt).astype(np.uint64)
+    sel = np.nonzero((idx != 0xFFFFFFFF) & (rng.random(nq) < 0.6))[0]
+    bq[idx[sel].astype(np.int64)] = sel.astype(np.uint64)
+    bwd = (rng.integers(0, 1 << 20, nt).astype(np.uint64) << np.uint64(32)) | bq
+
+    def host(a):
+        return None if a is None else a.view(np.uint32).reshape(-1, 2)
+
+    def device(a):
+        return None if a is None else torch.from_numpy(a.view(np.int32).reshape(-1, 2).copy()).to(dev)
+
+    for sec, bw in [(second, bwd), (None, bwd), (second, None)]:
+        ref = orc.filter_matches(host(fwd), host(sec), host(bw), (4, 5))
+        out, cnt = rs.filter_matches(device(fwd), device(sec), device(bw), nt, (4, 5))
+        n = int(cnt.item())
+        assert n == ref.shape[0]
+        assert np.array_equal(out[:n].cpu().numpy().view(np.uint32), ref)
