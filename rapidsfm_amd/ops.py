@@ -87,12 +87,19 @@ def bf_match_top2(pairs, stream=None, workspace=None):
     tp = ctypes.cast(tasks, ctypes.c_void_p)
     need = ctypes.c_size_t(0)
     check(lib.rsfm_bf_match_top2_workspace_size(tp, len(pairs), ctypes.byref(need)), "top2 workspace size")
-    if need.value > 0 and (workspace is None or workspace.numel() < need.value):
-        dev = pairs[0][0].device
-        workspace = torch.empty(need.value + 256, dtype=torch.uint8, device=dev)
     ws_ptr = 0
     if need.value > 0:
-        ws_ptr = (workspace.data_ptr() + 255) // 256 * 256
+        # the library wants a 256-B aligned pointer with need bytes behind it
+        fits = False
+        if workspace is not None:
+            ws_ptr = (workspace.data_ptr() + 255) // 256 * 256
+            fits = workspace.numel() * workspace.element_size() - (ws_ptr - workspace.data_ptr()) >= need.value
+        if not fits:
+            workspace = torch.empty(need.value + 256, dtype=torch.uint8, device=pairs[0][0].device)
+            ws_ptr = (workspace.data_ptr() + 255) // 256 * 256
+            if stream is not None:
+                # the temporary is freed on return: keep it until the work on stream is done
+                workspace.record_stream(stream)
     check(lib.rsfm_bf_match_top2(tp, len(pairs), ctypes.c_void_p(ws_ptr), need.value, _stream(stream)), "bf_match_top2")
     return outs
 

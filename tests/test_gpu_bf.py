@@ -222,3 +222,23 @@ def test_filter_trip_boundaries(rs, orc, dev, nq):
         n = int(cnt.item())
         assert n == ref.shape[0]
         assert np.array_equal(out[:n].cpu().numpy().view(np.uint32), ref)
+
+
+def test_top2_caller_workspace_and_stream(rs, orc, dev):
+    """A caller workspace at an unaligned offset, too small after alignment,
+    and a side stream: the wrapper must realign or replace it and the results
+    must not change."""
+    _say("top2 with caller workspaces and a side stream")
+    A = orc.gen_random_u8(301, 700)
+    B = orc.gen_random_u8(302, 1500)
+    qa, qs, qb = orc.bf_match_top2_u8(A, B)
+    pair = [(_to_dev(A, dev), _to_dev(B, dev))]
+    big = torch.empty(4 << 20, dtype=torch.uint8, device=dev)
+    side = torch.cuda.Stream()
+    for ws, st in [(big[3:], None), (big[: 1 << 10], None), (None, side), (big[5:], side)]:
+        with torch.cuda.stream(side if st is not None else torch.cuda.current_stream()):
+            (ha, hs, hb), = rs.bf_match_top2(pair, stream=st, workspace=ws)
+        torch.cuda.synchronize()
+        assert np.array_equal(_np(ha), qa)
+        assert np.array_equal(_np(hs), qs)
+        assert np.array_equal(_np(hb), qb)
