@@ -235,8 +235,10 @@ __device__ __forceinline__ void match_tile(const DevTask& T, uint32_t rowBase, u
     const uint8_t* aRow = sA + rloc * 128;
     const int aSw = swz(rloc);
 
-    // chunk c: stage c+2 (loaded one chunk earlier), MFMAs of chunk c (K-step major over both column
-    // blocks), then the keys of chunk c
+    // chunk c: staging (register path: store chunk c+2, loaded one chunk
+    // earlier, and load chunk c+3 after the MFMAs; direct path: see below),
+    // MFMAs of chunk c (K-step major over both column blocks), then the keys
+    // of chunk c
     // with prepared data the trip's first chunk c copies chunks c+2 and c+3
     // (their ring slots were last read before the previous trip's barrier) and
     // the trip's barrier waits for both
