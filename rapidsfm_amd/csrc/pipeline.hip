@@ -411,6 +411,10 @@ extern "C" int rsfm_pipeline_create(const rsfm_pipeline_pair* pairs, size_t nbPa
     const size_t oFill = take(std::max<size_t>(1, fill.size()) * sizeof(pipe::FillRange));
     const size_t oPart = take(std::max<size_t>(1, pl->partCap) * sizeof(uint64_t));
     const size_t oPrep = take(std::max<size_t>(1, worst.prepBytes));
+    // 8-point solver: every hypothesis' model, so the best-model kernel reads
+    // the winner instead of refitting it on one lane
+    const size_t modelFloats = (size_t)cfg->nbHyp * 9;
+    const size_t oModels = take(degensac ? 1 : std::max<size_t>(1, (size_t)P * modelFloats * sizeof(float)));
     rc = check_hip(hipMalloc(reinterpret_cast<void**>(&pl->arena), off), "pipeline workspace");
     if (rc != RSFM_OK) return fail_out(rc);
     pl->dTasks = reinterpret_cast<bf::DevTask*>(pl->arena + oTasks);
@@ -428,6 +432,7 @@ extern "C" int rsfm_pipeline_create(const rsfm_pipeline_pair* pairs, size_t nbPa
     pl->dFill = reinterpret_cast<pipe::FillRange*>(pl->arena + oFill);
     pl->dPart = reinterpret_cast<uint64_t*>(pl->arena + oPart);
     pl->dPrep = pl->arena + oPrep;
+    float* dModels = reinterpret_cast<float*>(pl->arena + oModels);
     for (auto& x : pl->pairs)
         if (x.nA > 0) pl->maxPrepB = std::max(pl->maxPrepB, x.nB);
 
@@ -449,7 +454,7 @@ extern "C" int rsfm_pipeline_create(const rsfm_pipeline_pair* pairs, size_t nbPa
                                 reinterpret_cast<float4*>(x.pts)};
         st[i] = ransac::SampleTask{x.sampleIdx, cfg->nbHyp, x.nA, x.count, cfg->seed + i, degensac ? 7u : 8u};
         rt[i] = ransac::DevTask{reinterpret_cast<const float4*>(x.pts), x.count, x.nA, cfg->nbHyp, x.sampleIdx,
-                                x.counts, nullptr, x.best, x.inlierMask};
+                                x.counts, degensac ? nullptr : dModels + i * modelFloats, x.best, x.inlierMask};
         if (x.distortion) {
             rsfm_real_camera c0{x.cam0, {}}, c1{x.cam1, {}};
             for (int k = 0; k < 5; k++) {
