@@ -173,6 +173,28 @@ int rsfm_cashash_find_best_match(const rsfm_cashash_image* queries, const uint32
                                  const uint32_t* maxQueryBucketSize, uint32_t maxTrainBucketSize, uint32_t nbSchemes,
                                  hipStream_t stream);
 
+/* Config-1 matching mode (no reference counterpart: the reference computes the
+ * 128-bit remap hash, computeHash.cu:474-496, but never matches with it, and its
+ * matchWithoutRemap.cu is empty): for every query descriptor the candidates are
+ * the train descriptors sharing its bucket in any of the first nbSchemes tables
+ * (6 in the cascade-hashing paper); they are ranked by the Hamming distance of
+ * the 128-bit remap hashes (ties: lower index), and the first `keep` (8, 16 or
+ * 32) are re-ranked by exact L2^2 (ties: lower index). results.train, when
+ * set, gets the same search with the roles of the images swapped. No
+ * candidate -> {~0u, UINT32_MAX}. At most 2^24 descriptors per image. */
+typedef struct rsfm_cashash_rerank_image {
+    const void* desc;           /* u8 [nbDesc][128], 16-byte aligned */
+    const uint8_t* codes;       /* [nbDesc][8] bucket codes, 8-byte aligned */
+    const uint32_t* remap;      /* [nbDesc][4] remap hash, 16-byte aligned */
+    const uint32_t* bucketEnds; /* [8][256] inclusive prefix */
+    const uint32_t* buckets;    /* [8][nbDesc] */
+    uint32_t nbDesc;
+} rsfm_cashash_rerank_image;
+int rsfm_cashash_find_best_match_rerank(const rsfm_cashash_rerank_image* queries,
+                                        const rsfm_cashash_rerank_image* train,
+                                        const rsfm_cashash_result* results, uint32_t nbTasks, uint32_t nbSchemes,
+                                        uint32_t keep, hipStream_t stream);
+
 /* ---------------------------------------------------------------------------
  * Match filtering
  * ------------------------------------------------------------------------ */
@@ -271,6 +293,47 @@ typedef struct rsfm_ransac_task {
  * mode 1: Sampson distance < t. */
 int rsfm_ransac_f8(const rsfm_ransac_task* tasks, size_t nbTasks, float threshold, int mode, float minSepSq0,
                    float minSepSq1, hipStream_t stream);
+
+/* The end of findEpipolarityRansacWithInsufficientTiePts<0> after the
+ * hypotheses of rsfm_ransac_f8 are scored (ImagePair.cpp:185-203, with the
+ * adaptive stop of ransac<F,8>, ransac.hpp:139-160):
+ *  - selection: with confidence > 0 the scored table is walked in hypothesis
+ *    order as the reference's loop walks its draws: zero counts are "insane"
+ *    and end the walk after more than min(128, maxIter) in a row; a strictly
+ *    larger count becomes the best and sets the required iteration count
+ *    min(maxIter, ceil(log2(1 - confidence) / log2(1 - (best / n)^8))); the
+ *    walk ends when the sane hypotheses seen reach it, or at the end of the
+ *    table. confidence <= 0 takes the whole table. The best hypothesis is the
+ *    first with the largest count among the hypotheses walked; with none
+ *    (every count 0) the reference's fallback sample {0, ..., 7} is fitted.
+ *    The logarithms are the library's own (frexp plus an atanh series in
+ *    FP64), so CPU and GPU agree on every cut;
+ *  - refit (refit != 0): with >= 8 inliers of the best F, F is refitted by
+ *    least squares on all of its inliers (Hartley normalisation and normal
+ *    matrix in FP64, null vector by Cholesky and inverse iteration, rank 2 by
+ *    a 3x3 Jacobi), and replaces the best F when it has more inliers. The
+ *    reference's mask2pts gathers rows by the running count instead of the
+ *    point index (legacy/geometry.cpp:341-354), i.e. refits on the first k
+ *    points; this entry refits on the inliers, as the call site intends.
+ * result: [0] best hypothesis (~0u when every count was 0), [1] inliers of
+ * its F, [2] hypotheses walked, [3] inliers of the final F (> [1] exactly
+ * when the refit was kept). F is row-major, unit Frobenius (zero when no F
+ * could be fitted); inlierMask holds the final F's inliers. n < 8 gives
+ * result {~0u, 0, 0, 0}. */
+typedef struct rsfm_ransac_finish_task {
+    const rsfm_pt_pair* pts;    /* nbPts correspondences, 16-byte aligned */
+    uint32_t nbPts;             /* capacity / count when nbPtsDev is NULL */
+    const uint32_t* nbPtsDev;   /* optional device count (capped at nbPts) */
+    const uint32_t* sampleIdx;  /* [nbHyp][8] the scored samples */
+    const uint32_t* counts;     /* [nbHyp] their counts (rsfm_ransac_f8, same threshold and mode) */
+    const float* models;        /* optional [nbHyp][9] their models; NULL: the best sample is refitted */
+    uint32_t nbHyp;
+    float* F;                   /* [9] final F */
+    uint32_t* result;           /* [4] */
+    uint8_t* inlierMask;        /* optional [nbPts] */
+} rsfm_ransac_finish_task;
+int rsfm_ransac_f8_finish(const rsfm_ransac_finish_task* tasks, size_t nbTasks, float threshold, int mode,
+                          float confidence, uint32_t maxIter, int refit, hipStream_t stream);
 
 /* 4-point homography hypotheses: replaces the per-hypothesis body of
  * ransac<H,4> in findHomographyRansacWithInsufficientTiePts<0>
@@ -498,6 +561,13 @@ typedef struct rsfm_pipeline_config {
     uint32_t nbCandidates;     /* DEGENSAC: best 7-point samples replayed */
     uint32_t nbPlaneSamples;   /* DEGENSAC: plane-and-parallax samples per candidate */
     float frontRatio;          /* pose stage: disambiguation ratio (the reference's 0.95) */
+    /* 8-point solver: the end of the reference's driver (rsfm_ransac_f8_finish) as a graph
+     * stage when refit != 0 or confidence > 0; every pair then needs F and solverResult,
+     * which receive the final F and the finisher's result [4], and inlierMask holds the
+     * final F's inliers */
+    float confidence;          /* adaptive cut of the hypothesis table, <= 0 = whole table */
+    uint32_t maxIter;          /* the reference's max_iter (5000) for the cut */
+    uint32_t refit;            /* refit F on the best sample's inliers */
 } rsfm_pipeline_config;
 #define RSFM_PIPELINE_WG_PER_TILE 0xFFFFFFFFu
 /* solvers: 8-point F RANSAC (samples, scores, best and inlier mask), or the
@@ -519,6 +589,8 @@ typedef struct rsfm_pipeline_info_t {
 typedef struct rsfm_pipeline rsfm_pipeline;
 int rsfm_pipeline_create(const rsfm_pipeline_pair* pairs, size_t nbPairs, const rsfm_pipeline_config* cfg,
                          rsfm_pipeline** out);
+/* one run in flight at a time: the graph's stages share the pipeline's task tables and
+ * partials; rsfm_pipeline_destroy waits for the last run's stream */
 int rsfm_pipeline_run(rsfm_pipeline* pipeline, hipStream_t stream);
 int rsfm_pipeline_info(const rsfm_pipeline* pipeline, rsfm_pipeline_info_t* info, rsfm_pipeline_tuning_entry* entries,
                        size_t maxEntries, size_t* nbEntries);

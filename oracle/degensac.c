@@ -576,3 +576,226 @@ void or_degensac_refine(const float* pts, uint32_t n, const uint32_t* idx, const
     free(c.bits);
     free(c.planar);
 }
+
+/* ---- the end of the 8-point F-RANSAC driver: adaptive cut and refit on all
+ * inliers (ransac.hpp:139-160, ImagePair.cpp:185-203), restated for
+ * rsfm_ransac_f8_finish ---- */
+
+/* log2 of x in (0, 1] (0 -> -inf, negative -> NaN): x = m 2^e with m in [sqrt(1/2), sqrt(2)], ln m =
+ * 2 atanh((m - 1) / (m + 1)) by its series to z^11 (z = s^2), evaluated as
+ * s (2 + z P(z)) in Horner order. The GPU repeats every operation; the result
+ * is pinned against glibc's log2 in tests. */
+double or_det_log2(double x)
+{
+    if (x == 1.0) return 0.0;
+    if (x == 0.0) return -INFINITY;
+    if (!(x > 0.0)) return NAN; /* negative or NaN, as libm */
+    uint64_t bits;
+    memcpy(&bits, &x, 8);
+    int e = (int)((bits >> 52) & 0x7FF) - 1023;
+    bits = (bits & 0x000FFFFFFFFFFFFFull) | 0x3FF0000000000000ull;
+    double m;
+    memcpy(&m, &bits, 8);
+    if (m > 1.4142135623730951) {
+        m = m * 0.5;
+        e = e + 1;
+    }
+    const double s = (m - 1.0) / (m + 1.0);
+    const double z = s * s;
+    static const double coef[11] = {2.0 / 23.0, 2.0 / 21.0, 2.0 / 19.0, 2.0 / 17.0, 2.0 / 15.0, 2.0 / 13.0,
+                                    2.0 / 11.0, 2.0 / 9.0,  2.0 / 7.0,  2.0 / 5.0,  2.0 / 3.0};
+    double p = coef[0];
+    for (int k = 1; k < 11; k++) p = p * z + coef[k];
+    const double lnm = s * (2.0 + z * p);
+    return (double)e + lnm * 1.4426950408889634;
+}
+
+/* ransac.hpp:155-157 with or_det_log2 and (best / n)^8 by three squarings.
+ * Non-finite quotients convert as the reference built for x86-64 does:
+ * -inf and NaN -> maxIter, +inf -> 0. */
+uint32_t or_ransac_required_iters_det(uint32_t best, uint32_t n, float conf, uint32_t max_iter)
+{
+    const double num = or_det_log2(1.0 - (double)conf);
+    const double p = (double)best / (double)n;
+    const double p2 = p * p, p4 = p2 * p2, p8 = p4 * p4;
+    const double q = num / or_det_log2(1.0 - p8);
+    if (q != q || q == -INFINITY) return max_iter;
+    if (q == INFINITY) return 0u;
+    const double c = ceil(q);
+    if (!(c > 0.0)) return 0u;
+    if (c >= (double)max_iter) return max_iter;
+    return (uint32_t)c;
+}
+
+/* the reference's loop (ransac.hpp:139-160) walking a scored table instead of
+ * drawing: returns the hypotheses walked; best_h = ~0u when none was sane */
+uint32_t or_ransac_adaptive_cut(const uint32_t* counts, uint32_t H, uint32_t n, float conf, uint32_t max_iter,
+                                uint32_t* best_h, uint32_t* best_count)
+{
+    uint32_t best = 0, bh = ~0u, required = max_iter, iters = 0, insane = 0;
+    const uint32_t cap = max_iter < 128u ? max_iter : 128u;
+    uint32_t h = 0;
+    while (h < H) {
+        const uint32_t r = counts[h++];
+        if (r == 0) {
+            insane++;
+            if (insane > cap) break;
+            continue;
+        }
+        insane = 0;
+        if (r > best) {
+            best = r;
+            bh = h - 1;
+            required = or_ransac_required_iters_det(best, n, conf, max_iter);
+        }
+        iters++;
+        if (iters >= required) break;
+    }
+    *best_h = bh;
+    *best_count = best;
+    return h;
+}
+
+#define FIN_THREADS 256u
+
+/* least-squares F on the inliers of F0 (m of them) in the GPU's summation
+ * order: thread t of 256 sums points t, t + 256, ... in index order, the 256
+ * partials are added in thread order */
+static int refit_all(const float* pts, uint32_t n, const float F0[9], float t, int mode, uint32_t m, float G[9])
+{
+    const double dm = (double)m;
+    double part[45][FIN_THREADS];
+    double tot[45];
+    for (uint32_t th = 0; th < FIN_THREADS; th++) {
+        double a[4] = {0.0, 0.0, 0.0, 0.0};
+        for (uint32_t i = th; i < n; i += FIN_THREADS) {
+            const float* p = pts + (size_t)i * 4;
+            if (!or_f_inlier_f32(F0, p, t, mode)) continue;
+            for (int k = 0; k < 4; k++) a[k] = a[k] + (double)p[k];
+        }
+        for (int k = 0; k < 4; k++) part[k][th] = a[k];
+    }
+    for (int k = 0; k < 4; k++) {
+        double a = 0.0;
+        for (uint32_t th = 0; th < FIN_THREADS; th++) a = a + part[k][th];
+        tot[k] = a;
+    }
+    dg_norm nm;
+    for (int k = 0; k < 4; k++) nm.c[k] = tot[k] / dm;
+    for (uint32_t th = 0; th < FIN_THREADS; th++) {
+        double d0 = 0.0, d1 = 0.0;
+        for (uint32_t i = th; i < n; i += FIN_THREADS) {
+            const float* p = pts + (size_t)i * 4;
+            if (!or_f_inlier_f32(F0, p, t, mode)) continue;
+            const double dx0 = (double)p[0] - nm.c[0], dy0 = (double)p[1] - nm.c[1];
+            const double dx1 = (double)p[2] - nm.c[2], dy1 = (double)p[3] - nm.c[3];
+            d0 = d0 + sqrt(dx0 * dx0 + dy0 * dy0);
+            d1 = d1 + sqrt(dx1 * dx1 + dy1 * dy1);
+        }
+        part[0][th] = d0;
+        part[1][th] = d1;
+    }
+    for (int k = 0; k < 2; k++) {
+        double a = 0.0;
+        for (uint32_t th = 0; th < FIN_THREADS; th++) a = a + part[k][th];
+        tot[k] = a;
+    }
+    const double n0 = tot[0] / dm, n1 = tot[1] / dm;
+    if (!(n0 > 0.0) || !(n1 > 0.0)) return 0;
+    nm.s0 = kSqrt2 / n0;
+    nm.s1 = kSqrt2 / n1;
+    for (uint32_t th = 0; th < FIN_THREADS; th++) {
+        double acc[45];
+        for (int e = 0; e < 45; e++) acc[e] = 0.0;
+        for (uint32_t i = th; i < n; i += FIN_THREADS) {
+            const float* p = pts + (size_t)i * 4;
+            if (!or_f_inlier_f32(F0, p, t, mode)) continue;
+            const double x0 = ((double)p[0] - nm.c[0]) * nm.s0, y0 = ((double)p[1] - nm.c[1]) * nm.s0;
+            const double x1 = ((double)p[2] - nm.c[2]) * nm.s1, y1 = ((double)p[3] - nm.c[3]) * nm.s1;
+            const double r[9] = {x1 * x0, y1 * x0, x0, x1 * y0, y1 * y0, y0, x1, y1, 1.0};
+            int e = 0;
+            for (int u = 0; u < 9; u++)
+                for (int v = u; v < 9; v++) {
+                    acc[e] = acc[e] + r[u] * r[v];
+                    e++;
+                }
+        }
+        for (int e = 0; e < 45; e++) part[e][th] = acc[e];
+    }
+    double N[9][9];
+    int e = 0;
+    for (int u = 0; u < 9; u++)
+        for (int v = u; v < 9; v++) {
+            double a = 0.0;
+            for (uint32_t th = 0; th < FIN_THREADS; th++) a = a + part[e][th];
+            N[u][v] = a;
+            N[v][u] = a;
+            e++;
+        }
+    double v[9];
+    return null_vector9(N, v) && finish_f(v, &nm, G);
+}
+
+static uint32_t count_inliers_f(const float* pts, uint32_t n, const float F[9], float t, int mode)
+{
+    uint32_t k = 0;
+    for (uint32_t i = 0; i < n; i++) k += (uint32_t)or_f_inlier_f32(F, pts + (size_t)i * 4, t, mode);
+    return k;
+}
+
+int or_f8_finish(const float* pts, uint32_t n, const uint32_t* idx, const uint32_t* counts, const float* models,
+                 uint32_t H, float t, int mode, float conf, uint32_t max_iter, int refit, float F[9],
+                 uint32_t result[4], uint8_t* mask)
+{
+    if (n < 8) {
+        memset(F, 0, 9 * sizeof(float));
+        result[0] = ~0u;
+        result[1] = result[2] = result[3] = 0u;
+        if (mask) memset(mask, 0, n);
+        return 0;
+    }
+    uint32_t bh = ~0u, bc = 0, walked = H;
+    if (conf > 0.f) {
+        walked = or_ransac_adaptive_cut(counts, H, n, conf, max_iter, &bh, &bc);
+    } else {
+        for (uint32_t h = 0; h < H; h++)
+            if (counts[h] > bc) {
+                bc = counts[h];
+                bh = h;
+            }
+    }
+    float F0[9];
+    int ok;
+    if (bh != ~0u && models) {
+        memcpy(F0, models + (size_t)bh * 9, sizeof F0);
+        ok = 1;
+    } else {
+        float p8[32];
+        for (int i = 0; i < 8; i++) {
+            const uint32_t k = bh != ~0u ? idx[(size_t)bh * 8 + i] : (uint32_t)i;
+            memcpy(p8 + 4 * i, pts + (size_t)(k < n ? k : 0) * 4, 16);
+        }
+        ok = or_f8_fit_f32(p8, F0);
+    }
+    if (!ok) memset(F0, 0, sizeof F0);
+    const uint32_t c0 = ok ? count_inliers_f(pts, n, F0, t, mode) : 0u;
+    uint32_t cf = c0;
+    if (refit && ok && c0 >= 8) {
+        float G[9];
+        if (refit_all(pts, n, F0, t, mode, c0, G)) {
+            const uint32_t c1 = count_inliers_f(pts, n, G, t, mode);
+            if (c1 > c0) {
+                cf = c1;
+                memcpy(F0, G, sizeof G);
+            }
+        }
+    }
+    memcpy(F, F0, sizeof F0);
+    result[0] = bh;
+    result[1] = c0;
+    result[2] = walked;
+    result[3] = cf;
+    if (mask)
+        for (uint32_t i = 0; i < n; i++) mask[i] = ok ? (uint8_t)or_f_inlier_f32(F0, pts + (size_t)i * 4, t, mode) : 0;
+    return ok;
+}

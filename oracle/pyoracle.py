@@ -21,7 +21,12 @@ def build(ref_tree="/root/reference"):
         subprocess.run(["make", "-s", "-C", HERE, "ref", "REF=" + ref_tree], check=True)
 
 
-if not os.path.exists(LIB_PATH):
+# a second instance of this module can bind another build of the same sources
+# (the CPU baseline's -march=native library): oracle/cpu_baseline.py
+_OTHER = os.environ.get("RSFM_ORACLE_LIB_OVERRIDE")
+if _OTHER:
+    LIB_PATH = _OTHER
+elif not os.path.exists(LIB_PATH):
     build()
 _lib = ctypes.CDLL(LIB_PATH)
 
@@ -103,6 +108,10 @@ _decl("or_h_from_f", I32, [P, P, P])
 _decl("or_f_from_h", I32, [P, P, P, P])
 _decl("or_h_inlier_f32", I32, [P, P, F32])
 _decl("or_degensac_refine", None, [P, U32, P, P, U32, F32, F32, I32, U32, U32, U64, P, P, P, P, P])
+_decl("or_det_log2", F64, [F64])
+_decl("or_ransac_required_iters_det", U32, [U32, U32, F32, U32])
+_decl("or_ransac_adaptive_cut", U32, [P, U32, U32, F32, U32, ctypes.POINTER(U32), ctypes.POINTER(U32)])
+_decl("or_f8_finish", I32, [P, U32, P, P, P, U32, F32, I32, F32, U32, I32, P, P, P])
 _decl("or_mlp_forward", None, [P, P, U32, P])
 _decl("or_mlp_loss_grad", F64, [P, P, P, U32, P])
 _decl("or_mlp_forward_bf16", None, [P, P, U32, P])
@@ -452,6 +461,38 @@ def f8_ransac_adaptive(pts, t, mode, confidence, max_iter, seed, min_sep_sq0=0.0
 
 def required_iters(best, n, k, conf, max_iter):
     return _lib.or_ransac_required_iters(best, n, k, conf, max_iter)
+
+
+def det_log2(x):
+    return _lib.or_det_log2(float(x))
+
+
+def required_iters_det(best, n, conf, max_iter):
+    return _lib.or_ransac_required_iters_det(best, n, conf, max_iter)
+
+
+def adaptive_cut(counts, n, conf, max_iter):
+    """(walked, best hypothesis or ~0, its count) of the reference loop over a scored table"""
+    counts = _c(counts, np.uint32)
+    bh, bc = U32(0), U32(0)
+    walked = _lib.or_ransac_adaptive_cut(_p(counts), counts.shape[0], n, conf, max_iter, ctypes.byref(bh),
+                                         ctypes.byref(bc))
+    return walked, bh.value, bc.value
+
+
+def f8_finish(pts, idx, counts, models, t, mode, conf=0.0, max_iter=5000, refit=True):
+    """rsfm_ransac_f8_finish for one pair: (F (3, 3), result (4,), mask (n,))"""
+    pts = _c(pts, np.float32).reshape(-1, 4)
+    idx = _c(idx, np.uint32).reshape(-1, 8)
+    counts = _c(counts, np.uint32)
+    mdl = _c(models, np.float32).reshape(-1, 9) if models is not None else None
+    n = pts.shape[0]
+    F = np.zeros(9, np.float32)
+    res = np.zeros(4, np.uint32)
+    mask = np.zeros(max(n, 1), np.uint8)
+    _lib.or_f8_finish(_p(pts), n, _p(idx), _p(counts), _p(mdl) if mdl is not None else None, counts.shape[0], t,
+                      mode, conf, max_iter, int(refit), _p(F), _p(res), _p(mask))
+    return F.reshape(3, 3), res, mask[:n]
 
 
 # ---------------- learned proposal MLP ----------------

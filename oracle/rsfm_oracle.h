@@ -201,6 +201,16 @@ void or_undistort_in_place(float* px, float* py, uint32_t n, const float cam[4],
 /* least-squares F (m >= 8) / H (m >= 4) on pts[sel[0..m)], unit Frobenius */
 int  or_fit_ls_f(const float* pts, const uint32_t* sel, uint32_t m, float F[9]);
 int  or_fit_ls_h(const float* pts, const uint32_t* sel, uint32_t m, float H[9]);
+/* end of the 8-point F-RANSAC driver (ransac.hpp:139-160, ImagePair.cpp:185-203):
+ * own FP64 log2, required iterations, the adaptive walk over a scored table and
+ * rsfm_ransac_f8_finish restated (result = best, its count, walked, final count) */
+double   or_det_log2(double x);
+uint32_t or_ransac_required_iters_det(uint32_t best, uint32_t n, float conf, uint32_t max_iter);
+uint32_t or_ransac_adaptive_cut(const uint32_t* counts, uint32_t H, uint32_t n, float conf, uint32_t max_iter,
+                                uint32_t* best_h, uint32_t* best_count);
+int      or_f8_finish(const float* pts, uint32_t n, const uint32_t* idx, const uint32_t* counts, const float* models,
+                      uint32_t H, float t, int mode, float conf, uint32_t max_iter, int refit, float F[9],
+                      uint32_t result[4], uint8_t* mask);
 /* H compatible with F through three correspondences (3 x 4) */
 int  or_h_from_f(const float F[9], const float* pts3, float H[9]);
 /* F = [e']x H through two correspondences */

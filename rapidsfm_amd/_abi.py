@@ -64,6 +64,21 @@ class RansacTask(ctypes.Structure):
     ]
 
 
+class FinishTask(ctypes.Structure):
+    _fields_ = [
+        ("pts", c_void_p),
+        ("nbPts", c_u32),
+        ("nbPtsDev", c_void_p),
+        ("sampleIdx", c_void_p),
+        ("counts", c_void_p),
+        ("models", c_void_p),
+        ("nbHyp", c_u32),
+        ("F", c_void_p),
+        ("result", c_void_p),
+        ("inlierMask", c_void_p),
+    ]
+
+
 class DegensacTask(ctypes.Structure):
     _fields_ = [
         ("pts", c_void_p),
@@ -182,6 +197,7 @@ _SIGS = {
         [c_void_p, c_void_p, c_u32, c_void_p, c_u32, c_u32, c_u32, c_void_p, c_void_p, c_void_p, c_size, c_void_p],
     ),
     "rsfm_ransac_f8": (c_int, [c_void_p, c_size, c_float, c_int, c_float, c_float, c_void_p]),
+    "rsfm_ransac_f8_finish": (c_int, [c_void_p, c_size, c_float, c_int, c_float, c_u32, c_int, c_void_p]),
     "rsfm_ransac_f7": (c_int, [c_void_p, c_size, c_float, c_int, c_float, c_float, c_void_p]),
     "rsfm_ransac_h4": (c_int, [c_void_p, c_size, c_float, c_float, c_float, c_void_p]),
     "rsfm_pose_from_f": (c_int, [c_void_p, c_size, c_float, c_void_p]),

@@ -22,6 +22,7 @@ SOURCES = [
     "bf_match.hip",
     "bf_aux.hip",
     "ransac.hip",
+    "ransac_finish.hip",
     "cashash.hip",
     "rmf.hip",
     "mlp.hip",

@@ -99,8 +99,31 @@ struct SampleTask {
     uint32_t k;  // indices per sample (4..8), the first k draws of the sequence
 };
 int launch_sample_table(const SampleTask* tasks, uint32_t nTasks, uint32_t maxH, hipStream_t stream);
+// scores (and models) of every hypothesis; withBest also launches the best-model / mask kernel
 int launch_score_table(const DevTask* tasks, uint32_t nTasks, uint32_t maxH, float threshold, int mode,
-                       float minSepSq0, float minSepSq1, hipStream_t stream);
+                       float minSepSq0, float minSepSq1, hipStream_t stream, bool withBest = true);
+// end of the 8-point driver (rsfm_ransac_f8_finish): adaptive cut, refit on the inliers
+struct FinishTask {
+    const float4* pts;
+    const uint32_t* nDev;
+    uint32_t nHost;
+    const uint32_t* idx;
+    const uint32_t* counts;
+    const float* models;
+    uint32_t H;
+    float* F;
+    uint32_t* result;
+    uint8_t* mask;
+    uint32_t* best;  // optional [2]: best hypothesis, its count
+};
+struct FinishParams {
+    float t2;
+    float conf;  // <= 0: the whole table
+    uint32_t maxIter;
+    int refit;
+};
+int make_finish_params(float threshold, int mode, float confidence, uint32_t maxIter, int refit, FinishParams& p);
+int launch_finish_table(const FinishTask* tasks, uint32_t nTasks, const FinishParams& p, int mode, hipStream_t stream);
 // 7-point scoring + best over a device task table (sampleIdx rows of 7)
 int launch_score7_table(const DevTask* tasks, uint32_t nTasks, uint32_t maxH, float threshold, int mode,
                         float minSepSq0, float minSepSq1, hipStream_t stream);

@@ -26,6 +26,9 @@
 //  * Pairs that carry lens distortion have their gathered keypoint pairs
 //    undistorted in place right after the filter (one launch over a device
 //    table), as the reference's makePtPair does before solving.
+//  * With the 8-point solver the end of the reference's driver can follow
+//    the scorer as one more stage (adaptive cut, refit on the inliers;
+//    ransac_finish.hip), one workgroup per pair.
 //  * With the DEGENSAC solver the verification stages are the reference's
 //    default pair solver instead: 7-point samples, the 7-point scorer, the
 //    DEGENSAC finisher (F and H with their inlier masks) and, optionally, the
@@ -72,6 +75,9 @@ struct rsfm_pipeline {
     aux::FilterTask* dFilter = nullptr;
     ransac::SampleTask* dSample = nullptr;
     ransac::DevTask* dRansac = nullptr;
+    ransac::FinishTask* dFinish = nullptr;  // 8-point solver with the finishing stage
+    ransac::FinishParams finParams{};
+    bool finish = false;
     pipe::FillRange* dFill = nullptr;
     uint32_t nFill = 0, maxFill = 0;
     uint64_t* dPart = nullptr;
@@ -94,6 +100,8 @@ struct rsfm_pipeline {
     uint32_t nPoseF = 0, nPoseH = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    hipEvent_t done = nullptr;  // recorded on the stream of every run
+    bool ran = false;
 };
 
 namespace {
@@ -292,7 +300,8 @@ int capture(rsfm_pipeline* pl)
     if (c.solver == RSFM_PIPELINE_SOLVER_F8) {
         if (rc == RSFM_OK)
             rc = ransac::launch_score_table(pl->dRansac, P, c.nbHyp, c.threshold, c.mode, c.minSepSq0, c.minSepSq1,
-                                            pl->stream);
+                                            pl->stream, !pl->finish);
+        if (rc == RSFM_OK && pl->finish) rc = ransac::launch_finish_table(pl->dFinish, P, pl->finParams, c.mode, pl->stream);
     } else {
         if (rc == RSFM_OK)
             rc = ransac::launch_score7_table(pl->dF7, P, c.nbHyp, c.threshold, c.mode, c.minSepSq0, c.minSepSq1,
@@ -313,6 +322,7 @@ int capture(rsfm_pipeline* pl)
 
 void release(rsfm_pipeline* pl)
 {
+    if (pl->done) (void)hipEventDestroy(pl->done);
     if (pl->exec) (void)hipGraphExecDestroy(pl->exec);
     if (pl->graph) (void)hipGraphDestroy(pl->graph);
     if (pl->arena) (void)hipFree(pl->arena);
@@ -338,6 +348,10 @@ extern "C" int rsfm_pipeline_create(const rsfm_pipeline_pair* pairs, size_t nbPa
     RSFM_RETURN_IF(cfg->solver != RSFM_PIPELINE_SOLVER_F8 && cfg->solver != RSFM_PIPELINE_SOLVER_DEGENSAC,
                    RSFM_ERR_INVALID, "pipeline: unknown solver");
     const bool degensac = cfg->solver == RSFM_PIPELINE_SOLVER_DEGENSAC;
+    const bool finish = !degensac && (cfg->refit != 0 || cfg->confidence > 0.f);
+    ransac::FinishParams finParams{};
+    if (finish) RSFM_CHECK(ransac::make_finish_params(cfg->threshold, cfg->mode, cfg->confidence, cfg->maxIter,
+                                                      cfg->refit != 0, finParams));
     for (size_t i = 0; i < nbPairs; i++) {
         const auto& x = pairs[i];
         RSFM_RETURN_IF(x.nA > 0 && (x.descA == nullptr || x.bestA == nullptr || x.secondA == nullptr || x.kpA == nullptr ||
@@ -351,6 +365,8 @@ extern "C" int rsfm_pipeline_create(const rsfm_pipeline_pair* pairs, size_t nbPa
                        "pipeline pair " + std::to_string(i) + ": sampleIdx/counts missing");
         RSFM_RETURN_IF(degensac && (x.F == nullptr || x.H == nullptr || x.solverResult == nullptr), RSFM_ERR_INVALID,
                        "pipeline pair " + std::to_string(i) + ": F/H/solverResult missing (DEGENSAC solver)");
+        RSFM_RETURN_IF(finish && (x.F == nullptr || x.solverResult == nullptr), RSFM_ERR_INVALID,
+                       "pipeline pair " + std::to_string(i) + ": F/solverResult missing (8-point finishing stage)");
         RSFM_RETURN_IF(!degensac && (x.posesF != nullptr || x.posesH != nullptr), RSFM_ERR_INVALID,
                        "pipeline pair " + std::to_string(i) + ": the pose stage needs the DEGENSAC solver");
         RSFM_RETURN_IF((x.posesF != nullptr && x.poseResultF == nullptr) ||
@@ -364,11 +380,14 @@ extern "C" int rsfm_pipeline_create(const rsfm_pipeline_pair* pairs, size_t nbPa
     rsfm_pipeline* pl = new rsfm_pipeline;
     pl->pairs.assign(pairs, pairs + nbPairs);
     pl->cfg = *cfg;
+    pl->finish = finish;
+    pl->finParams = finParams;
     auto fail_out = [&](int rc) {
         release(pl);
         return rc;
     };
     int rc = check_hip(hipStreamCreateWithFlags(&pl->stream, hipStreamNonBlocking), "pipeline stream");
+    if (rc == RSFM_OK) rc = check_hip(hipEventCreateWithFlags(&pl->done, hipEventDisableTiming), "pipeline event");
     if (rc != RSFM_OK) return fail_out(rc);
 
     // sizes: the finest candidate segment width bounds tiles and partials
@@ -403,6 +422,7 @@ extern "C" int rsfm_pipeline_create(const rsfm_pipeline_pair* pairs, size_t nbPa
     const size_t oFilter = take(P * sizeof(aux::FilterTask));
     const size_t oSample = take(P * sizeof(ransac::SampleTask));
     const size_t oRansac = take(P * sizeof(ransac::DevTask));
+    const size_t oFinish = take(P * sizeof(ransac::FinishTask));
     const size_t oF7 = take(P * sizeof(ransac::DevTask));
     const size_t oDg = take(P * sizeof(degensac::DevTask));
     const size_t oUnd = take(P * sizeof(undist::PairTask));
@@ -424,6 +444,7 @@ extern "C" int rsfm_pipeline_create(const rsfm_pipeline_pair* pairs, size_t nbPa
     pl->dFilter = reinterpret_cast<aux::FilterTask*>(pl->arena + oFilter);
     pl->dSample = reinterpret_cast<ransac::SampleTask*>(pl->arena + oSample);
     pl->dRansac = reinterpret_cast<ransac::DevTask*>(pl->arena + oRansac);
+    pl->dFinish = reinterpret_cast<ransac::FinishTask*>(pl->arena + oFinish);
     pl->dF7 = reinterpret_cast<ransac::DevTask*>(pl->arena + oF7);
     pl->dDg = reinterpret_cast<degensac::DevTask*>(pl->arena + oDg);
     pl->dUnd = reinterpret_cast<undist::PairTask*>(pl->arena + oUnd);
@@ -439,6 +460,7 @@ extern "C" int rsfm_pipeline_create(const rsfm_pipeline_pair* pairs, size_t nbPa
     std::vector<aux::FilterTask> ft(P);
     std::vector<ransac::SampleTask> st(P);
     std::vector<ransac::DevTask> rt(P), f7;
+    std::vector<ransac::FinishTask> fin;
     std::vector<degensac::DevTask> dg;
     std::vector<undist::PairTask> ud;
     std::vector<pose::DevTask> pf, ph;
@@ -455,6 +477,10 @@ extern "C" int rsfm_pipeline_create(const rsfm_pipeline_pair* pairs, size_t nbPa
         st[i] = ransac::SampleTask{x.sampleIdx, cfg->nbHyp, x.nA, x.count, cfg->seed + i, degensac ? 7u : 8u};
         rt[i] = ransac::DevTask{reinterpret_cast<const float4*>(x.pts), x.count, x.nA, cfg->nbHyp, x.sampleIdx,
                                 x.counts, degensac ? nullptr : dModels + i * modelFloats, x.best, x.inlierMask};
+        if (finish)
+            fin.push_back(ransac::FinishTask{reinterpret_cast<const float4*>(x.pts), x.count, x.nA, x.sampleIdx,
+                                             x.counts, dModels + i * modelFloats, cfg->nbHyp, x.F, x.solverResult,
+                                             x.inlierMask, x.best});
         if (x.distortion) {
             rsfm_real_camera c0{x.cam0, {}}, c1{x.cam1, {}};
             for (int k = 0; k < 5; k++) {
@@ -501,6 +527,7 @@ extern "C" int rsfm_pipeline_create(const rsfm_pipeline_pair* pairs, size_t nbPa
     if ((rc = upload(pl->dFilter, ft.data(), P * sizeof(aux::FilterTask))) != RSFM_OK ||
         (rc = upload(pl->dSample, st.data(), P * sizeof(ransac::SampleTask))) != RSFM_OK ||
         (rc = upload(pl->dRansac, rt.data(), P * sizeof(ransac::DevTask))) != RSFM_OK ||
+        (rc = upload(pl->dFinish, fin.data(), fin.size() * sizeof(ransac::FinishTask))) != RSFM_OK ||
         (rc = upload(pl->dF7, f7.data(), f7.size() * sizeof(ransac::DevTask))) != RSFM_OK ||
         (rc = upload(pl->dDg, dg.data(), dg.size() * sizeof(degensac::DevTask))) != RSFM_OK ||
         (rc = upload(pl->dUnd, ud.data(), ud.size() * sizeof(undist::PairTask))) != RSFM_OK ||
@@ -521,7 +548,9 @@ extern "C" int rsfm_pipeline_create(const rsfm_pipeline_pair* pairs, size_t nbPa
 extern "C" int rsfm_pipeline_run(rsfm_pipeline* pl, hipStream_t stream)
 {
     RSFM_RETURN_IF(pl == nullptr || pl->exec == nullptr, RSFM_ERR_INVALID, "pipeline: not created");
-    return check_hip(hipGraphLaunch(pl->exec, stream), "pipeline graph launch");
+    RSFM_CHECK(check_hip(hipGraphLaunch(pl->exec, stream), "pipeline graph launch"));
+    pl->ran = true;
+    return check_hip(hipEventRecord(pl->done, stream), "pipeline run event");
 }
 
 extern "C" int rsfm_pipeline_info(const rsfm_pipeline* pl, rsfm_pipeline_info_t* info, rsfm_pipeline_tuning_entry* entries,
@@ -541,7 +570,10 @@ extern "C" int rsfm_pipeline_info(const rsfm_pipeline* pl, rsfm_pipeline_info_t*
 extern "C" int rsfm_pipeline_destroy(rsfm_pipeline* pl)
 {
     if (pl == nullptr) return RSFM_OK;
-    const int rc = check_hip(hipStreamSynchronize(pl->stream), "pipeline sync");
+    // the graph's last run may still read the arena on the caller's stream
+    int rc = pl->ran ? check_hip(hipEventSynchronize(pl->done), "pipeline run sync") : RSFM_OK;
+    const int rc2 = check_hip(hipStreamSynchronize(pl->stream), "pipeline sync");
+    if (rc == RSFM_OK) rc = rc2;
     release(pl);
     return rc;
 }

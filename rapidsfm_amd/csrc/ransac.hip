@@ -636,7 +636,7 @@ int launch_sample_table(const SampleTask* tasks, uint32_t nTasks, uint32_t maxH,
 }
 
 int launch_score_table(const DevTask* tasks, uint32_t nTasks, uint32_t maxH, float threshold, int mode,
-                       float minSepSq0, float minSepSq1, hipStream_t stream)
+                       float minSepSq0, float minSepSq1, hipStream_t stream, bool withBest)
 {
     RSFM_RETURN_IF(mode != 0 && mode != 1, RSFM_ERR_INVALID, "ransac: mode must be 0 or 1");
     if (nTasks == 0) return RSFM_OK;
@@ -648,6 +648,7 @@ int launch_score_table(const DevTask* tasks, uint32_t nTasks, uint32_t maxH, flo
         else score_kernel<1><<<grid, kScoreBlock, 0, stream>>>(src, t2, minSepSq0, minSepSq1);
         RSFM_CHECK(check_launch("ransac score table"));
     }
+    if (!withBest) return RSFM_OK;
     if (mode == 0) best_kernel<0><<<nTasks, kBestThreads, 0, stream>>>(src, t2, minSepSq0, minSepSq1);
     else best_kernel<1><<<nTasks, kBestThreads, 0, stream>>>(src, t2, minSepSq0, minSepSq1);
     return check_launch("ransac best table");

@@ -298,6 +298,12 @@ extern "C" int rsfm_rmf_bin_host(int imgWidth, int imgHeight, uint32_t cellCols,
     const uint32_t cw = div_up(mx, cellCols);
     const uint32_t c = div_up((uint32_t)imgWidth, cw), r = div_up((uint32_t)imgHeight, cw);
     const uint32_t nCells = c * r;
+    RSFM_RETURN_IF(nbMatches > 0 && (matches == nullptr || cols == nullptr || rows == nullptr || cellWidth == nullptr ||
+                                     reordered == nullptr || bounds == nullptr || order == nullptr),
+                   RSFM_ERR_INVALID, "rmf bin: null argument");
+    for (uint32_t k = 0; k < nbMatches; k++)
+        RSFM_RETURN_IF(!std::isfinite(matches[k].x0) || !std::isfinite(matches[k].y0), RSFM_ERR_INVALID,
+                       "rmf bin: non-finite keypoint coordinate at match " + std::to_string(k));
     std::vector<uint32_t> cell(nbMatches), cnt(nCells + 1, 0);
     for (uint32_t k = 0; k < nbMatches; k++) {
         const float fi = std::min(std::max(std::ceil(matches[k].y0 / (float)cw), 0.f), (float)r - 1.f);

@@ -11,7 +11,7 @@ import numpy as np
 import torch
 
 from ._abi import WSampleTask, check, lib
-from .ops import _ptr, _stream
+from .ops import _ptr, _side, _stream
 
 N_PARAM = lib.rsfm_mlp_param_count()
 H = 64
@@ -51,15 +51,24 @@ def forward(params, x, stream=None):
 
 
 class GradWorkspace:
-    def __init__(self):
-        self.buf = None
+    """Gradient workspace, one buffer per (device, stream): calls on different
+    streams never share one."""
 
-    def get(self, n, device):
+    def __init__(self):
+        self.bufs = {}
+
+    def get(self, n, device, stream=None):
         need = ctypes.c_size_t(0)
         check(lib.rsfm_mlp_grad_workspace_size(n, ctypes.byref(need)), "rsfm_mlp_grad_workspace_size")
-        if self.buf is None or self.buf.numel() < need.value:
-            self.buf = torch.empty(need.value, dtype=torch.uint8, device=device)
-        return self.buf, need.value
+        s = stream or torch.cuda.current_stream(device)
+        key = (str(device), s.cuda_stream)
+        buf = self.bufs.get(key)
+        if buf is None or buf.numel() < need.value:
+            buf = torch.empty(need.value, dtype=torch.uint8, device=device)
+            if stream is not None:
+                buf.record_stream(stream)
+            self.bufs[key] = buf
+        return buf, need.value
 
 
 _ws = GradWorkspace()
@@ -71,7 +80,8 @@ def grad(params, x, y, stream=None, workspace=None):
     n = x.shape[0]
     g = torch.empty(N_PARAM, dtype=torch.float32, device=x.device)
     loss = torch.empty(1, dtype=torch.float32, device=x.device)
-    buf, nbytes = (workspace or _ws).get(n, x.device)
+    buf, nbytes = (workspace or _ws).get(n, x.device, stream)
+    _side(stream, g, loss)
     check(lib.rsfm_mlp_grad(_ptr(params), _ptr(x), _ptr(y), n, _ptr(g), _ptr(loss), _ptr(buf), nbytes,
                             _stream(stream)), "rsfm_mlp_grad")
     return g, loss
@@ -107,6 +117,7 @@ def sample_weighted(tasks, stream=None):
                              w.data_ptr() if w.numel() else None, cdf.data_ptr(), seed)
         outs.append(idx)
         keep.append(cdf)
+    _side(stream, *outs, *keep)
     check(lib.rsfm_ransac_sample_weighted_batch(ctypes.cast(arr, ctypes.c_void_p), len(tasks), _stream(stream)),
           "rsfm_ransac_sample_weighted_batch")
     return outs

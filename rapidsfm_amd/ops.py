@@ -8,13 +8,25 @@ import ctypes
 
 import torch
 
-from ._abi import BfTask, BfTop2Task, Camera, DegensacTask, PoseHTask, PoseTask, RansacTask, RealCamera, UndistortTask, check, lib
+from ._abi import BfTask, BfTop2Task, Camera, DegensacTask, FinishTask, PoseHTask, PoseTask, RansacTask, RealCamera, UndistortTask, check, lib
 
 
 def _stream(stream):
     if stream is None:
         stream = torch.cuda.current_stream()
     return ctypes.c_void_p(stream.cuda_stream)
+
+
+def _side(stream, *tensors):
+    """Work enqueued on an explicit stream other than the current one: that stream
+    first waits for the current stream (where the inputs and temporaries were
+    made), and the temporaries stay allocated until the stream's work is done."""
+    if stream is None or stream == torch.cuda.current_stream():
+        return
+    stream.wait_stream(torch.cuda.current_stream())
+    for t in tensors:
+        if t is not None:
+            t.record_stream(stream)
 
 
 def _ptr(t):
@@ -189,6 +201,7 @@ def pose_from_f(problems, front_ratio=0.95, stream=None):
         Fd = F.to(torch.float32).contiguous()
         o = {"poses": torch.empty((4, 12), dtype=torch.float32, device=pts.device),
              "result": torch.empty(10, dtype=torch.int32, device=pts.device), "_F": Fd}
+        _side(stream, Fd)
         tasks[i] = PoseTask(_ptr(Fd), Camera(*[float(v) for v in c0]), Camera(*[float(v) for v in c1]), _ptr(pts),
                             pts.shape[0], _ptr(ndev) if ndev is not None else None,
                             _ptr(mask) if mask is not None else None, _ptr(o["poses"]), _ptr(o["result"]))
@@ -215,6 +228,7 @@ def pose_from_h(problems, front_ratio=0.95, stream=None):
         o = {"poses": torch.empty((8, 12), dtype=torch.float32, device=pts.device),
              "normals": torch.empty((8, 3), dtype=torch.float32, device=pts.device),
              "result": torch.empty(19, dtype=torch.int32, device=pts.device), "_H": Hd}
+        _side(stream, Hd)
         tasks[i] = PoseHTask(_ptr(Hd), Camera(*[float(v) for v in c0]), Camera(*[float(v) for v in c1]), _ptr(pts),
                              pts.shape[0], _ptr(ndev) if ndev is not None else None,
                              _ptr(mask) if mask is not None else None, _ptr(o["poses"]), _ptr(o["normals"]),
@@ -243,6 +257,35 @@ def ransac_f8(problems, threshold, mode=1, min_sep_sq=(0.0, 0.0), want_models=Fa
     models (H, 3, 3) when asked.
     """
     return _ransac(lib.rsfm_ransac_f8, "rsfm_ransac_f8", problems, threshold, mode, min_sep_sq, want_models, stream)
+
+
+def ransac_f8_finish(problems, threshold, mode=1, confidence=0.0, max_iter=5000, refit=True, stream=None):
+    """End of the 8-point driver for problems of (pts, sampleIdx (H, 8), counts (H,)[, models (H, 3, 3)[, nDev]])
+    scored by ransac_f8 with the same threshold and mode: optional adaptive cut (confidence > 0), refit on the
+    best F's inliers (kept when it has more). Returns dicts with F (3, 3), result (4,) = [best hypothesis, its
+    inliers, hypotheses walked, final inliers] and mask (n,)."""
+    tasks = (FinishTask * max(1, len(problems)))()
+    outs = []
+    for i, prob in enumerate(problems):
+        pts, idx, counts = prob[0], prob[1], prob[2]
+        models = prob[3] if len(prob) > 3 else None
+        ndev = prob[4] if len(prob) > 4 else None
+        if pts.dtype != torch.float32 or pts.dim() != 2 or pts.shape[1] != 4 or not pts.is_contiguous():
+            raise ValueError("pts must be contiguous float32 (n, 4)")
+        if idx.dim() != 2 or idx.shape[1] != 8 or not idx.is_contiguous() or counts.shape[0] != idx.shape[0]:
+            raise ValueError("sampleIdx must be contiguous (H, 8) with counts (H,)")
+        n, dev = pts.shape[0], pts.device
+        o = {"F": torch.empty((3, 3), dtype=torch.float32, device=dev),
+             "result": torch.empty(4, dtype=torch.int32, device=dev),
+             "mask": torch.empty(n, dtype=torch.uint8, device=dev)}
+        tasks[i] = FinishTask(_ptr(pts), n, _ptr(ndev) if ndev is not None else None, _ptr(idx), _ptr(counts),
+                              _ptr(models) if models is not None else None, idx.shape[0], _ptr(o["F"]),
+                              _ptr(o["result"]), _ptr(o["mask"]))
+        outs.append(o)
+    check(lib.rsfm_ransac_f8_finish(ctypes.cast(tasks, ctypes.c_void_p), len(problems), float(threshold), int(mode),
+                                    float(confidence), int(max_iter), int(bool(refit)), _stream(stream)),
+          "rsfm_ransac_f8_finish")
+    return outs
 
 
 def _ransac_tasks(problems, want_models):

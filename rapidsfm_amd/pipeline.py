@@ -28,7 +28,8 @@ class PipelineConfig(ctypes.Structure):
     _fields_ = [("nbHyp", c_u32), ("threshold", c_f32), ("mode", ctypes.c_int), ("ratioNum", c_u32),
                 ("ratioDen", c_u32), ("seed", ctypes.c_uint64), ("minSepSq0", c_f32), ("minSepSq1", c_f32),
                 ("segCols", c_u32), ("wgPerXcd", c_u32), ("solver", c_u32), ("thresholdH", c_f32),
-                ("nbCandidates", c_u32), ("nbPlaneSamples", c_u32), ("frontRatio", c_f32)]
+                ("nbCandidates", c_u32), ("nbPlaneSamples", c_u32), ("frontRatio", c_f32), ("confidence", c_f32),
+                ("maxIter", c_u32), ("refit", c_u32)]
 
 
 class TuningEntry(ctypes.Structure):
@@ -57,6 +58,11 @@ class Pipeline:
     bestB (int32 (n,2) BestMatch bit patterns), matches (nA,3), count (1,),
     pts (nA,4), idx (H,8), counts (H,), best (2,), mask (nA,).
 
+    refit / confidence (8-point solver): the end of the reference's driver as a graph
+    stage (rsfm_ransac_f8_finish): F (3,3) and result (4,) = [best hypothesis, its
+    inliers, hypotheses walked, final inliers] are added and mask holds the final F's
+    inliers.
+
     solver="degensac" runs the DEGENSAC pair solver instead of 8-point F RANSAC:
     idx is then (H,7), mask holds F's inliers, and F (3,3), H (3,3), result (4,),
     maskH (nA,) are added; with cams (a list of (cam0, cam1) as (fx, fy, cx, cy)
@@ -70,9 +76,10 @@ class Pipeline:
 
     def __init__(self, pairs, n_hyp=4096, threshold=1.0, mode=1, ratio=(4, 5), seed=77, min_sep_sq=(0.0, 0.0),
                  seg_cols=0, wg_per_xcd=0, solver="f8", threshold_h=3.0, n_candidates=8, n_plane_samples=256,
-                 front_ratio=0.95, cams=None, distortion=None):
+                 front_ratio=0.95, cams=None, distortion=None, refit=False, confidence=0.0, max_iter=5000):
         dev = pairs[0][0].device
         dg = solver == "degensac"
+        fin = not dg and (refit or confidence > 0)
         k = 7 if dg else 8
         self.inputs = pairs
         self.out = []
@@ -92,6 +99,9 @@ class Pipeline:
                 "best": torch.empty(2, dtype=torch.int32, device=dev),
                 "mask": torch.empty(max(1, nA), dtype=torch.uint8, device=dev),
             }
+            if fin:
+                o.update(F=torch.zeros((3, 3), dtype=torch.float32, device=dev),
+                         result=torch.zeros(4, dtype=torch.int32, device=dev))
             if dg:
                 o.update(F=torch.zeros((3, 3), dtype=torch.float32, device=dev),
                          H=torch.zeros((3, 3), dtype=torch.float32, device=dev),
@@ -120,7 +130,8 @@ class Pipeline:
                                   ctypes.cast(dist, c_vp) if dist is not None else None)
             self.out.append(o)
         cfg = PipelineConfig(n_hyp, threshold, mode, ratio[0], ratio[1], seed, min_sep_sq[0], min_sep_sq[1], seg_cols,
-                             wg_per_xcd, SOLVERS[solver], threshold_h, n_candidates, n_plane_samples, front_ratio)
+                             wg_per_xcd, SOLVERS[solver], threshold_h, n_candidates, n_plane_samples, front_ratio,
+                             confidence, max_iter, int(bool(refit)))
         torch.cuda.synchronize()  # the inputs must be complete: creation times candidate tilings on them
         self.handle = c_vp()
         check(lib.rsfm_pipeline_create(ctypes.cast(arr, c_vp), len(pairs), ctypes.byref(cfg), ctypes.byref(self.handle)),

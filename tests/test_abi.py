@@ -46,3 +46,26 @@ def test_host_cross_check_matches_oracle(rs, orc):
     n2 = rs.lib.rsfm_remove_conflicts_host(fwd.ctypes.data_as(ctypes.c_void_p), 300, B.shape[0],
                                           out.ctypes.data_as(ctypes.c_void_p))
     assert np.array_equal(out[:n2], orc.remove_conflicts(fwd, B.shape[0]))
+
+
+def test_host_binning_matches_oracle_and_rejects_nan(rs, orc):
+    """rsfm_rmf_bin_host (RansacMatchFilter.cpp:42-67) on the CPU: the oracle's
+    cells and order; a non-finite keypoint is an error, not an out-of-range cell."""
+    import numpy as np
+    import pytest
+
+    from rapidsfm_amd import rmf
+
+    rng = np.random.default_rng(4)
+    m = np.stack([rng.uniform(0, 4000, 500), rng.uniform(0, 3000, 500), rng.uniform(0, 4000, 500),
+                  rng.uniform(0, 3000, 500)], 1).astype(np.float32)
+    m[:3, :2] = [[0, 0], [4000, 3000], [-5, 3100]]  # corners and out-of-image points clamp
+    reordered, bounds, order, cols, rows, cw = rmf.bin_matches(4000, 3000, m)
+    ref = orc.rmf_bin(4000, 3000, m, 16)
+    assert (cols, rows, cw) == tuple(ref[3:6])
+    assert np.array_equal(bounds, ref[1]) and np.array_equal(reordered, ref[0]) and np.array_equal(order, ref[2])
+    for bad in (np.nan, np.inf, -np.inf):
+        mm = m.copy()
+        mm[17, 1] = bad
+        with pytest.raises(rs.RsfmError, match="non-finite"):
+            rmf.bin_matches(4000, 3000, mm)

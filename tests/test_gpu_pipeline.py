@@ -316,3 +316,46 @@ def test_pipeline_undistorts_pairs(orc, dev):
     assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
     assert np.abs(got - raw).max() > 1.0
     pl.close()
+
+
+@pytest.mark.parametrize("conf,refit", [(0.0, True), (0.99, True), (0.995, False)])
+def test_pipeline_finish_stage(orc, dev, conf, refit):
+    """The 8-point graph with the end of the reference's driver as a stage:
+    identical to rsfm_ransac_f8 + rsfm_ransac_f8_finish over the same matches,
+    and pair 0 identical to the oracle's restatement."""
+    from rapidsfm_amd import ops
+    from rapidsfm_amd.pipeline import Pipeline
+
+    _say("pipeline finish stage conf", conf, "refit", refit)
+    sizes = SIZES[:4] + [(2500, 2600), (6000, 5000)]
+    pairs = _batch(dev, sizes, seed=700)
+    H = 1024
+    pl = Pipeline(pairs, n_hyp=H, seed=9, seg_cols=2048, wg_per_xcd=32, refit=refit, confidence=conf, max_iter=600)
+    pl.run()
+    outs = _matched(pairs)
+    _sample(pairs, outs, H, 9, 8)
+    sc = ops.ransac_f8([(o["pts"], o["idx"], o["count"]) for o in outs], 1.0, mode=1, want_models=True)
+    fin = ops.ransac_f8_finish([(o["pts"], o["idx"], s["counts"], s["models"], o["count"]) for o, s in zip(outs, sc)],
+                               1.0, 1, confidence=conf, max_iter=600, refit=refit)
+    torch.cuda.synchronize()
+    for i, (o, r, s, f) in enumerate(zip(pl.out, outs, sc, fin)):
+        nA = pairs[i][0].shape[0]
+        c = int(r["count"].item())
+        assert int(o["count"].item()) == c and torch.equal(o["pts"][:c], r["pts"][:c]), i
+        assert torch.equal(o["counts"], s["counts"]), i
+        assert torch.equal(o["F"].view(torch.int32), f["F"].view(torch.int32)), i
+        assert torch.equal(o["result"], f["result"]), i
+        assert torch.equal(o["mask"][:nA], f["mask"][:nA]), i
+        res = o["result"].cpu().numpy().view(np.uint32)
+        assert o["best"].cpu().numpy().view(np.uint32)[0] == res[0], i
+    # pair 0 against the oracle
+    o = pl.out[0]
+    c = int(o["count"].item())
+    pts = o["pts"][:c].cpu().numpy()
+    idx = orc.sample_indices_hash(9, c, H)
+    counts, models = orc.f8_score(pts, idx, 1.0, 1)
+    F, res, mask = orc.f8_finish(pts, idx, counts, models, 1.0, 1, conf, 600, refit)
+    assert np.array_equal(o["F"].cpu().numpy().view(np.uint32), F.view(np.uint32))
+    assert np.array_equal(o["result"].cpu().numpy().view(np.uint32), res)
+    assert np.array_equal(o["mask"][:c].cpu().numpy(), mask)
+    pl.close()
