@@ -17,6 +17,12 @@ class CasHashResult(ctypes.Structure):
     _fields_ = [("query", c_void_p), ("train", c_void_p)]
 
 
+class CasHashRerankImage(ctypes.Structure):
+    _fields_ = [("desc", c_void_p), ("codes", c_void_p), ("remap", c_void_p), ("bucketEnds", c_void_p),
+                ("buckets", c_void_p), ("nbDesc", c_u32)]
+
+
+bind_extra("rsfm_cashash_find_best_match_rerank", c_int, [c_void_p, c_void_p, c_void_p, c_u32, c_u32, c_u32, c_void_p])
 bind_extra("rsfm_cashash_accumulate_desc", c_int, [c_void_p, c_void_p, c_int, c_void_p, c_u32, c_void_p])
 bind_extra("rsfm_cashash_compute_bias", c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_u64, c_u32, c_void_p])
 bind_extra("rsfm_cashash_compute_hash_bucket_only", c_int,
@@ -116,4 +122,33 @@ def find_best_match(queries, train, n_schemes=8, bidirectional=True, stream=None
                                            ctypes.cast(ctypes.pointer(timg), c_void_p),
                                            train["n"], ctypes.cast(res, c_void_p), nq, None, 0, n_schemes,
                                            _stream(stream)), "find_best_match")
+    return outs
+
+
+def _rerank_image(im):
+    if im["remap"] is None:
+        raise ValueError("the re-rank mode needs the 192-channel hash (bucket codes and remap)")
+    return CasHashRerankImage(_ptr(im["desc"]), _ptr(im["codes"]), _ptr(im["remap"]), _ptr(im["ends"]),
+                              _ptr(im["buckets"]), im["n"])
+
+
+def find_best_match_rerank(queries, train, n_schemes=6, keep=8, bidirectional=True, stream=None):
+    """Config-1 mode: candidates from the first n_schemes bucket tables, the keep
+    Hamming-nearest (128-bit remap) re-ranked by exact L2^2. queries: list of
+    register_image() dicts built with 192-channel weights; train: one dict.
+    Returns a list of (bestQ, bestT) int32 (n, 2) tensors."""
+    nq = len(queries)
+    qimg = (CasHashRerankImage * max(1, nq))()
+    res = (CasHashResult * max(1, nq))()
+    outs = []
+    for i, q in enumerate(queries):
+        bq = torch.empty((max(1, q["n"]), 2), dtype=torch.int32, device=q["desc"].device)
+        bt = torch.empty((max(1, train["n"]), 2), dtype=torch.int32, device=q["desc"].device) if bidirectional else None
+        qimg[i] = _rerank_image(q)
+        res[i] = CasHashResult(_ptr(bq), _ptr(bt) if bt is not None else None)
+        outs.append((bq[: q["n"]], bt[: train["n"]] if bt is not None else None))
+    timg = _rerank_image(train)
+    check(lib.rsfm_cashash_find_best_match_rerank(ctypes.cast(qimg, c_void_p), ctypes.cast(ctypes.pointer(timg),
+                                                  c_void_p), ctypes.cast(res, c_void_p), nq, n_schemes, keep,
+                                                  _stream(stream)), "find_best_match_rerank")
     return outs

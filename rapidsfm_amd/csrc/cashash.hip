@@ -24,6 +24,7 @@
 //    atomicMin on (distance, global index) keys.
 #include "common.hpp"
 #include <algorithm>
+#include <vector>
 
 namespace rsfm_amd {
 namespace cashash {
@@ -479,6 +480,145 @@ __global__ void match_init_kernel(const MatchBatch batch)
     if (batch.resT[task] != nullptr && x < batch.t.n) batch.resT[task][x] = kNoMatch;
 }
 
+// ---------------------------------------------------------------------------
+// Config-1 mode (no reference counterpart: the reference computes the 128-bit
+// remap hash, computeHash.cu:474-496, but never matches with it; its
+// matchWithoutRemap.cu is empty): for every query the candidates are the train
+// descriptors sharing its bucket in any of the first nSchemes tables, ranked by
+// the Hamming distance of their 128-bit remap hashes (ties: lower index); the
+// kKeep first are re-ranked by exact L2^2 (ties: lower index).
+//
+// One wavefront per query. Lanes stride over the query's bucket in each table;
+// a train item already met in an earlier table (one of its first s code bytes
+// equals the query's) is skipped, so the union has no duplicates. Each lane
+// keeps its kKeep smallest keys (hamming << 24 | index) sorted in registers; kKeep
+// rounds of a wave minimum merge the 64 lists. The exact distances of the kept
+// candidates use eight lanes per candidate (16 bytes each, v_dot4 on u8) and a
+// wave minimum of (distance << 32 | index).
+struct RerankImage {
+    const uint8_t* desc;
+    const uint64_t* codes;  // 8 code bytes per descriptor
+    const uint4* remap;     // 128-bit remap hash per descriptor
+    const uint32_t* ends;   // [8][256] inclusive prefix
+    const uint32_t* buckets;
+    uint32_t n;
+};
+struct RerankTask {
+    RerankImage q, t;  // queries matched against the train image t
+    uint64_t* res;     // [q.n] (distance << 32 | train index)
+};
+constexpr int kRerankMaxTasks = 32;
+struct RerankBatch {
+    RerankTask t[kRerankMaxTasks];
+};
+constexpr int kRerankWaves = 4;
+
+__device__ inline uint32_t wave_min_u32(uint32_t x)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x = min(x, (uint32_t)__shfl_xor((int)x, o));
+    return x;
+}
+__device__ inline uint64_t wave_min_u64(uint64_t x)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const uint64_t y = __shfl_xor(x, o);
+        x = y < x ? y : x;
+    }
+    return x;
+}
+
+template <int kKeep>
+__global__ __launch_bounds__(kRerankWaves * 64) void rerank_kernel(const RerankBatch batch, uint32_t nSchemes)
+{
+    const RerankTask& T = batch.t[blockIdx.y];
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t qi = blockIdx.x * kRerankWaves + (threadIdx.x >> 6);
+    if (qi >= T.q.n) return;
+    const uint64_t qc = T.q.codes[qi];
+    const uint4 qr = T.q.remap[qi];
+    uint32_t top[kKeep];
+#pragma unroll
+    for (int k = 0; k < kKeep; k++) top[k] = ~0u;
+    for (uint32_t s = 0; s < nSchemes; s++) {
+        const uint32_t code = (uint32_t)(qc >> (8 * s)) & 0xFF;
+        const uint32_t end = T.t.ends[s * 256 + code];
+        const uint32_t beg = code == 0 ? 0u : T.t.ends[s * 256 + code - 1];
+        // code bytes of the earlier tables; bytes s.. are forced nonzero below
+        const uint64_t upper = s == 0 ? ~0ull : (s >= 8 ? 0ull : (~0ull << (8 * s)));
+        for (uint32_t p = beg + lane; p < end; p += 64) {
+            const uint32_t j = T.t.buckets[(size_t)s * T.t.n + p];
+            // zero byte in the first s bytes of (train codes ^ query codes): met before
+            const uint64_t x = (T.t.codes[j] ^ qc) | upper;
+            const bool seen = ((x - 0x0101010101010101ull) & ~x & 0x8080808080808080ull) != 0;
+            if (seen) continue;
+            const uint4 r = T.t.remap[j];
+            const uint32_t h = (uint32_t)(__popc(r.x ^ qr.x) + __popc(r.y ^ qr.y) + __popc(r.z ^ qr.z) +
+                                          __popc(r.w ^ qr.w));
+            uint32_t key = (h << 24) | j;
+#pragma unroll
+            for (int k = 0; k < kKeep; k++) {
+                const uint32_t lo = min(top[k], key);
+                key = max(top[k], key);
+                top[k] = lo;
+            }
+        }
+    }
+    // the kKeep smallest keys of the wave, in order
+    uint32_t sel[kKeep];
+#pragma unroll
+    for (int k = 0; k < kKeep; k++) {
+        const uint32_t m = wave_min_u32(top[0]);
+        sel[k] = m;
+        if (top[0] == m && m != ~0u) {
+#pragma unroll
+            for (int u = 0; u + 1 < kKeep; u++) top[u] = top[u + 1];
+            top[kKeep - 1] = ~0u;
+        }
+    }
+    // exact L2^2 of the kept candidates: lane = 8 x candidate slot + 16-byte part
+    const uint32_t part = lane & 7, slot = lane >> 3;
+    const uint4 a = *reinterpret_cast<const uint4*>(T.q.desc + (size_t)qi * 128 + part * 16);
+    uint32_t aa = 0;
+    aa = __builtin_amdgcn_udot4(a.x, a.x, aa, false);
+    aa = __builtin_amdgcn_udot4(a.y, a.y, aa, false);
+    aa = __builtin_amdgcn_udot4(a.z, a.z, aa, false);
+    aa = __builtin_amdgcn_udot4(a.w, a.w, aa, false);
+    uint64_t best = kNoMatch;
+#pragma unroll
+    for (int g = 0; g < kKeep / 8; g++) {
+        uint32_t key = sel[0];
+#pragma unroll
+        for (int k = 1; k < 8; k++)
+            if ((uint32_t)k == slot) key = sel[g * 8 + k];
+        if (slot == 0) key = sel[g * 8];
+        const bool valid = key != ~0u;
+        const uint32_t j = key & 0xFFFFFFu;
+        int32_t d = 0;
+        if (valid) {
+            const uint4 b = *reinterpret_cast<const uint4*>(T.t.desc + (size_t)j * 128 + part * 16);
+            uint32_t bb = 0, ab = 0;
+            bb = __builtin_amdgcn_udot4(b.x, b.x, bb, false);
+            bb = __builtin_amdgcn_udot4(b.y, b.y, bb, false);
+            bb = __builtin_amdgcn_udot4(b.z, b.z, bb, false);
+            bb = __builtin_amdgcn_udot4(b.w, b.w, bb, false);
+            ab = __builtin_amdgcn_udot4(a.x, b.x, ab, false);
+            ab = __builtin_amdgcn_udot4(a.y, b.y, ab, false);
+            ab = __builtin_amdgcn_udot4(a.z, b.z, ab, false);
+            ab = __builtin_amdgcn_udot4(a.w, b.w, ab, false);
+            d = (int32_t)(aa + bb) - 2 * (int32_t)ab;
+        }
+        d += __shfl_xor(d, 1);
+        d += __shfl_xor(d, 2);
+        d += __shfl_xor(d, 4);
+        const uint64_t k64 = valid ? make_key((uint32_t)d, j) : kNoMatch;
+        best = k64 < best ? k64 : best;
+    }
+    best = wave_min_u64(best);
+    if (lane == 0) T.res[qi] = best;
+}
+
 }  // namespace cashash
 }  // namespace rsfm_amd
 
@@ -623,6 +763,61 @@ extern "C" int rsfm_cashash_find_best_match(const rsfm_cashash_image* queries, c
         const uint32_t perScheme = std::min<uint32_t>(2048u, std::max<uint32_t>(16u, div_up(8192u, n * nbSchemes)));
         match_kernel<<<dim3(perScheme, nbSchemes, n), 64, 0, stream>>>(batch, nbSchemes);
         RSFM_CHECK(check_launch("find_best_match"));
+    }
+    return RSFM_OK;
+}
+
+extern "C" int rsfm_cashash_find_best_match_rerank(const rsfm_cashash_rerank_image* queries,
+                                                   const rsfm_cashash_rerank_image* train,
+                                                   const rsfm_cashash_result* results, uint32_t nbTasks,
+                                                   uint32_t nbSchemes, uint32_t keep, hipStream_t stream)
+{
+    using namespace cashash;
+    RSFM_RETURN_IF(nbTasks > 0 && (queries == nullptr || train == nullptr || results == nullptr), RSFM_ERR_INVALID,
+                   "find_best_match_rerank: null pointer");
+    RSFM_RETURN_IF(nbSchemes == 0 || nbSchemes > kSchemes, RSFM_ERR_INVALID,
+                   "find_best_match_rerank: nbSchemes must be 1..8");
+    RSFM_RETURN_IF(keep != 8 && keep != 16 && keep != 32, RSFM_ERR_INVALID,
+                   "find_best_match_rerank: keep must be 8, 16 or 32");
+    auto image = [](const rsfm_cashash_rerank_image& im, RerankImage& out) -> int {
+        RSFM_RETURN_IF(im.nbDesc > 0 && (im.desc == nullptr || im.codes == nullptr || im.remap == nullptr ||
+                                         im.bucketEnds == nullptr || im.buckets == nullptr),
+                       RSFM_ERR_INVALID, "find_best_match_rerank: image incomplete");
+        RSFM_RETURN_IF(im.nbDesc > (1u << 24), RSFM_ERR_INVALID,
+                       "find_best_match_rerank: at most 2^24 descriptors per image");
+        RSFM_RETURN_IF((reinterpret_cast<uintptr_t>(im.desc) | reinterpret_cast<uintptr_t>(im.remap)) % 16 ||
+                           reinterpret_cast<uintptr_t>(im.codes) % 8,
+                       RSFM_ERR_INVALID, "find_best_match_rerank: misaligned pointer");
+        out = RerankImage{static_cast<const uint8_t*>(im.desc), reinterpret_cast<const uint64_t*>(im.codes),
+                          reinterpret_cast<const uint4*>(im.remap), im.bucketEnds, im.buckets, im.nbDesc};
+        return RSFM_OK;
+    };
+    // every task's query direction, then (where asked) its train direction
+    std::vector<RerankTask> all;
+    RerankImage t{};
+    RSFM_CHECK(image(*train, t));
+    for (uint32_t i = 0; i < nbTasks; i++) {
+        RerankImage q{};
+        RSFM_CHECK(image(queries[i], q));
+        RSFM_RETURN_IF(q.n > 0 && results[i].query == nullptr, RSFM_ERR_INVALID,
+                       "find_best_match_rerank: results.query is null");
+        if (q.n > 0) all.push_back(RerankTask{q, t, reinterpret_cast<uint64_t*>(results[i].query)});
+        if (results[i].train != nullptr && t.n > 0)
+            all.push_back(RerankTask{t, q, reinterpret_cast<uint64_t*>(results[i].train)});
+    }
+    for (size_t b = 0; b < all.size(); b += kRerankMaxTasks) {
+        const size_t n = std::min<size_t>(kRerankMaxTasks, all.size() - b);
+        RerankBatch batch{};
+        uint32_t maxN = 0;
+        for (size_t i = 0; i < n; i++) {
+            batch.t[i] = all[b + i];
+            maxN = std::max(maxN, batch.t[i].q.n);
+        }
+        const dim3 grid(div_up(maxN, kRerankWaves), (uint32_t)n);
+        if (keep == 8) rerank_kernel<8><<<grid, kRerankWaves * 64, 0, stream>>>(batch, nbSchemes);
+        else if (keep == 16) rerank_kernel<16><<<grid, kRerankWaves * 64, 0, stream>>>(batch, nbSchemes);
+        else rerank_kernel<32><<<grid, kRerankWaves * 64, 0, stream>>>(batch, nbSchemes);
+        RSFM_CHECK(check_launch("find_best_match_rerank"));
     }
     return RSFM_OK;
 }

@@ -113,3 +113,50 @@ def test_six_tables_and_many_tasks(ch, orc, dev):
     for (gq, gt_), (rq, rt) in zip(outs, refs):
         assert np.array_equal(_h(gq, np.uint32), rq)
         assert np.array_equal(_h(gt_, np.uint32), rt)
+
+
+@pytest.mark.parametrize("keep", [8, 16])
+def test_config1_rerank_mode_8k(ch, orc, dev, keep):
+    """Config 1: 8192 vs 8192, 30% noisy true matches (2458), 192-channel seeded
+    Gaussian projections, 6 bucket tables, the `keep` Hamming-nearest of the
+    union by 128-bit remap, then exact L2^2; both directions bit-exact against
+    the oracle's restatement (the train direction as the oracle with the roles
+    swapped)."""
+    _say("config1 re-rank mode 8192x8192, 6 schemes, keep", keep)
+    w, base = orc.cashash_gaussian_weights(3, 192)
+    train, query, gt = orc.gen_sift_pair(1, 8192, 8192, 2458, 0.05)
+    acc_q = orc.cashash_accumulate(query)
+    bias_q = orc.cashash_compute_bias(w, base, acc_q, 8192)
+    acc_t = orc.cashash_accumulate(train, acc_q)
+    bias_t = orc.cashash_compute_bias(w, base, acc_t, 16384)
+    qi, qc, qr = _register(ch, orc, query, dev, w, bias_q)
+    ti, tc, tr = _register(ch, orc, train, dev, w, bias_t)
+    assert np.array_equal(_h(qi["remap"], np.uint32), qr) and np.array_equal(_h(ti["remap"], np.uint32), tr)
+    (gq, gt_), = ch.find_best_match_rerank([qi], ti, 6, keep)
+    rq = orc.cashash_find_best_rerank(query, qc, qr, train, tc, tr, 6, keep)
+    rt = orc.cashash_find_best_rerank(train, tc, tr, query, qc, qr, 6, keep)
+    assert np.array_equal(_h(gq, np.uint32), rq)
+    assert np.array_equal(_h(gt_, np.uint32), rt)
+    hit = (rq[:, 0].astype(np.int64) == gt) & (gt >= 0)
+    print("true matches recovered (keep %d): %d / %d" % (keep, hit.sum(), (gt >= 0).sum()))
+    assert hit.sum() > 0.6 * (gt >= 0).sum()
+
+
+def test_rerank_many_tasks_and_edges(ch, orc, dev):
+    _say("re-rank: 12 queries of uneven size vs one train, keep 32, empty query")
+    w, base = orc.cashash_gaussian_weights(7, 192)
+    train, _, _ = orc.gen_sift_pair(2, 3000, 10, 0, 0.05)
+    acc = orc.cashash_accumulate(train)
+    bias = orc.cashash_compute_bias(w, base, acc, 3000)
+    ti, tc, tr = _register(ch, orc, train, dev, w, bias)
+    qimgs, refs = [], []
+    for k in range(12):
+        _, q, _ = orc.gen_sift_pair(2, 3000, 1 + 251 * k, 40 + k, 0.05)
+        qi, qc, qr = _register(ch, orc, q, dev, w, bias)
+        qimgs.append(qi)
+        refs.append((orc.cashash_find_best_rerank(q, qc, qr, train, tc, tr, 5, 32),
+                     orc.cashash_find_best_rerank(train, tc, tr, q, qc, qr, 5, 32)))
+    outs = ch.find_best_match_rerank(qimgs, ti, 5, 32)
+    for (gq, gt_), (rq, rt) in zip(outs, refs):
+        assert np.array_equal(_h(gq, np.uint32), rq)
+        assert np.array_equal(_h(gt_, np.uint32), rt)
