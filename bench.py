@@ -7,11 +7,13 @@ resident in HBM, all on one stream:
   2. rsfm_filter_matches_batch    ratio test 0.8 + cross-check + keypoint gather
   3. rsfm_ransac_sample_indices_batch   H seeded 8-point samples per pair
   4. rsfm_ransac_f8               FP32 8-point fits + Sampson scoring (1 px),
-                                  every model stored, best model + inlier mask
-                                  per pair
-Synthetic data (rapidsfm_amd.synth, drawn on the device): SIFT-like
-descriptors (30% noisy true matches) with keypoints from one two-view geometry
-per pair (30% of true matches geometric outliers).
+                                  every count and model stored
+  5. rsfm_ransac_f8_finish        best hypothesis, FP64 least-squares refit of
+                                  F on its inliers (kept when it has more),
+                                  final F, result and inlier mask per pair
+Synthetic data (rapidsfm_amd.synth, drawn on the device): 64 distinct pairs of
+SIFT-like descriptors (30% noisy true matches) with keypoints from one
+two-view geometry per pair (30% of true matches geometric outliers).
 
 Prints one JSON line (metric, value, ..., roofline, cpu_baseline).
 """
@@ -27,11 +29,7 @@ sys.path.insert(0, ROOT)
 
 N_KPTS = 8192
 INT8_PEAK_TOPS = 5000.0  # MI355X dense int8 MFMA, 5.0 POPS
-# HBM bytes per matcher launch, measured with rocprofv3 --pmc FETCH_SIZE and
-# WRITE_SIZE (one pass each) at (keypoints, pairs per launch), each scaled by
-# the reading of a 256 MiB-in / 256 MiB-out calibration dispatch (FETCH x2.0,
-# WRITE x1.0); other shapes report null
-MEASURED_TRAFFIC = {(8192, 32): 136566 * 1024}
+MATCH_KERNEL = "bf::match_kernel<8,true,true>"
 
 
 def parse():
@@ -43,7 +41,8 @@ def parse():
     ap.add_argument("--hyp", type=int, default=4096, help="RANSAC hypotheses per pair")
     ap.add_argument("--n", type=int, default=N_KPTS, help="keypoints per image")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="bound of the CPU baseline sample")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="bound of the all-core CPU baseline sample")
+    ap.add_argument("--cpu-seconds-1core", type=float, default=8.0, help="bound of the one-thread CPU sample")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run one 256 MiB device copy (HBM-counter calibration for rocprofv3 --pmc)")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-pipeline leg")
@@ -77,7 +76,7 @@ def batched_leg(args, dev, log):
         pairs.append((pr["descA"], pr["descB"], kA, kB))
         truth.append((pr["src"], pr["geo"], pr["R"]))
     t0 = time.perf_counter()
-    pl = Pipeline(pairs, n_hyp=args.hyp, threshold=1.0, mode=1, seed=77)
+    pl = Pipeline(pairs, n_hyp=args.hyp, threshold=1.0, mode=1, seed=77, refit=True)
     t_create = time.perf_counter() - t0
     for _ in range(2):
         pl.run()
@@ -105,7 +104,8 @@ def batched_leg(args, dev, log):
     dg = None if args.no_degensac else degensac_batch(args, pairs, truth, info, log)
     return {
         "workload": "%d pairs, keypoints U[1024,16384] per image, top2+ratio0.8+crosscheck, %d-hypothesis "
-                    "F-RANSAC (Sampson 1px) per pair, one HIP graph launch" % (P, args.hyp),
+                    "F-RANSAC (Sampson 1px) + refit on the best sample's inliers per pair, one HIP graph launch"
+                    % (P, args.hyp),
         "value": round(P / (ms / 1e3), 1),
         "unit": "pairs/s",
         "ms_per_batch": round(ms, 4),
@@ -253,38 +253,78 @@ def cashash_leg(args, dev, log):
 
 def learned_leg(args, dev, log):
     """Learned proposals: train on plain-RANSAC labels, compare guided vs uniform
-    sampling on held-out 8K pairs at 70% outliers (plain scoring judges both)."""
+    sampling on held-out 8K pairs at 70% outliers (plain scoring judges both).
+    Ablations: the same data with the scale-ratio and orientation-difference
+    features (f6, f7) zeroed, and a generator variant whose outliers copy an
+    inlier's scale ratio and orientation difference (no such cue at all)."""
     import numpy as np
 
     from rapidsfm_amd import proposal
 
-    t0 = time.perf_counter()
-    rng = np.random.default_rng(31)
-    n = args.learned_train
-    train = proposal.make_split(17, n, rng.integers(1024, 2049, n), rng.uniform(0.5, 0.9, n), dev)
-    proposal.label_with_plain_ransac(train, H=4096)
-    t_data = time.perf_counter() - t0
-    label_acc = float(np.mean(np.concatenate([(s["y"].cpu().numpy() > 0.5) == s["gt"] for s in train])))
-    t1 = time.perf_counter()
-    params, losses = proposal.train(train, epochs=4, batch=8192)
-    t_train = time.perf_counter() - t1
-    n_samples = sum(int(s["x"].shape[0]) for s in train)
-    del train
-    held = proposal.make_split(7777, args.learned_eval, 8192, 0.7, dev)
-    rep = proposal.evaluate(params, held, H_plain=65536, H_guided=4096)
-    log("learned: gain %.2f (h99 plain %s, guided %s)" % (rep["gain"], rep["h99_plain"], rep["h99_guided"]))
-    rep.update({
-        "train_pairs": n,
-        "train_samples": n_samples,
-        "label_agreement_with_truth": round(label_acc, 4),
-        "train_loss": [round(v, 4) for v in losses],
-        "train_s": round(t_train, 2),
-        "data_s": round(t_data, 2),
-        "held_out": "%d pairs, 8192 keypoints per image, 70%% outliers among the kept matches" % len(held),
+    def run(hard, zero, tag):
+        t0 = time.perf_counter()
+        rng = np.random.default_rng(31)
+        n = args.learned_train
+        train = proposal.make_split(17, n, rng.integers(1024, 2049, n), rng.uniform(0.5, 0.9, n), dev,
+                                    hard_outliers=hard)
+        proposal.label_with_plain_ransac(train, H=4096)
+        t_data = time.perf_counter() - t0
+        label_acc = float(np.mean(np.concatenate([(s["y"].cpu().numpy() > 0.5) == s["gt"] for s in train])))
+        n_samples = sum(int(s["x"].shape[0]) for s in train)
+        held = proposal.make_split(7777, args.learned_eval, 8192, 0.7, dev, hard_outliers=hard)
+        reps = {}
+        for z in ([False, True] if zero else [False]):
+            tr = proposal.zero_features(train, (6, 7)) if z else train
+            he = proposal.zero_features(held, (6, 7)) if z else held
+            t1 = time.perf_counter()
+            curve = []
+            params, losses = proposal.train(tr, epochs=4, batch=8192, curve=curve)
+            t_train = time.perf_counter() - t1
+            rep = proposal.evaluate(params, he, H_plain=65536, H_guided=4096)
+            rep.update({
+                "train_pairs": n,
+                "train_samples": n_samples,
+                "label_agreement_with_truth": round(label_acc, 4),
+                "train_loss_epochs": [round(v, 4) for v in losses],
+                "train_loss_curve": [(k, round(v, 4)) for k, v in curve[:: max(1, len(curve) // 12)]],
+                "train_s": round(t_train, 2),
+                "data_s": round(t_data, 2),
+            })
+            key = tag + ("_no_scale_orientation" if z else "")
+            reps[key] = rep
+            log("learned %s: gain %.2f (h99 plain %s, guided %s)" % (key, rep["gain"], rep["h99_plain"],
+                                                                    rep["h99_guided"]))
+        return reps
+
+    reps = run(False, True, "default")
+    reps.update(run(True, False, "outliers_share_scale_orientation"))
+    out = dict(reps["default"])
+    out.update({
+        "held_out": "%d pairs, 8192 keypoints per image, 70%% outliers among the kept matches" % args.learned_eval,
         "h_plain": 65536,
         "h_guided": 4096,
+        "ablations": {k: {q: v[q] for q in ("gain", "gain_median_pair", "h99_plain", "h99_guided",
+                                            "guided_reaches_99pct", "q_plain", "q_guided", "train_loss_epochs")}
+                      for k, v in reps.items() if k != "default"},
     })
-    return rep
+    return out
+
+
+def measured_traffic(n, per_launch):
+    """HBM bytes per matcher launch from the committed counter profile
+    (profiles/match_8k_hbm_counters.md, two rocprofv3 --pmc passes, calibrated),
+    when it was taken for this kernel at this shape; else None."""
+    path = os.path.join(ROOT, "profiles", "match_8k_hbm_counters.md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                f = dict(kv.split("=") for kv in line.split(":", 1)[1].split())
+                if (f.get("kernel") == MATCH_KERNEL and int(f["keypoints"]) == n and
+                        int(f["pairs_per_launch"]) == per_launch):
+                    return int(f["bytes"])
+    except (OSError, ValueError, KeyError):
+        pass
+    return None
 
 
 def main():
@@ -297,22 +337,24 @@ def main():
     import numpy as np
     import torch
 
-    import rapidsfm_amd as rs
-    from rapidsfm_amd._abi import BfTask, BfTop2Task, FilterTask, RansacTask, SampleTask, check, lib
+    import rapidsfm_amd as rs  # noqa: F401  (fails loudly without the HIP library)
+    from rapidsfm_amd._abi import BfTask, BfTop2Task, FilterTask, FinishTask, RansacTask, SampleTask, check, lib
+
+    def log(msg):
+        print("[bench] " + msg, file=sys.stderr, flush=True)
 
     dev = torch.device("cuda:0")
     n, P, H = args.n, args.pairs, args.hyp
     stream = torch.cuda.current_stream()
     sp = ctypes.c_void_p(stream.cuda_stream)
 
-    # ---- synthetic pairs (generated on the device, resident in HBM)
+    # ---- synthetic pairs: P distinct pairs, generated on the device, resident in HBM
     from rapidsfm_amd import synth
 
-    n_unique = min(P, 8)
-    uniq = [synth.make_pair(1000 + u, n, n, copy_frac=0.3, geo_frac=0.7, device=dev) for u in range(n_unique)]
+    t_gen = time.perf_counter()
+    src = [synth.make_pair(1000 + p, n, n, copy_frac=0.3, geo_frac=0.7, device=dev) for p in range(P)]
     bufs = []
-    for p in range(P):
-        u = uniq[p % n_unique]
+    for u in src:
         b = {
             "dA": u["descA"], "dB": u["descB"],
             "kA": torch.from_numpy(np.ascontiguousarray(u["kpA"][:, :2])).to(dev),
@@ -325,25 +367,31 @@ def main():
             "idx": torch.empty((H, 8), dtype=torch.int32, device=dev),
             "counts": torch.empty(H, dtype=torch.int32, device=dev),
             "models": torch.empty((H, 9), dtype=torch.float32, device=dev),
-            "best": torch.empty(2, dtype=torch.int32, device=dev),
+            "F": torch.empty(9, dtype=torch.float32, device=dev),
+            "result": torch.empty(4, dtype=torch.int32, device=dev),
             "mask": torch.empty(n, dtype=torch.uint8, device=dev),
         }
         bufs.append(b)
+    torch.cuda.synchronize()
+    log("generated %d pairs in %.1f s" % (P, time.perf_counter() - t_gen))
     ptr = lambda t: t.data_ptr()  # noqa: E731
     top2 = (BfTop2Task * P)()
     filt = (FilterTask * P)()
     samp = (SampleTask * P)()
     rtask = (RansacTask * P)()
+    fin = (FinishTask * P)()
     for p, b in enumerate(bufs):
         top2[p] = BfTop2Task(BfTask(n, n, ptr(b["dA"]), ptr(b["dB"]), ptr(b["bestA"]), ptr(b["bestB"]), None, None),
                              ptr(b["secA"]))
         filt[p] = FilterTask(ptr(b["bestA"]), ptr(b["secA"]), n, ptr(b["bestB"]), n, None, ptr(b["count"]),
                              ptr(b["kA"]), ptr(b["kB"]), ptr(b["pts"]))
         samp[p] = SampleTask(ptr(b["idx"]), H, n, ptr(b["count"]), 77 + p)
-        # the scorer stores every model: the caller reads the best F as
-        # models[best[0]], and the best-model kernel takes it from there
+        # the scorer stores every hypothesis' count and model; the finisher picks
+        # the best, refits F on its inliers and writes F, result and the mask
         rtask[p] = RansacTask(ptr(b["pts"]), n, ptr(b["count"]), ptr(b["idx"]), H, ptr(b["counts"]),
-                              ptr(b["models"]), ptr(b["best"]), ptr(b["mask"]))
+                              ptr(b["models"]), None, None)
+        fin[p] = FinishTask(ptr(b["pts"]), n, ptr(b["count"]), ptr(b["idx"]), ptr(b["counts"]), ptr(b["models"]), H,
+                            ptr(b["F"]), ptr(b["result"]), ptr(b["mask"]))
     ws_need = ctypes.c_size_t(0)
     check(lib.rsfm_bf_match_top2_workspace_size(ctypes.cast(top2, ctypes.c_void_p), P, ctypes.byref(ws_need)), "ws")
     ws = torch.empty(ws_need.value + 256, dtype=torch.uint8, device=dev)
@@ -362,6 +410,7 @@ def main():
         check(lib.rsfm_filter_matches_batch(ctypes.cast(filt, ctypes.c_void_p), P, 4, 5, sp), "filter")
         check(lib.rsfm_ransac_sample_indices_batch(ctypes.cast(samp, ctypes.c_void_p), P, sp), "sample")
         check(lib.rsfm_ransac_f8(ctypes.cast(rtask, ctypes.c_void_p), P, 1.0, 1, 0.0, 0.0, sp), "ransac")
+        check(lib.rsfm_ransac_f8_finish(ctypes.cast(fin, ctypes.c_void_p), P, 1.0, 1, 0.0, 5000, 1, sp), "finish")
 
     for _ in range(args.warmup):
         step()
@@ -378,12 +427,13 @@ def main():
     value = P * args.steps / (ms_total / 1e3)
     # the matcher takes at most 32 pairs per kernel launch: report per launch
     launches = (P + 31) // 32
+    per_launch = min(P, 32)
     match_ms = sum(a.elapsed_time(b) for a, b in zip(ev_m0, ev_m1)) / args.steps / launches
     ops = 2.0 * n * n * 128 * P / launches
     achieved = ops / (match_ms / 1e3) / 1e12
 
-    b0 = bufs[0]
-    cnt = int(b0["count"].item())
+    cnts = np.array([int(b["count"].item()) for b in bufs])
+    res = np.stack([b["result"].cpu().numpy().view(np.uint32) for b in bufs])
     result = {
         "metric": "pairs/s at 8K kpts (match+RANSAC)",
         "value": round(value, 1),
@@ -395,28 +445,34 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "int8 (match, int32 accumulate) + fp32 (RANSAC)",
+        "dtype": "int8 (match, int32 accumulate) + fp32 (RANSAC) + fp64 (refit)",
         "data": "synthetic",
         "config": {
-            "workload": "bf-match top2+ratio0.8+crosscheck, then F 8-point RANSAC (Sampson 1px) per pair",
+            "workload": "bf-match top2+ratio0.8+crosscheck, then F 8-point RANSAC (Sampson 1px, fixed table of "
+                        "%d seeded hypotheses) + refit on the best sample's inliers, per pair" % H,
             "keypoints_per_image": n,
             "pairs_per_step": P,
+            "distinct_pairs": P,
             "ransac_hypotheses": H,
-            "matches_per_pair": cnt,
+            "matches_per_pair_median": int(np.median(cnts)),
+            "refit_kept_pairs": int((res[:, 3] > res[:, 1]).sum()),
             "parallelism": "single GPU",
         },
         "parity_pair0": None,
         "roofline": {
-            "kernel": "match_kernel<8,true,8,1> (rsfm_bf_match_top2)",
+            "kernel": MATCH_KERNEL + " (rsfm_bf_match_top2)",
             "bound": "mfma",
             "achieved": round(achieved, 1),
             "peak": INT8_PEAK_TOPS,
             "unit": "TOP/s",
             "frac": round(achieved / INT8_PEAK_TOPS, 4),
-            "traffic": MEASURED_TRAFFIC.get((n, min(P, 32))),
-            "traffic_unit": "bytes per launch (HBM, calibrated rocprofv3 counters)",
-            "algorithmic_bytes": min(P, 32) * (2 * n * 128 + 3 * n * 8),
+            "traffic": measured_traffic(n, per_launch),
+            "traffic_unit": "HBM bytes per launch (rocprofv3 FETCH_SIZE / WRITE_SIZE passes, calibrated; "
+                            "profiles/match_8k_hbm_counters.md)",
+            "algorithmic_bytes": per_launch * (2 * n * 128 + 3 * n * 8),
             "avg_launch_ms": round(match_ms, 4),
+            "timed": "HIP events around each rsfm_bf_match_top2 call on the launch stream (preparation kernel "
+                     "included), per launch of %d pairs" % per_launch,
         },
         "cpu_baseline": None,
     }
@@ -424,53 +480,47 @@ def main():
     if args.calib_copy:
         # one elementwise kernel that reads 256 MiB and writes 256 MiB: the
         # FETCH_SIZE / WRITE_SIZE readings of this dispatch calibrate the counters
-        src = torch.ones(64 << 20, dtype=torch.float32, device=dev)
-        dst = torch.empty_like(src)
-        torch.add(src, 1.0, out=dst)
+        srcb = torch.ones(64 << 20, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(srcb)
+        torch.add(srcb, 1.0, out=dst)
         torch.cuda.synchronize()
 
     if not args.no_cpu:
-        # ---- CPU baseline: the oracle's restatement of the same pass on the
-        # host cores, which also checks pair 0 of the measured GPU pass
+        # ---- parity of pair 0 of the measured pass, and the CPU baseline: the
+        # oracle's restatement of the same pass on the host cores
+        from oracle import cpu_baseline as cb
         from oracle import pyoracle as orc
 
-        host = [(u["descA"].cpu().numpy(), u["descB"].cpu().numpy(), u["kpA"][:, :2], u["kpB"][:, :2]) for u in uniq]
-        dA, dB, kA, kB = host[0]
-        ra, sa, rb = orc.bf_match_top2_fast(dA, dB)
+        host = [(u["descA"].cpu().numpy(), u["descB"].cpu().numpy(), u["kpA"][:, :2], u["kpB"][:, :2])
+                for u in src[:8]]
+        b0 = bufs[0]
+        F, r0, m0 = cb.one_pair(orc, *host[0], H, 77, 0)
+        ra, sa, rb = orc.bf_match_top2_fast(host[0][0], host[0][1])
         u32 = lambda t: t.cpu().numpy().view(np.uint32)  # noqa: E731
-        ok_match = (np.array_equal(u32(b0["bestA"]), ra) and np.array_equal(u32(b0["secA"]), sa)
-                    and np.array_equal(u32(b0["bestB"]), rb))
-        m = orc.filter_matches(ra, sa, rb, (4, 5))
-        pts_ref = np.concatenate([kA[m[:, 0]], kB[m[:, 1]]], 1).astype(np.float32)
-        ok_pts = cnt == len(m) and np.array_equal(b0["pts"][:cnt].cpu().numpy(), pts_ref)
-        idx_ref = orc.sample_indices_hash(77, cnt, H)
-        rc, _ = orc.f8_score(pts_ref, idx_ref, 1.0, 1, with_models=False)
-        ok_ransac = np.array_equal(u32(b0["counts"]), rc) and int(u32(b0["best"])[1]) == int(rc.max())
-        result["parity_pair0"] = bool(ok_match and ok_pts and ok_ransac)
-
+        cnt = int(cnts[0])
+        result["parity_pair0"] = bool(
+            np.array_equal(u32(b0["bestA"]), ra) and np.array_equal(u32(b0["secA"]), sa)
+            and np.array_equal(u32(b0["bestB"]), rb) and cnt == len(m0)
+            and np.array_equal(b0["F"].cpu().numpy().view(np.uint32), F.reshape(9).view(np.uint32))
+            and np.array_equal(u32(b0["result"]), r0) and np.array_equal(b0["mask"][:cnt].cpu().numpy(), m0))
+        native, how = cb.load_native()
         cores = len(os.sched_getaffinity(0))
-        t_cpu = 0.0
-        k = 0
-        while t_cpu < args.cpu_seconds and k < 4096:
-            dA, dB, kA, kB = host[k % n_unique]
-            t = time.perf_counter()
-            ra, sa, rb = orc.bf_match_top2_fast(dA, dB, cores)
-            mm = orc.filter_matches(ra, sa, rb, (4, 5))
-            pts = np.concatenate([kA[mm[:, 0]], kB[mm[:, 1]]], 1).astype(np.float32)
-            idx = orc.sample_indices_hash(77, len(mm), H)
-            orc.f8_score(pts, idx, 1.0, 1, nthreads=cores, with_models=False)
-            t_cpu += time.perf_counter() - t
-            k += 1
+        v_all, k_all, t_all = cb.measure(native, host, H, args.cpu_seconds, cores)
+        v_one, k_one, t_one = cb.measure(native, host, H, args.cpu_seconds_1core, 1)
         result["cpu_baseline"] = {
-            "value": round(k / t_cpu, 3),
+            "value": round(v_all, 3),
             "unit": "pairs/s",
             "cores": cores,
             "kind": "port",
-            "sample": "%d pairs of the same workload (%dx%d top-2 match + filter + %d-hypothesis F-RANSAC), "
-                      "%.1f s, OpenMP over %d threads" % (k, n, n, H, t_cpu, cores),
+            "sample": "%d pairs (8 distinct) of the same workload (%dx%d top-2 match + filter + %d-hypothesis "
+                      "F-RANSAC + refit), %.1f s, OpenMP over %d threads" % (k_all, n, n, H, t_all, cores),
+            "build": how,
+            "cpu_model": cb.cpu_model(),
+            "value_1core": round(v_one, 3),
+            "sample_1core": "%d pairs, %.1f s, one thread" % (k_one, t_one),
+            "gpu_over_cpu": round(value / v_all, 1),
         }
-    def log(msg):
-        print("[bench] " + msg, file=sys.stderr, flush=True)
+        log("cpu baseline: %.2f pairs/s on %d threads, %.3f on one (%s)" % (v_all, cores, v_one, how))
 
     if not args.no_batched:
         try:

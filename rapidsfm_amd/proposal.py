@@ -57,11 +57,16 @@ def _arr(cls, items):
     return a
 
 
-def match_pairs(pairs, device):
+def match_pairs(pairs, device, hard_outliers=None):
     """Top-2 match + cross-check (no ratio test) + features for a list of synth pairs.
 
     Returns a list of dicts (device tensors): matches (m, 3) int32 [query, train,
     distance bits], x (m, 8) features, pts (m, 4); m = kept matches.
+
+    hard_outliers: None, or a numpy Generator: then every kept outlier match takes
+    the scale ratio and orientation difference of a random kept inlier of its pair
+    (its query keypoint's scale and orientation are rewritten before the features
+    are computed), so outliers and inliers share the scale / orientation cue.
     """
     P = len(pairs)
     bufs = []
@@ -95,6 +100,23 @@ def match_pairs(pairs, device):
     filt = _arr(FilterTask, [FilterTask(p(b["bestA"]), None, b["nA"], p(b["bestB"]), b["nB"], p(b["out"]),
                                         p(b["count"]), p(b["kA2"]), p(b["kB2"]), p(b["pts"])) for b in bufs])
     check(lib.rsfm_filter_matches_batch(ctypes.cast(filt, ctypes.c_void_p), P, RATIO[0], RATIO[1], _sp()), "filter")
+    if hard_outliers is not None:
+        for pr, b in zip(pairs, bufs):
+            c = int(b["count"].item())
+            m = b["out"][:c].cpu().numpy()
+            q, t = m[:, 0], m[:, 1]
+            gt = (pr["src"][q] == t) & pr["geo"][q]
+            if gt.sum() == 0 or (~gt).sum() == 0:
+                continue
+            kA, kB = pr["kpA"].copy(), pr["kpB"]
+            qi, ti = q[gt], t[gt]
+            pick = hard_outliers.integers(0, len(qi), int((~gt).sum()))
+            ratio = kA[qi[pick], 2] / kB[ti[pick], 2]
+            delta = kA[qi[pick], 3] - kB[ti[pick], 3]
+            qo, to = q[~gt], t[~gt]
+            kA[qo, 2] = kB[to, 2] * ratio
+            kA[qo, 3] = np.angle(np.exp(1j * (kB[to, 3] + delta)))
+            b["kA"].copy_(torch.from_numpy(kA.astype(np.float32)))
     feats = _arr(FeatureTask, [FeatureTask(p(b["out"]), p(b["count"]), b["nA"], p(b["secA"]), p(b["kA"]), p(b["kB"]),
                                            synth.WIDTH, synth.HEIGHT, p(b["x"])) for b in bufs])
     check(lib.rsfm_proposal_features_batch(ctypes.cast(feats, ctypes.c_void_p), P, _sp()), "features")
@@ -155,19 +177,23 @@ def uniform_samples(ns, H, seeds, device):
     return idx
 
 
-def make_split(seed, n_pairs, kpts, outlier_rates, device, copy_frac=0.5, geo_frac=0.8, batch=32):
+def make_split(seed, n_pairs, kpts, outlier_rates, device, copy_frac=0.5, geo_frac=0.8, batch=32,
+               hard_outliers=False):
     """Pairs -> matched, featurised, subsampled to each pair's outlier rate.
 
     Returns a list of dicts: x (m,8), pts (m,4) device tensors, gt (m,) bool numpy.
+    hard_outliers: outliers copy an inlier's scale ratio and orientation
+    difference (match_pairs).
     """
     rng = np.random.default_rng(seed)
+    hard = np.random.default_rng(seed + 1) if hard_outliers else None
     out = []
     for b0 in range(0, n_pairs, batch):
         pairs = []
         for i in range(b0, min(n_pairs, b0 + batch)):
             k = int(kpts[i]) if hasattr(kpts, "__len__") else int(kpts)
             pairs.append(synth.make_pair(seed * 1000003 + i, k, k, copy_frac, geo_frac, device=device))
-        res = match_pairs(pairs, device)
+        res = match_pairs(pairs, device, hard)
         for i, (pr, r) in enumerate(zip(pairs, res)):
             gt = truth(pr, r["matches"])
             rate = outlier_rates[b0 + i] if hasattr(outlier_rates, "__len__") else outlier_rates
@@ -195,7 +221,20 @@ def label_with_plain_ransac(split, H=4096, seed=5, batch=32):
     return split
 
 
-def train(split, epochs=4, batch=8192, lr=3e-3, seed=1, log=None):
+def zero_features(split, cols):
+    """A copy of the split with the feature columns cols set to zero (ablation)."""
+    out = []
+    for s in split:
+        x = s["x"].clone()
+        x[:, list(cols)] = 0.0
+        out.append(dict(s, x=x))
+    return out
+
+
+def train(split, epochs=4, batch=8192, lr=3e-3, seed=1, log=None, curve=None):
+    """Adam on the whole split; returns (params, per-epoch loss on the first 65536
+    samples). curve: optional list that receives (step, loss) every 64 steps and
+    the loss of the initial network at step 0."""
     dev = split[0]["x"].device
     X = torch.cat([s["x"] for s in split]).contiguous()
     Y = torch.cat([s["y"] for s in split]).contiguous()
@@ -205,12 +244,18 @@ def train(split, epochs=4, batch=8192, lr=3e-3, seed=1, log=None):
     gen.manual_seed(seed)
     n = X.shape[0]
     losses = []
+    step = 0
+    if curve is not None:
+        curve.append((0, learned.grad(params, X[:65536].contiguous(), Y[:65536].contiguous())[1].item()))
     for ep in range(epochs):
         perm = torch.randperm(n, generator=gen, device=dev)
         for b0 in range(0, n - batch + 1, batch):
             sel = perm[b0:b0 + batch]
             g, loss = learned.grad(params, X[sel].contiguous(), Y[sel].contiguous())
             opt.step(g)
+            step += 1
+            if curve is not None and step % 64 == 0:
+                curve.append((step, loss.item()))
         losses.append(learned.grad(params, X[:65536].contiguous(), Y[:65536].contiguous())[1].item())
         if log:
             log("epoch %d loss %.4f" % (ep, losses[-1]))
