@@ -267,6 +267,33 @@ int rsfm_rmf_bin_host(int imgWidth, int imgHeight, uint32_t cellCols, const rsfm
                       uint32_t* cols, uint32_t* rows, uint32_t* cellWidth, rsfm_pt_pair* reordered, uint32_t* bounds,
                       uint32_t* order);
 
+/* Batched form of RansacMatchFilter::getInlierMask + the match selection of its
+ * caller (RansacMatchFilter.cpp:40-135, BuilderMethods.cpp:180-193), for pairs
+ * whose gathered keypoint pairs are already on the device (e.g. the output of
+ * rsfm_filter_matches_batch): per pair, device binning with the reference's
+ * rule and order (cellWidth = ceil(max(w, h) / cellCols), cell = clamped ceil
+ * of the first point over the cell width), the two cell kernels of
+ * rsfm_rmf_filter over all pairs in one launch each (threshold =
+ * relativeThreshold * cellWidth, LCG seed per pair), then the pairs with
+ * votes >= minVotes are kept: pts (and matches) are compacted in place in
+ * their original order and *nbPtsDev becomes the kept count (a pair with fewer
+ * than 16 points keeps none, as the reference's filter gives them no votes).
+ * Workspace: rsfm_rmf_filter_batch_workspace_size() bytes, 256-byte aligned
+ * (per pair: cell-ordered pairs, slots, bounds, affines, votes). */
+typedef struct rsfm_rmf_task {
+    rsfm_pt_pair* pts;          /* [nbPts] (first image x, y, second image x, y), 16-byte aligned */
+    rsfm_valid_match* matches;  /* optional [nbPts], compacted alongside */
+    uint32_t* nbPtsDev;         /* device count (capped at nbPts), rewritten with the kept count */
+    uint32_t nbPts;             /* capacity */
+    uint8_t* inlierMask;        /* optional [nbPts]: votes >= minVotes, original order */
+    int imgWidth, imgHeight;    /* first image */
+    uint32_t seed;
+} rsfm_rmf_task;
+int rsfm_rmf_filter_batch_workspace_size(const rsfm_rmf_task* tasks, size_t nbTasks, uint32_t cellCols, size_t* bytes);
+int rsfm_rmf_filter_batch(const rsfm_rmf_task* tasks, size_t nbTasks, uint32_t cellCols, uint32_t nbRansacTests,
+                          float relativeThreshold, uint32_t minVotes, int tryOtherAffines, void* workspace,
+                          size_t workspaceBytes, hipStream_t stream);
+
 /* ---------------------------------------------------------------------------
  * Fundamental-matrix RANSAC (8-point hypotheses)
  * ------------------------------------------------------------------------ */
@@ -545,6 +572,8 @@ typedef struct rsfm_pipeline_pair {
      * cam0/cam1 (rsfm_undistort_pt_pairs, 2 Gauss-Newton steps as the reference's
      * makePtPair) before sampling */
     const float* distortion;
+    /* first image size, for the optional affine match filter stage */
+    int width0, height0;
 } rsfm_pipeline_pair;
 typedef struct rsfm_pipeline_config {
     uint32_t nbHyp;            /* hypotheses per pair */
@@ -568,6 +597,16 @@ typedef struct rsfm_pipeline_config {
     float confidence;          /* adaptive cut of the hypothesis table, <= 0 = whole table */
     uint32_t maxIter;          /* the reference's max_iter (5000) for the cut */
     uint32_t refit;            /* refit F on the best sample's inliers */
+    /* optional affine match filter stage (rsfm_rmf_filter_batch, the reference's default
+     * RansacMatchFilter before pair solving, BuilderMethods.cpp:180-193) between the
+     * ratio/cross-check filter and the solver: the kept matches and keypoint pairs are
+     * compacted in place and count becomes the filtered count; LCG seed = seed + pair */
+    uint32_t matchFilter;      /* 0 off, 1 on (every pair then needs width0, height0) */
+    uint32_t mfCellCols;       /* 16 */
+    uint32_t mfTests;          /* 256 */
+    float mfThreshold;         /* relative to the cell width, 0.15 */
+    uint32_t mfMinVotes;       /* 2 */
+    uint32_t mfTryOther;       /* 1 */
 } rsfm_pipeline_config;
 #define RSFM_PIPELINE_WG_PER_TILE 0xFFFFFFFFu
 /* solvers: 8-point F RANSAC (samples, scores, best and inlier mask), or the

@@ -18,6 +18,7 @@
 #include <mutex>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <unordered_map>
 #include <utility>
 #include <vector>
@@ -134,6 +135,86 @@ inline void buildBuckets(const uint8_t* bucketIndices, uint32_t nbDesc, uint32_t
 {
     rsfmCheck(rsfm_cashash_build_buckets(bucketIndices, nbDesc, idxAllocator, buckets, stream), "cashash::buildBuckets");
 }
+// Overloads with the reference's own parameter types (casHash/cascadeHash.h:32-67,
+// called as at bruteForceMatcher.cpp:97-103): the reference passes
+// cudapp::KArray objects in device memory by reference or pointer. Any type of
+// the same size and trivial layout binds here (cudapp's KArray is a plain
+// array wrapper): KArray<uint64_t,128> (acc), KArray<int32_t,64> (bias, bias
+// base), KArray<int32_t,64,32> (weights = int8 [64][128]), KArray<int32_t,32>
+// (one weight row), KArray<uint32_t,32> (descriptor), KArray<uint8_t,8>
+// (bucket codes), KArray<uint32_t,8,256> (histogram, bounds, allocator).
+namespace detail {
+template <class T, size_t kBytes>
+using block_t = std::enable_if_t<std::is_trivially_copyable<T>::value && sizeof(T) == kBytes, int>;
+inline std::nullptr_t dev(std::nullptr_t) { return nullptr; }
+template <class T>
+inline T* dev(T* p) { return p; }
+}  // namespace detail
+
+template <class Acc, class Desc, class SqrAcc, detail::block_t<Acc, 1024> = 0, detail::block_t<Desc, 128> = 0>
+inline void accumuateDesc(Acc& acc, SqrAcc* sqrAcc, bool initAcc, const Desc* desc, uint32_t nbDesc, hipStream_t stream)
+{
+    static_assert(sizeof(SqrAcc) == 1024, "sqrAcc: KArray<uint64_t, 128>");
+    accumuateDesc(reinterpret_cast<uint64_t*>(&acc), reinterpret_cast<uint64_t*>(sqrAcc), initAcc, desc, nbDesc, stream);
+}
+template <class Acc, class Desc, detail::block_t<Acc, 1024> = 0, detail::block_t<Desc, 128> = 0>
+inline void accumuateDesc(Acc& acc, std::nullptr_t, bool initAcc, const Desc* desc, uint32_t nbDesc, hipStream_t stream)
+{
+    accumuateDesc(reinterpret_cast<uint64_t*>(&acc), nullptr, initAcc, desc, nbDesc, stream);
+}
+template <class Bias, class W, class Base, class Acc, detail::block_t<Bias, 256> = 0, detail::block_t<W, 8192> = 0,
+          detail::block_t<Base, 256> = 0, detail::block_t<Acc, 1024> = 0>
+inline void computeBias(Bias& bias, const W& weights, const Base& biasBase, const Acc& acc, uint64_t nbDesc,
+                        hipStream_t stream)
+{
+    computeBias(reinterpret_cast<int32_t*>(&bias), reinterpret_cast<const int8_t*>(&weights),
+                reinterpret_cast<const int32_t*>(&biasBase), reinterpret_cast<const uint64_t*>(&acc), nbDesc, stream);
+}
+template <class WRow, class Desc, class Codes, class Hist, detail::block_t<WRow, 128> = 0, detail::block_t<Desc, 128> = 0,
+          detail::block_t<Codes, 8> = 0>
+inline void computeHashForBucketOnly(const WRow* weights, const int32_t* bias, const Desc* desc, uint32_t nbDesc,
+                                     Codes* bucketIndices, Hist* histogram, hipStream_t stream)
+{
+    static_assert(std::is_same<Hist, std::nullptr_t>::value || sizeof(Hist) == 8192, "histogram: KArray<uint32_t, 8, 256>");
+    computeHashForBucketOnly(reinterpret_cast<const int8_t*>(weights), bias, desc, nbDesc,
+                             reinterpret_cast<uint8_t*>(bucketIndices), reinterpret_cast<uint32_t*>(histogram), stream);
+}
+template <class WRow, class Desc, class Codes, detail::block_t<WRow, 128> = 0, detail::block_t<Desc, 128> = 0,
+          detail::block_t<Codes, 8> = 0>
+inline void computeHashForBucketOnly(const WRow* weights, const int32_t* bias, const Desc* desc, uint32_t nbDesc,
+                                     Codes* bucketIndices, std::nullptr_t, hipStream_t stream)
+{
+    computeHashForBucketOnly(reinterpret_cast<const int8_t*>(weights), bias, desc, nbDesc,
+                             reinterpret_cast<uint8_t*>(bucketIndices), nullptr, stream);
+}
+template <class WRow, class Desc, class Codes, class Remap, detail::block_t<WRow, 128> = 0,
+          detail::block_t<Desc, 128> = 0, detail::block_t<Codes, 8> = 0, detail::block_t<Remap, 16> = 0>
+inline void computeHashForBucketAndRemap(const WRow* weights, const int32_t* bias, const Desc* desc, uint32_t nbDesc,
+                                         Codes* bucketIndices, Remap* remap, std::nullptr_t, hipStream_t stream)
+{
+    computeHashForBucketAndRemap(reinterpret_cast<const int8_t*>(weights), bias, desc, nbDesc,
+                                 reinterpret_cast<uint8_t*>(bucketIndices), reinterpret_cast<uint32_t*>(remap), nullptr,
+                                 stream);
+}
+template <class Hist, class Codes, detail::block_t<Hist, 8192> = 0, detail::block_t<Codes, 8> = 0>
+inline void buildHistogram(Hist* histogram, const Codes* bucketIndices, uint32_t nbDesc, hipStream_t stream)
+{
+    buildHistogram(reinterpret_cast<uint32_t*>(histogram), reinterpret_cast<const uint8_t*>(bucketIndices), nbDesc, stream);
+}
+template <class Hist, detail::block_t<Hist, 8192> = 0>
+inline void computeBucketBound(Hist& lower, Hist& upper, uint32_t* maxBucketSize, const Hist& histogram,
+                               hipStream_t stream)
+{
+    computeBucketBound(reinterpret_cast<uint32_t*>(&lower), reinterpret_cast<uint32_t*>(&upper), maxBucketSize,
+                       reinterpret_cast<const uint32_t*>(&histogram), stream);
+}
+template <class Codes, class Hist, detail::block_t<Codes, 8> = 0, detail::block_t<Hist, 8192> = 0>
+inline void buildBuckets(const Codes* bucketIndices, uint32_t nbDesc, Hist& idxAllocator, uint32_t* buckets,
+                         hipStream_t stream)
+{
+    buildBuckets(reinterpret_cast<const uint8_t*>(bucketIndices), nbDesc, reinterpret_cast<uint32_t*>(&idxAllocator),
+                 buckets, stream);
+}
 inline void findBestMatch(const ImageDesc* queries, const uint32_t* querySize, const ImageDesc& train, uint32_t trainSize,
                           const MatchResult* results, uint32_t nbTasks, const uint32_t* maxQueryBucketSize,
                           uint32_t maxTrainBucketSize, hipStream_t stream)
@@ -178,13 +259,40 @@ struct DeviceBuffer {
     {
         if (ptr) (void)hipFree(ptr);
     }
-    void reserve(size_t n)
+    // grows to at least n elements; true when it (re)allocated
+    bool reserve(size_t n)
     {
-        if (n <= count) return;
+        if (n <= count && ptr) return false;
         if (ptr) hipCheck(hipFree(ptr), "hipFree");
         ptr = nullptr;
         hipCheck(hipMalloc(&ptr, std::max<size_t>(n, 1) * sizeof(T)), "hipMalloc");
-        count = n;
+        count = std::max<size_t>(n, 1);
+        return true;
+    }
+};
+
+// page-locked host memory (asynchronous device-to-host copies)
+template <typename T>
+struct PinnedBuffer {
+    T* ptr = nullptr;
+    size_t count = 0;
+    PinnedBuffer() = default;
+    PinnedBuffer(const PinnedBuffer&) = delete;
+    PinnedBuffer& operator=(const PinnedBuffer&) = delete;
+    PinnedBuffer(PinnedBuffer&& o) noexcept : ptr(o.ptr), count(o.count) { o.ptr = nullptr; o.count = 0; }
+    ~PinnedBuffer()
+    {
+        if (ptr) (void)hipHostFree(ptr);
+    }
+    bool reserve(size_t n)
+    {
+        if (n <= count && ptr) return false;
+        if (ptr) hipCheck(hipHostFree(ptr), "hipHostFree");
+        ptr = nullptr;
+        hipCheck(hipHostMalloc(reinterpret_cast<void**>(&ptr), std::max<size_t>(n, 1) * sizeof(T), hipHostMallocDefault),
+                 "hipHostMalloc");
+        count = std::max<size_t>(n, 1);
+        return true;
     }
 };
 
@@ -244,9 +352,17 @@ private:
 using ImageHandle = uint32_t;
 
 // Per-image precompute cache and batched matching of queries against one
-// train image (bruteForceMatcher.cpp:163-443). Results come back as futures;
-// the cascade-hash method keeps the running descriptor mean in registration
-// order, as the reference does (bruteForceMatcher.cpp:97-100).
+// train image (bruteForceMatcher.cpp:163-443), as a GPU-resident service:
+//  - result storage per query slot (device, plus page-locked host memory for
+//    the copies back) grows by capacity and is reused across calls, as the
+//    reference's reserveStorage does (bruteForceMatcher.cpp:209-237); the
+//    private stream is synchronised only before storage grows;
+//  - results come back through page-locked memory and a host function enqueued
+//    on the private stream fulfils the promise (bruteForceMatcher.cpp:360-396),
+//    so no call blocks the host;
+//  - registration keeps its scratch (bias, codes, histogram, bounds) across
+//    calls and does not synchronise; the cascade-hash method keeps the running
+//    descriptor mean in registration order (bruteForceMatcher.cpp:97-100).
 class BruteForceMatcher {
 public:
     using Descriptor = std::array<uint32_t, 32>;
@@ -263,6 +379,7 @@ public:
         uint32_t nbDesc;
         ImageHandle hImage;
     };
+    using Result = std::vector<std::pair<std::vector<BestMatch>, std::vector<BestMatch>>>;
 
     BruteForceMatcher() : BruteForceMatcher(Config()) {}
     explicit BruteForceMatcher(Config cfg) : mCfg(std::move(cfg))
@@ -271,9 +388,10 @@ public:
         if (mCfg.useCascadeHash) {
             if (mCfg.casHashWeights.size() != 64 * 128 || mCfg.casHashBiasBase.size() != 64)
                 throw Exception("BruteForceMatcher: cascade hash needs 64x128 weights and 64 bias values");
-            mWeights.reserve(64 * 128);
-            mBiasBase.reserve(64);
-            mAcc.reserve(128);
+            grow(mWeights, 64 * 128);
+            grow(mBiasBase, 64);
+            grow(mAcc, 128);
+            grow(mBias, 64);
             hipCheck(hipMemcpy(mWeights.ptr, mCfg.casHashWeights.data(), 64 * 128, hipMemcpyHostToDevice), "H2D weights");
             hipCheck(hipMemcpy(mBiasBase.ptr, mCfg.casHashBiasBase.data(), 64 * 4, hipMemcpyHostToDevice), "H2D bias base");
         }
@@ -282,7 +400,10 @@ public:
     BruteForceMatcher& operator=(const BruteForceMatcher&) = delete;
     ~BruteForceMatcher()
     {
-        if (mStream) (void)hipStreamDestroy(mStream);
+        if (mStream) {
+            (void)hipStreamSynchronize(mStream);  // pending copies and host functions use the storage
+            (void)hipStreamDestroy(mStream);
+        }
     }
 
     void registerImage(ImageHandle hImage, const Descriptor* desc, uint32_t nbDesc, hipStream_t userStream)
@@ -292,65 +413,63 @@ public:
         PreComp pc;
         pc.nbDesc = nbDesc;
         if (mCfg.useSift4b) {
-            pc.desc4.reserve(size_t(nbDesc) * 64);
+            grow(pc.desc4, size_t(nbDesc) * 64);
             launchSiftDesc8bTo4b(pc.desc4.ptr, desc, nbDesc, mStream);
         }
         if (mCfg.useCascadeHash) {
             const bool first = mAccNbDesc == 0;
             cashash::accumuateDesc(mAcc.ptr, nullptr, first, desc, nbDesc, mStream);
             mAccNbDesc += nbDesc;
-            DeviceBuffer<int32_t> bias(64);
-            cashash::computeBias(bias.ptr, mWeights.ptr, mBiasBase.ptr, mAcc.ptr, mAccNbDesc, mStream);
-            DeviceBuffer<uint8_t> codes(size_t(nbDesc) * 8);
-            DeviceBuffer<uint32_t> hist(2048), lower(2048);
-            pc.ends.reserve(2048);
-            pc.buckets.reserve(size_t(nbDesc) * 8);
-            cashash::computeHashForBucketOnly(mWeights.ptr, bias.ptr, desc, nbDesc, codes.ptr, nullptr, mStream);
-            cashash::buildHistogram(hist.ptr, codes.ptr, nbDesc, mStream);
-            cashash::computeBucketBound(lower.ptr, pc.ends.ptr, nullptr, hist.ptr, mStream);
-            cashash::buildBuckets(codes.ptr, nbDesc, lower.ptr, pc.buckets.ptr, mStream);
-            hipCheck(hipStreamSynchronize(mStream), "registerImage");
+            cashash::computeBias(mBias.ptr, mWeights.ptr, mBiasBase.ptr, mAcc.ptr, mAccNbDesc, mStream);
+            if (mCodes.count < size_t(nbDesc) * 8 || mHist.count < 2048 || mLower.count < 2048) {
+                hipCheck(hipStreamSynchronize(mStream), "registerImage: grow scratch");
+                grow(mCodes, size_t(nbDesc) * 8);
+                grow(mHist, 2048);
+                grow(mLower, 2048);
+            }
+            grow(pc.ends, 2048);
+            grow(pc.buckets, size_t(nbDesc) * 8);
+            cashash::computeHashForBucketOnly(mWeights.ptr, mBias.ptr, desc, nbDesc, mCodes.ptr, nullptr, mStream);
+            cashash::buildHistogram(mHist.ptr, mCodes.ptr, nbDesc, mStream);
+            cashash::computeBucketBound(mLower.ptr, pc.ends.ptr, nullptr, mHist.ptr, mStream);
+            cashash::buildBuckets(mCodes.ptr, nbDesc, mLower.ptr, pc.buckets.ptr, mStream);
         }
         mPreComp[hImage] = std::move(pc);
         join(mStream, userStream);
     }
 
-    using Result = std::vector<std::pair<std::vector<BestMatch>, std::vector<BestMatch>>>;
-
     std::future<Result> match(const std::vector<Img>& queries, const Img& train, bool bidirectional, hipStream_t userStream)
     {
         std::lock_guard<std::mutex> lk(mLock);
-        join(userStream, mStream);
         (void)bidirectional;  // bidirectional is forced on, as in bruteForceMatcher.cpp:207
-        auto dev = matchImpl(queries, train);
-        auto res = std::make_shared<Result>(queries.size());
+        auto job = new MatchJob;
+        auto fut = job->promise.get_future();
+        if (queries.empty()) {
+            job->promise.set_value(Result{});
+            delete job;
+            return fut;
+        }
+        join(userStream, mStream);
+        matchImpl(queries, train);
         for (size_t i = 0; i < queries.size(); i++) {
-            (*res)[i].first.resize(queries[i].nbDesc);
-            (*res)[i].second.resize(train.nbDesc);
+            job->slots.push_back({mFwd[i].host.ptr, queries[i].nbDesc, mBwd[i].host.ptr, train.nbDesc});
             if (queries[i].nbDesc)
-                hipCheck(hipMemcpyAsync((*res)[i].first.data(), dev[i].first.ptr, queries[i].nbDesc * 8,
+                hipCheck(hipMemcpyAsync(mFwd[i].host.ptr, mFwd[i].dev.ptr, queries[i].nbDesc * sizeof(BestMatch),
                                         hipMemcpyDeviceToHost, mStream), "D2H fwd");
             if (train.nbDesc)
-                hipCheck(hipMemcpyAsync((*res)[i].second.data(), dev[i].second.ptr, train.nbDesc * 8,
+                hipCheck(hipMemcpyAsync(mBwd[i].host.ptr, mBwd[i].dev.ptr, train.nbDesc * sizeof(BestMatch),
                                         hipMemcpyDeviceToHost, mStream), "D2H bwd");
         }
-        hipEvent_t done;
-        hipCheck(hipEventCreateWithFlags(&done, hipEventDisableTiming), "event");
-        hipCheck(hipEventRecord(done, mStream), "record");
+        hipCheck(hipLaunchHostFunc(mStream, &MatchJob::run, job), "hipLaunchHostFunc");
         join(mStream, userStream);
-        auto keep = std::make_shared<std::vector<std::pair<DeviceBuffer<BestMatch>, DeviceBuffer<BestMatch>>>>(std::move(dev));
-        return std::async(std::launch::deferred, [done, res, keep]() {
-            hipCheck(hipEventSynchronize(done), "wait");
-            (void)hipEventDestroy(done);
-            return std::move(*res);
-        });
+        return fut;
     }
 
     // With crossCheck the filter runs on the device (rsfm_filter_matches_batch):
-    // only the match counts and the kept matches come back to the host, where
-    // the reference copies both full best-match arrays and filters on the CPU
-    // (bruteForceMatcher.cpp:354-369). Without crossCheck, removeMatchConflicts
-    // runs on the host as in the reference.
+    // the kept matches and their counts come back through page-locked memory,
+    // where the reference copies both full best-match arrays and filters on the
+    // CPU (bruteForceMatcher.cpp:354-369). Without crossCheck,
+    // removeMatchConflicts runs on the host as in the reference.
     std::future<std::vector<std::vector<ValidMatch>>> matchAndFilter(const std::vector<Img>& queries, const Img& train,
                                                                     bool crossCheck, hipStream_t userStream)
     {
@@ -366,46 +485,53 @@ public:
             });
         }
         std::lock_guard<std::mutex> lk(mLock);
-        join(userStream, mStream);
-        auto dev = std::make_shared<std::vector<std::pair<DeviceBuffer<BestMatch>, DeviceBuffer<BestMatch>>>>(
-            matchImpl(queries, train));
+        auto job = new FilterJob;
+        auto fut = job->promise.get_future();
         const size_t nq = queries.size();
-        auto kept = std::make_shared<std::vector<DeviceBuffer<ValidMatch>>>();
-        auto counts = std::make_shared<DeviceBuffer<uint32_t>>(std::max<size_t>(nq, 1));
-        std::vector<rsfm_filter_task> tasks(nq);
-        for (size_t i = 0; i < nq; i++) {
-            kept->emplace_back(std::max<uint32_t>(queries[i].nbDesc, 1));
-            tasks[i] = rsfm_filter_task{reinterpret_cast<const rsfm_best_match*>((*dev)[i].first.ptr), nullptr,
-                                        queries[i].nbDesc, reinterpret_cast<const rsfm_best_match*>((*dev)[i].second.ptr),
-                                        train.nbDesc, reinterpret_cast<rsfm_valid_match*>((*kept)[i].ptr),
-                                        counts->ptr + i, nullptr, nullptr, nullptr};
+        if (nq == 0) {
+            job->promise.set_value({});
+            delete job;
+            return fut;
         }
-        if (nq) rsfmCheck(rsfm_filter_matches_batch(tasks.data(), nq, 4, 5, mStream), "rsfm_filter_matches_batch");
-        auto hostCounts = std::make_shared<std::vector<uint32_t>>(nq);
-        if (nq)
-            hipCheck(hipMemcpyAsync(hostCounts->data(), counts->ptr, nq * sizeof(uint32_t), hipMemcpyDeviceToHost,
-                                    mStream),
-                     "D2H counts");
-        hipEvent_t done;
-        hipCheck(hipEventCreateWithFlags(&done, hipEventDisableTiming), "event");
-        hipCheck(hipEventRecord(done, mStream), "record");
-        join(mStream, userStream);
-        const hipStream_t st = mStream;
-        return std::async(std::launch::deferred, [done, dev, kept, counts, hostCounts, st]() {
-            hipCheck(hipEventSynchronize(done), "wait");
-            (void)hipEventDestroy(done);
-            std::vector<std::vector<ValidMatch>> out(hostCounts->size());
-            for (size_t i = 0; i < out.size(); i++) {
-                out[i].resize((*hostCounts)[i]);
-                if (!out[i].empty())
-                    hipCheck(hipMemcpyAsync(out[i].data(), (*kept)[i].ptr, out[i].size() * sizeof(ValidMatch),
-                                            hipMemcpyDeviceToHost, st),
-                             "D2H kept matches");
+        join(userStream, mStream);
+        matchImpl(queries, train);
+        bool regrow = mKept.size() < nq || mCounts.count < nq || mHostCounts.count < nq;
+        for (size_t i = 0; i < std::min(nq, mKept.size()); i++)
+            regrow |= mKept[i].dev.count < queries[i].nbDesc || mKept[i].host.count < queries[i].nbDesc;
+        if (regrow) {
+            hipCheck(hipStreamSynchronize(mStream), "matchAndFilter: grow storage");
+            if (mKept.size() < nq) mKept.resize(nq);
+            grow(mCounts, nq);
+            grow(mHostCounts, nq);
+            for (size_t i = 0; i < nq; i++) {
+                grow(mKept[i].dev, queries[i].nbDesc);
+                grow(mKept[i].host, queries[i].nbDesc);
             }
-            hipCheck(hipStreamSynchronize(st), "sync");
-            return out;
-        });
+        }
+        std::vector<rsfm_filter_task> tasks(nq);
+        for (size_t i = 0; i < nq; i++)
+            tasks[i] = rsfm_filter_task{mFwd[i].dev.ptr, nullptr, queries[i].nbDesc, mBwd[i].dev.ptr, train.nbDesc,
+                                        mKept[i].dev.ptr, mCounts.ptr + i, nullptr, nullptr, nullptr};
+        rsfmCheck(rsfm_filter_matches_batch(tasks.data(), nq, 4, 5, mStream), "rsfm_filter_matches_batch");
+        hipCheck(hipMemcpyAsync(mHostCounts.ptr, mCounts.ptr, nq * sizeof(uint32_t), hipMemcpyDeviceToHost, mStream),
+                 "D2H counts");
+        for (size_t i = 0; i < nq; i++) {
+            if (queries[i].nbDesc)
+                hipCheck(hipMemcpyAsync(mKept[i].host.ptr, mKept[i].dev.ptr, queries[i].nbDesc * sizeof(ValidMatch),
+                                        hipMemcpyDeviceToHost, mStream),
+                         "D2H kept matches");
+            job->kept.push_back(mKept[i].host.ptr);
+        }
+        job->counts = mHostCounts.ptr;
+        hipCheck(hipLaunchHostFunc(mStream, &FilterJob::run, job), "hipLaunchHostFunc");
+        join(mStream, userStream);
+        return fut;
     }
+
+    // device and page-locked allocations made so far (storage, scratch and
+    // per-image precompute data); constant across calls once storage is large
+    // enough
+    size_t allocations() const { return mAllocations; }
 
 private:
     struct PreComp {
@@ -414,6 +540,55 @@ private:
         DeviceBuffer<uint32_t> ends;
         DeviceBuffer<uint32_t> buckets;
     };
+    struct MatchStorage {
+        DeviceBuffer<BestMatch> dev;
+        PinnedBuffer<BestMatch> host;
+    };
+    struct KeptStorage {
+        DeviceBuffer<ValidMatch> dev;
+        PinnedBuffer<ValidMatch> host;
+    };
+    // results of one match() call, filled from page-locked memory by a host
+    // function that runs after the copies on the private stream
+    struct MatchJob {
+        struct Slot {
+            const BestMatch* fwd;
+            uint32_t nFwd;
+            const BestMatch* bwd;
+            uint32_t nBwd;
+        };
+        std::vector<Slot> slots;
+        std::promise<Result> promise;
+        static void run(void* p)
+        {
+            std::unique_ptr<MatchJob> job(static_cast<MatchJob*>(p));
+            Result r(job->slots.size());
+            for (size_t i = 0; i < job->slots.size(); i++) {
+                const Slot& s = job->slots[i];
+                r[i].first.assign(s.fwd, s.fwd + s.nFwd);
+                r[i].second.assign(s.bwd, s.bwd + s.nBwd);
+            }
+            job->promise.set_value(std::move(r));
+        }
+    };
+    struct FilterJob {
+        std::vector<const ValidMatch*> kept;
+        const uint32_t* counts = nullptr;
+        std::promise<std::vector<std::vector<ValidMatch>>> promise;
+        static void run(void* p)
+        {
+            std::unique_ptr<FilterJob> job(static_cast<FilterJob*>(p));
+            std::vector<std::vector<ValidMatch>> out(job->kept.size());
+            for (size_t i = 0; i < out.size(); i++) out[i].assign(job->kept[i], job->kept[i] + job->counts[i]);
+            job->promise.set_value(std::move(out));
+        }
+    };
+
+    template <class Buf>
+    void grow(Buf& b, size_t n)
+    {
+        if (b.reserve(n)) mAllocations++;
+    }
 
     static void join(hipStream_t from, hipStream_t to)
     {
@@ -425,12 +600,31 @@ private:
         hipCheck(hipEventDestroy(e), "destroy");
     }
 
-    std::vector<std::pair<DeviceBuffer<BestMatch>, DeviceBuffer<BestMatch>>> matchImpl(const std::vector<Img>& queries,
-                                                                                      const Img& train)
+    // query slot i's storage holds at least nbDesc (fwd) and the train size (bwd);
+    // the private stream is drained before anything is reallocated
+    void reserveStorage(const std::vector<Img>& queries, const Img& train)
     {
-        std::vector<std::pair<DeviceBuffer<BestMatch>, DeviceBuffer<BestMatch>>> out;
-        out.reserve(queries.size());
-        for (const auto& q : queries) out.emplace_back(DeviceBuffer<BestMatch>(q.nbDesc), DeviceBuffer<BestMatch>(train.nbDesc));
+        bool regrow = mFwd.size() < queries.size();
+        for (size_t i = 0; i < std::min(queries.size(), mFwd.size()); i++)
+            regrow |= mFwd[i].dev.count < queries[i].nbDesc || mBwd[i].dev.count < train.nbDesc ||
+                      mFwd[i].host.count < queries[i].nbDesc || mBwd[i].host.count < train.nbDesc;
+        if (!regrow) return;
+        hipCheck(hipStreamSynchronize(mStream), "reserveStorage");
+        if (mFwd.size() < queries.size()) {
+            mFwd.resize(queries.size());
+            mBwd.resize(queries.size());
+        }
+        for (size_t i = 0; i < queries.size(); i++) {
+            grow(mFwd[i].dev, queries[i].nbDesc);
+            grow(mFwd[i].host, queries[i].nbDesc);
+            grow(mBwd[i].dev, train.nbDesc);
+            grow(mBwd[i].host, train.nbDesc);
+        }
+    }
+
+    void matchImpl(const std::vector<Img>& queries, const Img& train)
+    {
+        reserveStorage(queries, train);
         if (mCfg.useCascadeHash) {
             std::vector<cashash::ImageDesc> qi(queries.size());
             std::vector<uint32_t> qs(queries.size());
@@ -440,7 +634,7 @@ private:
                 const PreComp& p = mPreComp.at(queries[i].hImage);
                 qi[i] = cashash::ImageDesc{queries[i].desc, p.ends.ptr, p.buckets.ptr, nullptr};
                 qs[i] = queries[i].nbDesc;
-                res[i] = cashash::MatchResult{out[i].first.ptr, out[i].second.ptr};
+                res[i] = cashash::MatchResult{mFwd[i].dev.ptr, mBwd[i].dev.ptr};
             }
             const cashash::ImageDesc ti{train.desc, tp.ends.ptr, tp.buckets.ptr, nullptr};
             cashash::findBestMatch(qi.data(), qs.data(), ti, train.nbDesc, res.data(), uint32_t(queries.size()), nullptr, 0,
@@ -454,8 +648,8 @@ private:
                 tasks[i].nbDescB = train.nbDesc;
                 tasks[i].descA = p.desc4.ptr;
                 tasks[i].descB = tp.desc4.ptr;
-                tasks[i].bestMatchA = out[i].first.ptr;
-                tasks[i].bestMatchB = out[i].second.ptr;
+                tasks[i].bestMatchA = mFwd[i].dev.ptr;
+                tasks[i].bestMatchB = mBwd[i].dev.ptr;
                 tasks[i].sqrNormDescA = nullptr;
                 tasks[i].sqrNormDescB = nullptr;
             }
@@ -464,20 +658,27 @@ private:
             std::vector<SiftBruteForceMatchTask> tasks(queries.size());
             for (size_t i = 0; i < queries.size(); i++)
                 tasks[i] = SiftBruteForceMatchTask{queries[i].nbDesc, train.nbDesc, queries[i].desc, train.desc,
-                                                   out[i].first.ptr, out[i].second.ptr, nullptr, nullptr};
+                                                   mFwd[i].dev.ptr, mBwd[i].dev.ptr, nullptr, nullptr};
             launchBruteForceMatchIMMA(tasks.data(), tasks.size(), mStream);
         }
-        return out;
     }
 
     Config mCfg;
     hipStream_t mStream = nullptr;
     std::mutex mLock;
     std::unordered_map<ImageHandle, PreComp> mPreComp;
+    std::vector<MatchStorage> mFwd, mBwd;
+    std::vector<KeptStorage> mKept;
+    DeviceBuffer<uint32_t> mCounts;
+    PinnedBuffer<uint32_t> mHostCounts;
     DeviceBuffer<int8_t> mWeights;
     DeviceBuffer<int32_t> mBiasBase;
     DeviceBuffer<uint64_t> mAcc;
+    DeviceBuffer<int32_t> mBias;
+    DeviceBuffer<uint8_t> mCodes;
+    DeviceBuffer<uint32_t> mHist, mLower;
     uint64_t mAccNbDesc = 0;
+    size_t mAllocations = 0;
 };
 
 }  // namespace rsfm

@@ -129,6 +129,36 @@ int launch_score7_table(const DevTask* tasks, uint32_t nTasks, uint32_t maxH, fl
                         float minSepSq0, float minSepSq1, hipStream_t stream);
 }  // namespace ransac
 
+namespace rmf {
+// batched affine match filter (rsfm_rmf_filter_batch) over one pair
+struct BatchTask {
+    float4* pts;                // gathered pairs, compacted in place
+    rsfm_valid_match* matches;  // optional, compacted alongside
+    uint32_t* count;            // device count, rewritten with the kept count
+    uint32_t nHost;
+    uint8_t* mask;              // optional, original order
+    float4* reordered;          // workspace: pairs in cell order
+    uint32_t* pos;              // workspace: slot of point i in cell order
+    uint32_t* bounds;           // workspace: [cells + 1]
+    float* affine;              // workspace: [cells][6]
+    uint8_t* votes;             // workspace: [nHost rounded up to 4], cell order
+    uint32_t cols, rows, cellWidth;
+    float threshold;            // pixels
+    uint32_t seed;
+};
+struct BatchParams {
+    uint32_t nbTests, minVotes;
+    int tryOther;
+};
+// argument checks and grid of one pair; its workspace bytes; its workspace pointers
+int make_batch_task(const rsfm_rmf_task& t, uint32_t cellCols, float relativeThreshold, BatchTask& out);
+size_t batch_workspace_bytes(uint32_t nPts, uint32_t nCells);
+void place_workspace(BatchTask& t, uint8_t*& ws);
+// binning, cell kernels and compaction over a device task table
+int launch_batch_table(const BatchTask* tasks, uint32_t nTasks, uint32_t maxCols, uint32_t maxRows, const BatchParams& p,
+                       hipStream_t stream);
+}  // namespace rmf
+
 namespace undist {
 struct Params {
     float fx, fy, cx, cy;      // pinhole

@@ -26,6 +26,9 @@
 //  * Pairs that carry lens distortion have their gathered keypoint pairs
 //    undistorted in place right after the filter (one launch over a device
 //    table), as the reference's makePtPair does before solving.
+//  * An optional affine match filter stage (the reference's RansacMatchFilter
+//    before pair solving) bins, filters and compacts every pair's gathered
+//    keypoint pairs on the device (rmf.hip), four launches for the batch.
 //  * With the 8-point solver the end of the reference's driver can follow
 //    the scorer as one more stage (adaptive cut, refit on the inliers;
 //    ransac_finish.hip), one workgroup per pair.
@@ -78,6 +81,9 @@ struct rsfm_pipeline {
     ransac::FinishTask* dFinish = nullptr;  // 8-point solver with the finishing stage
     ransac::FinishParams finParams{};
     bool finish = false;
+    rmf::BatchTask* dRmf = nullptr;  // affine match filter stage
+    rmf::BatchParams rmfParams{};
+    uint32_t rmfCols = 0, rmfRows = 0;
     pipe::FillRange* dFill = nullptr;
     uint32_t nFill = 0, maxFill = 0;
     uint64_t* dPart = nullptr;
@@ -295,6 +301,8 @@ int capture(rsfm_pipeline* pl)
     RSFM_CHECK(check_hip(hipStreamBeginCapture(pl->stream, hipStreamCaptureModeThreadLocal), "begin capture"));
     int rc = enqueue_match(pl, pl->stream);
     if (rc == RSFM_OK) rc = aux::launch_filter_table(pl->dFilter, P, c.ratioNum, c.ratioDen, pl->stream);
+    if (rc == RSFM_OK && c.matchFilter)
+        rc = rmf::launch_batch_table(pl->dRmf, P, pl->rmfCols, pl->rmfRows, pl->rmfParams, pl->stream);
     if (rc == RSFM_OK) rc = undist::launch_pairs_table(pl->dUnd, pl->nUnd, pl->maxUnd, pl->stream);
     if (rc == RSFM_OK && c.nbHyp > 0) rc = ransac::launch_sample_table(pl->dSample, P, c.nbHyp, pl->stream);
     if (c.solver == RSFM_PIPELINE_SOLVER_F8) {
@@ -352,6 +360,9 @@ extern "C" int rsfm_pipeline_create(const rsfm_pipeline_pair* pairs, size_t nbPa
     ransac::FinishParams finParams{};
     if (finish) RSFM_CHECK(ransac::make_finish_params(cfg->threshold, cfg->mode, cfg->confidence, cfg->maxIter,
                                                       cfg->refit != 0, finParams));
+    RSFM_RETURN_IF(cfg->matchFilter > 1, RSFM_ERR_INVALID, "pipeline: matchFilter must be 0 or 1");
+    RSFM_RETURN_IF(cfg->matchFilter && (cfg->mfCellCols == 0 || !(cfg->mfThreshold > 0.f)), RSFM_ERR_INVALID,
+                   "pipeline: match filter needs mfCellCols >= 1 and mfThreshold > 0");
     for (size_t i = 0; i < nbPairs; i++) {
         const auto& x = pairs[i];
         RSFM_RETURN_IF(x.nA > 0 && (x.descA == nullptr || x.bestA == nullptr || x.secondA == nullptr || x.kpA == nullptr ||
@@ -423,6 +434,25 @@ extern "C" int rsfm_pipeline_create(const rsfm_pipeline_pair* pairs, size_t nbPa
     const size_t oSample = take(P * sizeof(ransac::SampleTask));
     const size_t oRansac = take(P * sizeof(ransac::DevTask));
     const size_t oFinish = take(P * sizeof(ransac::FinishTask));
+    // affine match filter stage: task table and per-pair workspaces
+    std::vector<rmf::BatchTask> rt_mf;
+    size_t mfBytes = 0;
+    if (cfg->matchFilter) {
+        rt_mf.resize(P);
+        for (uint32_t i = 0; i < P; i++) {
+            const auto& x = pl->pairs[i];
+            rc = rmf::make_batch_task(rsfm_rmf_task{x.pts, x.matches, x.count, x.nA, nullptr, x.width0, x.height0,
+                                                    (uint32_t)(cfg->seed + i)},
+                                      cfg->mfCellCols, cfg->mfThreshold, rt_mf[i]);
+            if (rc != RSFM_OK) return fail_out(rc);
+            mfBytes += rmf::batch_workspace_bytes(x.nA, rt_mf[i].cols * rt_mf[i].rows);
+            pl->rmfCols = std::max(pl->rmfCols, rt_mf[i].cols);
+            pl->rmfRows = std::max(pl->rmfRows, rt_mf[i].rows);
+        }
+        pl->rmfParams = rmf::BatchParams{cfg->mfTests, cfg->mfMinVotes, (int)cfg->mfTryOther};
+    }
+    const size_t oRmf = take(std::max<size_t>(1, rt_mf.size()) * sizeof(rmf::BatchTask));
+    const size_t oRmfWs = take(std::max<size_t>(1, mfBytes));
     const size_t oF7 = take(P * sizeof(ransac::DevTask));
     const size_t oDg = take(P * sizeof(degensac::DevTask));
     const size_t oUnd = take(P * sizeof(undist::PairTask));
@@ -445,6 +475,11 @@ extern "C" int rsfm_pipeline_create(const rsfm_pipeline_pair* pairs, size_t nbPa
     pl->dSample = reinterpret_cast<ransac::SampleTask*>(pl->arena + oSample);
     pl->dRansac = reinterpret_cast<ransac::DevTask*>(pl->arena + oRansac);
     pl->dFinish = reinterpret_cast<ransac::FinishTask*>(pl->arena + oFinish);
+    pl->dRmf = reinterpret_cast<rmf::BatchTask*>(pl->arena + oRmf);
+    {
+        uint8_t* ws = pl->arena + oRmfWs;
+        for (auto& t : rt_mf) rmf::place_workspace(t, ws);
+    }
     pl->dF7 = reinterpret_cast<ransac::DevTask*>(pl->arena + oF7);
     pl->dDg = reinterpret_cast<degensac::DevTask*>(pl->arena + oDg);
     pl->dUnd = reinterpret_cast<undist::PairTask*>(pl->arena + oUnd);
@@ -528,6 +563,7 @@ extern "C" int rsfm_pipeline_create(const rsfm_pipeline_pair* pairs, size_t nbPa
         (rc = upload(pl->dSample, st.data(), P * sizeof(ransac::SampleTask))) != RSFM_OK ||
         (rc = upload(pl->dRansac, rt.data(), P * sizeof(ransac::DevTask))) != RSFM_OK ||
         (rc = upload(pl->dFinish, fin.data(), fin.size() * sizeof(ransac::FinishTask))) != RSFM_OK ||
+        (rc = upload(pl->dRmf, rt_mf.data(), rt_mf.size() * sizeof(rmf::BatchTask))) != RSFM_OK ||
         (rc = upload(pl->dF7, f7.data(), f7.size() * sizeof(ransac::DevTask))) != RSFM_OK ||
         (rc = upload(pl->dDg, dg.data(), dg.size() * sizeof(degensac::DevTask))) != RSFM_OK ||
         (rc = upload(pl->dUnd, ud.data(), ud.size() * sizeof(undist::PairTask))) != RSFM_OK ||

@@ -13,7 +13,7 @@
 // clock(); a cell with fewer than 9 neighbourhood matches stores a NaN affine
 // (the reference leaves it unwritten); every thread votes with the CTA-best
 // count (the reference tests partial per-thread counts before a barrier).
-#include "common.hpp"
+#include "internal.hpp"
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -147,18 +147,18 @@ __device__ inline void cta_find_max(Reducer& R, uint32_t& cnt, float T[6])
     }
 }
 
-__global__ __launch_bounds__(kCta) void best_affine_kernel(float* __restrict__ affine, const float4* __restrict__ m,
-                                                           const uint32_t* __restrict__ bounds, uint32_t cols,
-                                                           uint32_t rows, uint32_t nbTests, float threshold, uint32_t seed)
+// one cell (bx, by): the best of nbTests affine hypotheses
+__device__ inline void best_affine_cell(Reducer& R, float* __restrict__ affine, const float4* __restrict__ m,
+                                        const uint32_t* __restrict__ bounds, uint32_t cols, uint32_t rows,
+                                        uint32_t nbTests, float threshold, uint32_t seed, uint32_t bx, uint32_t by)
 {
     Cells c;
-    cells_init(c, bounds, cols, rows, blockIdx.x, blockIdx.y);
-    const uint32_t rank = blockIdx.y * cols + blockIdx.x;
+    cells_init(c, bounds, cols, rows, bx, by);
+    const uint32_t rank = by * cols + bx;
     if (c.sum < 9) {
         if (threadIdx.x < 6) affine[rank * 6 + threadIdx.x] = __builtin_nanf("");
         return;
     }
-    __shared__ Reducer R;
     float best[6] = {1.f, 0.f, 0.f, 0.f, 1.f, 0.f};
     uint32_t bestCnt = 0;
     uint32_t st = seed + (kCta * rank + threadIdx.x) * 22695477u;
@@ -200,22 +200,35 @@ __global__ __launch_bounds__(kCta) void best_affine_kernel(float* __restrict__ a
     }
 }
 
+__global__ __launch_bounds__(kCta) void best_affine_kernel(float* __restrict__ affine, const float4* __restrict__ m,
+                                                           const uint32_t* __restrict__ bounds, uint32_t cols,
+                                                           uint32_t rows, uint32_t nbTests, float threshold, uint32_t seed)
+{
+    __shared__ Reducer R;
+    best_affine_cell(R, affine, m, bounds, cols, rows, nbTests, threshold, seed, blockIdx.x, blockIdx.y);
+}
+
 __device__ inline void vote(uint32_t* votesWords, uint32_t idx)
 {
     atomicAdd(votesWords + (idx >> 2), 1u << (8 * (idx & 3)));
 }
 
-__global__ __launch_bounds__(kCta) void vote_kernel(uint32_t* __restrict__ votesWords, const float* __restrict__ affine,
-                                                    const float4* __restrict__ m, const uint32_t* __restrict__ bounds,
-                                                    uint32_t cols, uint32_t rows, float threshold, bool tryOther)
+struct VoteSmem {
+    Reducer R;
+    float best[6];
+    uint32_t bestCnt;
+};
+
+// one cell: its own affine (or, with tryOther, the best of every cell's affine
+// on its neighbourhood) votes for its inliers when it explains >= 6 of them
+__device__ inline void vote_cell(VoteSmem& S, uint32_t* __restrict__ votesWords, const float* __restrict__ affine,
+                                 const float4* __restrict__ m, const uint32_t* __restrict__ bounds, uint32_t cols,
+                                 uint32_t rows, float threshold, bool tryOther, uint32_t bx, uint32_t by)
 {
     Cells c;
-    cells_init(c, bounds, cols, rows, blockIdx.x, blockIdx.y);
-    const uint32_t rank = blockIdx.y * cols + blockIdx.x;
+    cells_init(c, bounds, cols, rows, bx, by);
+    const uint32_t rank = by * cols + bx;
     const uint32_t nCells = cols * rows;
-    __shared__ Reducer R;
-    __shared__ float sBest[6];
-    __shared__ uint32_t sBestCnt;
     float best[6];
 #pragma unroll
     for (int q = 0; q < 6; q++) best[q] = affine[rank * 6 + q];
@@ -232,25 +245,270 @@ __global__ __launch_bounds__(kCta) void vote_kernel(uint32_t* __restrict__ votes
                 for (int q = 0; q < 6; q++) best[q] = T[q];
             }
         }
-        cta_find_max(R, bestCnt, best);
+        cta_find_max(S.R, bestCnt, best);
     } else if (threadIdx.x == 0) {
         bestCnt = count_inliers(c, best, m, threshold);
     }
     if (threadIdx.x == 0) {
-        sBestCnt = bestCnt;
+        S.bestCnt = bestCnt;
 #pragma unroll
-        for (int q = 0; q < 6; q++) sBest[q] = best[q];
+        for (int q = 0; q < 6; q++) S.best[q] = best[q];
     }
     __syncthreads();
-    if (sBestCnt < kMinInliers) return;
+    if (S.bestCnt < kMinInliers) return;
     float T[6];
 #pragma unroll
-    for (int q = 0; q < 6; q++) T[q] = sBest[q];
+    for (int q = 0; q < 6; q++) T[q] = S.best[q];
     const float sq = sq_threshold(T, threshold);
     for (uint32_t k = threadIdx.x; k < c.sum; k += kCta) {
         const uint32_t mi = cells_map(c, k);
         if (is_inlier(T, m[mi], sq)) vote(votesWords, mi);
     }
+}
+
+__global__ __launch_bounds__(kCta) void vote_kernel(uint32_t* __restrict__ votesWords, const float* __restrict__ affine,
+                                                    const float4* __restrict__ m, const uint32_t* __restrict__ bounds,
+                                                    uint32_t cols, uint32_t rows, float threshold, bool tryOther)
+{
+    __shared__ VoteSmem S;
+    vote_cell(S, votesWords, affine, m, bounds, cols, rows, threshold, tryOther, blockIdx.x, blockIdx.y);
+}
+
+// ---------------------------------------------------------------------------
+// Batched form (no reference counterpart: the reference bins each pair's
+// matches on the host and runs one pair per call with a blocking sync,
+// RansacMatchFilter.cpp:42-133). Per pair: device binning with the
+// reference's rule and order, the two cell kernels over all pairs in one
+// launch each, then the mask (votes >= minVotes) and an in-place stable
+// compaction of the gathered pairs and matches with the kept count.
+constexpr int kBinThreads = 1024;
+constexpr int kBinWaves = kBinThreads / 64;
+constexpr uint32_t kMaxCells = 1024;
+
+__device__ inline uint32_t task_n(const BatchTask& T) { return min(*T.count, T.nHost); }
+
+// cell of a point: the reference's ceil rule, clamped (RansacMatchFilter.cpp:47-52)
+__device__ inline uint32_t cell_of(float x, float y, const BatchTask& T)
+{
+    const float cw = (float)T.cellWidth;
+    const float fi = fminf(fmaxf(ceilf(y / cw), 0.f), (float)T.rows - 1.f);
+    const float fj = fminf(fmaxf(ceilf(x / cw), 0.f), (float)T.cols - 1.f);
+    return (uint32_t)fi * T.cols + (uint32_t)fj;
+}
+
+struct BinSmem {
+    uint32_t cnt[kMaxCells];
+    uint32_t cursor[kMaxCells];
+    uint16_t waveCnt[kBinWaves][kMaxCells];
+    uint32_t scan[kBinThreads];
+};
+
+// one workgroup per task: votes cleared, cell counts, bounds (exclusive
+// prefix, cells in row-major order), then a stable scatter: points in index
+// order, a point's slot = its cell's start + earlier points of the same cell
+// (ballot peer groups inside a wave, per-wave cell counts across the waves of a
+// round of 1024 points, a running cursor across rounds)
+template <class Src>
+__global__ __launch_bounds__(kBinThreads) void bin_kernel(const Src batch)
+{
+    __shared__ BinSmem S;
+    const BatchTask& T = batch[blockIdx.x];
+    const uint32_t n = task_n(T);
+    const uint32_t nCells = T.cols * T.rows;
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (uint32_t w = tid; w < (T.nHost + 3) / 4; w += kBinThreads) reinterpret_cast<uint32_t*>(T.votes)[w] = 0;
+    for (uint32_t c = tid; c < kMaxCells; c += kBinThreads) S.cnt[c] = 0;
+    __syncthreads();
+    for (uint32_t i = tid; i < n; i += kBinThreads) {
+        const float4 p = T.pts[i];
+        atomicAdd(&S.cnt[cell_of(p.x, p.y, T)], 1u);
+    }
+    __syncthreads();
+    // exclusive prefix over the cells (nCells <= kMaxCells = kBinThreads)
+    const uint32_t v = tid < nCells ? S.cnt[tid] : 0u;
+    S.scan[tid] = v;
+    __syncthreads();
+    for (uint32_t off = 1; off < kBinThreads; off <<= 1) {
+        const uint32_t a = tid >= off ? S.scan[tid - off] : 0u;
+        __syncthreads();
+        S.scan[tid] += a;
+        __syncthreads();
+    }
+    if (tid < nCells) {
+        const uint32_t start = S.scan[tid] - v;
+        S.cursor[tid] = start;
+        T.bounds[tid] = start;
+    }
+    if (tid == 0) T.bounds[nCells] = n;
+    __syncthreads();
+    for (uint32_t base = 0; base < n; base += kBinThreads) {
+        const uint32_t i = base + tid;
+        const bool valid = i < n;
+        float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
+        uint32_t cell = ~0u;
+        if (valid) {
+            p = T.pts[i];
+            cell = cell_of(p.x, p.y, T);
+        }
+        for (uint32_t c = lane; c < nCells; c += 64) S.waveCnt[wave][c] = 0;
+        // lanes of this wave with the same cell
+        uint64_t peers = 0;
+        uint64_t todo = __ballot(valid);
+        while (todo) {
+            const int leader = __ffsll((unsigned long long)todo) - 1;
+            const uint32_t lc = (uint32_t)__shfl((int)cell, leader);
+            const uint64_t same = __ballot(valid && cell == lc);
+            if (valid && cell == lc) peers = same;
+            todo &= ~same;
+        }
+        const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+        const uint32_t rank = (uint32_t)__popcll(peers & below);
+        __syncthreads();  // waveCnt cleared by every wave
+        if (valid && rank == 0) S.waveCnt[wave][cell] = (uint16_t)__popcll(peers);
+        __syncthreads();
+        if (valid) {
+            uint32_t pos = S.cursor[cell] + rank;
+            for (uint32_t w = 0; w < wave; w++) pos += S.waveCnt[w][cell];
+            T.reordered[pos] = p;
+            T.pos[i] = pos;
+        }
+        __syncthreads();
+        for (uint32_t c = tid; c < nCells; c += kBinThreads) {
+            uint32_t add = 0;
+            for (uint32_t w = 0; w < kBinWaves; w++) add += S.waveCnt[w][c];
+            S.cursor[c] += add;
+        }
+        __syncthreads();
+    }
+}
+
+template <class Src>
+__global__ __launch_bounds__(kCta) void best_affine_batch_kernel(const Src batch, BatchParams prm)
+{
+    __shared__ Reducer R;
+    const BatchTask& T = batch[blockIdx.z];
+    if (blockIdx.x >= T.cols || blockIdx.y >= T.rows || task_n(T) < 16) return;
+    best_affine_cell(R, T.affine, T.reordered, T.bounds, T.cols, T.rows, prm.nbTests, T.threshold, T.seed, blockIdx.x,
+                     blockIdx.y);
+}
+
+template <class Src>
+__global__ __launch_bounds__(kCta) void vote_batch_kernel(const Src batch, BatchParams prm)
+{
+    __shared__ VoteSmem S;
+    const BatchTask& T = batch[blockIdx.z];
+    if (blockIdx.x >= T.cols || blockIdx.y >= T.rows || task_n(T) < 16) return;
+    vote_cell(S, reinterpret_cast<uint32_t*>(T.votes), T.affine, T.reordered, T.bounds, T.cols, T.rows, T.threshold,
+              prm.tryOther != 0, blockIdx.x, blockIdx.y);
+}
+
+// one workgroup per task: keep[i] = votes of point i >= minVotes; the kept
+// points (and matches) move to the front in index order, in place (a round
+// reads all of its 1024 entries before any is written, and writes only below
+// the next round's first entry); the count becomes the kept count
+template <class Src>
+__global__ __launch_bounds__(kBinThreads) void compact_kernel(const Src batch, BatchParams prm)
+{
+    __shared__ uint32_t sWave[kBinWaves];
+    __shared__ uint32_t sBase;
+    const BatchTask& T = batch[blockIdx.x];
+    const uint32_t n = task_n(T);
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (tid == 0) sBase = 0;
+    __syncthreads();
+    for (uint32_t base = 0; base < n; base += kBinThreads) {
+        const uint32_t i = base + tid;
+        bool keep = false;
+        float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
+        rsfm_valid_match mv{};
+        if (i < n) {
+            keep = n >= 16 && T.votes[T.pos[i]] >= prm.minVotes;
+            p = T.pts[i];
+            if (T.matches) mv = T.matches[i];
+            if (T.mask) T.mask[i] = keep ? 1 : 0;
+        }
+        const uint64_t b = __ballot(keep);
+        const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+        if (lane == 0) sWave[wave] = (uint32_t)__popcll(b);
+        __syncthreads();
+        uint32_t off = sBase;
+        for (uint32_t w = 0; w < wave; w++) off += sWave[w];
+        if (keep) {
+            const uint32_t o = off + (uint32_t)__popcll(b & below);
+            T.pts[o] = p;
+            if (T.matches) T.matches[o] = mv;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            uint32_t t = 0;
+            for (uint32_t w = 0; w < kBinWaves; w++) t += sWave[w];
+            sBase += t;
+        }
+        __syncthreads();
+    }
+    if (T.mask)
+        for (uint32_t i = n + tid; i < T.nHost; i += kBinThreads) T.mask[i] = 0;
+    if (tid == 0) *T.count = sBase;
+}
+
+int launch_batch_table(const BatchTask* tasks, uint32_t nTasks, uint32_t maxCols, uint32_t maxRows, const BatchParams& p,
+                       hipStream_t stream)
+{
+    if (nTasks == 0) return RSFM_OK;
+    const InTable<BatchTask> src{tasks};
+    bin_kernel<<<nTasks, kBinThreads, 0, stream>>>(src);
+    RSFM_CHECK(check_launch("rmf bin table"));
+    best_affine_batch_kernel<<<dim3(maxCols, maxRows, nTasks), kCta, 0, stream>>>(src, p);
+    RSFM_CHECK(check_launch("rmf best affine table"));
+    vote_batch_kernel<<<dim3(maxCols, maxRows, nTasks), kCta, 0, stream>>>(src, p);
+    RSFM_CHECK(check_launch("rmf vote table"));
+    compact_kernel<<<nTasks, kBinThreads, 0, stream>>>(src, p);
+    return check_launch("rmf compact table");
+}
+
+size_t batch_workspace_bytes(uint32_t nPts, uint32_t nCells)
+{
+    return align_up((size_t)nPts * 16, 256) + align_up((size_t)nPts * 4, 256) + align_up(((size_t)nCells + 1) * 4, 256) +
+           align_up((size_t)nCells * 24, 256) + align_up(((size_t)nPts + 3) / 4 * 4, 256);
+}
+
+void place_workspace(BatchTask& t, uint8_t*& ws)
+{
+    const uint32_t nCells = t.cols * t.rows;
+    t.reordered = reinterpret_cast<float4*>(ws);
+    ws += align_up((size_t)t.nHost * 16, 256);
+    t.pos = reinterpret_cast<uint32_t*>(ws);
+    ws += align_up((size_t)t.nHost * 4, 256);
+    t.bounds = reinterpret_cast<uint32_t*>(ws);
+    ws += align_up(((size_t)nCells + 1) * 4, 256);
+    t.affine = reinterpret_cast<float*>(ws);
+    ws += align_up((size_t)nCells * 24, 256);
+    t.votes = ws;
+    ws += align_up(((size_t)t.nHost + 3) / 4 * 4, 256);
+}
+
+int make_batch_task(const rsfm_rmf_task& t, uint32_t cellCols, float relativeThreshold, BatchTask& out)
+{
+    RSFM_RETURN_IF(t.imgWidth <= 0 || t.imgHeight <= 0, RSFM_ERR_INVALID, "rmf batch: bad image size");
+    RSFM_RETURN_IF(t.nbPtsDev == nullptr, RSFM_ERR_INVALID, "rmf batch: nbPtsDev (device count) is required");
+    RSFM_RETURN_IF(t.nbPts > 0 && t.pts == nullptr, RSFM_ERR_INVALID, "rmf batch: pts is null");
+    RSFM_RETURN_IF(reinterpret_cast<uintptr_t>(t.pts) % 16, RSFM_ERR_INVALID, "rmf batch: pts must be 16-byte aligned");
+    const uint32_t mx = (uint32_t)std::max(t.imgWidth, t.imgHeight);
+    const uint32_t cw = div_up(mx, cellCols);
+    const uint32_t cols = div_up((uint32_t)t.imgWidth, cw), rows = div_up((uint32_t)t.imgHeight, cw);
+    RSFM_RETURN_IF(cols * rows > kMaxCells, RSFM_ERR_INVALID, "rmf batch: more than 1024 cells");
+    out = BatchTask{};
+    out.pts = reinterpret_cast<float4*>(t.pts);
+    out.matches = t.matches;
+    out.count = t.nbPtsDev;
+    out.nHost = t.nbPts;
+    out.mask = t.inlierMask;
+    out.cols = cols;
+    out.rows = rows;
+    out.cellWidth = cw;
+    out.threshold = (float)cw * relativeThreshold;
+    out.seed = t.seed;
+    return RSFM_OK;
 }
 
 __global__ void zero_words_kernel(uint32_t* p, uint32_t n)
@@ -321,5 +579,57 @@ extern "C" int rsfm_rmf_bin_host(int imgWidth, int imgHeight, uint32_t cellCols,
     *cols = c;
     *rows = r;
     *cellWidth = cw;
+    return RSFM_OK;
+}
+
+extern "C" int rsfm_rmf_filter_batch_workspace_size(const rsfm_rmf_task* tasks, size_t nbTasks, uint32_t cellCols,
+                                                    size_t* bytes)
+{
+    RSFM_RETURN_IF(bytes == nullptr || (nbTasks > 0 && tasks == nullptr), RSFM_ERR_INVALID, "rmf batch: null argument");
+    RSFM_RETURN_IF(cellCols == 0, RSFM_ERR_INVALID, "rmf batch: cellCols must be >= 1");
+    size_t total = 0;
+    for (size_t i = 0; i < nbTasks; i++) {
+        rmf::BatchTask t;
+        RSFM_CHECK(rmf::make_batch_task(tasks[i], cellCols, 1.f, t));
+        total += rmf::batch_workspace_bytes(t.nHost, t.cols * t.rows);
+    }
+    *bytes = total;
+    return RSFM_OK;
+}
+
+extern "C" int rsfm_rmf_filter_batch(const rsfm_rmf_task* tasks, size_t nbTasks, uint32_t cellCols,
+                                     uint32_t nbRansacTests, float relativeThreshold, uint32_t minVotes,
+                                     int tryOtherAffines, void* workspace, size_t workspaceBytes, hipStream_t stream)
+{
+    using namespace rmf;
+    size_t need = 0;
+    RSFM_CHECK(rsfm_rmf_filter_batch_workspace_size(tasks, nbTasks, cellCols, &need));
+    RSFM_RETURN_IF(workspaceBytes < need, RSFM_ERR_INVALID,
+                   "rmf batch: workspace too small (" + std::to_string(workspaceBytes) + " < " + std::to_string(need) + ")");
+    RSFM_RETURN_IF(need > 0 && (workspace == nullptr || reinterpret_cast<uintptr_t>(workspace) % 256),
+                   RSFM_ERR_INVALID, "rmf batch: workspace must be non-null and 256-byte aligned");
+    RSFM_RETURN_IF(!(relativeThreshold > 0.f), RSFM_ERR_INVALID, "rmf batch: threshold must be > 0");
+    const BatchParams prm{nbRansacTests, minVotes, tryOtherAffines};
+    uint8_t* ws = static_cast<uint8_t*>(workspace);
+    constexpr size_t kMax = 32;
+    for (size_t b = 0; b < nbTasks; b += kMax) {
+        const size_t n = std::min(kMax, nbTasks - b);
+        ByValue<BatchTask, kMax> batch{};
+        uint32_t maxCols = 0, maxRows = 0;
+        for (size_t i = 0; i < n; i++) {
+            RSFM_CHECK(make_batch_task(tasks[b + i], cellCols, relativeThreshold, batch.t[i]));
+            place_workspace(batch.t[i], ws);
+            maxCols = std::max(maxCols, batch.t[i].cols);
+            maxRows = std::max(maxRows, batch.t[i].rows);
+        }
+        bin_kernel<<<(uint32_t)n, kBinThreads, 0, stream>>>(batch);
+        RSFM_CHECK(check_launch("rmf bin"));
+        best_affine_batch_kernel<<<dim3(maxCols, maxRows, (uint32_t)n), kCta, 0, stream>>>(batch, prm);
+        RSFM_CHECK(check_launch("rmf best affine batch"));
+        vote_batch_kernel<<<dim3(maxCols, maxRows, (uint32_t)n), kCta, 0, stream>>>(batch, prm);
+        RSFM_CHECK(check_launch("rmf vote batch"));
+        compact_kernel<<<(uint32_t)n, kBinThreads, 0, stream>>>(batch, prm);
+        RSFM_CHECK(check_launch("rmf compact"));
+    }
     return RSFM_OK;
 }

@@ -21,7 +21,7 @@ class PipelinePair(ctypes.Structure):
                 ("pts", c_vp), ("sampleIdx", c_vp), ("counts", c_vp), ("best", c_vp), ("inlierMask", c_vp),
                 ("F", c_vp), ("H", c_vp), ("solverResult", c_vp), ("inlierMaskH", c_vp), ("cam0", Camera),
                 ("cam1", Camera), ("posesF", c_vp), ("poseResultF", c_vp), ("posesH", c_vp), ("normalsH", c_vp),
-                ("poseResultH", c_vp), ("distortion", c_vp)]
+                ("poseResultH", c_vp), ("distortion", c_vp), ("width0", ctypes.c_int), ("height0", ctypes.c_int)]
 
 
 class PipelineConfig(ctypes.Structure):
@@ -29,7 +29,8 @@ class PipelineConfig(ctypes.Structure):
                 ("ratioDen", c_u32), ("seed", ctypes.c_uint64), ("minSepSq0", c_f32), ("minSepSq1", c_f32),
                 ("segCols", c_u32), ("wgPerXcd", c_u32), ("solver", c_u32), ("thresholdH", c_f32),
                 ("nbCandidates", c_u32), ("nbPlaneSamples", c_u32), ("frontRatio", c_f32), ("confidence", c_f32),
-                ("maxIter", c_u32), ("refit", c_u32)]
+                ("maxIter", c_u32), ("refit", c_u32), ("matchFilter", c_u32), ("mfCellCols", c_u32),
+                ("mfTests", c_u32), ("mfThreshold", c_f32), ("mfMinVotes", c_u32), ("mfTryOther", c_u32)]
 
 
 class TuningEntry(ctypes.Structure):
@@ -69,6 +70,11 @@ class Pipeline:
     per pair) also posesF (4,12), poseF (10,), posesH (8,12), normalsH (8,3),
     poseH (19,).
 
+    match_filter: optional dict with img_size (a (width, height) of the first image per
+    pair) and cell_cols / n_tests / threshold / min_votes / try_other (defaults 16 / 256 /
+    0.15 / 2 / True, the reference's): the affine match filter stage after the ratio and
+    cross-check filter; matches, pts and count then hold the filtered matches.
+
     distortion: optional list (one entry per pair, None for none) of 10 floats,
     k1, k2, p1, p2, k3 of camera 0 then camera 1; those pairs' gathered keypoint
     pairs are undistorted (with cams, required then) before sampling.
@@ -76,10 +82,16 @@ class Pipeline:
 
     def __init__(self, pairs, n_hyp=4096, threshold=1.0, mode=1, ratio=(4, 5), seed=77, min_sep_sq=(0.0, 0.0),
                  seg_cols=0, wg_per_xcd=0, solver="f8", threshold_h=3.0, n_candidates=8, n_plane_samples=256,
-                 front_ratio=0.95, cams=None, distortion=None, refit=False, confidence=0.0, max_iter=5000):
+                 front_ratio=0.95, cams=None, distortion=None, refit=False, confidence=0.0, max_iter=5000,
+                 match_filter=None):
         dev = pairs[0][0].device
         dg = solver == "degensac"
         fin = not dg and (refit or confidence > 0)
+        mf = dict(cell_cols=16, n_tests=256, threshold=0.15, min_votes=2, try_other=True)
+        img_size = None
+        if match_filter is not None:
+            mf.update({k: v for k, v in match_filter.items() if k != "img_size"})
+            img_size = match_filter["img_size"]
         k = 7 if dg else 8
         self.inputs = pairs
         self.out = []
@@ -127,11 +139,13 @@ class Pipeline:
                                   p(o["best"]), p(o["mask"]), q("F"), q("H"), q("result"), q("maskH"),
                                   Camera(*[float(v) for v in c0]), Camera(*[float(v) for v in c1]), q("posesF"),
                                   q("poseF"), q("posesH"), q("normalsH"), q("poseH"),
-                                  ctypes.cast(dist, c_vp) if dist is not None else None)
+                                  ctypes.cast(dist, c_vp) if dist is not None else None,
+                                  *(img_size[i] if img_size is not None else (0, 0)))
             self.out.append(o)
         cfg = PipelineConfig(n_hyp, threshold, mode, ratio[0], ratio[1], seed, min_sep_sq[0], min_sep_sq[1], seg_cols,
                              wg_per_xcd, SOLVERS[solver], threshold_h, n_candidates, n_plane_samples, front_ratio,
-                             confidence, max_iter, int(bool(refit)))
+                             confidence, max_iter, int(bool(refit)), int(match_filter is not None), mf["cell_cols"],
+                             mf["n_tests"], mf["threshold"], mf["min_votes"], int(mf["try_other"]))
         torch.cuda.synchronize()  # the inputs must be complete: creation times candidate tilings on them
         self.handle = c_vp()
         check(lib.rsfm_pipeline_create(ctypes.cast(arr, c_vp), len(pairs), ctypes.byref(cfg), ctypes.byref(self.handle)),

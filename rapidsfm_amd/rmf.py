@@ -56,3 +56,44 @@ def get_inlier_mask(img_size0, matches, min_votes=2, n_tests=256, threshold=0.15
     mask = np.zeros(len(order), bool)
     mask[order] = v >= min_votes
     return mask
+
+
+class RmfTask(ctypes.Structure):
+    _fields_ = [("pts", c_void_p), ("matches", c_void_p), ("nbPtsDev", c_void_p), ("nbPts", c_u32),
+                ("inlierMask", c_void_p), ("imgWidth", c_int), ("imgHeight", c_int), ("seed", c_u32)]
+
+
+bind_extra("rsfm_rmf_filter_batch_workspace_size", c_int, [c_void_p, ctypes.c_size_t, c_u32,
+                                                            ctypes.POINTER(ctypes.c_size_t)])
+bind_extra("rsfm_rmf_filter_batch", c_int, [c_void_p, ctypes.c_size_t, c_u32, c_u32, c_float, c_u32, c_int, c_void_p,
+                                            ctypes.c_size_t, c_void_p])
+
+
+def filter_batch(problems, min_votes=2, n_tests=256, threshold=0.15, cell_cols=16, try_other=True, stream=None):
+    """Batched affine match filter on device-resident gathered pairs.
+
+    problems: list of (pts float32 (n, 4) CUDA, count int32 (1,) CUDA, (width, height) of the first image, seed[,
+    matches int32 (n, 3) CUDA]). pts (and matches) are compacted in place and count becomes the kept count.
+    Returns the masks (uint8 (n,), original order)."""
+    tasks = (RmfTask * max(1, len(problems)))()
+    masks = []
+    for i, prob in enumerate(problems):
+        pts, count, (w, h), seed = prob[:4]
+        matches = prob[4] if len(prob) > 4 else None
+        if pts.dtype != torch.float32 or pts.dim() != 2 or pts.shape[1] != 4 or not pts.is_contiguous():
+            raise ValueError("pts must be contiguous float32 (n, 4)")
+        mask = torch.empty(max(1, pts.shape[0]), dtype=torch.uint8, device=pts.device)
+        tasks[i] = RmfTask(_ptr(pts), _ptr(matches) if matches is not None else None, _ptr(count), pts.shape[0],
+                           _ptr(mask), int(w), int(h), int(seed) & 0xFFFFFFFF)
+        masks.append(mask[: pts.shape[0]])
+    tp = ctypes.cast(tasks, c_void_p)
+    need = ctypes.c_size_t(0)
+    check(lib.rsfm_rmf_filter_batch_workspace_size(tp, len(problems), cell_cols, ctypes.byref(need)), "rmf batch ws")
+    dev = problems[0][0].device
+    ws = torch.empty(need.value + 256, dtype=torch.uint8, device=dev)
+    wsp = (ws.data_ptr() + 255) // 256 * 256
+    if stream is not None:
+        ws.record_stream(stream)
+    check(lib.rsfm_rmf_filter_batch(tp, len(problems), cell_cols, n_tests, float(threshold), min_votes, int(try_other),
+                                    c_void_p(wsp), need.value, _stream(stream)), "rsfm_rmf_filter_batch")
+    return masks

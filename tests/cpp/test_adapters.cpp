@@ -2,6 +2,7 @@
 // launch functions, cross-check, BruteForceMatcher (default, 4-bit and
 // cascade-hash methods), RansacMatchFilter and error reporting. Prints one
 // "PASS <name>" / "FAIL <name>" line per check; exit status 0 iff all pass.
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <random>
@@ -9,6 +10,22 @@
 #include "rsfm_amd.hpp"
 
 using namespace rsfm;
+
+// Layout stand-ins for cudapp::KArray (the reference's extern/cuda-utils, absent
+// here): a plain array wrapper with a .data member, as the reference's call
+// sites use it (casHashWeights.data, bias->data).
+namespace cudapp {
+template <typename T, uint32_t... Dims>
+struct KArray;
+template <typename T, uint32_t D0>
+struct KArray<T, D0> {
+    T data[D0];
+};
+template <typename T, uint32_t D0, uint32_t D1>
+struct KArray<T, D0, D1> {
+    KArray<T, D1> data[D0];
+};
+}  // namespace cudapp
 
 static int g_fail = 0;
 static void report(bool ok, const char* name)
@@ -143,6 +160,122 @@ int main()
             ok2 &= filt2[0][k].idxQuery == rc[k].idxQuery && filt2[0][k].idxTrain == rc[k].idxTrain;
         report(ok2, "BruteForceMatcher::matchAndFilter (conflicts)");
         hipCheck(hipFree(dA2), "free");
+    }
+    // GPU-resident service: after a warm-up call the storage is reused, no
+    // allocation happens over ten further calls (including smaller ones), and
+    // every result is right
+    {
+        BruteForceMatcher m;
+        using D = BruteForceMatcher::Descriptor;
+        m.registerImage(0, (const D*)dA, nA, s);
+        m.registerImage(2, (const D*)dB, nB, s);
+        auto cc = crossCheckMatches(ra.data(), nA, rb.data(), nB);
+        (void)m.match({{(const D*)dA, nA, 0}, {(const D*)dA, nA, 0}}, {(const D*)dB, nB, 2}, true, s).get();
+        (void)m.matchAndFilter({{(const D*)dA, nA, 0}, {(const D*)dA, nA, 0}}, {(const D*)dB, nB, 2}, true, s).get();
+        const size_t warm = m.allocations();
+        bool ok = warm > 0;
+        std::vector<std::future<BruteForceMatcher::Result>> pending;
+        for (int it = 0; it < 10; it++) {
+            if (it % 2 == 0) {
+                pending.push_back(m.match({{(const D*)dA, nA, 0}}, {(const D*)dB, nB, 2}, true, s));
+            } else {
+                auto f = m.matchAndFilter({{(const D*)dA, nA, 0}, {(const D*)dA, 300, 0}}, {(const D*)dB, nB, 2}, true, s)
+                             .get();
+                ok &= f.size() == 2 && f[0].size() == cc.size();
+                for (size_t k = 0; ok && k < cc.size(); k++)
+                    ok &= f[0][k].idxQuery == cc[k].idxQuery && f[0][k].idxTrain == cc[k].idxTrain;
+            }
+        }
+        for (auto& f : pending) {
+            auto r = f.get();
+            ok &= r.size() == 1 && same(r[0].first, ra) && same(r[0].second, rb);
+        }
+        ok &= m.allocations() == warm;
+        report(ok, "BruteForceMatcher storage reused over 10 calls (no allocation after warm-up)");
+    }
+    // the reference's cascade-hash registration sequence, written as at
+    // bruteForceMatcher.cpp:97-103 against KArray-typed device objects, gives
+    // the same codes, bounds and bucket sets as the raw-pointer entry points
+    {
+        using cudapp::KArray;
+        std::vector<int8_t> w(64 * 128);
+        std::vector<int32_t> base(64);
+        std::normal_distribution<float> nd(0.f, 32.f);
+        for (auto& x : w) x = int8_t(std::max(-128.f, std::min(127.f, std::round(nd(rng)))));
+        for (auto& x : base) x = int32_t(rng() % 2001) - 1000;
+        auto* casHashWeightsPtr = reinterpret_cast<KArray<int32_t, 64, 32>*>(upload(w));
+        auto* casHashBiasBasePtr = reinterpret_cast<KArray<int32_t, 64>*>(upload(base));
+        const KArray<int32_t, 64, 32>& casHashWeights = *casHashWeightsPtr;
+        const KArray<int32_t, 64>& casHashBiasBase = *casHashBiasBasePtr;
+        KArray<uint64_t, 128>* accPtr = nullptr;
+        KArray<int32_t, 64>* bias = nullptr;
+        KArray<uint8_t, cashash::nbScheme>* bucketIndices = nullptr;
+        KArray<uint32_t, cashash::nbScheme, cashash::nbBuckets>* histAndBounds = nullptr;
+        KArray<uint32_t, cashash::nbScheme, cashash::nbBuckets>* casHashBucketEnds = nullptr;
+        uint32_t* casHashBuckets = nullptr;
+        uint32_t* pinnedMaxBucketSize = nullptr;
+        hipCheck(hipMalloc(&accPtr, sizeof(*accPtr)), "malloc");
+        hipCheck(hipMalloc(&bias, sizeof(*bias)), "malloc");
+        hipCheck(hipMalloc(&bucketIndices, sizeof(*bucketIndices) * nB), "malloc");
+        hipCheck(hipMalloc(&histAndBounds, sizeof(*histAndBounds) * 2), "malloc");
+        hipCheck(hipMalloc(&casHashBucketEnds, sizeof(*casHashBucketEnds)), "malloc");
+        hipCheck(hipMalloc(&casHashBuckets, sizeof(uint32_t) * nB * cashash::nbScheme), "malloc");
+        hipCheck(hipHostMalloc(reinterpret_cast<void**>(&pinnedMaxBucketSize), 4, hipHostMallocDefault), "pinned");
+        KArray<uint64_t, 128>& acc = *accPtr;
+        auto* const histogram = histAndBounds;
+        auto* const idxAllocator = histAndBounds + 1;
+        const auto* src = reinterpret_cast<const KArray<uint32_t, 32>*>(dB);
+        const uint32_t nbDesc = nB;
+        uint64_t accNbDesc = 0;
+        const bool isFirst = accNbDesc == 0;
+        hipStream_t stream = s;
+        // the reference's lines, unchanged apart from the stream type
+        cashash::accumuateDesc(acc, nullptr, isFirst, src, nbDesc, stream);
+        accNbDesc += nbDesc;
+        cashash::computeBias(bias[0], casHashWeights, casHashBiasBase, acc, accNbDesc, stream);
+        cashash::computeHashForBucketOnly(casHashWeights.data, bias->data, src, nbDesc, bucketIndices, nullptr, stream);
+        cashash::buildHistogram(histogram, bucketIndices, nbDesc, stream);
+        cashash::computeBucketBound(*idxAllocator, *casHashBucketEnds, pinnedMaxBucketSize, *histogram, stream);
+        cashash::buildBuckets(bucketIndices, nbDesc, *idxAllocator, casHashBuckets, stream);
+        // the same through the raw-pointer entry points
+        DeviceBuffer<uint64_t> acc2(128);
+        DeviceBuffer<int32_t> bias2(64);
+        DeviceBuffer<uint8_t> codes2(size_t(nB) * 8);
+        DeviceBuffer<uint32_t> hist2(2048), lower2(2048), ends2(2048), buckets2(size_t(nB) * 8);
+        rsfmCheck(rsfm_cashash_accumulate_desc(acc2.ptr, nullptr, 1, dB, nB, s), "acc");
+        rsfmCheck(rsfm_cashash_compute_bias(bias2.ptr, reinterpret_cast<const int8_t*>(casHashWeightsPtr),
+                                            reinterpret_cast<const int32_t*>(casHashBiasBasePtr), acc2.ptr, nB, 64, s),
+                  "bias");
+        rsfmCheck(rsfm_cashash_compute_hash_bucket_only(reinterpret_cast<const int8_t*>(casHashWeightsPtr), bias2.ptr, dB,
+                                                        nB, codes2.ptr, nullptr, s),
+                  "hash");
+        rsfmCheck(rsfm_cashash_build_histogram(hist2.ptr, codes2.ptr, nB, s), "hist");
+        rsfmCheck(rsfm_cashash_compute_bucket_bound(lower2.ptr, ends2.ptr, nullptr, hist2.ptr, s), "bounds");
+        rsfmCheck(rsfm_cashash_build_buckets(codes2.ptr, nB, lower2.ptr, buckets2.ptr, s), "buckets");
+        std::vector<uint8_t> c1(size_t(nB) * 8), c2(size_t(nB) * 8);
+        std::vector<uint32_t> e1(2048), e2(2048), b1(size_t(nB) * 8), b2(size_t(nB) * 8);
+        hipCheck(hipMemcpyAsync(c1.data(), bucketIndices, c1.size(), hipMemcpyDeviceToHost, s), "D2H");
+        hipCheck(hipMemcpyAsync(c2.data(), codes2.ptr, c2.size(), hipMemcpyDeviceToHost, s), "D2H");
+        hipCheck(hipMemcpyAsync(e1.data(), casHashBucketEnds, 8192, hipMemcpyDeviceToHost, s), "D2H");
+        hipCheck(hipMemcpyAsync(e2.data(), ends2.ptr, 8192, hipMemcpyDeviceToHost, s), "D2H");
+        hipCheck(hipMemcpyAsync(b1.data(), casHashBuckets, b1.size() * 4, hipMemcpyDeviceToHost, s), "D2H");
+        hipCheck(hipMemcpyAsync(b2.data(), buckets2.ptr, b2.size() * 4, hipMemcpyDeviceToHost, s), "D2H");
+        hipCheck(hipStreamSynchronize(s), "sync");
+        bool ok = c1 == c2 && e1 == e2 && *pinnedMaxBucketSize > 0;
+        for (uint32_t sc = 0; ok && sc < 8; sc++)
+            for (uint32_t bk = 0; bk < 256; bk++) {
+                const uint32_t lo = bk ? e1[sc * 256 + bk - 1] : 0, hi = e1[sc * 256 + bk];
+                std::vector<uint32_t> x(b1.begin() + sc * nB + lo, b1.begin() + sc * nB + hi);
+                std::vector<uint32_t> y(b2.begin() + sc * nB + lo, b2.begin() + sc * nB + hi);
+                std::sort(x.begin(), x.end());
+                std::sort(y.begin(), y.end());
+                ok &= x == y;
+            }
+        report(ok, "cashash call sites with KArray-typed arguments (bruteForceMatcher.cpp:97-103)");
+        for (void* p : {(void*)casHashWeightsPtr, (void*)casHashBiasBasePtr, (void*)accPtr, (void*)bias,
+                        (void*)bucketIndices, (void*)histAndBounds, (void*)casHashBucketEnds, (void*)casHashBuckets})
+            hipCheck(hipFree(p), "free");
+        hipCheck(hipHostFree(pinnedMaxBucketSize), "free");
     }
     // 4-bit method: L2 on the high nibbles
     {

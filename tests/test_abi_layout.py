@@ -32,6 +32,9 @@ PAIRS = [
     ("rsfm_pipeline_tuning_entry", "pipeline", "TuningEntry"),
     ("rsfm_pipeline_info_t", "pipeline", "PipelineInfo"),
     ("rsfm_feature_task", "proposal", "FeatureTask"),
+    ("rsfm_ransac_finish_task", "_abi", "FinishTask"),
+    ("rsfm_cashash_rerank_image", "cashash", "CasHashRerankImage"),
+    ("rsfm_rmf_task", "rmf", "RmfTask"),
 ]
 
 

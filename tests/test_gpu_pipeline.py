@@ -359,3 +359,42 @@ def test_pipeline_finish_stage(orc, dev, conf, refit):
     assert np.array_equal(o["result"].cpu().numpy().view(np.uint32), res)
     assert np.array_equal(o["mask"][:c].cpu().numpy(), mask)
     pl.close()
+
+
+def test_pipeline_match_filter_stage(orc, dev):
+    """The affine match filter as a graph stage: identical to the entry-point
+    chain (filter, rsfm_rmf_filter_batch with seed + pair, samples, scorer),
+    and pair 0's kept pairs identical to the oracle's binning + filter."""
+    from rapidsfm_amd import ops, rmf
+    from rapidsfm_amd.pipeline import Pipeline
+
+    _say("pipeline with the affine match filter stage")
+    sizes = SIZES[:4] + [(2500, 2600), (6000, 5000), (30, 40)]
+    pairs = _batch(dev, sizes, seed=900)
+    H = 512
+    img = [(4000, 3000)] * len(pairs)
+    pl = Pipeline(pairs, n_hyp=H, seed=13, seg_cols=2048, wg_per_xcd=32, match_filter={"img_size": img})
+    pl.run()
+    outs = _matched(pairs)
+    torch.cuda.synchronize()
+    raw = [(int(o["count"].item()), o["pts"].clone()) for o in outs]
+    rmf.filter_batch([(o["pts"], o["count"], img[i], 13 + i, o["matches"]) for i, o in enumerate(outs)])
+    _sample(pairs, outs, H, 13, 8)
+    sc = ops.ransac_f8([(o["pts"], o["idx"], o["count"]) for o in outs], 1.0, mode=1)
+    torch.cuda.synchronize()
+    for i, (o, r, s) in enumerate(zip(pl.out, outs, sc)):
+        c = int(r["count"].item())
+        assert int(o["count"].item()) == c, i
+        assert torch.equal(o["matches"][:c], r["matches"][:c]) and torch.equal(o["pts"][:c], r["pts"][:c]), i
+        assert torch.equal(o["idx"], r["idx"]) and torch.equal(o["counts"], s["counts"]), i
+        assert torch.equal(o["best"], s["best"]), i
+    n0, p0 = raw[0]
+    m = p0[:n0].cpu().numpy()
+    reordered, bounds, order, cols, rows, cw = orc.rmf_bin(4000, 3000, m, 16)
+    votes, _ = orc.rmf_filter(reordered, bounds, cols, rows, 256, cw * 0.15, True, 13)
+    keep = np.zeros(n0, bool)
+    keep[order] = votes >= 2
+    c0 = int(pl.out[0]["count"].item())
+    assert c0 == int(keep.sum()) and np.array_equal(pl.out[0]["pts"][:c0].cpu().numpy(), m[keep])
+    assert int(pl.out[-1]["count"].item()) == 0 or raw[-1][0] >= 16
+    pl.close()

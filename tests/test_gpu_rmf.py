@@ -61,3 +61,54 @@ def test_small_inputs(dev):
     m = _scene(9, 10, 3)
     mask = rmf.get_inlier_mask((4000, 3000), m, 2, 256, 0.15, 16, True, seed=1, device=dev)
     assert not mask.any()
+
+
+def _batch_ref(orc, m, w, h, seed, min_votes=2, try_other=True):
+    """the oracle's binning + filter, then the caller's selection: kept pairs in order"""
+    reordered, bounds, order, cols, rows, cw = orc.rmf_bin(w, h, m, 16)
+    votes, _ = orc.rmf_filter(reordered, bounds, cols, rows, 256, cw * 0.15, try_other, seed)
+    mask = np.zeros(len(m), bool)
+    mask[order] = votes >= min_votes
+    return mask
+
+
+@pytest.mark.parametrize("try_other", [True, False])
+def test_batch_filter_matches_oracle(orc, dev, try_other):
+    """rsfm_rmf_filter_batch over 10 pairs of uneven size and image shape (one
+    below 16 points, one empty), with a device count below the capacity on
+    some: masks, compacted pairs and matches, kept counts identical to the
+    oracle's binning + filter + selection."""
+    from rapidsfm_amd import rmf
+
+    _say("rmf batch, 10 pairs, tryOther", try_other)
+    specs = [(3000, 1000, 4000, 3000), (800, 1600, 4000, 3000), (40, 10, 4000, 3000), (5, 4, 4000, 3000),
+             (0, 0, 4000, 3000), (1500, 500, 3000, 4000), (2500, 2500, 1920, 1080), (6000, 3000, 4000, 3000),
+             (700, 300, 640, 480), (9000, 7000, 4000, 3000)]
+    probs, refs = [], []
+    for k, (ni, no, w, h) in enumerate(specs):
+        m = _scene(100 + k, ni, no, w=w, h=h) if ni + no else np.zeros((0, 4), np.float32)
+        cap = len(m) + (37 if k % 3 == 0 else 0)
+        buf = np.zeros((max(1, cap), 4), np.float32)
+        buf[: len(m)] = m
+        matches = np.zeros((max(1, cap), 3), np.int32)
+        matches[: len(m), 0] = np.arange(len(m))
+        matches[: len(m), 1] = np.arange(len(m)) * 7
+        pd = torch.from_numpy(buf).to(dev)
+        md = torch.from_numpy(matches).to(dev)
+        cnt = torch.tensor([len(m)], dtype=torch.int32, device=dev)
+        probs.append((pd, cnt, (w, h), 50 + k, md))
+        refs.append((m, _batch_ref(orc, m, w, h, 50 + k, try_other=try_other) if len(m) else np.zeros(0, bool),
+                     matches[: len(m)]))
+    masks = rmf.filter_batch(probs, try_other=try_other)
+    torch.cuda.synchronize()
+    for k, ((pd, cnt, _, _, md), (m, mask, mt), gm) in enumerate(zip(probs, refs, masks)):
+        n = len(m)
+        got = gm.cpu().numpy().astype(bool)
+        assert np.array_equal(got[:n], mask), k
+        assert not got[n:].any(), k
+        kept = int(mask.sum())
+        assert int(cnt.item()) == kept, k
+        assert np.array_equal(pd[:kept].cpu().numpy(), m[mask]), k
+        assert np.array_equal(md[:kept].cpu().numpy(), mt[mask]), k
+    # the first pair keeps its inliers and sheds most outliers
+    assert int(probs[0][1].item()) > 2500
