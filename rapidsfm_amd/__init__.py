@@ -23,6 +23,7 @@ from .ops import (  # noqa: F401
     desc8b_to_4b,
     filter_matches,
     ransac_f8,
+    ransac_f8_finish,
     ransac_sample_indices,
     sqrnorm_u4,
     sqrnorm_u8,
@@ -40,6 +41,7 @@ __all__ = [
     "desc8b_to_4b",
     "filter_matches",
     "ransac_f8",
+    "ransac_f8_finish",
     "ransac_sample_indices",
     "sqrnorm_u8",
     "sqrnorm_u4",
