@@ -110,6 +110,14 @@ int rsfm_bf_match_u4(const rsfm_bf_task* tasks, size_t nbTasks, hipStream_t stre
 int rsfm_bf_match_top2_workspace_size(const rsfm_bf_top2_task* tasks, size_t nbTasks, size_t* bytes);
 int rsfm_bf_match_top2(const rsfm_bf_top2_task* tasks, size_t nbTasks, void* workspace,
                        size_t workspaceBytes, hipStream_t stream);
+/* rsfm_bf_match_top2 without a caller workspace and without any allocation (no
+ * reference counterpart; for callers that cannot allocate: code under HIP graph
+ * capture, tight serving loops): the workspace lives in a fixed 64 MiB device
+ * array of the library, held on every GPU the library runs on until the process
+ * ends. Tasks needing more than 64 MiB of workspace (rsfm_bf_match_top2_workspace_size)
+ * are rejected with RSFM_ERR_INVALID. Calls that overlap on one GPU race on the
+ * array: serialise them (one stream, or events). */
+int rsfm_bf_match_top2_noalloc(const rsfm_bf_top2_task* tasks, size_t nbTasks, hipStream_t stream);
 
 /* ---------------------------------------------------------------------------
  * Cascade hashing (rsfm::cashash, casHash/cascadeHash.h:32-107)

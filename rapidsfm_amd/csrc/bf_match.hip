@@ -32,6 +32,7 @@
 #include "internal.hpp"
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 namespace rsfm_amd {
@@ -737,4 +738,52 @@ extern "C" int rsfm_bf_match_top2(const rsfm_bf_top2_task* tasks, size_t nbTasks
         }
     }
     return RSFM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// rsfm_bf_match_top2_noalloc: the top-2 matcher with its workspace (train
+// staging form and segment partials) in a fixed device array instead of a
+// caller buffer, for callers that cannot allocate. The array is this file's
+// `scratch`; its device address is looked up once per device, with that device
+// current, and only that address is handed to kernels. Every call on one GPU
+// shares the array (calls that overlap on one GPU race); nothing in it needs
+// clearing (the preparation kernel writes the whole staging form and every
+// partial that the merge reads is written by a tile first).
+namespace {
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+constexpr int kMaxDevices = 64;
+std::mutex g_scratchLock;
+void* g_scratchPtr[kMaxDevices] = {};
+}  // namespace
+
+static __device__ __align__(256) uint8_t scratch[kNoAllocScratchBytes];
+
+static int noalloc_scratch(void** out)
+{
+    int dev = 0;
+    RSFM_CHECK(check_hip(hipGetDevice(&dev), "top2 noalloc: hipGetDevice"));
+    RSFM_RETURN_IF(dev < 0 || dev >= kMaxDevices, RSFM_ERR_UNSUPPORTED, "top2 noalloc: device ordinal out of range");
+    std::lock_guard<std::mutex> lk(g_scratchLock);
+    if (g_scratchPtr[dev] == nullptr) {
+        void* p = nullptr;
+        RSFM_CHECK(check_hip(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "top2 noalloc: hipGetSymbolAddress"));
+        RSFM_RETURN_IF(p == nullptr || reinterpret_cast<uintptr_t>(p) % 256 != 0, RSFM_ERR_HIP,
+                       "top2 noalloc: scratch address unusable");
+        g_scratchPtr[dev] = p;
+    }
+    *out = g_scratchPtr[dev];
+    return RSFM_OK;
+}
+
+extern "C" int rsfm_bf_match_top2_noalloc(const rsfm_bf_top2_task* tasks, size_t nbTasks, hipStream_t stream)
+{
+    RSFM_RETURN_IF(nbTasks > 0 && tasks == nullptr, RSFM_ERR_INVALID, "bf top2 noalloc: tasks is null");
+    const size_t need = bf::top2_workspace(tasks, nbTasks);
+    RSFM_RETURN_IF(need > kNoAllocScratchBytes, RSFM_ERR_INVALID,
+                   "bf top2 noalloc: these tasks need " + std::to_string(need) +
+                       " bytes of workspace, the fixed scratch holds " + std::to_string(kNoAllocScratchBytes) +
+                       " (64 MiB); use rsfm_bf_match_top2 with a caller workspace");
+    void* ws = nullptr;
+    if (need > 0) RSFM_CHECK(noalloc_scratch(&ws));
+    return rsfm_bf_match_top2(tasks, nbTasks, ws, need, stream);
 }
