@@ -55,6 +55,24 @@ def test_bf_ties_lowest_index(orc):
     assert bb[0, 0] == 5
 
 
+@pytest.mark.parametrize("nA,nB", [(1, 1), (7, 300), (300, 9), (2048, 2048)])
+def test_bf_top2_fast_matches_plain(orc, nA, nB):
+    """The cache-blocked oracle path (eight rows per pass, per-thread column
+    minima merged at the end) against the plain one, including planted ties
+    (duplicated train and query rows) across the 8-row blocks."""
+    train, query, _ = orc.gen_sift_pair(5, max(nB, 1), max(nA, 1), min(nA, nB) // 3, 0.05)
+    A, B = query[:nA].copy(), train[:nB].copy()
+    if nA >= 16 and nB >= 16:
+        B[nB // 2] = B[3]
+        A[nA - 1] = A[2]
+        A[9] = B[5]
+    r_plain = orc.bf_match_top2_u8(A, B)
+    for nthreads in (1, 3):
+        r_fast = orc.bf_match_top2_fast(A, B, nthreads)
+        for x, y in zip(r_fast, r_plain):
+            assert np.array_equal(x, y)
+
+
 def test_empty_inputs(orc):
     A = np.zeros((0, 128), np.uint8)
     B = orc.gen_random_u8(1, 5)
