@@ -19,13 +19,23 @@
 //    tile copies it into the LDS ring with direct global-to-LDS loads, so
 //    staging costs neither registers nor VALU in the main loop. Top-1 (no
 //    workspace) loads, transforms and stores each chunk itself.
-//  * Epilogue per accumulator: one v_lshl_add makes the row key
+//  * Epilogue per accumulator, top-1 path: one v_lshl_add makes the row key
 //    ((NB'_j + 2dot) << 8 | column-block tag), one makes the column key
-//    ((NA'_i + 2dot) << 8 | local row); v_min3 folds top-1, v_med3 + v_min
-//    fold top-2. Row state lives in registers for the whole segment and is
-//    reduced at the end through an LDS transpose (one thread per row).
-//    Column partials are reduced across the waves through LDS and merged
-//    globally with one 64-bit atomicMin per column per workgroup.
+//    ((NA'_i + 2dot) << 8 | local row); v_min3 folds top-1. Row state lives
+//    in registers for the whole segment and is reduced at the end through an
+//    LDS transpose (one thread per row). Column partials are reduced across
+//    the waves through LDS and merged globally with one 64-bit atomicMin per
+//    column per workgroup.
+//  * Top-2 path: the MFMA chain starts from H_i = NA'_i >> 1 instead of zero,
+//    so the single key R = (acc << 9) + (NB'_j << 8 | tag) = 256 (d - p_i) +
+//    tag (p_i = NA'_i & 1) orders each row's columns exactly and each
+//    column's rows up to the parity p_i. One v_lshl_add per accumulator,
+//    v_med3 + v_min3 for the row top-2 and v_min3 for the column minimum: the
+//    column side needs no key of its own. Per column and row block the
+//    kernel stores E = (smallest R) >> 8; two small kernels then list each
+//    column under the row blocks that can hold its best row (E <= e + 1) and
+//    recompute those blocks' exact distances to the listed columns on MFMA
+//    (resolve_block_kernel).
 //  * Segments split the train range when one launch has too few row blocks
 //    to fill 256 CUs; row results then merge by 64-bit atomicMin (top-1) or
 //    through a small workspace and a merge kernel (top-2).
@@ -39,6 +49,16 @@ namespace rsfm_amd {
 namespace bf {
 
 constexpr int32_t kNone = INT32_MAX;
+
+// Prepared (top-2) column offset of the columns past nB, whose staged operands
+// are zero: with the accumulators started at H_i = NA'_i >> 1 such a column's
+// key is 256 (NA'_i - p_i) + kPastCols + tag. Over all u8 b,
+// |a - b|^2 - NA' <= 128 * 48896 (at a byte of 255), so kPastCols above
+// 256 * 128 * 48896 + 255 keeps that key above every real column's key
+// 256 (d - p_i) + tag, and NA' <= 128 * 128^2 keeps it below 2^31.
+constexpr int32_t kPastCols = 0x5FC00000;
+static_assert(256LL * 128 * 48896 + 255 < kPastCols, "past-nB key must exceed every real key");
+static_assert(256LL * 128 * 16384 + kPastCols + 255 <= INT32_MAX, "past-nB key must not wrap");
 
 // 16 bytes of the (expanded) descriptor at byte offset off of the 128-value row
 template <int kBits>
@@ -160,26 +180,29 @@ __device__ __forceinline__ void match_tile(const DevTask& T, uint32_t rowBase, u
     const int l32 = lane & 31, hi = lane >> 5;
     const int rloc = wave * 32 + l32;
 
-    // ---- query rows: operands u = 127 - a into LDS, offsets NA' = |a - 128|^2
+    // ---- query rows: operands u = 127 - a into LDS, offsets NA' = |a - 128|^2.
+    // Prepared (top-2) form: rows past nA repeat row nA - 1, so they tie with
+    // it and never beat it on the train side, and sQ holds NA' itself.
     {
         const uint32_t row = rowBase + rloc;
         const bool valid = row < T.nA;
+        const uint32_t src = kPrep ? min(row, T.nA - 1) : row;
         int32_t nap = 0;
 #pragma unroll
         for (int kc = 0; kc < 4; kc++) {
             v4i x = {0, 0, 0, 0};
-            if (valid) x = load16<kBits>(T.A + (size_t)row * kDescBytes, kc * 32 + hi * 16);
+            if (kPrep || valid) x = load16<kBits>(T.A + (size_t)src * kDescBytes, kc * 32 + hi * 16);
             v4i u;
 #pragma unroll
             for (int q = 0; q < 4; q++) {
                 const int xa = x[q] ^ (int)0x80808080u;
                 nap = __builtin_amdgcn_sdot4(xa, xa, nap, false);
-                u[q] = valid ? (x[q] ^ 0x7F7F7F7F) : 0;
+                u[q] = (kPrep || valid) ? (x[q] ^ 0x7F7F7F7F) : 0;
             }
             *reinterpret_cast<v4i*>(sA + rloc * 128 + (((2 * kc + hi) ^ swz(rloc)) << 4)) = u;
         }
         const int32_t na = nap + __shfl_xor(nap, 32);
-        if (hi == 0) sQ[rloc] = valid ? (na * 256 + rloc) : kNone;
+        if (hi == 0) sQ[rloc] = kPrep ? na : (valid ? (na * 256 + rloc) : kNone);
     }
 
     int32_t m1[16], m2[16];
@@ -188,6 +211,7 @@ __device__ __forceinline__ void match_tile(const DevTask& T, uint32_t rowBase, u
         m1[v] = kNone;
         m2[v] = kNone;
     }
+    v16i Hv = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};  // prepared form: accumulator start
 
     const int sc = tid >> 3, sp = tid & 7;
     auto stage_load = [&](uint32_t c) -> v4i {
@@ -232,6 +256,24 @@ __device__ __forceinline__ void match_tile(const DevTask& T, uint32_t rowBase, u
             atomicMin(reinterpret_cast<unsigned long long*>(T.bestB + gcol), (unsigned long long)make_key(dist, row));
         }
     };
+    // prepared (top-2) form: wave k % 8 reduces the 8 waves' minima of chunk
+    // k's 64 columns (one lane per column): the block's E = Rmin >> 8 goes to
+    // T.colMin by atomicMin, and to this block's row of T.colPart with the
+    // column tag replaced by the first wave holding Rmin (bits 0-2) and bit 3
+    // set when another wave comes within one unit of it (E_w <= E + 1)
+    auto flush_prep = [&](uint32_t k) {
+        if (wave != (int)(k & 7)) return;
+        const int32_t* src = &sCol[k % kColSlots][0][lane];
+        int32_t k1 = kNone, k2 = kNone;
+#pragma unroll
+        for (int w = 0; w < kWaves; w += 2)
+            top2_fold2(k1, k2, (src[w * kColStride] & ~0xFF) | w, (src[(w + 1) * kColStride] & ~0xFF) | (w + 1));
+        const uint32_t gcol = colBeg + k * kBN + lane;
+        if (gcol < colEnd) {
+            T.colPart[(size_t)(rowBase / kBM) * T.nB + gcol] = k1 | ((k2 >> 8) <= (k1 >> 8) + 1 ? 8 : 0);
+            atomicMin(T.colMin + gcol, k1 >> 8);
+        }
+    };
     const bool doCols = T.bestB != nullptr;
     const uint8_t* aRow = sA + rloc * 128;
     const int aSw = swz(rloc);
@@ -255,8 +297,13 @@ __device__ __forceinline__ void match_tile(const DevTask& T, uint32_t rowBase, u
         // the previous trip's column partials: flushed after the staging store,
         // so that store's wait on the staged load does not also wait on these atomics
         if (flushPrev) {
-            flush(c - 2);
-            flush(c - 1);
+            if constexpr (kPrep) {
+                flush_prep(c - 2);
+                flush_prep(c - 1);
+            } else {
+                flush(c - 2);
+                flush(c - 1);
+            }
         }
         const uint8_t* sb = sU + (c % kRing) * kChunkBytes;
         v16i acc[2];
@@ -269,7 +316,7 @@ __device__ __forceinline__ void match_tile(const DevTask& T, uint32_t rowBase, u
                 const int cl = cb * 32 + l32;
                 const v4i b = *reinterpret_cast<const v4i*>(sb + cl * 128 + ((s ^ swz(cl)) << 4));
                 const v16i zero = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-                acc[cb] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b, kc == 0 ? zero : acc[cb], 0, 0, 0);
+                acc[cb] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b, kc == 0 ? (kPrep ? Hv : zero) : acc[cb], 0, 0, 0);
             }
             __builtin_amdgcn_sched_barrier(0);
         }
@@ -282,21 +329,40 @@ __device__ __forceinline__ void match_tile(const DevTask& T, uint32_t rowBase, u
             if constexpr (kPrep) P[cb] |= (int32_t)(c * 2 + cb);
         }
         int32_t cm0 = kNone, cm1 = kNone;
+        if constexpr (kPrep) {
+            // the accumulators started at H_i = NA'_i >> 1, so one key
+            // R = (acc << 9) + P = 256 (d - p_i) + tag serves both sides: per
+            // row across columns (p_i is the row's constant) and, tag aside,
+            // per column across rows up to the parity p_i, which the
+            // train-side resolution settles
 #pragma unroll
-        for (int g = 0; g < 4; g++) {
-            const v4i q4 = *reinterpret_cast<const v4i*>(sQ + wave * 32 + 8 * g + 4 * hi);
+            for (int v = 0; v < 16; v += 2) {
+                const int32_t r0 = shl9_add(acc[0][v], P[0]);
+                const int32_t r1 = shl9_add(acc[1][v], P[1]);
+                const int32_t s0 = shl9_add(acc[0][v + 1], P[0]);
+                const int32_t s1 = shl9_add(acc[1][v + 1], P[1]);
+                top2_fold2(m1[v], m2[v], r0, r1);
+                top2_fold2(m1[v + 1], m2[v + 1], s0, s1);
+                asm("v_min3_i32 %0, %0, %1, %2" : "+v"(cm0) : "v"(r0), "v"(s0));
+                asm("v_min3_i32 %0, %0, %1, %2" : "+v"(cm1) : "v"(r1), "v"(s1));
+            }
+        } else {
 #pragma unroll
-            for (int j = 0; j < 4; j++) {
-                const int v = 4 * g + j;
-                const int32_t a0 = acc[0][v], a1 = acc[1][v];
-                const int32_t r0 = shl9_add(a0, P[0]);
-                const int32_t r1 = shl9_add(a1, P[1]);
-                if constexpr (kTop2)
-                    top2_fold2(m1[v], m2[v], r0, r1);
-                else
-                    m1[v] = imin(m1[v], imin(r0, r1));
-                cm0 = imin(cm0, shl9_add_asm(a0, q4[j]));
-                cm1 = imin(cm1, shl9_add_asm(a1, q4[j]));
+            for (int g = 0; g < 4; g++) {
+                const v4i q4 = *reinterpret_cast<const v4i*>(sQ + wave * 32 + 8 * g + 4 * hi);
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const int v = 4 * g + j;
+                    const int32_t a0 = acc[0][v], a1 = acc[1][v];
+                    const int32_t r0 = shl9_add(a0, P[0]);
+                    const int32_t r1 = shl9_add(a1, P[1]);
+                    if constexpr (kTop2)
+                        top2_fold2(m1[v], m2[v], r0, r1);
+                    else
+                        m1[v] = imin(m1[v], imin(r0, r1));
+                    cm0 = imin(cm0, shl9_add_asm(a0, q4[j]));
+                    cm1 = imin(cm1, shl9_add_asm(a1, q4[j]));
+                }
             }
         }
         const auto sw = __builtin_amdgcn_permlane32_swap((uint32_t)cm0, (uint32_t)cm1, false, false);
@@ -320,6 +386,15 @@ __device__ __forceinline__ void match_tile(const DevTask& T, uint32_t rowBase, u
         xs = stage_load(2);
     }
     __syncthreads();
+    if constexpr (kPrep) {
+        // accumulator start H_i = NA'_i >> 1 of the lane's 16 rows (MFMA layout)
+#pragma unroll
+        for (int g = 0; g < 4; g++) {
+            const v4i q4 = *reinterpret_cast<const v4i*>(sQ + wave * 32 + 8 * g + 4 * hi);
+#pragma unroll
+            for (int j = 0; j < 4; j++) Hv[4 * g + j] = q4[j] >> 1;
+        }
+    }
 
     // two chunks per trip, barrier after the odd one; the column partials of a
     // trip's chunks are complete after its barrier and flushed at the next trip
@@ -335,7 +410,12 @@ __device__ __forceinline__ void match_tile(const DevTask& T, uint32_t rowBase, u
     }
     __syncthreads();
     if (doCols)
-        for (uint32_t k = firstUnflushed; k < nChunks; k++) flush(k);
+        for (uint32_t k = firstUnflushed; k < nChunks; k++) {
+            if constexpr (kPrep)
+                flush_prep(k);
+            else
+                flush(k);
+        }
 
     // ---- row results: transpose the per-lane (row, column-class) bests through LDS
     int32_t* L = reinterpret_cast<int32_t*>(sU);
@@ -378,9 +458,13 @@ __device__ __forceinline__ void match_tile(const DevTask& T, uint32_t rowBase, u
     if (tid < kBM) {
         const uint32_t row = rowBase + tid;
         if (row < T.nA) {
-            const int32_t na = sQ[tid] >> 8;
+            // plain keys: (NB' + 2 dot) << 8 | tag, plus NA'. Prepared keys:
+            // 256 (d - p) + tag, and keys of columns past nB start at
+            // 256 (NA' - p) + kPastCols, above every real column's key
+            const int32_t na = kPrep ? (sQ[tid] & 1) : (sQ[tid] >> 8);
+            const int32_t past = kPrep ? (sQ[tid] - na) * 256 + kPastCols : kNone;
             auto to_key = [&](int32_t m, int cls) -> uint64_t {
-                if (m == kNone) return kNoMatch;
+                if (m == kNone || (kPrep && m >= past)) return kNoMatch;
                 return make_key((uint32_t)((m >> 8) + na), colBeg + (uint32_t)(m & 0xFF) * 32u + (uint32_t)cls);
             };
             const uint64_t k1 = to_key(b1, c1);
@@ -400,6 +484,112 @@ __device__ __forceinline__ void match_tile(const DevTask& T, uint32_t rowBase, u
                     T.bestA[row] = k1;
                 else if (k1 != kNoMatch)
                     atomicMin(reinterpret_cast<unsigned long long*>(T.bestA + row), (unsigned long long)k1);
+            }
+        }
+    }
+
+    // ---- prepared (top-2) form, train side. For column j this block's
+    // entry holds E = min over its rows of (d - p_i); e = min over all
+    // blocks (T.colMin). The lowest row of smallest d has d - p <= e + 1 (a
+    // row with E = e has d <= e + 1), so only blocks with E <= e + 1 hold it,
+    // and in such a block only the first wave named in the entry (all 8 when
+    // the entry's bit 3 is set). T.colMin still decreases while other blocks
+    // run, so this block takes every column with E <= colMin + 1 as read
+    // now: a superset of its true share. Each listed (column, wave) gets its
+    // exact (d, row) minimum over the wave's 32 rows on MFMA, merged into
+    // bestB by a 64-bit atomicMin.
+    if constexpr (kPrep) {
+        if (doCols) {
+            // the segment's candidates: column offset (16 bits) and wave mask
+            // (8 bits) per entry, kMaxSegCols entries at most
+            constexpr uint32_t kBuf = 96, kPer = kMaxSegCols / (kWaves * 64);
+            uint16_t* sListJ = reinterpret_cast<uint16_t*>(sU);
+            uint8_t* sListM = reinterpret_cast<uint8_t*>(sListJ + kMaxSegCols);
+            uint32_t* sBuf = reinterpret_cast<uint32_t*>(sListM + kMaxSegCols);  // [kWaves][kBuf]
+            static_assert(kMaxSegCols * 3 + kWaves * kBuf * 4 <= kUnionBytes, "resolution lists exceed the union");
+            __shared__ uint32_t sNres;  // entries listed (outside sU, which the row results may still read)
+            uint32_t* sN = &sNres;
+            const int32_t* part = T.colPart + (size_t)(rowBase / kBM) * T.nB;
+            uint32_t* buf = sBuf + wave * kBuf;
+            // one 32-column batch of this wave's list: d << 8 | local row
+            // = (u.w << 9) + (NA' << 8 | local row) + (NB' << 8)
+            auto resolve_batch = [&](uint32_t cnt) {
+                const uint32_t j = buf[min((uint32_t)l32, cnt - 1)];
+                const int32_t nb = T.NB[j];
+                int32_t key = kNone;
+                v16i acc = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+                for (int kc = 0; kc < 4; kc++) {
+                    const int sl = 2 * kc + hi;
+                    const v4i a = *reinterpret_cast<const v4i*>(aRow + ((sl ^ aSw) << 4));
+                    const v4i b = *reinterpret_cast<const v4i*>(T.W + (size_t)j * 128 + sl * 16);
+                    acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b, acc, 0, 0, 0);
+                }
+#pragma unroll
+                for (int g = 0; g < 4; g++) {
+                    const v4i q4 = *reinterpret_cast<const v4i*>(sQ + wave * 32 + 8 * g + 4 * hi);
+#pragma unroll
+                    for (int q = 0; q < 4; q++)
+                        key = imin(key, shl9_add(acc[4 * g + q], q4[q] * 256 + wave * 32 + 8 * g + 4 * hi + q));
+                }
+                key = imin(key, __shfl_xor(key, 32));
+                const uint32_t dk = (uint32_t)(key + nb);
+                if (hi == 0 && (uint32_t)l32 < cnt)
+                    atomicMin(reinterpret_cast<unsigned long long*>(T.bestB + j),
+                              (unsigned long long)make_key(dk >> 8, rowBase + (dk & 0xFF)));
+            };
+            if (tid == 0) *sN = 0;
+            // every load of the scan in flight at once
+            int32_t x[kPer], e[kPer];
+#pragma unroll
+            for (uint32_t q = 0; q < kPer; q++) {
+                const uint32_t j = colBeg + q * (kWaves * 64) + (uint32_t)tid;
+                x[q] = j < colEnd ? part[j] : INT32_MAX;
+                e[q] = j < colEnd ? T.colMin[j] : 0;
+            }
+            __syncthreads();  // the row results' reads of sU, and *sN
+#pragma unroll
+            for (uint32_t q = 0; q < kPer; q++) {
+                const bool cand = (x[q] >> 8) <= e[q] + 1;
+                const uint64_t m = __ballot(cand);
+                if (m == 0) continue;
+                uint32_t base = 0;
+                if (lane == 0) base = atomicAdd(sN, (uint32_t)__popcll(m));
+                base = (uint32_t)__shfl((int)base, 0);
+                if (cand) {
+                    const uint32_t pos = base + (uint32_t)__popcll(m & ((1ull << lane) - 1));
+                    sListJ[pos] = (uint16_t)(q * (kWaves * 64) + (uint32_t)tid);
+                    sListM[pos] = (uint8_t)((x[q] & 8) ? 0xFFu : (1u << (x[q] & 7)));
+                }
+            }
+            __syncthreads();
+            const uint32_t n = *sN;
+            {
+                // this wave's entries, gathered 64 at a time into its buffer
+                uint32_t pend = 0;
+                for (uint32_t e0 = 0; e0 < n; e0 += 64) {
+                    const uint32_t ei = e0 + (uint32_t)lane;
+                    const bool sel = ei < n && ((sListM[ei] >> wave) & 1);
+                    const uint64_t m = __ballot(sel);
+                    if (sel) buf[pend + __popcll(m & ((1ull << lane) - 1))] = colBeg + sListJ[ei];
+                    pend += (uint32_t)__popcll(m);
+                    if (pend >= 32) {
+                        resolve_batch(32);
+                        // move the rest (at most 63) to the front
+                        const uint32_t rest = pend - 32;
+                        const uint32_t v = (uint32_t)lane < rest ? buf[32 + lane] : 0u;
+                        if ((uint32_t)lane < rest) buf[lane] = v;
+                        pend = rest;
+                        if (pend >= 32) {
+                            resolve_batch(32);
+                            const uint32_t rest2 = pend - 32;
+                            const uint32_t v2 = (uint32_t)lane < rest2 ? buf[32 + lane] : 0u;
+                            if ((uint32_t)lane < rest2) buf[lane] = v2;
+                            pend = rest2;
+                        }
+                    }
+                }
+                if (pend > 0) resolve_batch(pend);
             }
         }
     }
@@ -501,23 +691,32 @@ __global__ __launch_bounds__(256) void prep_train_kernel(const Src batch, bool i
     acc += dpp_xor1(acc);
     acc += dpp_xor2(acc);
     acc += dpp_mirror8(acc);
-    if (sp == 0) T.NB[col] = valid ? acc * 256 : kNone;
+    if (sp == 0) {
+        T.NB[col] = valid ? acc * 256 : kPastCols;
+        if (valid) T.colMin[col] = INT32_MAX;
+    }
 }
 
-size_t prep_bytes(uint32_t nB)
+size_t prep_bytes(uint32_t nA, uint32_t nB)
 {
+    if (nA == 0 || nB == 0) return 0;
     const size_t nPad = div_up(nB, (uint32_t)kBN) * (size_t)kBN;
-    return align_up(nPad * 128, 256) + align_up(nPad * 4, 256);
+    const size_t nRB = div_up(nA, (uint32_t)kBM);
+    return align_up(nPad * 128, 256) + align_up(nPad * 4, 256) + align_up((size_t)nB * 4, 256) +
+           align_up(nRB * nB * 4, 256);
 }
 
-// W and NB of a task from its part of a workspace
-static void place_prep(DevTask& t, uint8_t*& ws)
+void place_prep(DevTask& t, uint8_t* ws)
 {
-    if (t.nB == 0) return;
+    if (t.nA == 0 || t.nB == 0) return;
     const size_t nPad = div_up(t.nB, (uint32_t)kBN) * (size_t)kBN;
     t.W = ws;
-    t.NB = reinterpret_cast<int32_t*>(ws + align_up(nPad * 128, 256));
-    ws += prep_bytes(t.nB);
+    ws += align_up(nPad * 128, 256);
+    t.NB = reinterpret_cast<int32_t*>(ws);
+    ws += align_up(nPad * 4, 256);
+    t.colMin = reinterpret_cast<int32_t*>(ws);
+    ws += align_up((size_t)t.nB * 4, 256);
+    t.colPart = reinterpret_cast<int32_t*>(ws);
 }
 
 int launch_prep_table(const DevTask* tasks, uint32_t nTasks, uint32_t maxB, hipStream_t stream)
@@ -527,6 +726,8 @@ int launch_prep_table(const DevTask* tasks, uint32_t nTasks, uint32_t maxB, hipS
     prep_train_kernel<<<dim3(nPad * 8 / 256, 1, nTasks), 256, 0, stream>>>(TaskTable{tasks}, false);
     return check_launch("bf prep table");
 }
+
+
 
 template <class Src>
 __global__ void merge_top2_kernel(const Src batch)
@@ -634,7 +835,7 @@ static size_t top2_workspace(const rsfm_bf_top2_task* tasks, size_t nbTasks)
         const Plan p = make_plan(t, n);
         for (size_t i = 0; i < n; i++) {
             if (p.nSeg > 1) bytes += align_up((size_t)p.nSeg * t[i].nA * 16, 256);
-            if (t[i].nA > 0) bytes += prep_bytes(t[i].nB);
+            bytes += prep_bytes(t[i].nA, t[i].nB);
         }
     }
     return bytes;
@@ -721,7 +922,8 @@ extern "C" int rsfm_bf_match_top2(const rsfm_bf_top2_task* tasks, size_t nbTasks
                 batch.t[i].part = reinterpret_cast<uint64_t*>(ws);
                 ws += align_up((size_t)p.nSeg * batch.t[i].nA * 16, 256);
             }
-            if (batch.t[i].nA > 0) place_prep(batch.t[i], ws);
+            place_prep(batch.t[i], ws);
+            ws += prep_bytes(batch.t[i].nA, batch.t[i].nB);
         }
         // one launch: results to "no match" where no tile writes them, and
         // the staging form of every task's train descriptors

@@ -35,10 +35,16 @@ struct DevTask {
     uint32_t nSeg;      // column segments of this task (1: results written directly)
     uint32_t segCols;   // columns per segment (multiple of kBN, <= kMaxSegCols)
     // top-2 staging form of B, written by the preparation kernel: w = b - 128
-    // as signed bytes for nB rounded up to kBN columns (zero past nB) and per
-    // column NB' << 8 with NB' = |b - 127|^2 - 128 (INT32_MAX past nB)
+    // as signed bytes for nB rounded up to kBN columns (zero past nB), per
+    // column NB' << 8 with NB' = |b - 127|^2 - 128 (a fixed offset past nB)
+    // and |b|^2
     uint8_t* W;
     int32_t* NB;
+    // top-2 train side: per query row block of kBM rows and train column the
+    // block's smallest d - p with the wave holding it ([ceil(nA/kBM)][nB]),
+    // and per column the smallest over the blocks (merged by atomicMin)
+    int32_t* colPart;
+    int32_t* colMin;
 };
 typedef ByValue<DevTask, kMaxTasks> TaskBatch;
 typedef InTable<DevTask> TaskTable;
@@ -54,10 +60,14 @@ int launch_match_tiles_top2(const DevTask* tasks, const TileDesc* tiles, const u
 // one workgroup per tile of the same table; maxList = longest XCD list
 int launch_match_tile_grid_top2(const DevTask* tasks, const TileDesc* tiles, const uint32_t* xcdOff, uint32_t maxList,
                                 hipStream_t stream);
-// bytes of the staging form of nB train descriptors (W then NB, 256-B aligned parts)
-size_t prep_bytes(uint32_t nB);
-// staging form of B for the listed tasks (W, NB set); maxB = largest nB
+// workspace bytes of one top-2 task's staging form and train-side partials
+// (W, NB, colMin, colPart; 256-B aligned parts)
+size_t prep_bytes(uint32_t nA, uint32_t nB);
+// point W, NB, colMin and colPart of t into ws (prep_bytes(t.nA, t.nB) bytes)
+void place_prep(DevTask& t, uint8_t* ws);
+// staging form of B for the listed tasks (W, NB set, colMin cleared); maxB = largest nB
 int launch_prep_table(const DevTask* tasks, uint32_t nTasks, uint32_t maxB, hipStream_t stream);
+
 // merge of the segment partials of the listed tasks (all with nSeg > 1)
 int launch_merge_top2_table(const DevTask* tasks, uint32_t nTasks, uint32_t maxA, hipStream_t stream);
 }  // namespace bf

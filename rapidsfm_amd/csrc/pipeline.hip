@@ -151,14 +151,8 @@ Plan make_plan(const std::vector<rsfm_pipeline_pair>& pairs, uint32_t segCols, u
             t.part = part ? part + pl.partWords : nullptr;
             pl.partWords += (size_t)t.nSeg * x.nA * 2;
         }
-        if (x.nA > 0 && x.nB > 0) {
-            const size_t nPad = (size_t)div_up(x.nB, (uint32_t)bf::kBN) * bf::kBN;
-            if (prep) {
-                t.W = prep + pl.prepBytes;
-                t.NB = reinterpret_cast<int32_t*>(prep + pl.prepBytes + align_up(nPad * 128, 256));
-            }
-            pl.prepBytes += bf::prep_bytes(x.nB);
-        }
+        if (prep) bf::place_prep(t, prep + pl.prepBytes);
+        pl.prepBytes += bf::prep_bytes(x.nA, x.nB);
     }
     // pairs to XCDs: longest processing time first
     std::vector<size_t> order(P);
@@ -490,7 +484,7 @@ extern "C" int rsfm_pipeline_create(const rsfm_pipeline_pair* pairs, size_t nbPa
     pl->dPrep = pl->arena + oPrep;
     float* dModels = reinterpret_cast<float*>(pl->arena + oModels);
     for (auto& x : pl->pairs)
-        if (x.nA > 0) pl->maxPrepB = std::max(pl->maxPrepB, x.nB);
+        if (x.nA > 0 && x.nB > 0) pl->maxPrepB = std::max(pl->maxPrepB, x.nB);
 
     std::vector<aux::FilterTask> ft(P);
     std::vector<ransac::SampleTask> st(P);
