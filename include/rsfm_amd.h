@@ -590,7 +590,7 @@ typedef struct rsfm_pipeline_config {
     uint32_t ratioNum, ratioDen; /* ratio test num/den (4, 5 = 0.8) */
     uint64_t seed;             /* pair p samples with seed + p */
     float minSepSq0, minSepSq1;  /* sample sanity gate, 0 = off */
-    uint32_t segCols;          /* matcher tile width (multiple of 64, <= 8192); 0 = autotune */
+    uint32_t segCols;          /* matcher tile width (multiple of 64, <= 4096); 0 = autotune */
     uint32_t wgPerXcd;         /* persistent matcher workgroups per XCD, RSFM_PIPELINE_WG_PER_TILE
                                   (one workgroup per tile, no persistent loop); 0 = autotune */
     uint32_t solver;           /* RSFM_PIPELINE_SOLVER_F8 or RSFM_PIPELINE_SOLVER_DEGENSAC */

@@ -140,23 +140,25 @@ __device__ inline void dma4(const void* src, void* lds)
 // s_waitcnt vmcnt(0) (gfx9 encoding: expcnt and lgkmcnt left at their maxima)
 __device__ inline void wait_vm0() { __builtin_amdgcn_s_waitcnt(0x0F70); }
 
-// sCol row stride in words: the flush reads 8 consecutive columns from each of
-// the 8 waves; a stride of 72 puts those 64 words in distinct banks
+// sCol row stride of the top-1 tile in words: its flush reads 8 consecutive
+// columns from each of the 8 waves; a stride of 72 puts those 64 words in
+// distinct banks
 constexpr int kColStride = 72;
 
-// One workgroup = 8 waves x 32 query rows. The query tile (operands and
-// column-key offsets) sits in LDS and the column segment streams through an
-// LDS ring in 64-column chunks. Per chunk a wave issues its 8 MFMAs (K-step
-// major over the two 32-column blocks, operands read from LDS per K-step) into
-// one accumulator pair, then folds the keys of that chunk. With one
-// accumulator pair and streamed operands a wave fits in 128 VGPRs, so two
-// workgroups (four waves per SIMD) share a CU and the waves overlap each
-// other's matrix and vector work; the earlier layout software-pipelined two
-// accumulator pairs inside each wave at 210 VGPRs and two waves per SIMD, and
-// ran 10% slower on top-2. Column partials of chunk c are written to LDS in
-// iteration c and merged to global memory by all waves (8 columns each) after
-// the next barrier.
-template <int kBits, bool kTop2, bool kPrep>
+// Top-1 tile (rsfm_bf_match_u8 / _u4). One workgroup = 8 waves x 32 query
+// rows. The query tile (operands and column-key offsets) sits in LDS and the
+// column segment streams through an LDS ring in 64-column chunks, loaded,
+// transformed and stored by the tile itself (register staging). Per chunk a
+// wave issues its 8 MFMAs (K-step major over the two 32-column blocks,
+// operands read from LDS per K-step) into one accumulator pair, then folds
+// the keys of that chunk: one v_lshl_add makes the row key
+// ((NB'_j + 2dot) << 8 | column-block tag), one the column key
+// ((NA'_i + 2dot) << 8 | local row), v_min3 folds both. The wave fits in 128
+// VGPRs, so two workgroups (four waves per SIMD) share a CU. Column partials
+// of chunk c are written to LDS in iteration c and merged to global memory by
+// all waves (8 columns each, one 64-bit atomicMin per column) after the next
+// barrier.
+template <int kBits>
 __device__ __forceinline__ void match_tile(const DevTask& T, uint32_t rowBase, uint32_t colBeg, uint32_t seg)
 {
     constexpr int kWaves = 8;
@@ -180,38 +182,31 @@ __device__ __forceinline__ void match_tile(const DevTask& T, uint32_t rowBase, u
     const int l32 = lane & 31, hi = lane >> 5;
     const int rloc = wave * 32 + l32;
 
-    // ---- query rows: operands u = 127 - a into LDS, offsets NA' = |a - 128|^2.
-    // Prepared (top-2) form: rows past nA repeat row nA - 1, so they tie with
-    // it and never beat it on the train side, and sQ holds NA' itself.
+    // ---- query rows: operands u = 127 - a into LDS, offsets NA' = |a - 128|^2
     {
         const uint32_t row = rowBase + rloc;
         const bool valid = row < T.nA;
-        const uint32_t src = kPrep ? min(row, T.nA - 1) : row;
         int32_t nap = 0;
 #pragma unroll
         for (int kc = 0; kc < 4; kc++) {
             v4i x = {0, 0, 0, 0};
-            if (kPrep || valid) x = load16<kBits>(T.A + (size_t)src * kDescBytes, kc * 32 + hi * 16);
+            if (valid) x = load16<kBits>(T.A + (size_t)row * kDescBytes, kc * 32 + hi * 16);
             v4i u;
 #pragma unroll
             for (int q = 0; q < 4; q++) {
                 const int xa = x[q] ^ (int)0x80808080u;
                 nap = __builtin_amdgcn_sdot4(xa, xa, nap, false);
-                u[q] = (kPrep || valid) ? (x[q] ^ 0x7F7F7F7F) : 0;
+                u[q] = valid ? (x[q] ^ 0x7F7F7F7F) : 0;
             }
             *reinterpret_cast<v4i*>(sA + rloc * 128 + (((2 * kc + hi) ^ swz(rloc)) << 4)) = u;
         }
         const int32_t na = nap + __shfl_xor(nap, 32);
-        if (hi == 0) sQ[rloc] = kPrep ? na : (valid ? (na * 256 + rloc) : kNone);
+        if (hi == 0) sQ[rloc] = valid ? (na * 256 + rloc) : kNone;
     }
 
-    int32_t m1[16], m2[16];
+    int32_t m1[16];
 #pragma unroll
-    for (int v = 0; v < 16; v++) {
-        m1[v] = kNone;
-        m2[v] = kNone;
-    }
-    v16i Hv = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};  // prepared form: accumulator start
+    for (int v = 0; v < 16; v++) m1[v] = kNone;
 
     const int sc = tid >> 3, sp = tid & 7;
     auto stage_load = [&](uint32_t c) -> v4i {
@@ -234,15 +229,6 @@ __device__ __forceinline__ void match_tile(const DevTask& T, uint32_t rowBase, u
         acc += dpp_mirror8(acc);
         sP[c % kPSlots][sc] = valid ? (acc * 256 + (int32_t)(c * 2 + (sc >> 5))) : kNone;
     };
-    // prepared train data: lane L of wave w fills column w*8 + L/8, slot L%8 of
-    // the chunk's ring slot, i.e. it reads source slot (L%8) ^ swz(column);
-    // wave 0 also copies the chunk's 64 column offsets (NB' << 8, kNone past nB)
-    auto stage_dma = [&](uint32_t c) {
-        const int col = wave * 8 + (lane >> 3);
-        const uint8_t* src = T.W + (size_t)(colBeg + c * kBN + col) * 128 + (((lane & 7) ^ swz(col)) << 4);
-        dma16(src, sU + (c % kRing) * kChunkBytes + wave * 1024);
-        if (wave == 0) dma4(T.NB + colBeg + c * kBN + lane, &sP[c % kPSlots][0]);
-    };
     auto flush = [&](uint32_t k) {
         const int col = wave * 8 + (lane >> 3);
         int32_t x = sCol[k % kColSlots][lane & 7][col];
@@ -256,54 +242,19 @@ __device__ __forceinline__ void match_tile(const DevTask& T, uint32_t rowBase, u
             atomicMin(reinterpret_cast<unsigned long long*>(T.bestB + gcol), (unsigned long long)make_key(dist, row));
         }
     };
-    // prepared (top-2) form: wave k % 8 reduces the 8 waves' minima of chunk
-    // k's 64 columns (one lane per column): the block's E = Rmin >> 8 goes to
-    // T.colMin by atomicMin, and to this block's row of T.colPart with the
-    // column tag replaced by the first wave holding Rmin (bits 0-2) and bit 3
-    // set when another wave comes within one unit of it (E_w <= E + 1)
-    auto flush_prep = [&](uint32_t k) {
-        if (wave != (int)(k & 7)) return;
-        const int32_t* src = &sCol[k % kColSlots][0][lane];
-        int32_t k1 = kNone, k2 = kNone;
-#pragma unroll
-        for (int w = 0; w < kWaves; w += 2)
-            top2_fold2(k1, k2, (src[w * kColStride] & ~0xFF) | w, (src[(w + 1) * kColStride] & ~0xFF) | (w + 1));
-        const uint32_t gcol = colBeg + k * kBN + lane;
-        if (gcol < colEnd) {
-            T.colPart[(size_t)(rowBase / kBM) * T.nB + gcol] = k1 | ((k2 >> 8) <= (k1 >> 8) + 1 ? 8 : 0);
-            atomicMin(T.colMin + gcol, k1 >> 8);
-        }
-    };
     const bool doCols = T.bestB != nullptr;
     const uint8_t* aRow = sA + rloc * 128;
     const int aSw = swz(rloc);
 
-    // chunk c: staging (register path: store chunk c+2, loaded one chunk
-    // earlier, and load chunk c+3 after the MFMAs; direct path: see below),
-    // MFMAs of chunk c (K-step major over both column blocks), then the keys
-    // of chunk c
-    // with prepared data the trip's first chunk c copies chunks c+2 and c+3
-    // (their ring slots were last read before the previous trip's barrier) and
-    // the trip's barrier waits for both
-    auto body = [&](uint32_t c, v4i& xs, bool sync, bool flushPrev, bool issue) {
-        if constexpr (kPrep) {
-            if (issue) {
-                if (c + 2 < nChunks) stage_dma(c + 2);
-                if (c + 3 < nChunks) stage_dma(c + 3);
-            }
-        } else {
-            stage_store(c + 2, xs);
-        }
-        // the previous trip's column partials: flushed after the staging store,
-        // so that store's wait on the staged load does not also wait on these atomics
+    // chunk c: store chunk c+2 (loaded one chunk earlier), flush the previous
+    // trip's column partials (after the staging store, so that store's wait on
+    // its staged load does not also wait on these atomics), MFMAs of chunk c
+    // (K-step major over both column blocks), load chunk c+3, keys of chunk c
+    auto body = [&](uint32_t c, v4i& xs, bool sync, bool flushPrev) {
+        stage_store(c + 2, xs);
         if (flushPrev) {
-            if constexpr (kPrep) {
-                flush_prep(c - 2);
-                flush_prep(c - 1);
-            } else {
-                flush(c - 2);
-                flush(c - 1);
-            }
+            flush(c - 2);
+            flush(c - 1);
         }
         const uint8_t* sb = sU + (c % kRing) * kChunkBytes;
         v16i acc[2];
@@ -316,106 +267,54 @@ __device__ __forceinline__ void match_tile(const DevTask& T, uint32_t rowBase, u
                 const int cl = cb * 32 + l32;
                 const v4i b = *reinterpret_cast<const v4i*>(sb + cl * 128 + ((s ^ swz(cl)) << 4));
                 const v16i zero = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-                acc[cb] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b, kc == 0 ? (kPrep ? Hv : zero) : acc[cb], 0, 0, 0);
+                acc[cb] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b, kc == 0 ? zero : acc[cb], 0, 0, 0);
             }
             __builtin_amdgcn_sched_barrier(0);
         }
-        if constexpr (!kPrep) xs = stage_load(c + 3);
+        xs = stage_load(c + 3);
         int32_t P[2];
 #pragma unroll
-        for (int cb = 0; cb < 2; cb++) {
-            P[cb] = sP[c % kPSlots][cb * 32 + l32];
-            // the prepared offsets carry no column-block tag (kNone keeps its value)
-            if constexpr (kPrep) P[cb] |= (int32_t)(c * 2 + cb);
-        }
+        for (int cb = 0; cb < 2; cb++) P[cb] = sP[c % kPSlots][cb * 32 + l32];
         int32_t cm0 = kNone, cm1 = kNone;
-        if constexpr (kPrep) {
-            // the accumulators started at H_i = NA'_i >> 1, so one key
-            // R = (acc << 9) + P = 256 (d - p_i) + tag serves both sides: per
-            // row across columns (p_i is the row's constant) and, tag aside,
-            // per column across rows up to the parity p_i, which the
-            // train-side resolution settles
-#pragma unroll
-            for (int v = 0; v < 16; v += 2) {
-                const int32_t r0 = shl9_add(acc[0][v], P[0]);
-                const int32_t r1 = shl9_add(acc[1][v], P[1]);
-                const int32_t s0 = shl9_add(acc[0][v + 1], P[0]);
-                const int32_t s1 = shl9_add(acc[1][v + 1], P[1]);
-                top2_fold2(m1[v], m2[v], r0, r1);
-                top2_fold2(m1[v + 1], m2[v + 1], s0, s1);
-                asm("v_min3_i32 %0, %0, %1, %2" : "+v"(cm0) : "v"(r0), "v"(s0));
-                asm("v_min3_i32 %0, %0, %1, %2" : "+v"(cm1) : "v"(r1), "v"(s1));
-            }
-        } else {
-#pragma unroll
-            for (int g = 0; g < 4; g++) {
-                const v4i q4 = *reinterpret_cast<const v4i*>(sQ + wave * 32 + 8 * g + 4 * hi);
-#pragma unroll
-                for (int j = 0; j < 4; j++) {
-                    const int v = 4 * g + j;
-                    const int32_t a0 = acc[0][v], a1 = acc[1][v];
-                    const int32_t r0 = shl9_add(a0, P[0]);
-                    const int32_t r1 = shl9_add(a1, P[1]);
-                    if constexpr (kTop2)
-                        top2_fold2(m1[v], m2[v], r0, r1);
-                    else
-                        m1[v] = imin(m1[v], imin(r0, r1));
-                    cm0 = imin(cm0, shl9_add_asm(a0, q4[j]));
-                    cm1 = imin(cm1, shl9_add_asm(a1, q4[j]));
-                }
-            }
-        }
-        const auto sw = __builtin_amdgcn_permlane32_swap((uint32_t)cm0, (uint32_t)cm1, false, false);
-        sCol[c % kColSlots][wave][lane] = imin((int32_t)sw[0], (int32_t)sw[1]);
-        if (sync) {
-            if constexpr (kPrep) wait_vm0();
-            __syncthreads();
-        }
-    };
-
-    v4i xs = {0, 0, 0, 0};
-    if constexpr (kPrep) {
-        stage_dma(0);
-        if (nChunks > 1) stage_dma(1);
-        wait_vm0();
-    } else {
-        xs = stage_load(0);
-        stage_store(0, xs);
-        xs = stage_load(1);
-        stage_store(1, xs);
-        xs = stage_load(2);
-    }
-    __syncthreads();
-    if constexpr (kPrep) {
-        // accumulator start H_i = NA'_i >> 1 of the lane's 16 rows (MFMA layout)
 #pragma unroll
         for (int g = 0; g < 4; g++) {
             const v4i q4 = *reinterpret_cast<const v4i*>(sQ + wave * 32 + 8 * g + 4 * hi);
 #pragma unroll
-            for (int j = 0; j < 4; j++) Hv[4 * g + j] = q4[j] >> 1;
+            for (int j = 0; j < 4; j++) {
+                const int v = 4 * g + j;
+                const int32_t a0 = acc[0][v], a1 = acc[1][v];
+                m1[v] = imin(m1[v], imin(shl9_add(a0, P[0]), shl9_add(a1, P[1])));
+                cm0 = imin(cm0, shl9_add_asm(a0, q4[j]));
+                cm1 = imin(cm1, shl9_add_asm(a1, q4[j]));
+            }
         }
-    }
+        const auto sw = __builtin_amdgcn_permlane32_swap((uint32_t)cm0, (uint32_t)cm1, false, false);
+        sCol[c % kColSlots][wave][lane] = imin((int32_t)sw[0], (int32_t)sw[1]);
+        if (sync) __syncthreads();
+    };
+
+    v4i xs = stage_load(0);
+    stage_store(0, xs);
+    xs = stage_load(1);
+    stage_store(1, xs);
+    xs = stage_load(2);
+    __syncthreads();
 
     // two chunks per trip, barrier after the odd one; the column partials of a
     // trip's chunks are complete after its barrier and flushed at the next trip
     uint32_t c = 0;
     for (; c + 2 <= nChunks; c += 2) {
-        body(c, xs, false, doCols && c >= 2, true);
-        body(c + 1, xs, true, false, false);
+        body(c, xs, false, doCols && c >= 2);
+        body(c + 1, xs, true, false);
     }
     uint32_t firstUnflushed = c >= 2 ? c - 2 : 0;
     if (c < nChunks) {
-        body(c, xs, false, doCols && c >= 2, true);
+        body(c, xs, false, doCols && c >= 2);
         firstUnflushed = c;
     }
     __syncthreads();
     if (doCols)
-        for (uint32_t k = firstUnflushed; k < nChunks; k++) {
-            if constexpr (kPrep)
-                flush_prep(k);
-            else
-                flush(k);
-        }
+        for (uint32_t k = firstUnflushed; k < nChunks; k++) flush(k);
 
     // ---- row results: transpose the per-lane (row, column-class) bests through LDS
     int32_t* L = reinterpret_cast<int32_t*>(sU);
@@ -426,10 +325,260 @@ __device__ __forceinline__ void match_tile(const DevTask& T, uint32_t rowBase, u
         L[rl * kTailStride + l32] = m1[v];
     }
     __syncthreads();
+    if (tid < kBM) {
+        int32_t b1 = kNone;
+        int c1 = 0;
+        for (int cc = 0; cc < 32; cc++) {
+            const int32_t x = L[tid * kTailStride + cc];
+            if (x < b1) {
+                b1 = x;
+                c1 = cc;
+            }
+        }
+        const uint32_t row = rowBase + tid;
+        if (row < T.nA) {
+            const int32_t na = sQ[tid] >> 8;
+            const uint64_t k1 = b1 == kNone ? kNoMatch
+                                            : make_key((uint32_t)((b1 >> 8) + na),
+                                                       colBeg + (uint32_t)(b1 & 0xFF) * 32u + (uint32_t)c1);
+            if (T.nSeg == 1)
+                T.bestA[row] = k1;
+            else if (k1 != kNoMatch)
+                atomicMin(reinterpret_cast<unsigned long long*>(T.bestA + row), (unsigned long long)k1);
+        }
+    }
+    (void)seg;
+}
+
+// Top-2 tile over the prepared train form (rsfm_bf_match_top2 and the batched
+// pipeline), on v_mfma_i32_16x16x64_i8. One workgroup = 8 waves x 32 query
+// rows, two workgroups (four waves per SIMD) per CU. Lane l of a wave works
+// on column class l % 16 and row group q = l / 16:
+//  * query operands stay in registers: for row block rb (16 rows) and K-step
+//    ks (64 bytes) the lane holds bytes [64 ks + 16 q, +16) of row
+//    rb * 16 + l % 16 as u = 127 - a (rows past nA repeat row nA - 1, so they
+//    tie with it and never beat it on the train side);
+//  * the segment streams through a 4-slot LDS ring in 64-column chunks copied
+//    by direct global-to-LDS loads of the prepared form (w = b - 128 and
+//    NB' << 8 | tag per column, written once per call by prep_train_kernel);
+//    a chunk is 4 column blocks of 16, so a chunk's 16 MFMAs (2 row blocks x
+//    4 column blocks x 2 K-steps) give the lane 32 products: rows
+//    rb * 16 + 4 q + j (j < 4) of columns cb * 16 + l % 16;
+//  * the MFMA chains start at H_i = NA'_i >> 1, so the single key
+//    R = (acc << 9) + (NB'_j << 8 | tag) = 256 (d - p_i) + tag
+//    (p_i = NA'_i & 1, tag = 4 chunk + cb within the segment, hence at most
+//    4096 columns per segment) orders each row's columns exactly and each
+//    column's rows up to p_i: one v_lshl_add per product, v_med3 + v_min3 for
+//    the row top-2 per (row, class), v_min3 for the column minimum, and two
+//    lane swaps (v_permlane32_swap, v_permlane16_swap) that leave lane L with
+//    column L's minimum over the wave's rows;
+//  * train side: per column and row block the smallest R with the first wave
+//    holding it, a global per-column minimum by atomicMin, and at the end of
+//    the tile the exact distances of the candidate (column, wave) pairs on
+//    MFMA, merged into bestB by atomicMin.
+__device__ __forceinline__ void match_tile_top2(const DevTask& T, uint32_t rowBase, uint32_t colBeg, uint32_t seg)
+{
+    constexpr int kWaves = 8;
+    constexpr int kChunkBytes = kBN * 128;
+    constexpr int kRing = 4;  // ring slot of chunk c: c % 4
+    constexpr int kPSlots = 8;
+    constexpr int kColSlots = 4;
+    constexpr int kCS = 72;
+    constexpr int kTailStride = 17;
+    constexpr int kUnionBytes = kRing * kChunkBytes;
+    static_assert(kBM * kTailStride * 4 <= kUnionBytes, "row transpose exceeds the ring");
+
+    const uint32_t colEnd = min(colBeg + T.segCols, T.nB);
+    const uint32_t nChunks = (colEnd - colBeg + kBN - 1) / kBN;
+
+    __shared__ __attribute__((aligned(16))) uint8_t sU[kUnionBytes];
+    __shared__ int32_t sP[kPSlots][kBN];
+    __shared__ int32_t sCol[kColSlots][kWaves][kCS];
+    __shared__ __attribute__((aligned(16))) int32_t sQ[kBM];  // NA'
+    __shared__ uint32_t sNres;
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int c16 = lane & 15, q = lane >> 4;
+
+    // ---- query operands and NA' = |a - 128|^2 (four lanes per row)
+    v4i aop[2][2];
+#pragma unroll
+    for (int rb = 0; rb < 2; rb++) {
+        const uint32_t row = min(rowBase + wave * 32 + rb * 16 + c16, T.nA - 1);
+        int32_t nap = 0;
+#pragma unroll
+        for (int ks = 0; ks < 2; ks++) {
+            const v4i x = load16<8>(T.A + (size_t)row * 128, ks * 64 + q * 16);
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const int xa = x[k] ^ (int)0x80808080u;
+                nap = __builtin_amdgcn_sdot4(xa, xa, nap, false);
+                aop[rb][ks][k] = x[k] ^ 0x7F7F7F7F;
+            }
+        }
+        nap += __shfl_xor(nap, 16);
+        nap += __shfl_xor(nap, 32);
+        if (q == 0) sQ[wave * 32 + rb * 16 + c16] = nap;
+    }
+
+    int32_t m1[2][4], m2[2][4];
+#pragma unroll
+    for (int rb = 0; rb < 2; rb++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            m1[rb][j] = kNone;
+            m2[rb][j] = kNone;
+        }
+    v4i Hv[2];  // accumulator start H = NA' >> 1 of rows rb * 16 + 4 q + j
+
+    // lane L of wave w fills column w*8 + L/8, slot L%8 of the chunk's ring
+    // slot, i.e. it reads source slot (L%8) ^ swz(column); wave 0 also copies
+    // the chunk's 64 column offsets
+    auto stage_dma = [&](uint32_t c) {
+        const int col = wave * 8 + (lane >> 3);
+        const uint8_t* src = T.W + (size_t)(colBeg + c * kBN + col) * 128 + (((lane & 7) ^ swz(col)) << 4);
+        dma16(src, sU + (c % kRing) * kChunkBytes + wave * 1024);
+        if (wave == 0) dma4(T.NB + colBeg + c * kBN + lane, &sP[c % kPSlots][0]);
+    };
+    // wave k % 8 reduces chunk k's 8 wave minima per column (one lane per
+    // column): E = Rmin >> 8 to T.colMin by atomicMin, and to this block's
+    // row of T.colPart with the tag replaced by the first wave holding Rmin
+    // (bits 0-2) and bit 3 set when another wave comes within one unit
+    // (E_w <= E + 1)
+    auto flush = [&](uint32_t k) {
+        if (wave != (int)(k & 7)) return;
+        const int32_t* src = &sCol[k % kColSlots][0][lane];
+        int32_t k1 = kNone, k2 = kNone;
+#pragma unroll
+        for (int w = 0; w < kWaves; w += 2)
+            top2_fold2(k1, k2, (src[w * kCS] & ~0xFF) | w, (src[(w + 1) * kCS] & ~0xFF) | (w + 1));
+        const uint32_t gcol = colBeg + k * kBN + lane;
+        if (gcol < colEnd) {
+            T.colPart[(size_t)(rowBase / kBM) * T.nB + gcol] = k1 | ((k2 >> 8) <= (k1 >> 8) + 1 ? 8 : 0);
+            atomicMin(T.colMin + gcol, k1 >> 8);
+        }
+    };
+    // lane L ends with the minimum over the wave's rows of column L, from the
+    // lanes' per-class minima x[cb] (rows of group q, column cb * 16 + l % 16)
+    auto col_reduce = [&](const int32_t (&x)[4]) -> int32_t {
+        const auto a = __builtin_amdgcn_permlane32_swap((uint32_t)x[0], (uint32_t)x[2], false, false);
+        const auto b = __builtin_amdgcn_permlane32_swap((uint32_t)x[1], (uint32_t)x[3], false, false);
+        const int32_t t02 = imin((int32_t)a[0], (int32_t)a[1]);
+        const int32_t t13 = imin((int32_t)b[0], (int32_t)b[1]);
+        const auto s = __builtin_amdgcn_permlane16_swap((uint32_t)t02, (uint32_t)t13, false, false);
+        return imin((int32_t)s[0], (int32_t)s[1]);
+    };
+    const bool doCols = T.bestB != nullptr;
+
+    // chunk c: the trip's first chunk copies chunks c+2 and c+3 (their slots
+    // were last read before the previous trip's barrier) and flushes the
+    // previous trip's column partials; the chunk's 16 MFMAs (operand reads of
+    // one K-step at a time); keys of chunk c; the trip's barrier waits for
+    // the copies
+    auto body = [&](uint32_t c, bool sync, bool flushPrev, bool issue) {
+        if (issue) {
+            if (c + 2 < nChunks) stage_dma(c + 2);
+            if (c + 3 < nChunks) stage_dma(c + 3);
+        }
+        if (flushPrev) {
+            flush(c - 2);
+            flush(c - 1);
+        }
+        const uint8_t* sb = sU + (c % kRing) * kChunkBytes;
+        v4i acc[2][4];
+#pragma unroll
+        for (int ks = 0; ks < 2; ks++) {
+            v4i b[4];
+#pragma unroll
+            for (int cb = 0; cb < 4; cb++) {
+                const int col = cb * 16 + c16;
+                b[cb] = *reinterpret_cast<const v4i*>(sb + col * 128 + (((4 * ks + q) ^ swz(col)) << 4));
+            }
+#pragma unroll
+            for (int rb = 0; rb < 2; rb++)
+#pragma unroll
+                for (int cb = 0; cb < 4; cb++)
+                    acc[rb][cb] = __builtin_amdgcn_mfma_i32_16x16x64_i8(aop[rb][ks], b[cb], ks == 0 ? Hv[rb] : acc[rb][cb], 0, 0, 0);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        int32_t P[4];
+#pragma unroll
+        for (int cb = 0; cb < 4; cb++) P[cb] = sP[c % kPSlots][cb * 16 + c16];
+        int32_t cm[4];
+#pragma unroll
+        for (int rb = 0; rb < 2; rb++)
+#pragma unroll
+            for (int jp = 0; jp < 4; jp += 2) {
+                int32_t r[2][4];
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    const int j = jp + h;
+#pragma unroll
+                    for (int cb = 0; cb < 4; cb++) r[h][cb] = shl9_add(acc[rb][cb][j], P[cb]);
+                    // two pairs into the running top-2: m2 takes the second
+                    // of each pair's fold in one v_min3
+                    const int32_t t1 = med3(m1[rb][j], r[h][0], r[h][1]);
+                    asm("v_min3_i32 %0, %0, %1, %2" : "+v"(m1[rb][j]) : "v"(r[h][0]), "v"(r[h][1]));
+                    const int32_t t2 = med3(m1[rb][j], r[h][2], r[h][3]);
+                    asm("v_min3_i32 %0, %0, %1, %2" : "+v"(m1[rb][j]) : "v"(r[h][2]), "v"(r[h][3]));
+                    asm("v_min3_i32 %0, %0, %1, %2" : "+v"(m2[rb][j]) : "v"(t1), "v"(t2));
+                }
+                // column minima over the lane's rows, two rows per v_min3
+#pragma unroll
+                for (int cb = 0; cb < 4; cb++) {
+                    if (rb == 0 && jp == 0)
+                        cm[cb] = imin(r[0][cb], r[1][cb]);
+                    else
+                        asm("v_min3_i32 %0, %0, %1, %2" : "+v"(cm[cb]) : "v"(r[0][cb]), "v"(r[1][cb]));
+                }
+            }
+        sCol[c % kColSlots][wave][lane] = col_reduce(cm);
+        if (sync) {
+            wait_vm0();
+            __syncthreads();
+        }
+    };
+
+    stage_dma(0);
+    if (nChunks > 1) stage_dma(1);
+    wait_vm0();
+    __syncthreads();
+#pragma unroll
+    for (int rb = 0; rb < 2; rb++) {
+        const v4i q4 = *reinterpret_cast<const v4i*>(sQ + wave * 32 + rb * 16 + 4 * q);
+#pragma unroll
+        for (int j = 0; j < 4; j++) Hv[rb][j] = q4[j] >> 1;
+    }
+
+    // two chunks per trip, barrier after the odd one; the column partials of a
+    // trip's chunks are complete after its barrier and flushed at the next trip
+    uint32_t c = 0;
+    for (; c + 2 <= nChunks; c += 2) {
+        body(c, false, doCols && c >= 2, true);
+        body(c + 1, true, false, false);
+    }
+    uint32_t firstUnflushed = c >= 2 ? c - 2 : 0;
+    if (c < nChunks) {
+        body(c, false, doCols && c >= 2, true);
+        firstUnflushed = c;
+    }
+    __syncthreads();
+    if (doCols)
+        for (uint32_t k = firstUnflushed; k < nChunks; k++) flush(k);
+
+    // ---- row results: transpose the per-(row, class) bests through LDS
+    int32_t* L = reinterpret_cast<int32_t*>(sU);
+    __syncthreads();
+#pragma unroll
+    for (int rb = 0; rb < 2; rb++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) L[(wave * 32 + rb * 16 + 4 * q + j) * kTailStride + c16] = m1[rb][j];
+    __syncthreads();
     int32_t b1 = kNone, b2 = kNone;
     int c1 = 0, c2 = 0;
     if (tid < kBM) {
-        for (int cc = 0; cc < 32; cc++) {
+        for (int cc = 0; cc < 16; cc++) {
             const int32_t x = L[tid * kTailStride + cc];
             if (x < b1) {
                 b2 = b1; c2 = c1;
@@ -439,166 +588,140 @@ __device__ __forceinline__ void match_tile(const DevTask& T, uint32_t rowBase, u
             }
         }
     }
-    if constexpr (kTop2) {
-        __syncthreads();
+    __syncthreads();
 #pragma unroll
-        for (int v = 0; v < 16; v++) {
-            const int rl = wave * 32 + 8 * (v >> 2) + 4 * hi + (v & 3);
-            L[rl * kTailStride + l32] = m2[v];
-        }
-        __syncthreads();
-        if (tid < kBM) {
-            const int32_t x = L[tid * kTailStride + c1];
-            if (x < b2 || (x == b2 && c1 < c2)) {
-                b2 = x;
-                c2 = c1;
-            }
-        }
-    }
+    for (int rb = 0; rb < 2; rb++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) L[(wave * 32 + rb * 16 + 4 * q + j) * kTailStride + c16] = m2[rb][j];
+    __syncthreads();
     if (tid < kBM) {
+        const int32_t x = L[tid * kTailStride + c1];
+        if (x < b2 || (x == b2 && c1 < c2)) {
+            b2 = x;
+            c2 = c1;
+        }
         const uint32_t row = rowBase + tid;
         if (row < T.nA) {
-            // plain keys: (NB' + 2 dot) << 8 | tag, plus NA'. Prepared keys:
-            // 256 (d - p) + tag, and keys of columns past nB start at
+            // keys 256 (d - p) + tag; keys of columns past nB start at
             // 256 (NA' - p) + kPastCols, above every real column's key
-            const int32_t na = kPrep ? (sQ[tid] & 1) : (sQ[tid] >> 8);
-            const int32_t past = kPrep ? (sQ[tid] - na) * 256 + kPastCols : kNone;
+            const int32_t p = sQ[tid] & 1;
+            const int32_t past = (sQ[tid] - p) * 256 + kPastCols;
             auto to_key = [&](int32_t m, int cls) -> uint64_t {
-                if (m == kNone || (kPrep && m >= past)) return kNoMatch;
-                return make_key((uint32_t)((m >> 8) + na), colBeg + (uint32_t)(m & 0xFF) * 32u + (uint32_t)cls);
+                if (m == kNone || m >= past) return kNoMatch;
+                return make_key((uint32_t)((m >> 8) + p), colBeg + (uint32_t)(m & 0xFF) * 16u + (uint32_t)cls);
             };
-            const uint64_t k1 = to_key(b1, c1);
-            const bool direct = T.nSeg == 1;
-            if constexpr (kTop2) {
-                const uint64_t k2 = to_key(b2, c2);
-                if (direct) {
-                    T.bestA[row] = k1;
-                    T.secondA[row] = k2;
-                } else {
-                    uint64_t* p = T.part + ((size_t)seg * T.nA + row) * 2;
-                    p[0] = k1;
-                    p[1] = k2;
-                }
+            const uint64_t k1 = to_key(b1, c1), k2 = to_key(b2, c2);
+            if (T.nSeg == 1) {
+                T.bestA[row] = k1;
+                T.secondA[row] = k2;
             } else {
-                if (direct)
-                    T.bestA[row] = k1;
-                else if (k1 != kNoMatch)
-                    atomicMin(reinterpret_cast<unsigned long long*>(T.bestA + row), (unsigned long long)k1);
+                uint64_t* pp = T.part + ((size_t)seg * T.nA + row) * 2;
+                pp[0] = k1;
+                pp[1] = k2;
             }
         }
     }
 
-    // ---- prepared (top-2) form, train side. For column j this block's
-    // entry holds E = min over its rows of (d - p_i); e = min over all
-    // blocks (T.colMin). The lowest row of smallest d has d - p <= e + 1 (a
-    // row with E = e has d <= e + 1), so only blocks with E <= e + 1 hold it,
-    // and in such a block only the first wave named in the entry (all 8 when
-    // the entry's bit 3 is set). T.colMin still decreases while other blocks
-    // run, so this block takes every column with E <= colMin + 1 as read
-    // now: a superset of its true share. Each listed (column, wave) gets its
-    // exact (d, row) minimum over the wave's 32 rows on MFMA, merged into
-    // bestB by a 64-bit atomicMin.
-    if constexpr (kPrep) {
-        if (doCols) {
-            // the segment's candidates: column offset (16 bits) and wave mask
-            // (8 bits) per entry, kMaxSegCols entries at most
-            constexpr uint32_t kBuf = 96, kPer = kMaxSegCols / (kWaves * 64);
-            uint16_t* sListJ = reinterpret_cast<uint16_t*>(sU);
-            uint8_t* sListM = reinterpret_cast<uint8_t*>(sListJ + kMaxSegCols);
-            uint32_t* sBuf = reinterpret_cast<uint32_t*>(sListM + kMaxSegCols);  // [kWaves][kBuf]
-            static_assert(kMaxSegCols * 3 + kWaves * kBuf * 4 <= kUnionBytes, "resolution lists exceed the union");
-            __shared__ uint32_t sNres;  // entries listed (outside sU, which the row results may still read)
-            uint32_t* sN = &sNres;
-            const int32_t* part = T.colPart + (size_t)(rowBase / kBM) * T.nB;
-            uint32_t* buf = sBuf + wave * kBuf;
-            // one 32-column batch of this wave's list: d << 8 | local row
-            // = (u.w << 9) + (NA' << 8 | local row) + (NB' << 8)
-            auto resolve_batch = [&](uint32_t cnt) {
-                const uint32_t j = buf[min((uint32_t)l32, cnt - 1)];
-                const int32_t nb = T.NB[j];
-                int32_t key = kNone;
-                v16i acc = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    // ---- train side. For column j this block's entry holds E = min over its
+    // rows of (d - p_i); e = min over all blocks (T.colMin). The lowest row of
+    // smallest d has d - p <= e + 1 (a row with E = e has d <= e + 1), so
+    // only blocks with E <= e + 1 hold it, and in such a block only the first
+    // wave named in the entry (all 8 when its bit 3 is set). T.colMin still
+    // decreases while other blocks run, so this block takes every column
+    // with E <= colMin + 1 as read now: a superset of its true share. Each
+    // listed (column, wave) gets its exact (d, row) minimum over the wave's
+    // 32 rows on MFMA, merged into bestB by a 64-bit atomicMin.
+    if (!doCols) return;
+    constexpr uint32_t kBuf = 128, kPer = kMaxSegColsTop2 / (kWaves * 64);
+    uint16_t* sListJ = reinterpret_cast<uint16_t*>(sU);
+    uint8_t* sListM = reinterpret_cast<uint8_t*>(sListJ + kMaxSegColsTop2);
+    uint32_t* sBuf = reinterpret_cast<uint32_t*>(sListM + kMaxSegColsTop2);  // [kWaves][kBuf]
+    static_assert(kMaxSegColsTop2 * 3 + kWaves * kBuf * 4 <= kUnionBytes, "resolution lists exceed the ring");
+    const int32_t* part = T.colPart + (size_t)(rowBase / kBM) * T.nB;
+    uint32_t* buf = sBuf + wave * kBuf;
+    // one batch of up to 64 listed columns of this wave: lane L ends with
+    // column L's d << 8 | local row = (u.w << 9) + (NA' << 8 | local row) +
+    // (NB' << 8), minimised over the wave's rows
+    auto resolve_batch = [&](uint32_t cnt) {
+        int32_t cm[4];
 #pragma unroll
-                for (int kc = 0; kc < 4; kc++) {
-                    const int sl = 2 * kc + hi;
-                    const v4i a = *reinterpret_cast<const v4i*>(aRow + ((sl ^ aSw) << 4));
-                    const v4i b = *reinterpret_cast<const v4i*>(T.W + (size_t)j * 128 + sl * 16);
-                    acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b, acc, 0, 0, 0);
-                }
+        for (int cb = 0; cb < 4; cb++) {
+            const uint32_t jc = buf[min((uint32_t)(cb * 16 + c16), cnt - 1)];
+            v4i acc[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
 #pragma unroll
-                for (int g = 0; g < 4; g++) {
-                    const v4i q4 = *reinterpret_cast<const v4i*>(sQ + wave * 32 + 8 * g + 4 * hi);
+            for (int ks = 0; ks < 2; ks++) {
+                const v4i b = *reinterpret_cast<const v4i*>(T.W + (size_t)jc * 128 + (4 * ks + q) * 16);
 #pragma unroll
-                    for (int q = 0; q < 4; q++)
-                        key = imin(key, shl9_add(acc[4 * g + q], q4[q] * 256 + wave * 32 + 8 * g + 4 * hi + q));
-                }
-                key = imin(key, __shfl_xor(key, 32));
-                const uint32_t dk = (uint32_t)(key + nb);
-                if (hi == 0 && (uint32_t)l32 < cnt)
-                    atomicMin(reinterpret_cast<unsigned long long*>(T.bestB + j),
-                              (unsigned long long)make_key(dk >> 8, rowBase + (dk & 0xFF)));
-            };
-            if (tid == 0) *sN = 0;
-            // every load of the scan in flight at once
-            int32_t x[kPer], e[kPer];
-#pragma unroll
-            for (uint32_t q = 0; q < kPer; q++) {
-                const uint32_t j = colBeg + q * (kWaves * 64) + (uint32_t)tid;
-                x[q] = j < colEnd ? part[j] : INT32_MAX;
-                e[q] = j < colEnd ? T.colMin[j] : 0;
+                for (int rb = 0; rb < 2; rb++) acc[rb] = __builtin_amdgcn_mfma_i32_16x16x64_i8(aop[rb][ks], b, acc[rb], 0, 0, 0);
             }
-            __syncthreads();  // the row results' reads of sU, and *sN
+            int32_t k = kNone;
 #pragma unroll
-            for (uint32_t q = 0; q < kPer; q++) {
-                const bool cand = (x[q] >> 8) <= e[q] + 1;
-                const uint64_t m = __ballot(cand);
-                if (m == 0) continue;
-                uint32_t base = 0;
-                if (lane == 0) base = atomicAdd(sN, (uint32_t)__popcll(m));
-                base = (uint32_t)__shfl((int)base, 0);
-                if (cand) {
-                    const uint32_t pos = base + (uint32_t)__popcll(m & ((1ull << lane) - 1));
-                    sListJ[pos] = (uint16_t)(q * (kWaves * 64) + (uint32_t)tid);
-                    sListM[pos] = (uint8_t)((x[q] & 8) ? 0xFFu : (1u << (x[q] & 7)));
-                }
+            for (int rb = 0; rb < 2; rb++) {
+                const v4i q4 = *reinterpret_cast<const v4i*>(sQ + wave * 32 + rb * 16 + 4 * q);
+#pragma unroll
+                for (int j = 0; j < 4; j++)
+                    k = imin(k, shl9_add(acc[rb][j], q4[j] * 256 + wave * 32 + rb * 16 + 4 * q + j));
             }
-            __syncthreads();
-            const uint32_t n = *sN;
-            {
-                // this wave's entries, gathered 64 at a time into its buffer
-                uint32_t pend = 0;
-                for (uint32_t e0 = 0; e0 < n; e0 += 64) {
-                    const uint32_t ei = e0 + (uint32_t)lane;
-                    const bool sel = ei < n && ((sListM[ei] >> wave) & 1);
-                    const uint64_t m = __ballot(sel);
-                    if (sel) buf[pend + __popcll(m & ((1ull << lane) - 1))] = colBeg + sListJ[ei];
-                    pend += (uint32_t)__popcll(m);
-                    if (pend >= 32) {
-                        resolve_batch(32);
-                        // move the rest (at most 63) to the front
-                        const uint32_t rest = pend - 32;
-                        const uint32_t v = (uint32_t)lane < rest ? buf[32 + lane] : 0u;
-                        if ((uint32_t)lane < rest) buf[lane] = v;
-                        pend = rest;
-                        if (pend >= 32) {
-                            resolve_batch(32);
-                            const uint32_t rest2 = pend - 32;
-                            const uint32_t v2 = (uint32_t)lane < rest2 ? buf[32 + lane] : 0u;
-                            if ((uint32_t)lane < rest2) buf[lane] = v2;
-                            pend = rest2;
-                        }
-                    }
-                }
-                if (pend > 0) resolve_batch(pend);
-            }
+            cm[cb] = k;
+        }
+        const int32_t key = col_reduce(cm);
+        if ((uint32_t)lane < cnt) {
+            const uint32_t j = buf[lane];
+            const uint32_t dk = (uint32_t)(key + (T.NB[j] & ~0xFF));
+            atomicMin(reinterpret_cast<unsigned long long*>(T.bestB + j),
+                      (unsigned long long)make_key(dk >> 8, rowBase + (dk & 0xFF)));
+        }
+    };
+    if (tid == 0) sNres = 0;
+    // every load of the scan in flight at once
+    int32_t x[kPer], e[kPer];
+#pragma unroll
+    for (uint32_t t = 0; t < kPer; t++) {
+        const uint32_t j = colBeg + t * (kWaves * 64) + (uint32_t)tid;
+        x[t] = j < colEnd ? part[j] : INT32_MAX;
+        e[t] = j < colEnd ? T.colMin[j] : 0;
+    }
+    __syncthreads();  // the row results' reads of sU, and sNres
+#pragma unroll
+    for (uint32_t t = 0; t < kPer; t++) {
+        const bool cand = (x[t] >> 8) <= e[t] + 1;
+        const uint64_t m = __ballot(cand);
+        if (m == 0) continue;
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(&sNres, (uint32_t)__popcll(m));
+        base = (uint32_t)__shfl((int)base, 0);
+        if (cand) {
+            const uint32_t pos = base + (uint32_t)__popcll(m & ((1ull << lane) - 1));
+            sListJ[pos] = (uint16_t)(t * (kWaves * 64) + (uint32_t)tid);
+            sListM[pos] = (uint8_t)((x[t] & 8) ? 0xFFu : (1u << (x[t] & 7)));
         }
     }
+    __syncthreads();
+    const uint32_t n = sNres;
+    // this wave's entries, gathered 64 at a time into its buffer
+    uint32_t pend = 0;
+    for (uint32_t e0 = 0; e0 < n; e0 += 64) {
+        const uint32_t ei = e0 + (uint32_t)lane;
+        const bool sel = ei < n && ((sListM[ei] >> wave) & 1);
+        const uint64_t m = __ballot(sel);
+        if (sel) buf[pend + __popcll(m & ((1ull << lane) - 1))] = colBeg + sListJ[ei];
+        pend += (uint32_t)__popcll(m);
+        if (pend >= 64) {
+            resolve_batch(64);
+            // move the rest (at most 63) to the front
+            const uint32_t rest = pend - 64;
+            const uint32_t v = (uint32_t)lane < rest ? buf[64 + lane] : 0u;
+            if ((uint32_t)lane < rest) buf[lane] = v;
+            pend = rest;
+        }
+    }
+    if (pend > 0) resolve_batch(pend);
 }
 
 // grid launch: x = row block, y = segment, z = task (XCD-local order).
 // __launch_bounds__(512, 4): 512 threads, at least four waves per SIMD, i.e.
 // at most 128 VGPRs, so two workgroups fit a CU.
-template <int kBits, bool kTop2, bool kPrep>
+template <int kBits>
 __global__ __launch_bounds__(512, 4) void match_kernel(const TaskBatch batch)
 {
     const Wg3 wg = xcd_local_block();
@@ -606,7 +729,17 @@ __global__ __launch_bounds__(512, 4) void match_kernel(const TaskBatch batch)
     const uint32_t rowBase = wg.x * kBM;
     const uint32_t colBeg = wg.y * T.segCols;
     if (rowBase >= T.nA || colBeg >= T.nB) return;
-    match_tile<kBits, kTop2, kPrep>(T, rowBase, colBeg, wg.y);
+    match_tile<kBits>(T, rowBase, colBeg, wg.y);
+}
+
+__global__ __launch_bounds__(512, 4) void match_top2_kernel(const TaskBatch batch)
+{
+    const Wg3 wg = xcd_local_block();
+    const DevTask& T = batch.t[wg.z];
+    const uint32_t rowBase = wg.x * kBM;
+    const uint32_t colBeg = wg.y * T.segCols;
+    if (rowBase >= T.nA || colBeg >= T.nB) return;
+    match_tile_top2(T, rowBase, colBeg, wg.y);
 }
 
 // Persistent launch over a tile table: the tiles are split into kNumXcd lists
@@ -614,7 +747,6 @@ __global__ __launch_bounds__(512, 4) void match_kernel(const TaskBatch batch)
 // (the XCD the hardware dispatches it to), striding by the workgroups of that
 // XCD, so all tiles of one task share one L2. gridDim.x is a multiple of
 // kNumXcd.
-template <int kBits, bool kTop2>
 __global__ __launch_bounds__(512, 4) void match_tiles_kernel(const DevTask* __restrict__ tasks,
                                                           const TileDesc* __restrict__ tiles,
                                                           const uint32_t* __restrict__ xcdOff)
@@ -625,14 +757,13 @@ __global__ __launch_bounds__(512, 4) void match_tiles_kernel(const DevTask* __re
         const TileDesc d = tiles[i];
         const DevTask T = tasks[d.task];
         __syncthreads();  // the previous tile's last LDS reads
-        match_tile<kBits, kTop2, true>(T, d.rowBase, d.colBeg, d.seg);
+        match_tile_top2(T, d.rowBase, d.colBeg, d.seg);
     }
 }
 
 // The same tile table, one workgroup per tile: workgroup b takes element b / 8
 // of XCD list b % 8 (the XCD the hardware dispatches it to). The body is
-// match_tile alone, without the persistent loop's extra live registers.
-template <int kBits, bool kTop2>
+// the tile alone, without the persistent loop's extra live registers.
 __global__ __launch_bounds__(512, 4) void match_tile_grid_kernel(const DevTask* __restrict__ tasks,
                                                               const TileDesc* __restrict__ tiles,
                                                               const uint32_t* __restrict__ xcdOff)
@@ -642,7 +773,7 @@ __global__ __launch_bounds__(512, 4) void match_tile_grid_kernel(const DevTask* 
     if (i >= xcdOff[xcd + 1]) return;
     const TileDesc d = tiles[i];
     const DevTask T = tasks[d.task];
-    match_tile<kBits, kTop2, true>(T, d.rowBase, d.colBeg, d.seg);
+    match_tile_top2(T, d.rowBase, d.colBeg, d.seg);
 }
 
 template <class Src>
@@ -692,7 +823,9 @@ __global__ __launch_bounds__(256) void prep_train_kernel(const Src batch, bool i
     acc += dpp_xor2(acc);
     acc += dpp_mirror8(acc);
     if (sp == 0) {
-        T.NB[col] = valid ? acc * 256 : kPastCols;
+        // 16-column block within the segment, the low byte of a row key
+        const int32_t tag = (int32_t)((col % T.segCols) >> 4);
+        T.NB[col] = (valid ? acc * 256 : kPastCols) | tag;
         if (valid) T.colMin[col] = INT32_MAX;
     }
 }
@@ -754,7 +887,7 @@ struct Plan {
     uint32_t maxA = 0, maxB = 0;
 };
 
-static Plan make_plan(const DevTask* t, size_t n)
+static Plan make_plan(const DevTask* t, size_t n, uint32_t maxSegCols)
 {
     Plan p;
     for (size_t i = 0; i < n; i++) {
@@ -767,10 +900,10 @@ static Plan make_plan(const DevTask* t, size_t n)
     const uint32_t wg = p.gridX * (uint32_t)n;
     uint32_t nSeg = div_up(512, wg);
     nSeg = std::min(nSeg, std::max(1u, p.maxB / 1024));
-    nSeg = std::max(nSeg, div_up(p.maxB, kMaxSegCols));
+    nSeg = std::max(nSeg, div_up(p.maxB, maxSegCols));
     uint32_t seg = div_up(p.maxB, nSeg);
     seg = div_up(seg, kBN) * kBN;
-    p.segCols = std::min(seg, kMaxSegCols);
+    p.segCols = std::min(seg, maxSegCols);
     p.nSeg = div_up(p.maxB, p.segCols);
     return p;
 }
@@ -809,7 +942,7 @@ static int launch_top1(const rsfm_bf_task* tasks, size_t nbTasks, hipStream_t st
         const size_t n = std::min<size_t>(kMaxTasks, nbTasks - b);
         TaskBatch batch{};
         for (size_t i = 0; i < n; i++) batch.t[i] = to_dev(tasks[b + i]);
-        const Plan p = make_plan(batch.t, n);
+        const Plan p = make_plan(batch.t, n, kMaxSegCols);
         for (size_t i = 0; i < n; i++) {
             batch.t[i].nSeg = p.nSeg;
             batch.t[i].segCols = p.segCols;
@@ -819,7 +952,7 @@ static int launch_top1(const rsfm_bf_task* tasks, size_t nbTasks, hipStream_t st
         init_kernel<<<dim3(initX, 1, (uint32_t)n), 256, 0, stream>>>(batch, true);
         RSFM_CHECK(check_launch("bf init"));
         if (p.gridX == 0 || p.maxB == 0) continue;
-        match_kernel<kBits, false, false><<<dim3(p.gridX, p.nSeg, (uint32_t)n), 512, 0, stream>>>(batch);
+        match_kernel<kBits><<<dim3(p.gridX, p.nSeg, (uint32_t)n), 512, 0, stream>>>(batch);
         RSFM_CHECK(check_launch("bf match"));
     }
     return RSFM_OK;
@@ -832,7 +965,7 @@ static size_t top2_workspace(const rsfm_bf_top2_task* tasks, size_t nbTasks)
         const size_t n = std::min<size_t>(kMaxTasks, nbTasks - b);
         DevTask t[kMaxTasks];
         for (size_t i = 0; i < n; i++) t[i] = to_dev(tasks[b + i].base);
-        const Plan p = make_plan(t, n);
+        const Plan p = make_plan(t, n, kMaxSegColsTop2);
         for (size_t i = 0; i < n; i++) {
             if (p.nSeg > 1) bytes += align_up((size_t)p.nSeg * t[i].nA * 16, 256);
             bytes += prep_bytes(t[i].nA, t[i].nB);
@@ -845,7 +978,7 @@ int launch_match_tiles_top2(const DevTask* tasks, const TileDesc* tiles, const u
                             hipStream_t stream)
 {
     RSFM_RETURN_IF(grid == 0 || grid % kNumXcd != 0, RSFM_ERR_INVALID, "match tiles: grid must be a multiple of 8");
-    match_tiles_kernel<8, true><<<grid, 512, 0, stream>>>(tasks, tiles, xcdOff);
+    match_tiles_kernel<<<grid, 512, 0, stream>>>(tasks, tiles, xcdOff);
     return check_launch("bf match tiles");
 }
 
@@ -853,7 +986,7 @@ int launch_match_tile_grid_top2(const DevTask* tasks, const TileDesc* tiles, con
                                 hipStream_t stream)
 {
     if (maxList == 0) return RSFM_OK;
-    match_tile_grid_kernel<8, true><<<maxList * kNumXcd, 512, 0, stream>>>(tasks, tiles, xcdOff);
+    match_tile_grid_kernel<<<maxList * kNumXcd, 512, 0, stream>>>(tasks, tiles, xcdOff);
     return check_launch("bf match tile grid");
 }
 
@@ -912,7 +1045,7 @@ extern "C" int rsfm_bf_match_top2(const rsfm_bf_top2_task* tasks, size_t nbTasks
             batch.t[i] = to_dev(tasks[b + i].base);
             batch.t[i].secondA = reinterpret_cast<uint64_t*>(tasks[b + i].secondMatchA);
         }
-        const Plan p = make_plan(batch.t, n);
+        const Plan p = make_plan(batch.t, n, kMaxSegColsTop2);
         for (size_t i = 0; i < n; i++) {
             batch.t[i].nSeg = p.nSeg;
             batch.t[i].segCols = p.segCols;
@@ -932,7 +1065,7 @@ extern "C" int rsfm_bf_match_top2(const rsfm_bf_top2_task* tasks, size_t nbTasks
         prep_train_kernel<<<dim3(div_up(prepThreads, 256), 1, (uint32_t)n), 256, 0, stream>>>(batch, true);
         RSFM_CHECK(check_launch("bf top2 prep"));
         if (p.gridX == 0 || p.maxB == 0) continue;
-        match_kernel<8, true, true><<<dim3(p.gridX, p.nSeg, (uint32_t)n), 512, 0, stream>>>(batch);
+        match_top2_kernel<<<dim3(p.gridX, p.nSeg, (uint32_t)n), 512, 0, stream>>>(batch);
         RSFM_CHECK(check_launch("bf top2 match"));
         if (p.nSeg > 1) {
             merge_top2_kernel<<<dim3(div_up(p.maxA, 256), 1, (uint32_t)n), 256, 0, stream>>>(batch);

@@ -239,7 +239,7 @@ int tune(rsfm_pipeline* pl)
     if (pl->cfg.segCols)
         segs.push_back(pl->cfg.segCols);
     else
-        segs = {1024, 2048, 4096, 8192};
+        segs = {1024, 2048, 4096};
     if (pl->cfg.wgPerXcd)
         wgs.push_back(pl->cfg.wgPerXcd);
     else
@@ -343,8 +343,8 @@ extern "C" int rsfm_pipeline_create(const rsfm_pipeline_pair* pairs, size_t nbPa
     RSFM_RETURN_IF(cfg->mode != 0 && cfg->mode != 1, RSFM_ERR_INVALID, "pipeline: mode must be 0 or 1");
     RSFM_RETURN_IF(!(cfg->threshold > 0.f), RSFM_ERR_INVALID, "pipeline: threshold must be > 0");
     RSFM_RETURN_IF(cfg->ratioNum == 0 || cfg->ratioDen == 0, RSFM_ERR_INVALID, "pipeline: ratio must be > 0");
-    RSFM_RETURN_IF(cfg->segCols % bf::kBN != 0 || cfg->segCols > bf::kMaxSegCols, RSFM_ERR_INVALID,
-                   "pipeline: segCols must be 0 or a multiple of 64 up to 8192");
+    RSFM_RETURN_IF(cfg->segCols % bf::kBN != 0 || cfg->segCols > bf::kMaxSegColsTop2, RSFM_ERR_INVALID,
+                   "pipeline: segCols must be 0 or a multiple of 64 up to 4096");
     RSFM_RETURN_IF(cfg->wgPerXcd > 1024 && cfg->wgPerXcd != RSFM_PIPELINE_WG_PER_TILE, RSFM_ERR_INVALID,
                    "pipeline: wgPerXcd must be <= 1024 or RSFM_PIPELINE_WG_PER_TILE");
     RSFM_RETURN_IF(cfg->solver != RSFM_PIPELINE_SOLVER_F8 && cfg->solver != RSFM_PIPELINE_SOLVER_DEGENSAC,
