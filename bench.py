@@ -29,7 +29,7 @@ sys.path.insert(0, ROOT)
 
 N_KPTS = 8192
 INT8_PEAK_TOPS = 5000.0  # MI355X dense int8 MFMA, 5.0 POPS
-MATCH_KERNEL = "bf::match_kernel<8,true,true>"
+MATCH_KERNEL = "bf::match_top2_kernel"
 
 
 def parse():
@@ -49,6 +49,7 @@ def parse():
     ap.add_argument("--batched-pairs", type=int, default=256)
     ap.add_argument("--no-degensac", action="store_true", help="skip the batched DEGENSAC + pose run")
     ap.add_argument("--no-cashash", action="store_true", help="skip the cascade-hashing leg")
+    ap.add_argument("--no-aux", action="store_true", help="skip the top-1 / 4-bit / IDP matcher and affine-filter leg")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-proposal leg")
     ap.add_argument("--learned-train", type=int, default=10000, help="training pairs (1K-2K keypoints)")
     ap.add_argument("--learned-eval", type=int, default=1024, help="held-out 8K-keypoint pairs at 70%% outliers")
@@ -175,6 +176,75 @@ def degensac_batch(args, pairs, truth, info, log):
         "pairs": P,
         "create_s": round(t_create, 2),
     }
+
+
+def aux_leg(args, dev, log):
+    """The other matchers and the affine match filter at the benchmark's size:
+    top-1 bidirectional int8 MFMA (rsfm_bf_match_u8), 4-bit (rsfm_bf_match_u4,
+    descriptors halved first), packed-dot VALU (rsfm_bf_match_idp_u8), each on
+    distinct 8192 x 8192 pairs, and the batched affine filter
+    (rsfm_rmf_filter_batch) on 64 pairs of 2500 correspondences (60% on the
+    pair's geometry, the rest uniform over the image)."""
+    import numpy as np
+    import torch
+
+    import rapidsfm_amd as rs
+    from rapidsfm_amd import synth
+    from rapidsfm_amd.rmf import filter_batch
+
+    n = args.n
+
+    def timed(fn, reps):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    out = {}
+    base = [synth.make_pair(9000 + i, n, n, 0.3, 0.7, device=dev) for i in range(32)]
+    pairs = [(p["descA"], p["descB"]) for p in base]
+    ops = 2.0 * n * n * 128
+    for impl, P, reps in (("mfma", 32, 5), ("u4", 32, 5), ("idp", 4, 2)):
+        pp = pairs[:P]
+        if impl == "u4":
+            pp = [(rs.desc8b_to_4b(a), rs.desc8b_to_4b(b)) for a, b in pp]
+        ms = timed(lambda: rs.bf_match(pp, impl=impl), reps)
+        out["bf_" + impl] = {"pairs_per_call": P, "ms_per_call": round(ms, 4),
+                             "ms_per_pair": round(ms / P, 4),
+                             "tops": round(ops * P / (ms / 1e3) / 1e12, 1)}
+        log("aux %s: %.3f ms per %d pairs" % (impl, ms, P))
+    rng = np.random.default_rng(31)
+    probs = []
+    for i in range(64):
+        p = base[i % 32]
+        idx = np.nonzero(p["geo"])[0][:1500]
+        a = p["kpA"][idx, :2]
+        b = p["kpB"][p["src"][idx], :2]
+        k = 2500 - len(idx)
+        ra = rng.uniform((0, 0), (synth.WIDTH, synth.HEIGHT), (k, 2)).astype(np.float32)
+        rb = rng.uniform((0, 0), (synth.WIDTH, synth.HEIGHT), (k, 2)).astype(np.float32)
+        pts = np.concatenate([np.concatenate([a, b], 1), np.concatenate([ra, rb], 1)]).astype(np.float32)
+        probs.append(torch.from_numpy(np.ascontiguousarray(pts)).to(dev))
+    cnt = [torch.zeros(1, dtype=torch.int32, device=dev) for _ in probs]
+    work = [x.clone() for x in probs]
+
+    def rmf_once():
+        for w, x, c in zip(work, probs, cnt):
+            w.copy_(x)
+            c.fill_(x.shape[0])
+        filter_batch([(w, c, (int(synth.WIDTH), int(synth.HEIGHT)), 5 + j) for j, (w, c) in enumerate(zip(work, cnt))])
+
+    ms = timed(rmf_once, 5)
+    kept = sum(int(c.item()) for c in cnt)
+    out["rmf_batch"] = {"pairs_per_call": len(probs), "correspondences_per_pair": 2500, "ms_per_call": round(ms, 4),
+                        "ms_per_pair": round(ms / len(probs), 4), "kept_per_pair": round(kept / len(probs), 1)}
+    log("aux rmf: %.3f ms per %d pairs" % (ms, len(probs)))
+    return out
 
 
 def cashash_leg(args, dev, log):
@@ -532,6 +602,11 @@ def main():
             result["cashash"] = cashash_leg(args, dev, log)
         except Exception as e:
             result["cashash"] = {"error": repr(e)}
+    if not args.no_aux:
+        try:
+            result["aux"] = aux_leg(args, dev, log)
+        except Exception as e:
+            result["aux"] = {"error": repr(e)}
     if not args.no_learned:
         try:
             result["learned"] = learned_leg(args, dev, log)

@@ -372,10 +372,10 @@ __device__ __forceinline__ void match_tile(const DevTask& T, uint32_t rowBase, u
 //    the row top-2 per (row, class), v_min3 for the column minimum, and two
 //    lane swaps (v_permlane32_swap, v_permlane16_swap) that leave lane L with
 //    column L's minimum over the wave's rows;
-//  * train side: per column and row block the smallest R with the first wave
-//    holding it, a global per-column minimum by atomicMin, and at the end of
-//    the tile the exact distances of the candidate (column, wave) pairs on
-//    MFMA, merged into bestB by atomicMin.
+//  * train side: per column the block's smallest R with the first wave
+//    holding it (in LDS), a global per-column minimum by atomicMin, and at
+//    the end of the tile the exact distances of the candidate (column, wave)
+//    pairs on MFMA, merged into bestB by atomicMin.
 __device__ __forceinline__ void match_tile_top2(const DevTask& T, uint32_t rowBase, uint32_t colBeg, uint32_t seg)
 {
     constexpr int kWaves = 8;
@@ -395,6 +395,7 @@ __device__ __forceinline__ void match_tile_top2(const DevTask& T, uint32_t rowBa
     __shared__ int32_t sP[kPSlots][kBN];
     __shared__ int32_t sCol[kColSlots][kWaves][kCS];
     __shared__ __attribute__((aligned(16))) int32_t sQ[kBM];  // NA'
+    __shared__ int32_t sPart[kMaxSegColsTop2];                 // this block's train-side entries
     __shared__ uint32_t sNres;
 
     const int tid = threadIdx.x;
@@ -442,8 +443,8 @@ __device__ __forceinline__ void match_tile_top2(const DevTask& T, uint32_t rowBa
         if (wave == 0) dma4(T.NB + colBeg + c * kBN + lane, &sP[c % kPSlots][0]);
     };
     // wave k % 8 reduces chunk k's 8 wave minima per column (one lane per
-    // column): E = Rmin >> 8 to T.colMin by atomicMin, and to this block's
-    // row of T.colPart with the tag replaced by the first wave holding Rmin
+    // column): E = Rmin >> 8 to T.colMin by atomicMin, and to the block's
+    // entry in sPart with the tag replaced by the first wave holding Rmin
     // (bits 0-2) and bit 3 set when another wave comes within one unit
     // (E_w <= E + 1)
     auto flush = [&](uint32_t k) {
@@ -455,7 +456,7 @@ __device__ __forceinline__ void match_tile_top2(const DevTask& T, uint32_t rowBa
             top2_fold2(k1, k2, (src[w * kCS] & ~0xFF) | w, (src[(w + 1) * kCS] & ~0xFF) | (w + 1));
         const uint32_t gcol = colBeg + k * kBN + lane;
         if (gcol < colEnd) {
-            T.colPart[(size_t)(rowBase / kBM) * T.nB + gcol] = k1 | ((k2 >> 8) <= (k1 >> 8) + 1 ? 8 : 0);
+            sPart[k * kBN + lane] = k1 | ((k2 >> 8) <= (k1 >> 8) + 1 ? 8 : 0);
             atomicMin(T.colMin + gcol, k1 >> 8);
         }
     };
@@ -622,8 +623,8 @@ __device__ __forceinline__ void match_tile_top2(const DevTask& T, uint32_t rowBa
         }
     }
 
-    // ---- train side. For column j this block's entry holds E = min over its
-    // rows of (d - p_i); e = min over all blocks (T.colMin). The lowest row of
+    // ---- train side. For column j this block's entry (sPart) holds E = min
+    // over its rows of (d - p_i); e = min over all blocks (T.colMin). The lowest row of
     // smallest d has d - p <= e + 1 (a row with E = e has d <= e + 1), so
     // only blocks with E <= e + 1 hold it, and in such a block only the first
     // wave named in the entry (all 8 when its bit 3 is set). T.colMin still
@@ -637,7 +638,6 @@ __device__ __forceinline__ void match_tile_top2(const DevTask& T, uint32_t rowBa
     uint8_t* sListM = reinterpret_cast<uint8_t*>(sListJ + kMaxSegColsTop2);
     uint32_t* sBuf = reinterpret_cast<uint32_t*>(sListM + kMaxSegColsTop2);  // [kWaves][kBuf]
     static_assert(kMaxSegColsTop2 * 3 + kWaves * kBuf * 4 <= kUnionBytes, "resolution lists exceed the ring");
-    const int32_t* part = T.colPart + (size_t)(rowBase / kBM) * T.nB;
     uint32_t* buf = sBuf + wave * kBuf;
     // one batch of up to 64 listed columns of this wave: lane L ends with
     // column L's d << 8 | local row = (u.w << 9) + (NA' << 8 | local row) +
@@ -678,7 +678,7 @@ __device__ __forceinline__ void match_tile_top2(const DevTask& T, uint32_t rowBa
 #pragma unroll
     for (uint32_t t = 0; t < kPer; t++) {
         const uint32_t j = colBeg + t * (kWaves * 64) + (uint32_t)tid;
-        x[t] = j < colEnd ? part[j] : INT32_MAX;
+        x[t] = j < colEnd ? sPart[j - colBeg] : INT32_MAX;
         e[t] = j < colEnd ? T.colMin[j] : 0;
     }
     __syncthreads();  // the row results' reads of sU, and sNres
@@ -834,9 +834,7 @@ size_t prep_bytes(uint32_t nA, uint32_t nB)
 {
     if (nA == 0 || nB == 0) return 0;
     const size_t nPad = div_up(nB, (uint32_t)kBN) * (size_t)kBN;
-    const size_t nRB = div_up(nA, (uint32_t)kBM);
-    return align_up(nPad * 128, 256) + align_up(nPad * 4, 256) + align_up((size_t)nB * 4, 256) +
-           align_up(nRB * nB * 4, 256);
+    return align_up(nPad * 128, 256) + align_up(nPad * 4, 256) + align_up((size_t)nB * 4, 256);
 }
 
 void place_prep(DevTask& t, uint8_t* ws)
@@ -848,8 +846,6 @@ void place_prep(DevTask& t, uint8_t* ws)
     t.NB = reinterpret_cast<int32_t*>(ws);
     ws += align_up(nPad * 4, 256);
     t.colMin = reinterpret_cast<int32_t*>(ws);
-    ws += align_up((size_t)t.nB * 4, 256);
-    t.colPart = reinterpret_cast<int32_t*>(ws);
 }
 
 int launch_prep_table(const DevTask* tasks, uint32_t nTasks, uint32_t maxB, hipStream_t stream)

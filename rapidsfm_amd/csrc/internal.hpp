@@ -41,10 +41,8 @@ struct DevTask {
     // and |b|^2
     uint8_t* W;
     int32_t* NB;
-    // top-2 train side: per query row block of kBM rows and train column the
-    // block's smallest d - p with the wave holding it ([ceil(nA/kBM)][nB]),
-    // and per column the smallest over the blocks (merged by atomicMin)
-    int32_t* colPart;
+    // top-2 train side: per train column the smallest d - p over the query
+    // row blocks (merged by atomicMin)
     int32_t* colMin;
 };
 typedef ByValue<DevTask, kMaxTasks> TaskBatch;
@@ -62,9 +60,9 @@ int launch_match_tiles_top2(const DevTask* tasks, const TileDesc* tiles, const u
 int launch_match_tile_grid_top2(const DevTask* tasks, const TileDesc* tiles, const uint32_t* xcdOff, uint32_t maxList,
                                 hipStream_t stream);
 // workspace bytes of one top-2 task's staging form and train-side partials
-// (W, NB, colMin, colPart; 256-B aligned parts)
+// (W, NB, colMin; 256-B aligned parts)
 size_t prep_bytes(uint32_t nA, uint32_t nB);
-// point W, NB, colMin and colPart of t into ws (prep_bytes(t.nA, t.nB) bytes)
+// point W, NB and colMin of t into ws (prep_bytes(t.nA, t.nB) bytes)
 void place_prep(DevTask& t, uint8_t* ws);
 // staging form of B for the listed tasks (W, NB set, colMin cleared); maxB = largest nB
 int launch_prep_table(const DevTask* tasks, uint32_t nTasks, uint32_t maxB, hipStream_t stream);
