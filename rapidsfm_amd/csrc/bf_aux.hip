@@ -61,11 +61,26 @@ __global__ void desc8b_to_4b_kernel(uint8_t* __restrict__ dst, const uint8_t* __
 }
 
 // ---------------------------------------------------------------------------
-// Vector-ALU matcher (the IDP path): one query row per lane held in
-// registers, train descriptors staged through LDS 64 at a time, u8 dot
-// products with v_dot4_u32_u8, L2^2 = |a|^2 + |b|^2 - 2 a.b.
+// Vector-ALU matcher (the IDP path, bruteForceMatch.cu:560-620): u8 dot
+// products with v_dot4_u32_u8, L2^2 = |a|^2 + |b|^2 - 2 a.b, exact in u32.
+// A workgroup (256 threads) takes 128 query rows against a segment of 256
+// train columns in four tiles of 64. Both operands sit in LDS K-major
+// ([dword][row or column]); thread (tr = tid / 16, tc = tid % 16) owns rows
+// 8 tr .. 8 tr + 7 and columns 4 tc .. 4 tc + 3 of a tile, so per K-dword two
+// 16-byte reads of A and one of B feed 32 dot products.
+//  * row side: key distance << 8 | column within the segment (u32, so ties go
+//    to the lower column), a running minimum per row over the segment, then a
+//    16-lane butterfly and one 64-bit atomicMin per row and segment;
+//  * column side: key distance << 7 | row within the workgroup, minimum over
+//    the thread's 8 rows, the wave's 4 row groups by lane swaps and the 4
+//    waves through LDS, one 64-bit atomicMin per column and tile.
+// Positions past the end load zeros and take the norm 2^23, so their
+// distances are >= 2^23 > 128 * 255^2, the largest real one.
 constexpr int kIdpThreads = 256;
-constexpr int kIdpTile = 64;
+constexpr int kIdpTM = 128;   // query rows per workgroup
+constexpr int kIdpTN = 64;    // train columns per tile
+constexpr int kIdpSeg = 256;  // train columns per workgroup
+constexpr uint32_t kIdpPastNorm = 1u << 23;
 constexpr int kIdpMaxTasks = 32;
 
 struct IdpTask {
@@ -82,56 +97,129 @@ struct IdpBatch {
 __global__ __launch_bounds__(kIdpThreads) void idp_kernel(const IdpBatch batch)
 {
     const IdpTask& T = batch.t[blockIdx.z];
-    const uint32_t rowBase = blockIdx.x * kIdpThreads;
-    if (rowBase >= T.nA || T.nB == 0) return;
-    __shared__ uint32_t sB[kIdpTile][33];  // padded rows
-    __shared__ uint32_t sNB[kIdpTile];
-    __shared__ unsigned long long sColBest[kIdpTile];
-    const uint32_t row = rowBase + threadIdx.x;
-    const bool valid = row < T.nA;
-    uint32_t a[32];
-    uint32_t na = 0;
+    const uint32_t rowBase = blockIdx.x * kIdpTM;
+    const uint32_t colBeg = blockIdx.y * kIdpSeg;
+    if (rowBase >= T.nA || colBeg >= T.nB) return;
+    const uint32_t nTiles = (min(colBeg + kIdpSeg, T.nB) - colBeg + kIdpTN - 1) / kIdpTN;
+    __shared__ __attribute__((aligned(16))) uint32_t sA[32][kIdpTM];
+    __shared__ __attribute__((aligned(16))) uint32_t sB[2][32][kIdpTN];
+    __shared__ __attribute__((aligned(16))) uint32_t sNA[kIdpTM];
+    __shared__ __attribute__((aligned(16))) uint32_t sNB[2][kIdpTN];
+    __shared__ uint32_t sCol[2][4][kIdpTN];
+    const int tid = threadIdx.x, tr = tid >> 4, tc = tid & 15, lane = tid & 63, wave = tid >> 6;
+
+    // A: thread t stages 16 dwords of row t / 2 (the descriptors are only
+    // 4-byte aligned, so dword loads)
+    {
+        const uint32_t r = (uint32_t)tid >> 1, row = rowBase + r;
+        const bool valid = row < T.nA;
+        const uint32_t* src = reinterpret_cast<const uint32_t*>(T.A + (size_t)(valid ? row : 0u) * 128) + (tid & 1) * 16;
+        uint32_t x[16], na = 0;
 #pragma unroll
-    for (int q = 0; q < 32; q++) {
-        a[q] = valid ? reinterpret_cast<const uint32_t*>(T.A + (size_t)row * 128)[q] : 0u;
-        na = __builtin_amdgcn_udot4(a[q], a[q], na, false);
+        for (int k = 0; k < 16; k++) x[k] = valid ? src[k] : 0u;
+#pragma unroll
+        for (int k = 0; k < 16; k++) {
+            sA[(tid & 1) * 16 + k][r] = x[k];
+            na = __builtin_amdgcn_udot4(x[k], x[k], na, false);
+        }
+        na += (uint32_t)__shfl_xor((int)na, 1);
+        if ((tid & 1) == 0) sNA[r] = valid ? na : kIdpPastNorm;
     }
-    uint64_t best = kNoMatch;
-    for (uint32_t c0 = 0; c0 < T.nB; c0 += kIdpTile) {
-        __syncthreads();
-        // stage 64 descriptors: 256 lanes x 32 B
-        {
-            const uint32_t col = threadIdx.x >> 2, part = threadIdx.x & 3;
-            const uint32_t g = c0 + col;
+    // B tile of 64 columns: thread t loads 8 dwords of column t / 4
+    uint32_t xb[8];
+    auto load_b = [&](uint32_t tile) {
+        const uint32_t col = colBeg + tile * kIdpTN + ((uint32_t)tid >> 2);
+        const bool valid = col < T.nB;
+        const uint32_t* src = reinterpret_cast<const uint32_t*>(T.B + (size_t)(valid ? col : 0u) * 128) + (tid & 3) * 8;
 #pragma unroll
-            for (int q = 0; q < 8; q++)
-                sB[col][part * 8 + q] = g < T.nB ? reinterpret_cast<const uint32_t*>(T.B + (size_t)g * 128)[part * 8 + q] : 0u;
-            if (threadIdx.x < kIdpTile) sColBest[threadIdx.x] = kNoMatch;
+        for (int k = 0; k < 8; k++) xb[k] = valid ? src[k] : 0u;
+    };
+    auto store_b = [&](uint32_t tile, int buf) {
+        const uint32_t c = (uint32_t)tid >> 2;
+        uint32_t nb = 0;
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            sB[buf][(tid & 3) * 8 + k][c] = xb[k];
+            nb = __builtin_amdgcn_udot4(xb[k], xb[k], nb, false);
         }
-        __syncthreads();
-        if (threadIdx.x < kIdpTile) {
-            uint32_t nb = 0;
+        nb += (uint32_t)__shfl_xor((int)nb, 1);
+        nb += (uint32_t)__shfl_xor((int)nb, 2);
+        if ((tid & 3) == 0) sNB[buf][c] = colBeg + tile * kIdpTN + c < T.nB ? nb : kIdpPastNorm;
+    };
+    load_b(0);
+    store_b(0, 0);
+    __syncthreads();
+
+    uint32_t na[8];
 #pragma unroll
-            for (int q = 0; q < 32; q++) nb = __builtin_amdgcn_udot4(sB[threadIdx.x][q], sB[threadIdx.x][q], nb, false);
-            sNB[threadIdx.x] = nb;
+    for (int i = 0; i < 8; i++) na[i] = sNA[tr * 8 + i];
+    uint32_t rk[8];  // row tr * 8 + i: distance << 8 | column within the segment
+#pragma unroll
+    for (int i = 0; i < 8; i++) rk[i] = UINT32_MAX;
+
+    for (uint32_t t = 0; t < nTiles; t++) {
+        const int buf = (int)(t & 1);
+        if (t + 1 < nTiles) load_b(t + 1);
+        uint32_t acc[8][4];
+#pragma unroll
+        for (int i = 0; i < 8; i++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) acc[i][j] = 0;
+#pragma unroll 4
+        for (int k = 0; k < 32; k++) {
+            const uint4 a0 = *reinterpret_cast<const uint4*>(&sA[k][tr * 8]);
+            const uint4 a1 = *reinterpret_cast<const uint4*>(&sA[k][tr * 8 + 4]);
+            const uint4 bv = *reinterpret_cast<const uint4*>(&sB[buf][k][tc * 4]);
+            const uint32_t av[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
+            const uint32_t bw[4] = {bv.x, bv.y, bv.z, bv.w};
+#pragma unroll
+            for (int i = 0; i < 8; i++)
+#pragma unroll
+                for (int j = 0; j < 4; j++) acc[i][j] = __builtin_amdgcn_udot4(av[i], bw[j], acc[i][j], false);
         }
-        __syncthreads();
-        const uint32_t ncol = min((uint32_t)kIdpTile, T.nB - c0);
-        for (uint32_t j = 0; j < ncol; j++) {
-            uint32_t dot = 0;
+        const uint4 nbv = *reinterpret_cast<const uint4*>(&sNB[buf][tc * 4]);
+        const uint32_t nb[4] = {nbv.x, nbv.y, nbv.z, nbv.w};
+        uint32_t ck[4] = {UINT32_MAX, UINT32_MAX, UINT32_MAX, UINT32_MAX};
 #pragma unroll
-            for (int q = 0; q < 32; q++) dot = __builtin_amdgcn_udot4(a[q], sB[j][q], dot, false);
-            const uint32_t dist = na + sNB[j] - 2u * dot;
-            if (valid) {
-                best = min(best, make_key(dist, c0 + j));
-                if (T.bestB) atomicMin(&sColBest[j], (unsigned long long)make_key(dist, row));
+        for (int i = 0; i < 8; i++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const uint32_t d = na[i] + nb[j] - 2u * acc[i][j];
+                rk[i] = min(rk[i], (d << 8) | (t * kIdpTN + tc * 4 + j));
+                ck[j] = min(ck[j], (d << 7) | (uint32_t)(tr * 8 + i));
             }
+        // columns: the wave's 4 row groups (lanes 16 apart), then the 4 waves
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            ck[j] = min(ck[j], (uint32_t)__shfl_xor((int)ck[j], 16));
+            ck[j] = min(ck[j], (uint32_t)__shfl_xor((int)ck[j], 32));
         }
+        if (lane < 16) {
+#pragma unroll
+            for (int j = 0; j < 4; j++) sCol[buf][wave][tc * 4 + j] = ck[j];
+        }
+        if (t + 1 < nTiles) store_b(t + 1, buf ^ 1);
         __syncthreads();
-        if (T.bestB && threadIdx.x < ncol && sColBest[threadIdx.x] != kNoMatch)
-            atomicMin(reinterpret_cast<unsigned long long*>(T.bestB + c0 + threadIdx.x), sColBest[threadIdx.x]);
+        if (T.bestB != nullptr && tid < kIdpTN) {
+            const uint32_t m = min(min(sCol[buf][0][tid], sCol[buf][1][tid]), min(sCol[buf][2][tid], sCol[buf][3][tid]));
+            const uint32_t col = colBeg + t * kIdpTN + (uint32_t)tid;
+            if (col < T.nB && m < (1u << 30))
+                atomicMin(reinterpret_cast<unsigned long long*>(T.bestB + col),
+                          (unsigned long long)make_key(m >> 7, rowBase + (m & 127)));
+        }
     }
-    if (valid) T.bestA[row] = best;
+    // rows: minimum over the 16 column groups, then lane tc < 8 takes row tr * 8 + tc
+#pragma unroll
+    for (int i = 0; i < 8; i++)
+#pragma unroll
+        for (int o = 1; o < 16; o <<= 1) rk[i] = min(rk[i], (uint32_t)__shfl_xor((int)rk[i], o));
+    uint32_t mine = rk[0];
+#pragma unroll
+    for (int i = 1; i < 8; i++) mine = tc == i ? rk[i] : mine;
+    const uint32_t row = rowBase + tr * 8 + tc;
+    if (tc < 8 && row < T.nA && mine < (1u << 31))
+        atomicMin(reinterpret_cast<unsigned long long*>(T.bestA + row),
+                  (unsigned long long)make_key(mine >> 8, colBeg + (mine & 255)));
 }
 
 __global__ void idp_init_kernel(const IdpBatch batch)
@@ -349,7 +437,7 @@ extern "C" int rsfm_bf_match_idp_u8(const rsfm_bf_task* tasks, size_t nbTasks, h
         idp_init_kernel<<<dim3(initX, 1, (uint32_t)n), 256, 0, stream>>>(batch);
         RSFM_CHECK(check_launch("bf idp init"));
         if (maxA == 0 || maxB == 0) continue;
-        idp_kernel<<<dim3(div_up(maxA, kIdpThreads), 1, (uint32_t)n), kIdpThreads, 0, stream>>>(batch);
+        idp_kernel<<<dim3(div_up(maxA, kIdpTM), div_up(maxB, kIdpSeg), (uint32_t)n), kIdpThreads, 0, stream>>>(batch);
         RSFM_CHECK(check_launch("bf idp"));
     }
     return RSFM_OK;

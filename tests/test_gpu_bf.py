@@ -100,9 +100,10 @@ def test_extreme_values_and_ties(rs, orc, dev):
     assert np.array_equal(_np(ga), ra)
     assert np.array_equal(_np(gs), sa)
     assert np.array_equal(_np(gb), rb)
-    (ta, tb), = rs.bf_match([(_to_dev(A, dev), _to_dev(B, dev))])
-    assert np.array_equal(_np(ta), ra)
-    assert np.array_equal(_np(tb), rb)
+    for impl in ("mfma", "idp"):
+        (ta, tb), = rs.bf_match([(_to_dev(A, dev), _to_dev(B, dev))], impl=impl)
+        assert np.array_equal(_np(ta), ra), impl
+        assert np.array_equal(_np(tb), rb), impl
 
 
 def test_empty_sides(rs, orc, dev):
@@ -172,6 +173,8 @@ def test_8k_properties(rs, orc, dev):
     ra, sa, rb = orc.bf_match_top2_fast(query, train)
     assert np.array_equal(a, ra) and np.array_equal(s, sa) and np.array_equal(b, rb)
     assert np.array_equal(a, _np(ta)) and np.array_equal(b, _np(tb))
+    (pa, pb), = rs.bf_match([(qd, td)], impl="idp")
+    assert np.array_equal(a, _np(pa)) and np.array_equal(b, _np(pb))
     # determinism
     (ga2, gs2, gb2), = rs.bf_match_top2([(qd, td)])
     assert np.array_equal(a, _np(ga2)) and np.array_equal(s, _np(gs2)) and np.array_equal(b, _np(gb2))

@@ -115,6 +115,30 @@ def test_six_tables_and_many_tasks(ch, orc, dev):
         assert np.array_equal(_h(gt_, np.uint32), rt)
 
 
+def test_planted_ties_both_directions(ch, orc, dev):
+    """Duplicated descriptors on both sides: a duplicate lands in the same
+    buckets as its original, so every query that meets one meets both at the
+    same distance and the lower index must win, in both directions."""
+    _say("planted duplicate descriptors, 3000 x 2000, 6 tables")
+    w, base = orc.cashash_gaussian_weights(7, 192)
+    train, query, _ = orc.gen_sift_pair(4, 3000, 2000, 600, 0.05)
+    train, query = train.copy(), query.copy()
+    train[2000:2100] = train[0:100]
+    train[500:600] = train[2900:3000]
+    query[1500:1600] = query[10:110]
+    query[300:340] = query[1960:2000]
+    acc = orc.cashash_accumulate(train)
+    bias = orc.cashash_compute_bias(w, base, acc, 3000)
+    ti, tc, _ = _register(ch, orc, train, dev, w, bias)
+    qi, qc, _ = _register(ch, orc, query, dev, w, bias)
+    (gq, gt_), = ch.find_best_match([qi], ti, 6)
+    rq, rt = orc.cashash_find_best(query, qc, train, tc, 6)
+    assert np.array_equal(_h(gq, np.uint32), rq)
+    assert np.array_equal(_h(gt_, np.uint32), rt)
+    # the planted copies exist in the oracle's answer (ties were exercised)
+    assert np.isin(rq[:, 0], np.arange(0, 100)).any() and np.isin(rt[:, 0], np.arange(10, 110)).any()
+
+
 @pytest.mark.parametrize("keep", [8, 16])
 def test_config1_rerank_mode_8k(ch, orc, dev, keep):
     """Config 1: 8192 vs 8192, 30% noisy true matches (2458), 192-channel seeded
