@@ -7,35 +7,27 @@
 //
 // Design (own, for CDNA4):
 //  * u8 descriptors are mapped to signed bytes u = 127 - a (rows) and
-//    w = b - 128 (columns) so one v_mfma_i32_32x32x32_i8 chain gives
+//    w = b - 128 (columns) so one v_mfma_i32_16x16x64_i8 chain gives
 //    dot = u.w and  L2^2 = NA' + NB' + 2 dot  with NA' = |a-128|^2 and
-//    NB' = |b-127|^2 - 128 (both in [-128, 2^21]).
-//  * A workgroup keeps 256 query rows resident in LDS (one 32-row MFMA block
-//    per wave, K = 128 = 4 MFMAs each) and streams the train descriptors of
-//    its column segment through LDS in 64-column chunks (a ring of four,
-//    XOR-swizzled 16-B slots); two workgroups share a CU.
-//  * Top-2: the call first writes the train side in its staging form (w and
-//    NB' << 8 per column, padded to whole chunks) into the workspace, and the
-//    tile copies it into the LDS ring with direct global-to-LDS loads, so
-//    staging costs neither registers nor VALU in the main loop. Top-1 (no
-//    workspace) loads, transforms and stores each chunk itself.
-//  * Epilogue per accumulator, top-1 path: one v_lshl_add makes the row key
-//    ((NB'_j + 2dot) << 8 | column-block tag), one makes the column key
-//    ((NA'_i + 2dot) << 8 | local row); v_min3 folds top-1. Row state lives
-//    in registers for the whole segment and is reduced at the end through an
-//    LDS transpose (one thread per row). Column partials are reduced across
-//    the waves through LDS and merged globally with one 64-bit atomicMin per
-//    column per workgroup.
-//  * Top-2 path: the MFMA chain starts from H_i = NA'_i >> 1 instead of zero,
-//    so the single key R = (acc << 9) + (NB'_j << 8 | tag) = 256 (d - p_i) +
-//    tag (p_i = NA'_i & 1) orders each row's columns exactly and each
-//    column's rows up to the parity p_i. One v_lshl_add per accumulator,
-//    v_med3 + v_min3 for the row top-2 and v_min3 for the column minimum: the
-//    column side needs no key of its own. Per column and row block the
-//    kernel stores E = (smallest R) >> 8; two small kernels then list each
-//    column under the row blocks that can hold its best row (E <= e + 1) and
-//    recompute those blocks' exact distances to the listed columns on MFMA
-//    (resolve_block_kernel).
+//    NB' = |b-127|^2 - 128 (both in [-128, 2^21]); 4-bit descriptors are
+//    widened to bytes as they are loaded.
+//  * Every call first writes the train side in its staging form (w and
+//    NB' << 8 | tag per column, padded to whole 64-column chunks) into a
+//    workspace (the caller's for top-2, a stream-ordered allocation for
+//    top-1); the tile copies it into an LDS ring with direct global-to-LDS
+//    loads, so staging costs neither registers nor VALU in the main loop.
+//  * A workgroup is 8 waves x 32 query rows, their operands resident in
+//    registers, over a segment of at most 4096 train columns; two
+//    workgroups share a CU.
+//  * The MFMA chain starts from H_i = NA'_i >> 1 instead of zero, so the
+//    single key R = (acc << 9) + (NB'_j << 8 | tag) = 256 (d - p_i) + tag
+//    (p_i = NA'_i & 1) orders each row's columns exactly and each column's
+//    rows up to the parity p_i. One v_lshl_add per product; v_min3 folds
+//    the row best (v_med3 + v_min3 the row top-2) and the column minimum.
+//  * Train side: per column the workgroup keeps its smallest d - p (and the
+//    first wave holding it) in LDS and lowers a global per-column minimum
+//    by atomicMin; at the end of the tile every column within 1 of that
+//    minimum is recomputed exactly on MFMA and merged by a 64-bit atomicMin.
 //  * Segments split the train range when one launch has too few row blocks
 //    to fill 256 CUs; row results then merge by 64-bit atomicMin (top-1) or
 //    through a small workspace and a merge kernel (top-2).
@@ -50,7 +42,7 @@ namespace bf {
 
 constexpr int32_t kNone = INT32_MAX;
 
-// Prepared (top-2) column offset of the columns past nB, whose staged operands
+// Prepared column offset of the columns past nB, whose staged operands
 // are zero: with the accumulators started at H_i = NA'_i >> 1 such a column's
 // key is 256 (NA'_i - p_i) + kPastCols + tag. Over all u8 b,
 // |a - b|^2 - NA' <= 128 * 48896 (at a byte of 255), so kPastCols above
@@ -140,218 +132,9 @@ __device__ inline void dma4(const void* src, void* lds)
 // s_waitcnt vmcnt(0) (gfx9 encoding: expcnt and lgkmcnt left at their maxima)
 __device__ inline void wait_vm0() { __builtin_amdgcn_s_waitcnt(0x0F70); }
 
-// sCol row stride of the top-1 tile in words: its flush reads 8 consecutive
-// columns from each of the 8 waves; a stride of 72 puts those 64 words in
-// distinct banks
-constexpr int kColStride = 72;
-
-// Top-1 tile (rsfm_bf_match_u8 / _u4). One workgroup = 8 waves x 32 query
-// rows. The query tile (operands and column-key offsets) sits in LDS and the
-// column segment streams through an LDS ring in 64-column chunks, loaded,
-// transformed and stored by the tile itself (register staging). Per chunk a
-// wave issues its 8 MFMAs (K-step major over the two 32-column blocks,
-// operands read from LDS per K-step) into one accumulator pair, then folds
-// the keys of that chunk: one v_lshl_add makes the row key
-// ((NB'_j + 2dot) << 8 | column-block tag), one the column key
-// ((NA'_i + 2dot) << 8 | local row), v_min3 folds both. The wave fits in 128
-// VGPRs, so two workgroups (four waves per SIMD) share a CU. Column partials
-// of chunk c are written to LDS in iteration c and merged to global memory by
-// all waves (8 columns each, one 64-bit atomicMin per column) after the next
-// barrier.
-template <int kBits>
-__device__ __forceinline__ void match_tile(const DevTask& T, uint32_t rowBase, uint32_t colBeg, uint32_t seg)
-{
-    constexpr int kWaves = 8;
-    constexpr int kDescBytes = kBits * 16;
-    constexpr int kChunkBytes = kBN * 128;
-
-    const uint32_t colEnd = min(colBeg + T.segCols, T.nB);
-    const uint32_t nChunks = (colEnd - colBeg + kBN - 1) / kBN;
-
-    constexpr int kRing = 4, kColSlots = 4, kPSlots = 8;
-    constexpr int kTailStride = 33;
-    constexpr int kUnionBytes = (kRing * kChunkBytes > kBM * kTailStride * 4) ? kRing * kChunkBytes : kBM * kTailStride * 4;
-    __shared__ __attribute__((aligned(16))) uint8_t sU[kUnionBytes];
-    __shared__ __attribute__((aligned(16))) uint8_t sA[kBM * 128];
-    __shared__ int32_t sP[kPSlots][kBN];
-    __shared__ int32_t sCol[kColSlots][kWaves][kColStride];
-    __shared__ __attribute__((aligned(16))) int32_t sQ[kBM];  // NA' << 8 | local row, kNone past nA
-
-    const int tid = threadIdx.x;
-    const int lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const int l32 = lane & 31, hi = lane >> 5;
-    const int rloc = wave * 32 + l32;
-
-    // ---- query rows: operands u = 127 - a into LDS, offsets NA' = |a - 128|^2
-    {
-        const uint32_t row = rowBase + rloc;
-        const bool valid = row < T.nA;
-        int32_t nap = 0;
-#pragma unroll
-        for (int kc = 0; kc < 4; kc++) {
-            v4i x = {0, 0, 0, 0};
-            if (valid) x = load16<kBits>(T.A + (size_t)row * kDescBytes, kc * 32 + hi * 16);
-            v4i u;
-#pragma unroll
-            for (int q = 0; q < 4; q++) {
-                const int xa = x[q] ^ (int)0x80808080u;
-                nap = __builtin_amdgcn_sdot4(xa, xa, nap, false);
-                u[q] = valid ? (x[q] ^ 0x7F7F7F7F) : 0;
-            }
-            *reinterpret_cast<v4i*>(sA + rloc * 128 + (((2 * kc + hi) ^ swz(rloc)) << 4)) = u;
-        }
-        const int32_t na = nap + __shfl_xor(nap, 32);
-        if (hi == 0) sQ[rloc] = valid ? (na * 256 + rloc) : kNone;
-    }
-
-    int32_t m1[16];
-#pragma unroll
-    for (int v = 0; v < 16; v++) m1[v] = kNone;
-
-    const int sc = tid >> 3, sp = tid & 7;
-    auto stage_load = [&](uint32_t c) -> v4i {
-        const uint32_t gcol = min(colBeg + c * kBN + sc, colEnd - 1);
-        return load16<kBits>(T.B + (size_t)gcol * kDescBytes, sp * 16);
-    };
-    auto stage_store = [&](uint32_t c, const v4i& x) {
-        const bool valid = colBeg + c * kBN + sc < colEnd;
-        int32_t acc = 0;
-        v4i w;
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-            w[q] = valid ? (x[q] ^ (int)0x80808080u) : 0;
-            acc = __builtin_amdgcn_sdot4(w[q], w[q], acc, false);
-            acc = __builtin_amdgcn_sdot4(w[q], 0x02020202, acc, false);
-        }
-        *reinterpret_cast<v4i*>(sU + (c % kRing) * kChunkBytes + sc * 128 + ((sp ^ swz(sc)) << 4)) = w;
-        acc += dpp_xor1(acc);
-        acc += dpp_xor2(acc);
-        acc += dpp_mirror8(acc);
-        sP[c % kPSlots][sc] = valid ? (acc * 256 + (int32_t)(c * 2 + (sc >> 5))) : kNone;
-    };
-    auto flush = [&](uint32_t k) {
-        const int col = wave * 8 + (lane >> 3);
-        int32_t x = sCol[k % kColSlots][lane & 7][col];
-        x = imin(x, dpp_xor1(x));
-        x = imin(x, dpp_xor2(x));
-        x = imin(x, dpp_mirror8(x));
-        const uint32_t gcol = colBeg + k * kBN + col;
-        if ((lane & 7) == 0 && gcol < colEnd && x != kNone) {
-            const uint32_t dist = (uint32_t)((x >> 8) + (sP[k % kPSlots][col] >> 8));
-            const uint32_t row = rowBase + (uint32_t)(x & 0xFF);
-            atomicMin(reinterpret_cast<unsigned long long*>(T.bestB + gcol), (unsigned long long)make_key(dist, row));
-        }
-    };
-    const bool doCols = T.bestB != nullptr;
-    const uint8_t* aRow = sA + rloc * 128;
-    const int aSw = swz(rloc);
-
-    // chunk c: store chunk c+2 (loaded one chunk earlier), flush the previous
-    // trip's column partials (after the staging store, so that store's wait on
-    // its staged load does not also wait on these atomics), MFMAs of chunk c
-    // (K-step major over both column blocks), load chunk c+3, keys of chunk c
-    auto body = [&](uint32_t c, v4i& xs, bool sync, bool flushPrev) {
-        stage_store(c + 2, xs);
-        if (flushPrev) {
-            flush(c - 2);
-            flush(c - 1);
-        }
-        const uint8_t* sb = sU + (c % kRing) * kChunkBytes;
-        v16i acc[2];
-#pragma unroll
-        for (int kc = 0; kc < 4; kc++) {
-            const int s = 2 * kc + hi;
-            const v4i a = *reinterpret_cast<const v4i*>(aRow + ((s ^ aSw) << 4));
-#pragma unroll
-            for (int cb = 0; cb < 2; cb++) {
-                const int cl = cb * 32 + l32;
-                const v4i b = *reinterpret_cast<const v4i*>(sb + cl * 128 + ((s ^ swz(cl)) << 4));
-                const v16i zero = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-                acc[cb] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b, kc == 0 ? zero : acc[cb], 0, 0, 0);
-            }
-            __builtin_amdgcn_sched_barrier(0);
-        }
-        xs = stage_load(c + 3);
-        int32_t P[2];
-#pragma unroll
-        for (int cb = 0; cb < 2; cb++) P[cb] = sP[c % kPSlots][cb * 32 + l32];
-        int32_t cm0 = kNone, cm1 = kNone;
-#pragma unroll
-        for (int g = 0; g < 4; g++) {
-            const v4i q4 = *reinterpret_cast<const v4i*>(sQ + wave * 32 + 8 * g + 4 * hi);
-#pragma unroll
-            for (int j = 0; j < 4; j++) {
-                const int v = 4 * g + j;
-                const int32_t a0 = acc[0][v], a1 = acc[1][v];
-                m1[v] = imin(m1[v], imin(shl9_add(a0, P[0]), shl9_add(a1, P[1])));
-                cm0 = imin(cm0, shl9_add_asm(a0, q4[j]));
-                cm1 = imin(cm1, shl9_add_asm(a1, q4[j]));
-            }
-        }
-        const auto sw = __builtin_amdgcn_permlane32_swap((uint32_t)cm0, (uint32_t)cm1, false, false);
-        sCol[c % kColSlots][wave][lane] = imin((int32_t)sw[0], (int32_t)sw[1]);
-        if (sync) __syncthreads();
-    };
-
-    v4i xs = stage_load(0);
-    stage_store(0, xs);
-    xs = stage_load(1);
-    stage_store(1, xs);
-    xs = stage_load(2);
-    __syncthreads();
-
-    // two chunks per trip, barrier after the odd one; the column partials of a
-    // trip's chunks are complete after its barrier and flushed at the next trip
-    uint32_t c = 0;
-    for (; c + 2 <= nChunks; c += 2) {
-        body(c, xs, false, doCols && c >= 2);
-        body(c + 1, xs, true, false);
-    }
-    uint32_t firstUnflushed = c >= 2 ? c - 2 : 0;
-    if (c < nChunks) {
-        body(c, xs, false, doCols && c >= 2);
-        firstUnflushed = c;
-    }
-    __syncthreads();
-    if (doCols)
-        for (uint32_t k = firstUnflushed; k < nChunks; k++) flush(k);
-
-    // ---- row results: transpose the per-lane (row, column-class) bests through LDS
-    int32_t* L = reinterpret_cast<int32_t*>(sU);
-    __syncthreads();
-#pragma unroll
-    for (int v = 0; v < 16; v++) {
-        const int rl = wave * 32 + 8 * (v >> 2) + 4 * hi + (v & 3);
-        L[rl * kTailStride + l32] = m1[v];
-    }
-    __syncthreads();
-    if (tid < kBM) {
-        int32_t b1 = kNone;
-        int c1 = 0;
-        for (int cc = 0; cc < 32; cc++) {
-            const int32_t x = L[tid * kTailStride + cc];
-            if (x < b1) {
-                b1 = x;
-                c1 = cc;
-            }
-        }
-        const uint32_t row = rowBase + tid;
-        if (row < T.nA) {
-            const int32_t na = sQ[tid] >> 8;
-            const uint64_t k1 = b1 == kNone ? kNoMatch
-                                            : make_key((uint32_t)((b1 >> 8) + na),
-                                                       colBeg + (uint32_t)(b1 & 0xFF) * 32u + (uint32_t)c1);
-            if (T.nSeg == 1)
-                T.bestA[row] = k1;
-            else if (k1 != kNoMatch)
-                atomicMin(reinterpret_cast<unsigned long long*>(T.bestA + row), (unsigned long long)k1);
-        }
-    }
-    (void)seg;
-}
-
-// Top-2 tile over the prepared train form (rsfm_bf_match_top2 and the batched
-// pipeline), on v_mfma_i32_16x16x64_i8. One workgroup = 8 waves x 32 query
+// Matcher tile over the prepared train form (rsfm_bf_match_top2 and the
+// batched pipeline with kTop2; rsfm_bf_match_u8 / _u4 without: the row side
+// then keeps only its best), on v_mfma_i32_16x16x64_i8. One workgroup = 8 waves x 32 query
 // rows, two workgroups (four waves per SIMD) per CU. Lane l of a wave works
 // on column class l % 16 and row group q = l / 16:
 //  * query operands stay in registers: for row block rb (16 rows) and K-step
@@ -376,6 +159,7 @@ __device__ __forceinline__ void match_tile(const DevTask& T, uint32_t rowBase, u
 //    holding it (in LDS), a global per-column minimum by atomicMin, and at
 //    the end of the tile the exact distances of the candidate (column, wave)
 //    pairs on MFMA, merged into bestB by atomicMin.
+template <int kBits, bool kTop2>
 __device__ __forceinline__ void match_tile_top2(const DevTask& T, uint32_t rowBase, uint32_t colBeg, uint32_t seg)
 {
     constexpr int kWaves = 8;
@@ -410,7 +194,7 @@ __device__ __forceinline__ void match_tile_top2(const DevTask& T, uint32_t rowBa
         int32_t nap = 0;
 #pragma unroll
         for (int ks = 0; ks < 2; ks++) {
-            const v4i x = load16<8>(T.A + (size_t)row * 128, ks * 64 + q * 16);
+            const v4i x = load16<kBits>(T.A + (size_t)row * (kBits * 16), ks * 64 + q * 16);
 #pragma unroll
             for (int k = 0; k < 4; k++) {
                 const int xa = x[k] ^ (int)0x80808080u;
@@ -517,13 +301,18 @@ __device__ __forceinline__ void match_tile_top2(const DevTask& T, uint32_t rowBa
                     const int j = jp + h;
 #pragma unroll
                     for (int cb = 0; cb < 4; cb++) r[h][cb] = shl9_add(acc[rb][cb][j], P[cb]);
-                    // two pairs into the running top-2: m2 takes the second
-                    // of each pair's fold in one v_min3
-                    const int32_t t1 = med3(m1[rb][j], r[h][0], r[h][1]);
-                    asm("v_min3_i32 %0, %0, %1, %2" : "+v"(m1[rb][j]) : "v"(r[h][0]), "v"(r[h][1]));
-                    const int32_t t2 = med3(m1[rb][j], r[h][2], r[h][3]);
-                    asm("v_min3_i32 %0, %0, %1, %2" : "+v"(m1[rb][j]) : "v"(r[h][2]), "v"(r[h][3]));
-                    asm("v_min3_i32 %0, %0, %1, %2" : "+v"(m2[rb][j]) : "v"(t1), "v"(t2));
+                    if constexpr (kTop2) {
+                        // two pairs into the running top-2: m2 takes the
+                        // second of each pair's fold in one v_min3
+                        const int32_t t1 = med3(m1[rb][j], r[h][0], r[h][1]);
+                        asm("v_min3_i32 %0, %0, %1, %2" : "+v"(m1[rb][j]) : "v"(r[h][0]), "v"(r[h][1]));
+                        const int32_t t2 = med3(m1[rb][j], r[h][2], r[h][3]);
+                        asm("v_min3_i32 %0, %0, %1, %2" : "+v"(m1[rb][j]) : "v"(r[h][2]), "v"(r[h][3]));
+                        asm("v_min3_i32 %0, %0, %1, %2" : "+v"(m2[rb][j]) : "v"(t1), "v"(t2));
+                    } else {
+                        asm("v_min3_i32 %0, %0, %1, %2" : "+v"(m1[rb][j]) : "v"(r[h][0]), "v"(r[h][1]));
+                        asm("v_min3_i32 %0, %0, %1, %2" : "+v"(m1[rb][j]) : "v"(r[h][2]), "v"(r[h][3]));
+                    }
                 }
                 // column minima over the lane's rows, two rows per v_min3
 #pragma unroll
@@ -589,17 +378,21 @@ __device__ __forceinline__ void match_tile_top2(const DevTask& T, uint32_t rowBa
             }
         }
     }
-    __syncthreads();
+    if constexpr (kTop2) {
+        __syncthreads();
 #pragma unroll
-    for (int rb = 0; rb < 2; rb++)
+        for (int rb = 0; rb < 2; rb++)
 #pragma unroll
-        for (int j = 0; j < 4; j++) L[(wave * 32 + rb * 16 + 4 * q + j) * kTailStride + c16] = m2[rb][j];
-    __syncthreads();
+            for (int j = 0; j < 4; j++) L[(wave * 32 + rb * 16 + 4 * q + j) * kTailStride + c16] = m2[rb][j];
+        __syncthreads();
+    }
     if (tid < kBM) {
-        const int32_t x = L[tid * kTailStride + c1];
-        if (x < b2 || (x == b2 && c1 < c2)) {
-            b2 = x;
-            c2 = c1;
+        if constexpr (kTop2) {
+            const int32_t x = L[tid * kTailStride + c1];
+            if (x < b2 || (x == b2 && c1 < c2)) {
+                b2 = x;
+                c2 = c1;
+            }
         }
         const uint32_t row = rowBase + tid;
         if (row < T.nA) {
@@ -611,14 +404,22 @@ __device__ __forceinline__ void match_tile_top2(const DevTask& T, uint32_t rowBa
                 if (m == kNone || m >= past) return kNoMatch;
                 return make_key((uint32_t)((m >> 8) + p), colBeg + (uint32_t)(m & 0xFF) * 16u + (uint32_t)cls);
             };
-            const uint64_t k1 = to_key(b1, c1), k2 = to_key(b2, c2);
-            if (T.nSeg == 1) {
+            const uint64_t k1 = to_key(b1, c1);
+            if constexpr (kTop2) {
+                const uint64_t k2 = to_key(b2, c2);
+                if (T.nSeg == 1) {
+                    T.bestA[row] = k1;
+                    T.secondA[row] = k2;
+                } else {
+                    uint64_t* pp = T.part + ((size_t)seg * T.nA + row) * 2;
+                    pp[0] = k1;
+                    pp[1] = k2;
+                }
+            } else if (T.nSeg == 1) {
                 T.bestA[row] = k1;
-                T.secondA[row] = k2;
-            } else {
-                uint64_t* pp = T.part + ((size_t)seg * T.nA + row) * 2;
-                pp[0] = k1;
-                pp[1] = k2;
+            } else if (k1 != kNoMatch) {
+                // several segments: merged in place (results start at "no match")
+                atomicMin(reinterpret_cast<unsigned long long*>(T.bestA + row), (unsigned long long)k1);
             }
         }
     }
@@ -721,15 +522,16 @@ __device__ __forceinline__ void match_tile_top2(const DevTask& T, uint32_t rowBa
 // grid launch: x = row block, y = segment, z = task (XCD-local order).
 // __launch_bounds__(512, 4): 512 threads, at least four waves per SIMD, i.e.
 // at most 128 VGPRs, so two workgroups fit a CU.
+// Top-1 (rsfm_bf_match_u8 / _u4, 8- or 4-bit descriptors).
 template <int kBits>
-__global__ __launch_bounds__(512, 4) void match_kernel(const TaskBatch batch)
+__global__ __launch_bounds__(512, 4) void match_top1_kernel(const TaskBatch batch)
 {
     const Wg3 wg = xcd_local_block();
     const DevTask& T = batch.t[wg.z];
     const uint32_t rowBase = wg.x * kBM;
     const uint32_t colBeg = wg.y * T.segCols;
     if (rowBase >= T.nA || colBeg >= T.nB) return;
-    match_tile<kBits>(T, rowBase, colBeg, wg.y);
+    match_tile_top2<kBits, false>(T, rowBase, colBeg, wg.y);
 }
 
 __global__ __launch_bounds__(512, 4) void match_top2_kernel(const TaskBatch batch)
@@ -739,7 +541,7 @@ __global__ __launch_bounds__(512, 4) void match_top2_kernel(const TaskBatch batc
     const uint32_t rowBase = wg.x * kBM;
     const uint32_t colBeg = wg.y * T.segCols;
     if (rowBase >= T.nA || colBeg >= T.nB) return;
-    match_tile_top2(T, rowBase, colBeg, wg.y);
+    match_tile_top2<8, true>(T, rowBase, colBeg, wg.y);
 }
 
 // Persistent launch over a tile table: the tiles are split into kNumXcd lists
@@ -757,7 +559,7 @@ __global__ __launch_bounds__(512, 4) void match_tiles_kernel(const DevTask* __re
         const TileDesc d = tiles[i];
         const DevTask T = tasks[d.task];
         __syncthreads();  // the previous tile's last LDS reads
-        match_tile_top2(T, d.rowBase, d.colBeg, d.seg);
+        match_tile_top2<8, true>(T, d.rowBase, d.colBeg, d.seg);
     }
 }
 
@@ -773,42 +575,37 @@ __global__ __launch_bounds__(512, 4) void match_tile_grid_kernel(const DevTask* 
     if (i >= xcdOff[xcd + 1]) return;
     const TileDesc d = tiles[i];
     const DevTask T = tasks[d.task];
-    match_tile_top2(T, d.rowBase, d.colBeg, d.seg);
-}
-
-template <class Src>
-__global__ void init_kernel(const Src batch, bool initA)
-{
-    const DevTask& T = batch[blockIdx.z];
-    const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
-    if (initA && x < T.nA) T.bestA[x] = kNoMatch;
-    if (T.bestB != nullptr && x < T.nB) T.bestB[x] = kNoMatch;
+    match_tile_top2<8, true>(T, d.rowBase, d.colBeg, d.seg);
 }
 
 // staging form of B (DevTask::W, NB): eight lanes per column, one 16-B slot
-// each; a workgroup of 256 covers 32 columns and the padded column count is a
-// multiple of 64, so a workgroup is wholly inside or outside it. With
-// initResults the same threads also set the train-side results to "no match"
-// (they are merged by atomicMin) and, for a task without train descriptors,
-// the query-side results (no tile writes them).
-template <class Src>
-__global__ __launch_bounds__(256) void prep_train_kernel(const Src batch, bool initResults)
+// each (4-bit descriptors widened to bytes); a workgroup of 256 covers 32
+// columns and the padded column count is a multiple of 64, so a workgroup is
+// wholly inside or outside it. The same threads also set results to "no
+// match": kInitNone leaves them (the pipeline clears its own), kInitTop2 sets
+// the train-side results (merged by atomicMin) and, for a task without train
+// descriptors, the query-side ones (no tile writes them), kInitTop1 sets the
+// train-side and all query-side results (several segments merge them by
+// atomicMin).
+enum : int { kInitNone = 0, kInitTop2 = 1, kInitTop1 = 2 };
+template <int kBits, class Src>
+__global__ __launch_bounds__(256) void prep_train_kernel(const Src batch, int init)
 {
     const DevTask& T = batch[blockIdx.z];
     const uint32_t nPad = (T.nB + kBN - 1) / kBN * kBN;
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (initResults) {
+    if (init != kInitNone) {
         if (T.bestB != nullptr && t < T.nB) T.bestB[t] = kNoMatch;
-        if (T.nB == 0 && t < T.nA) {
+        if ((T.nB == 0 || init == kInitTop1) && t < T.nA) {
             T.bestA[t] = kNoMatch;
-            T.secondA[t] = kNoMatch;
+            if (T.secondA != nullptr) T.secondA[t] = kNoMatch;
         }
     }
     const uint32_t col = t >> 3, sp = t & 7;
     if (T.W == nullptr || col >= nPad) return;
     const bool valid = col < T.nB;
     v4i x = {0, 0, 0, 0};
-    if (valid) x = *reinterpret_cast<const v4i*>(T.B + (size_t)col * 128 + sp * 16);
+    if (valid) x = load16<kBits>(T.B + (size_t)col * (kBits * 16), sp * 16);
     int32_t acc = 0;
     v4i w;
 #pragma unroll
@@ -852,7 +649,7 @@ int launch_prep_table(const DevTask* tasks, uint32_t nTasks, uint32_t maxB, hipS
 {
     if (nTasks == 0 || maxB == 0) return RSFM_OK;
     const uint32_t nPad = div_up(maxB, (uint32_t)kBN) * kBN;
-    prep_train_kernel<<<dim3(nPad * 8 / 256, 1, nTasks), 256, 0, stream>>>(TaskTable{tasks}, false);
+    prep_train_kernel<8><<<dim3(nPad * 8 / 256, 1, nTasks), 256, 0, stream>>>(TaskTable{tasks}, kInitNone);
     return check_launch("bf prep table");
 }
 
@@ -929,6 +726,10 @@ static DevTask to_dev(const rsfm_bf_task& t)
     return d;
 }
 
+// Top-1 entry points: the tile needs the train sets' staging form, which the
+// reference interface has no workspace for, so the call takes it from the
+// stream-ordered allocator (hipMallocAsync / hipFreeAsync on the same stream:
+// a pooled block, no device-wide synchronisation).
 template <int kBits>
 static int launch_top1(const rsfm_bf_task* tasks, size_t nbTasks, hipStream_t stream)
 {
@@ -938,18 +739,33 @@ static int launch_top1(const rsfm_bf_task* tasks, size_t nbTasks, hipStream_t st
         const size_t n = std::min<size_t>(kMaxTasks, nbTasks - b);
         TaskBatch batch{};
         for (size_t i = 0; i < n; i++) batch.t[i] = to_dev(tasks[b + i]);
-        const Plan p = make_plan(batch.t, n, kMaxSegCols);
+        const Plan p = make_plan(batch.t, n, kMaxSegColsTop2);
+        size_t bytes = 0;
         for (size_t i = 0; i < n; i++) {
             batch.t[i].nSeg = p.nSeg;
             batch.t[i].segCols = p.segCols;
+            bytes += prep_bytes(batch.t[i].nA, batch.t[i].nB);
         }
-        const uint32_t initX = div_up(std::max(p.maxA, p.maxB), 256);
-        if (initX == 0) continue;
-        init_kernel<<<dim3(initX, 1, (uint32_t)n), 256, 0, stream>>>(batch, true);
-        RSFM_CHECK(check_launch("bf init"));
-        if (p.gridX == 0 || p.maxB == 0) continue;
-        match_kernel<kBits><<<dim3(p.gridX, p.nSeg, (uint32_t)n), 512, 0, stream>>>(batch);
-        RSFM_CHECK(check_launch("bf match"));
+        const uint32_t prepThreads = std::max(div_up(p.maxB, (uint32_t)kBN) * kBN * 8, std::max(p.maxA, p.maxB));
+        if (prepThreads == 0) continue;
+        void* ws = nullptr;
+        if (bytes > 0) RSFM_CHECK(check_hip(hipMallocAsync(&ws, bytes, stream), "bf match: staging workspace"));
+        uint8_t* w = static_cast<uint8_t*>(ws);
+        for (size_t i = 0; i < n; i++) {
+            place_prep(batch.t[i], w);
+            w += prep_bytes(batch.t[i].nA, batch.t[i].nB);
+        }
+        prep_train_kernel<kBits><<<dim3(div_up(prepThreads, 256), 1, (uint32_t)n), 256, 0, stream>>>(batch, kInitTop1);
+        int rc = check_launch("bf prep");
+        if (rc == RSFM_OK && p.gridX > 0 && p.maxB > 0) {
+            match_top1_kernel<kBits><<<dim3(p.gridX, p.nSeg, (uint32_t)n), 512, 0, stream>>>(batch);
+            rc = check_launch("bf match");
+        }
+        if (ws != nullptr) {
+            const int rf = check_hip(hipFreeAsync(ws, stream), "bf match: staging workspace release");
+            if (rc == RSFM_OK) rc = rf;
+        }
+        RSFM_CHECK(rc);
     }
     return RSFM_OK;
 }
@@ -1058,7 +874,7 @@ extern "C" int rsfm_bf_match_top2(const rsfm_bf_top2_task* tasks, size_t nbTasks
         // the staging form of every task's train descriptors
         const uint32_t prepThreads = std::max(div_up(p.maxB, (uint32_t)kBN) * kBN * 8, std::max(p.maxA, p.maxB));
         if (prepThreads == 0) continue;
-        prep_train_kernel<<<dim3(div_up(prepThreads, 256), 1, (uint32_t)n), 256, 0, stream>>>(batch, true);
+        prep_train_kernel<8><<<dim3(div_up(prepThreads, 256), 1, (uint32_t)n), 256, 0, stream>>>(batch, kInitTop2);
         RSFM_CHECK(check_launch("bf top2 prep"));
         if (p.gridX == 0 || p.maxB == 0) continue;
         match_top2_kernel<<<dim3(p.gridX, p.nSeg, (uint32_t)n), 512, 0, stream>>>(batch);

@@ -21,8 +21,7 @@ struct InTable {
 namespace bf {
 constexpr int kBM = 256;                // query rows per workgroup tile
 constexpr int kBN = 64;                 // train columns per staged chunk
-constexpr uint32_t kMaxSegCols = 8192;      // top-1: 256 blocks of 32, the 8-bit tag of a row key
-constexpr uint32_t kMaxSegColsTop2 = 4096;  // top-2: 256 blocks of 16
+constexpr uint32_t kMaxSegColsTop2 = 4096;  // columns per segment: 256 blocks of 16, the 8-bit tag of a row key
 constexpr int kMaxTasks = 32;
 
 struct DevTask {
@@ -34,8 +33,8 @@ struct DevTask {
     uint64_t* bestB;    // may be null
     uint64_t* part;     // top-2 with several segments: [nSeg][nA][2]
     uint32_t nSeg;      // column segments of this task (1: results written directly)
-    uint32_t segCols;   // columns per segment (multiple of kBN, <= kMaxSegCols; top-2 <= kMaxSegColsTop2)
-    // top-2 staging form of B, written by the preparation kernel: w = b - 128
+    uint32_t segCols;   // columns per segment (multiple of kBN, <= kMaxSegColsTop2)
+    // staging form of B, written by the preparation kernel: w = b - 128
     // as signed bytes for nB rounded up to kBN columns (zero past nB), per
     // column NB' << 8 with NB' = |b - 127|^2 - 128 (a fixed offset past nB)
     // and |b|^2
