@@ -32,6 +32,16 @@
 // The reference's adaptive iteration bound (degensac.cpp:230) is replaced by
 // the caller's fixed hypothesis count.
 //
+// Everything that body computes for a candidate depends on that candidate
+// alone; only whether it runs (the candidate beats the best so far) and what
+// it replaces depend on the earlier ones. So three launches: the nbCandidates
+// best samples of each pair (select_kernel), one workgroup per (pair,
+// candidate) that runs the whole body for its candidate and stores a record
+// (candidate_kernel), and one workgroup per pair that replays the records in
+// order with the reference's acceptance rules and writes the results and
+// masks (replay_kernel): the same outputs as one workgroup walking the
+// candidates in turn, with nbCandidates times the parallelism.
+//
 // GPU layout: every per-point pass is spread over the 256 lanes (ballot
 // inlier bits into LDS bitmaps, integer counts), the plane search scores 256
 // F per round like the hypothesis scorer (one wavefront per 64 F, lanes over
@@ -450,118 +460,192 @@ __device__ bool plane_search(Smem& s, const float4* pts, uint32_t n, uint32_t mO
     return true;
 }
 
+// the nCand best samples (count, then lowest index; count > 0) of each pair
+template <class Src>
+__global__ __launch_bounds__(kThreads) void select_kernel(const Src batch, const Params p)
+{
+    __shared__ uint64_t sKey[kThreads];
+    const DevTask& T = batch[blockIdx.x];
+    const uint32_t n = task_n(T);
+    uint64_t prevKey = ~0ull;
+    for (uint32_t c = 0; c < p.nCand; c++) {
+        uint64_t key = 0;
+        if (n >= 7)
+            for (uint32_t h = threadIdx.x; h < T.H; h += kThreads) {
+                const uint32_t cnt = T.counts[h];
+                const uint64_t k = ((uint64_t)cnt << 32) | (uint32_t)~h;
+                if (cnt > 0 && k < prevKey && k > key) key = k;
+            }
+        sKey[threadIdx.x] = key;
+        __syncthreads();
+        for (int off = kThreads / 2; off > 0; off >>= 1) {
+            if ((int)threadIdx.x < off) sKey[threadIdx.x] = max(sKey[threadIdx.x], sKey[threadIdx.x + off]);
+            __syncthreads();
+        }
+        key = sKey[0];
+        __syncthreads();
+        if (threadIdx.x == 0) T.cand[c].h = key == 0 ? ~0u : ~(uint32_t)key;
+        prevKey = key == 0 ? 0ull : key;  // nothing is below 0: the rest stay empty
+    }
+}
+
+// the per-new-best body of degensac.cpp:126-231 for candidate blockIdx.x of
+// pair blockIdx.y, as if it beat the best so far; its record in T.cand
 template <int kMode, class Src>
-__global__ __launch_bounds__(kThreads) void degensac_kernel(const Src batch, const Params p)
+__global__ __launch_bounds__(kThreads) void candidate_kernel(const Src batch, const Params p)
 {
     __shared__ Smem s;
+    const DevTask& T = batch[blockIdx.y];
+    const uint32_t c = blockIdx.x;
+    CandRec& R = T.cand[c];
+    const uint32_t h = R.h;
+    if (h == ~0u) return;
+    const uint32_t n = task_n(T);
+    const float4* pts = T.pts;
+    bool inRange = true;
+    for (int i = 0; i < 7; i++) inRange = inRange && T.idx[(size_t)h * 7 + i] < n;
+    if (threadIdx.x < 7) s.sample[threadIdx.x] = pts[inRange ? T.idx[(size_t)h * 7 + threadIdx.x] : 0];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float4 q[7];
+        for (int i = 0; i < 7; i++) q[i] = s.sample[i];
+        float Fs[3][9];
+        const int nm = inRange ? fgeom::fit7(q, Fs) : 0;
+        for (int m = 0; m < nm; m++)
+            for (int k = 0; k < 9; k++) s.models[m][k] = Fs[m][k];
+        s.nm = nm;
+    }
+    __syncthreads();
+    const int nm = s.nm;
+    uint32_t cntF = 0;
+    int bm = -1;
+    for (int m = 0; m < nm; m++) {
+        const uint32_t cm = optimise_f<kMode>(s, pts, n, s.models[m], p.tF2);
+        if (cm > cntF) {
+            cntF = cm;
+            bm = m;
+        }
+    }
+    if (threadIdx.x == 0) {
+        R.cntF = cntF;
+        R.planar = 0;
+        R.cntH = 0;
+        R.plane = 0;
+        R.cntFh = 0;
+    }
+    if (bm < 0) return;
+    copy9(s.bestF, s.models[bm]);
+    copy9(R.F, s.models[bm]);
+    __syncthreads();
+    // degeneracy test of the sample under its best F
+    if (threadIdx.x == 0) {
+        uint32_t nbPlanar = 3;
+        for (int i = 0; i < 5; i++) {
+            const float4 tri[3] = {s.sample[kTri[i][0]], s.sample[kTri[i][1]], s.sample[kTri[i][2]]};
+            float Hc[9];
+            if (!hgeom::h_from_f(s.bestF, tri, Hc)) continue;
+            uint32_t cc = 3;
+            for (int k = 0; k < 4; k++) cc += hgeom::inlier_h(Hc, s.sample[kRem[i][k]], p.tH2) ? 1u : 0u;
+            if (cc > nbPlanar) {
+                nbPlanar = cc;
+                for (int q = 0; q < 9; q++) s.curH[q] = Hc[q];
+            }
+        }
+        s.nbPlanar = nbPlanar;
+    }
+    __syncthreads();
+    if (s.nbPlanar < 5) return;
+    const uint32_t cntH = optimise_h(s, pts, n, p.tH2);
+    if (threadIdx.x == 0) {
+        R.planar = 1;
+        R.cntH = cntH;
+    }
+    copy9(R.H, s.curH);
+    const uint32_t mOff = prefix_scan(s, s.planar, true, n);
+    if (mOff < 2) return;
+    if (!plane_search<kMode>(s, pts, n, mOff, p, c)) return;
+    const uint32_t cFh = optimise_f<kMode>(s, pts, n, s.fh, p.tF2);
+    if (threadIdx.x == 0) {
+        R.plane = 1;
+        R.cntFh = cFh;
+    }
+    copy9(R.Fh, s.fh);
+}
+
+// the reference's sequence over the records (degensac.cpp:126-231): a
+// candidate runs its body only when its count beats the best so far, and its
+// H and plane-and-parallax F replace the best ones when they have more
+// inliers; then the results and both inlier masks
+template <int kMode, class Src>
+__global__ __launch_bounds__(kThreads) void replay_kernel(const Src batch, const Params p)
+{
+    __shared__ float sF[9], sH[9];
+    __shared__ uint32_t sRes[4];
     const DevTask& T = batch[blockIdx.x];
     const uint32_t n = task_n(T);
     const float4* pts = T.pts;
-    if (threadIdx.x < 9) {
-        s.bestF[threadIdx.x] = 0.f;
-        s.bestH[threadIdx.x] = 0.f;
-    }
-    uint32_t bestCntF = 0, bestCntH = 0, source = 0, bestHyp = ~0u;
-    uint64_t prevKey = ~0ull;
-    for (uint32_t cand = 0; n >= 7 && cand < p.nCand; cand++) {
-        // next sample: largest (count, lowest index) below the previous one
-        uint64_t key = 0;
-        for (uint32_t h = threadIdx.x; h < T.H; h += kThreads) {
-            const uint32_t c = T.counts[h];
-            const uint64_t k = ((uint64_t)c << 32) | (uint32_t)~h;
-            if (c > 0 && k < prevKey && k > key) key = k;
-        }
-        key = block_max(s, key);
-        if (key == 0) break;
-        prevKey = key;
-        const uint32_t h = ~(uint32_t)key;
-        bool inRange = true;
-        for (int i = 0; i < 7; i++) inRange = inRange && T.idx[(size_t)h * 7 + i] < n;
-        if (threadIdx.x < 7) s.sample[threadIdx.x] = pts[inRange ? T.idx[(size_t)h * 7 + threadIdx.x] : 0];
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            float4 q[7];
-            for (int i = 0; i < 7; i++) q[i] = s.sample[i];
-            float Fs[3][9];
-            const int nm = inRange ? fgeom::fit7(q, Fs) : 0;
-            for (int m = 0; m < nm; m++)
-                for (int k = 0; k < 9; k++) s.models[m][k] = Fs[m][k];
-            s.nm = nm;
-        }
-        __syncthreads();
-        const int nm = s.nm;
-        uint32_t cntF = 0;
-        int bm = -1;
-        for (int m = 0; m < nm; m++) {
-            const uint32_t c = optimise_f<kMode>(s, pts, n, s.models[m], p.tF2);
-            if (c > cntF) {
-                cntF = c;
-                bm = m;
+    if (threadIdx.x == 0) {
+        uint32_t bestCntF = 0, bestCntH = 0, source = 0, bestHyp = ~0u;
+        const float* bF = nullptr;
+        const float* bH = nullptr;
+        for (uint32_t c = 0; n >= 7 && c < p.nCand; c++) {
+            const CandRec& R = T.cand[c];
+            if (R.h == ~0u) break;
+            if (R.cntF <= bestCntF) continue;
+            bestCntF = R.cntF;
+            source = 0;
+            bestHyp = R.h;
+            bF = R.F;
+            if (!R.planar) continue;
+            if (R.cntH > bestCntH) {
+                bestCntH = R.cntH;
+                bH = R.H;
+            }
+            if (!R.plane) continue;
+            if (R.cntFh > bestCntF) {
+                bestCntF = R.cntFh;
+                source = 1;
+                bF = R.Fh;
             }
         }
-        if (bm < 0 || cntF <= bestCntF) continue;
-        bestCntF = cntF;
-        source = 0;
-        bestHyp = h;
-        copy9(s.bestF, s.models[bm]);
-        __syncthreads();
-        // degeneracy test of the sample under its best F
-        if (threadIdx.x == 0) {
-            uint32_t nbPlanar = 3;
-            for (int i = 0; i < 5; i++) {
-                const float4 tri[3] = {s.sample[kTri[i][0]], s.sample[kTri[i][1]], s.sample[kTri[i][2]]};
-                float Hc[9];
-                if (!hgeom::h_from_f(s.bestF, tri, Hc)) continue;
-                uint32_t c = 3;
-                for (int k = 0; k < 4; k++) c += hgeom::inlier_h(Hc, s.sample[kRem[i][k]], p.tH2) ? 1u : 0u;
-                if (c > nbPlanar) {
-                    nbPlanar = c;
-                    for (int q = 0; q < 9; q++) s.curH[q] = Hc[q];
-                }
-            }
-            s.nbPlanar = nbPlanar;
+        for (int q = 0; q < 9; q++) {
+            sF[q] = bF ? bF[q] : 0.f;
+            sH[q] = bH ? bH[q] : 0.f;
         }
-        __syncthreads();
-        if (s.nbPlanar < 5) continue;
-        const uint32_t cntH = optimise_h(s, pts, n, p.tH2);
-        if (cntH > bestCntH) {
-            bestCntH = cntH;
-            copy9(s.bestH, s.curH);
-            __syncthreads();
-        }
-        const uint32_t mOff = prefix_scan(s, s.planar, true, n);
-        if (mOff < 2) continue;
-        if (!plane_search<kMode>(s, pts, n, mOff, p, cand)) continue;
-        const uint32_t cFh = optimise_f<kMode>(s, pts, n, s.fh, p.tF2);
-        if (cFh > bestCntF) {
-            bestCntF = cFh;
-            source = 1;
-            copy9(s.bestF, s.fh);
-            __syncthreads();
-        }
+        sRes[0] = bestCntF;
+        sRes[1] = bestCntH;
+        sRes[2] = source;
+        sRes[3] = bestHyp;
     }
     __syncthreads();
     if (threadIdx.x < 9) {
-        T.F[threadIdx.x] = s.bestF[threadIdx.x];
-        T.Hm[threadIdx.x] = s.bestH[threadIdx.x];
+        T.F[threadIdx.x] = sF[threadIdx.x];
+        T.Hm[threadIdx.x] = sH[threadIdx.x];
     }
-    if (threadIdx.x == 0) {
-        T.result[0] = bestCntF;
-        T.result[1] = bestCntH;
-        T.result[2] = source;
-        T.result[3] = bestHyp;
-    }
+    if (threadIdx.x < 4) T.result[threadIdx.x] = sRes[threadIdx.x];
+    const uint32_t bestCntF = sRes[0], bestCntH = sRes[1];
     float F[9], Hh[9];
 #pragma unroll
     for (int q = 0; q < 9; q++) {
-        F[q] = s.bestF[q];
-        Hh[q] = s.bestH[q];
+        F[q] = sF[q];
+        Hh[q] = sH[q];
     }
     for (uint32_t i = threadIdx.x; i < T.nHost; i += kThreads) {
         const bool live = i < n;
         if (T.maskF) T.maskF[i] = (live && bestCntF > 0) ? (uint8_t)fgeom::inlier<kMode>(F, pts[i], p.tF2) : 0;
         if (T.maskH) T.maskH[i] = (live && bestCntH > 0) ? (uint8_t)hgeom::inlier_h(Hh, pts[i], p.tH2) : 0;
     }
+}
+
+template <int kMode, class Src>
+static int launch_all(const Src& src, uint32_t nTasks, const Params& p, hipStream_t stream)
+{
+    select_kernel<<<nTasks, kThreads, 0, stream>>>(src, p);
+    RSFM_CHECK(check_launch("degensac select"));
+    candidate_kernel<kMode><<<dim3(p.nCand, nTasks), kThreads, 0, stream>>>(src, p);
+    RSFM_CHECK(check_launch("degensac candidates"));
+    replay_kernel<kMode><<<nTasks, kThreads, 0, stream>>>(src, p);
+    return check_launch("degensac replay");
 }
 
 }  // namespace degensac
@@ -603,9 +687,7 @@ int launch_degensac_table(const DevTask* tasks, uint32_t nTasks, const Params& p
 {
     if (nTasks == 0) return RSFM_OK;
     const InTable<DevTask> src{tasks};
-    if (mode == 0) degensac_kernel<0><<<nTasks, kThreads, 0, stream>>>(src, p);
-    else degensac_kernel<1><<<nTasks, kThreads, 0, stream>>>(src, p);
-    return check_launch("degensac table");
+    return mode == 0 ? launch_all<0>(src, nTasks, p, stream) : launch_all<1>(src, nTasks, p, stream);
 }
 
 }  // namespace degensac
@@ -619,13 +701,19 @@ extern "C" int rsfm_degensac_refine(const rsfm_degensac_task* tasks, size_t nbTa
     RSFM_RETURN_IF(nbTasks > 0 && tasks == nullptr, RSFM_ERR_INVALID, "degensac: tasks is null");
     Params prm;
     RSFM_CHECK(make_params(thresholdF, thresholdH, mode, nbCandidates, nbPlaneSamples, seed, prm));
+    // the candidate records come from the stream-ordered allocator (the
+    // reference interface has no workspace)
     for (size_t b = 0; b < nbTasks; b += kMaxTasks) {
         const size_t n = std::min<size_t>(kMaxTasks, nbTasks - b);
         Batch batch{};
         for (size_t i = 0; i < n; i++) RSFM_CHECK(make_task(tasks[b + i], batch.t[i]));
-        if (mode == 0) degensac_kernel<0><<<(uint32_t)n, kThreads, 0, stream>>>(batch, prm);
-        else degensac_kernel<1><<<(uint32_t)n, kThreads, 0, stream>>>(batch, prm);
-        RSFM_CHECK(check_launch("degensac refine"));
+        void* ws = nullptr;
+        RSFM_CHECK(check_hip(hipMallocAsync(&ws, n * prm.nCand * sizeof(CandRec), stream), "degensac: candidate records"));
+        for (size_t i = 0; i < n; i++) batch.t[i].cand = static_cast<CandRec*>(ws) + i * prm.nCand;
+        int rc = mode == 0 ? launch_all<0>(batch, (uint32_t)n, prm, stream) : launch_all<1>(batch, (uint32_t)n, prm, stream);
+        const int rf = check_hip(hipFreeAsync(ws, stream), "degensac: candidate records release");
+        if (rc == RSFM_OK) rc = rf;
+        RSFM_CHECK(rc);
     }
     return RSFM_OK;
 }

@@ -208,6 +208,18 @@ int launch_pose_table(const DevTask* tasks, uint32_t nTasks, int kind, float fro
 }  // namespace pose
 
 namespace degensac {
+// what the per-new-best body of the reference computes for one candidate
+// sample; it depends on the candidate alone, so the candidates are worked in
+// parallel and their records replayed in order afterwards
+struct CandRec {
+    uint32_t h;       // sample (hypothesis) index; ~0u: no candidate
+    uint32_t cntF;    // optimised count of the sample's best F (0: none usable)
+    uint32_t planar;  // the degeneracy test put >= 5 of the 7 points on one H
+    uint32_t cntH;    // optimised count of that H
+    uint32_t plane;   // the plane-and-parallax search gave an F
+    uint32_t cntFh;   // its optimised count
+    float F[9], H[9], Fh[9];
+};
 struct DevTask {
     const float4* pts;
     const uint32_t* nDev;
@@ -220,6 +232,7 @@ struct DevTask {
     uint32_t* result;
     uint8_t* maskF;
     uint8_t* maskH;
+    CandRec* cand;  // [nCand] workspace
 };
 struct Params {
     float tF2, tH2;

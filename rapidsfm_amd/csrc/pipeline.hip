@@ -449,6 +449,7 @@ extern "C" int rsfm_pipeline_create(const rsfm_pipeline_pair* pairs, size_t nbPa
     const size_t oRmfWs = take(std::max<size_t>(1, mfBytes));
     const size_t oF7 = take(P * sizeof(ransac::DevTask));
     const size_t oDg = take(P * sizeof(degensac::DevTask));
+    const size_t oCand = take(degensac ? (size_t)P * cfg->nbCandidates * sizeof(degensac::CandRec) : 1);
     const size_t oUnd = take(P * sizeof(undist::PairTask));
     const size_t oPoseF = take(P * sizeof(pose::DevTask));
     const size_t oPoseH = take(P * sizeof(pose::DevTask));
@@ -538,6 +539,7 @@ extern "C" int rsfm_pipeline_create(const rsfm_pipeline_pair* pairs, size_t nbPa
                                                          x.H, x.solverResult, x.inlierMask, x.inlierMaskH},
                                       dg.back())) != RSFM_OK)
             return fail_out(rc);
+        dg.back().cand = reinterpret_cast<degensac::CandRec*>(pl->arena + oCand) + (size_t)(dg.size() - 1) * cfg->nbCandidates;
         if (x.posesF) {
             pf.emplace_back();
             if ((rc = pose::make_task(rsfm_pose_task{x.F, x.cam0, x.cam1, x.pts, x.nA, x.count, x.inlierMask,
