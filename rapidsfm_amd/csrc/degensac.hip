@@ -40,12 +40,15 @@
 // (candidate_kernel), and one workgroup per pair that replays the records in
 // order with the reference's acceptance rules and writes the results and
 // masks (replay_kernel): the same outputs as one workgroup walking the
-// candidates in turn, with nbCandidates times the parallelism.
+// candidates in turn, with nbCandidates times the parallelism. The
+// plane-and-parallax search of a planar candidate that can still lead is
+// itself split over workgroups of 16 samples (plane_kernel) and its winner
+// optimised after (plane_finish_kernel).
 //
 // GPU layout: every per-point pass is spread over the 256 lanes (ballot
-// inlier bits into LDS bitmaps, integer counts), the plane search scores 256
-// F per round like the hypothesis scorer (one wavefront per 64 F, lanes over
-// LDS point tiles), and the small dense solves run on one lane in FP64
+// inlier bits into LDS bitmaps, integer counts), the plane search keeps its
+// samples' F in LDS and every wavefront scores all of them over a quarter of
+// each LDS point tile, and the small dense solves run on one lane in FP64
 // (hgeom.hpp).
 #include "internal.hpp"
 #include "fgeom.hpp"
@@ -407,57 +410,57 @@ __device__ inline bool plane_sample(const Smem& s, const float4* pts, uint32_t n
     return hgeom::f_from_h(s.curH, pts[ia], pts[ib], F);
 }
 
-// plane-and-parallax search over the points off s.curH (mOff >= 2 of them):
-// best F in s.fh, false when no sample gave a usable F
+// plane-and-parallax samples per workgroup: a candidate's nbPlaneSamples are
+// split over ceil(nbPlaneSamples / kPlaneChunk) workgroups (16: the largest
+// pairs' searches set the launch's length, 64 per workgroup was 3x slower)
+constexpr uint32_t kPlaneChunk = 16;
+
+// scores the plane-and-parallax samples [s0, s1) (s1 - s0 <= kThreads) over
+// the points off s.curH (mOff >= 2 of them, inlier bits in s.planar, scanned
+// into s.prefix): the samples' F in LDS, every wave takes all of them over
+// its quarter of each point tile. Returns the best key count << 32 | ~sample
+// (0 when no sample gave a usable F).
 template <int kMode>
-__device__ bool plane_search(Smem& s, const float4* pts, uint32_t n, uint32_t mOff, const Params& p, uint32_t cand)
+__device__ uint64_t plane_score(Smem& s, const float4* pts, uint32_t n, uint32_t mOff, const Params& p, uint32_t cand,
+                                uint32_t s0, uint32_t s1)
 {
     PlaneScratch& ps = s.u.plane;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    uint64_t best = 0;
-    for (uint32_t chunk = 0; chunk < p.nPlane; chunk += kThreads) {
-        const uint32_t sidx = chunk + threadIdx.x;
+    const uint32_t nHyp = s1 - s0;
+    {
+        const uint32_t sidx = s0 + threadIdx.x;
         float F[9];
-        const bool ok = sidx < p.nPlane && plane_sample(s, pts, n, mOff, p.seed, cand, sidx, F);
+        const bool ok = threadIdx.x < nHyp && plane_sample(s, pts, n, mOff, p.seed, cand, sidx, F);
 #pragma unroll
         for (int q = 0; q < 9; q++) ps.Fh[threadIdx.x][q] = ok ? F[q] : 0.f;
         ps.ok[threadIdx.x] = ok;
         ps.cnt[threadIdx.x] = 0;
-        for (uint32_t base = 0; base < n; base += kTile) {
-            const uint32_t tn = min((uint32_t)kTile, n - base);
-            __syncthreads();
-            for (uint32_t i = threadIdx.x; i < tn; i += kThreads) ps.tile[i] = pts[base + i];
-            __syncthreads();
-            for (int hh = 0; hh < 64; hh++) {
-                const int hyp = wave * 64 + hh;
-                if (!ps.ok[hyp]) continue;
-                float G[9];
-#pragma unroll
-                for (int q = 0; q < 9; q++) G[q] = ps.Fh[hyp][q];
-                uint32_t cnt = 0;
-                for (uint32_t q = lane; q < tn; q += 64) {
-                    const uint32_t i = base + q;
-                    const bool off = !((s.planar[i >> 5] >> (i & 31)) & 1u);
-                    const bool in = off && fgeom::inlier<kMode>(G, ps.tile[q], p.tF2);
-                    cnt += (uint32_t)__popcll(__ballot(in));
-                }
-                if (lane == 0) ps.cnt[hyp] += cnt;
-            }
-        }
-        __syncthreads();
-        const uint64_t key = ps.ok[threadIdx.x] ? (((uint64_t)ps.cnt[threadIdx.x] << 32) | (uint32_t)~sidx) : 0ull;
-        const uint64_t k = block_max(s, key);
-        best = k > best ? k : best;
     }
-    if (best == 0) return false;
-    if (threadIdx.x == 0) {
-        float F[9];
-        plane_sample(s, pts, n, mOff, p.seed, cand, ~(uint32_t)best, F);
+    constexpr uint32_t kQuarter = kTile / 4;
+    for (uint32_t base = 0; base < n; base += kTile) {
+        const uint32_t tn = min((uint32_t)kTile, n - base);
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < tn; i += kThreads) ps.tile[i] = pts[base + i];
+        __syncthreads();
+        const uint32_t q0 = wave * kQuarter, q1 = min(q0 + kQuarter, tn);
+        for (uint32_t hh = 0; hh < nHyp; hh++) {
+            if (!ps.ok[hh]) continue;
+            float G[9];
 #pragma unroll
-        for (int q = 0; q < 9; q++) s.fh[q] = F[q];
+            for (int q = 0; q < 9; q++) G[q] = ps.Fh[hh][q];
+            uint32_t cnt = 0;
+            for (uint32_t q = q0 + lane; q < q1; q += 64) {
+                const uint32_t i = base + q;
+                const bool off = !((s.planar[i >> 5] >> (i & 31)) & 1u);
+                const bool in = off && fgeom::inlier<kMode>(G, ps.tile[q], p.tF2);
+                cnt += (uint32_t)__popcll(__ballot(in));
+            }
+            if (lane == 0 && cnt) atomicAdd(&ps.cnt[hh], cnt);
+        }
     }
     __syncthreads();
-    return true;
+    const uint64_t key = ps.ok[threadIdx.x] ? (((uint64_t)ps.cnt[threadIdx.x] << 32) | (uint32_t)~(s0 + threadIdx.x)) : 0ull;
+    return block_max(s, key);
 }
 
 // the nCand best samples (count, then lowest index; count > 0) of each pair
@@ -489,8 +492,9 @@ __global__ __launch_bounds__(kThreads) void select_kernel(const Src batch, const
     }
 }
 
-// the per-new-best body of degensac.cpp:126-231 for candidate blockIdx.x of
-// pair blockIdx.y, as if it beat the best so far; its record in T.cand
+// the per-new-best body of degensac.cpp:126-194 for candidate blockIdx.x of
+// pair blockIdx.y, as if it beat the best so far (through the optimised H);
+// its record in T.cand
 template <int kMode, class Src>
 __global__ __launch_bounds__(kThreads) void candidate_kernel(const Src batch, const Params p)
 {
@@ -532,6 +536,7 @@ __global__ __launch_bounds__(kThreads) void candidate_kernel(const Src batch, co
         R.cntH = 0;
         R.plane = 0;
         R.cntFh = 0;
+        R.planeKey = 0;
     }
     if (bm < 0) return;
     copy9(s.bestF, s.models[bm]);
@@ -561,9 +566,71 @@ __global__ __launch_bounds__(kThreads) void candidate_kernel(const Src batch, co
         R.cntH = cntH;
     }
     copy9(R.H, s.curH);
-    const uint32_t mOff = prefix_scan(s, s.planar, true, n);
+}
+
+// whether candidate c can become the best at all: the best count before it
+// is at least every earlier candidate's count, so it needs a count above all
+// of them (a necessary condition; the replay decides)
+__device__ inline bool may_lead(const DevTask& T, uint32_t c)
+{
+    uint32_t before = 0;
+    for (uint32_t k = 0; k < c; k++) before = max(before, T.cand[k].cntF);
+    return T.cand[c].cntF > before;
+}
+
+// the off-plane set of candidate c's optimised H again: inlier bits into
+// s.planar (s.curH = the stored H) and their complement scanned into
+// s.prefix; returns its size
+__device__ uint32_t off_plane(Smem& s, const CandRec& R, const float4* pts, uint32_t n, const Params& p)
+{
+    if (threadIdx.x < 9) s.curH[threadIdx.x] = R.H[threadIdx.x];
+    __syncthreads();
+    count_pass<2>(s, pts, n, s.curH, p.tH2, s.planar);
+    return prefix_scan(s, s.planar, true, n);
+}
+
+// the plane-and-parallax search (degensac.cpp:195-228) of a planar
+// candidate, its samples split over workgroups: workgroup x of pair y scores
+// samples [kPlaneChunk g, kPlaneChunk (g + 1)) of candidate x / nChunks
+// (g = x % nChunks) and merges its best key into the record
+template <int kMode, class Src>
+__global__ __launch_bounds__(kThreads) void plane_kernel(const Src batch, const Params p, uint32_t nChunks)
+{
+    __shared__ Smem s;
+    const DevTask& T = batch[blockIdx.y];
+    const uint32_t c = blockIdx.x / nChunks, g = blockIdx.x % nChunks;
+    CandRec& R = T.cand[c];
+    if (R.h == ~0u || !R.planar || !may_lead(T, c)) return;
+    const uint32_t n = task_n(T);
+    const uint32_t mOff = off_plane(s, R, T.pts, n, p);
     if (mOff < 2) return;
-    if (!plane_search<kMode>(s, pts, n, mOff, p, c)) return;
+    const uint32_t s0 = g * kPlaneChunk, s1 = min(s0 + kPlaneChunk, p.nPlane);
+    const uint64_t key = plane_score<kMode>(s, T.pts, n, mOff, p, c, s0, s1);
+    if (threadIdx.x == 0 && key != 0)
+        atomicMax(reinterpret_cast<unsigned long long*>(&R.planeKey), (unsigned long long)key);
+}
+
+// the best plane-and-parallax F of a candidate, optimised on all points
+template <int kMode, class Src>
+__global__ __launch_bounds__(kThreads) void plane_finish_kernel(const Src batch, const Params p)
+{
+    __shared__ Smem s;
+    const DevTask& T = batch[blockIdx.y];
+    const uint32_t c = blockIdx.x;
+    CandRec& R = T.cand[c];
+    if (R.h == ~0u || !R.planar || !may_lead(T, c)) return;
+    const uint64_t key = R.planeKey;
+    if (key == 0) return;
+    const uint32_t n = task_n(T);
+    const float4* pts = T.pts;
+    const uint32_t mOff = off_plane(s, R, pts, n, p);
+    if (threadIdx.x == 0) {
+        float F[9];
+        plane_sample(s, pts, n, mOff, p.seed, c, ~(uint32_t)key, F);
+#pragma unroll
+        for (int q = 0; q < 9; q++) s.fh[q] = F[q];
+    }
+    __syncthreads();
     const uint32_t cFh = optimise_f<kMode>(s, pts, n, s.fh, p.tF2);
     if (threadIdx.x == 0) {
         R.plane = 1;
@@ -644,6 +711,11 @@ static int launch_all(const Src& src, uint32_t nTasks, const Params& p, hipStrea
     RSFM_CHECK(check_launch("degensac select"));
     candidate_kernel<kMode><<<dim3(p.nCand, nTasks), kThreads, 0, stream>>>(src, p);
     RSFM_CHECK(check_launch("degensac candidates"));
+    const uint32_t nChunks = (p.nPlane + kPlaneChunk - 1) / kPlaneChunk;
+    plane_kernel<kMode><<<dim3(p.nCand * nChunks, nTasks), kThreads, 0, stream>>>(src, p, nChunks);
+    RSFM_CHECK(check_launch("degensac plane search"));
+    plane_finish_kernel<kMode><<<dim3(p.nCand, nTasks), kThreads, 0, stream>>>(src, p);
+    RSFM_CHECK(check_launch("degensac plane finish"));
     replay_kernel<kMode><<<nTasks, kThreads, 0, stream>>>(src, p);
     return check_launch("degensac replay");
 }

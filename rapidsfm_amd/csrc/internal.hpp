@@ -212,6 +212,7 @@ namespace degensac {
 // sample; it depends on the candidate alone, so the candidates are worked in
 // parallel and their records replayed in order afterwards
 struct CandRec {
+    uint64_t planeKey;  // best plane-and-parallax sample: count << 32 | ~index (0: none usable)
     uint32_t h;       // sample (hypothesis) index; ~0u: no candidate
     uint32_t cntF;    // optimised count of the sample's best F (0: none usable)
     uint32_t planar;  // the degeneracy test put >= 5 of the 7 points on one H
