@@ -83,7 +83,8 @@ int rsfm_sqrnorm_u8(uint32_t* sqrNorm, const void* desc, uint32_t nbDesc, hipStr
 /* replaces rsfm::launchBruteForceMatchIMMA(const SiftBruteForceMatchTask*, ...)
  * (bruteForceMatch.h:136, bruteForceMatchIMMA.cu:626-633): bidirectional top-1
  * L2^2, lowest index on ties; outputs start at {~0u, UINT32_MAX}. Any number
- * of tasks (internally batched). */
+ * of tasks (internally batched). The train sets' staging form (132 B per
+ * descriptor) comes from hipMallocAsync / hipFreeAsync on the stream. */
 int rsfm_bf_match_u8(const rsfm_bf_task* tasks, size_t nbTasks, hipStream_t stream);
 
 /* replaces rsfm::launchBruteForceMatchIDP (bruteForceMatch.h:118,
@@ -100,7 +101,8 @@ int rsfm_desc8b_to_4b(void* dst, const void* src, uint32_t nbDesc, hipStream_t s
 int rsfm_sqrnorm_u4(uint32_t* sqrNorm, const void* desc4, uint32_t nbDesc, hipStream_t stream);
 
 /* replaces rsfm::launchBruteForceMatchIMMA(const Sift4bBruteForceMatchTask*, ...)
- * (bruteForceMatch.h:134): descA/descB point to 64-B 4-bit descriptors */
+ * (bruteForceMatch.h:134): descA/descB point to 64-B 4-bit descriptors; the
+ * staging form as for rsfm_bf_match_u8 */
 int rsfm_bf_match_u4(const rsfm_bf_task* tasks, size_t nbTasks, hipStream_t stream);
 
 /* top-2 query side + top-1 train side (new). Needs a workspace of
@@ -410,7 +412,9 @@ int rsfm_ransac_f7(const rsfm_ransac_task* tasks, size_t nbTasks, float threshol
  * unit Frobenius, zero when none), result = {F inliers, H inliers, 1 when F
  * came from plane and parallax else 0, sample index of the best 7-point F or
  * ~0u}, and the optional inlier masks of F and H (DegensacSolution,
- * degensac.h). At most 65536 correspondences per pair; one workgroup per pair. */
+ * degensac.h). At most 65536 correspondences per pair; one workgroup per pair
+ * and candidate, whose 132-B records come from hipMallocAsync / hipFreeAsync on
+ * the stream. */
 typedef struct rsfm_degensac_task {
     const rsfm_pt_pair* pts;    /* nbPts correspondences, 16-byte aligned */
     uint32_t nbPts;             /* capacity / count when nbPtsDev is NULL */
