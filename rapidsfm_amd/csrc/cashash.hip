@@ -488,11 +488,14 @@ __global__ void match_init_kernel(const MatchBatch batch)
 // the Hamming distance of their 128-bit remap hashes (ties: lower index); the
 // kKeep first are re-ranked by exact L2^2 (ties: lower index).
 //
-// One wavefront per query. Lanes stride over the query's bucket in each table;
-// a train item already met in an earlier table (one of its first s code bytes
-// equals the query's) is skipped, so the union has no duplicates. Each lane
-// keeps its kKeep smallest keys (hamming << 24 | index) sorted in registers; kKeep
-// rounds of a wave minimum merge the 64 lists. The exact distances of the kept
+// One wavefront per query. Lanes stride over the query's bucket in each table,
+// reading the bucket-ordered records of the train image (its members' indices
+// and remap hashes, gathered once per call). Each lane keeps its kKeep smallest
+// distinct keys (hamming << 24 | index) sorted in registers: an item met again
+// in a later table has the same key, so it is either still in the list or
+// larger than the list's last key, and is not inserted twice. kKeep rounds of a
+// wave minimum merge the 64 lists; lanes holding the minimum all drop it, so an
+// item met by two lanes is selected once. The exact distances of the kept
 // candidates use eight lanes per candidate (16 bytes each, v_dot4 on u8) and a
 // wave minimum of (distance << 32 | index).
 struct RerankImage {
@@ -502,6 +505,11 @@ struct RerankImage {
     const uint32_t* ends;   // [8][256] inclusive prefix
     const uint32_t* buckets;
     uint32_t n;
+    // the members' remap hashes and indices in bucket order, [nSchemes][n]
+    // (rerank_recs_kernel), so that lanes striding over a bucket read
+    // consecutive entries
+    uint4* recRemap;
+    uint32_t* recIdx;
 };
 struct RerankTask {
     RerankImage q, t;  // queries matched against the train image t
@@ -529,6 +537,21 @@ __device__ inline uint64_t wave_min_u64(uint64_t x)
     return x;
 }
 
+// records of image blockIdx.y (0: the train image, 1 + i: query image i) in
+// bucket order, one per (table, position)
+struct RecBatch {
+    RerankImage im[kRerankMaxTasks + 1];
+};
+__global__ __launch_bounds__(256) void rerank_recs_kernel(const RecBatch batch, uint32_t nSchemes)
+{
+    const RerankImage& im = batch.im[blockIdx.y];
+    const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (im.recIdx == nullptr || x >= nSchemes * im.n) return;
+    const uint32_t j = im.buckets[x];
+    im.recIdx[x] = j;
+    im.recRemap[x] = im.remap[j];
+}
+
 template <int kKeep>
 __global__ __launch_bounds__(kRerankWaves * 64) void rerank_kernel(const RerankBatch batch, uint32_t nSchemes)
 {
@@ -545,18 +568,20 @@ __global__ __launch_bounds__(kRerankWaves * 64) void rerank_kernel(const RerankB
         const uint32_t code = (uint32_t)(qc >> (8 * s)) & 0xFF;
         const uint32_t end = T.t.ends[s * 256 + code];
         const uint32_t beg = code == 0 ? 0u : T.t.ends[s * 256 + code - 1];
-        // code bytes of the earlier tables; bytes s.. are forced nonzero below
-        const uint64_t upper = s == 0 ? ~0ull : (s >= 8 ? 0ull : (~0ull << (8 * s)));
+        const uint4* recR = T.t.recRemap + (size_t)s * T.t.n;
+        const uint32_t* recI = T.t.recIdx + (size_t)s * T.t.n;
         for (uint32_t p = beg + lane; p < end; p += 64) {
-            const uint32_t j = T.t.buckets[(size_t)s * T.t.n + p];
-            // zero byte in the first s bytes of (train codes ^ query codes): met before
-            const uint64_t x = (T.t.codes[j] ^ qc) | upper;
-            const bool seen = ((x - 0x0101010101010101ull) & ~x & 0x8080808080808080ull) != 0;
-            if (seen) continue;
-            const uint4 r = T.t.remap[j];
+            const uint32_t j = recI[p];
+            const uint4 r = recR[p];
             const uint32_t h = (uint32_t)(__popc(r.x ^ qr.x) + __popc(r.y ^ qr.y) + __popc(r.z ^ qr.z) +
                                           __popc(r.w ^ qr.w));
             uint32_t key = (h << 24) | j;
+            if (key >= top[kKeep - 1]) continue;
+            // an item met in an earlier table (same key) is listed once
+            bool dup = false;
+#pragma unroll
+            for (int k = 0; k < kKeep; k++) dup = dup || top[k] == key;
+            if (dup) continue;
 #pragma unroll
             for (int k = 0; k < kKeep; k++) {
                 const uint32_t lo = min(top[k], key);
@@ -792,20 +817,52 @@ extern "C" int rsfm_cashash_find_best_match_rerank(const rsfm_cashash_rerank_ima
                           reinterpret_cast<const uint4*>(im.remap), im.bucketEnds, im.buckets, im.nbDesc};
         return RSFM_OK;
     };
+    std::vector<RerankImage> imgs(1 + (size_t)nbTasks);  // the train image, then the query images
+    RSFM_CHECK(image(*train, imgs[0]));
+    for (uint32_t i = 0; i < nbTasks; i++) {
+        RSFM_CHECK(image(queries[i], imgs[1 + i]));
+        RSFM_RETURN_IF(imgs[1 + i].n > 0 && results[i].query == nullptr, RSFM_ERR_INVALID,
+                       "find_best_match_rerank: results.query is null");
+    }
+    // every image's members in bucket order (20 B per member and table), from
+    // the stream-ordered allocator
+    size_t nRecs = 0;
+    for (const auto& im : imgs) nRecs += (size_t)nbSchemes * im.n;
+    if (nRecs == 0) return RSFM_OK;
+    void* ws = nullptr;
+    RSFM_CHECK(check_hip(hipMallocAsync(&ws, nRecs * (sizeof(uint4) + sizeof(uint32_t)), stream),
+                         "find_best_match_rerank: records"));
+    uint4* rr = static_cast<uint4*>(ws);
+    uint32_t* ri = reinterpret_cast<uint32_t*>(rr + nRecs);
+    for (auto& im : imgs) {
+        im.recRemap = im.n > 0 ? rr : nullptr;
+        im.recIdx = im.n > 0 ? ri : nullptr;
+        rr += (size_t)nbSchemes * im.n;
+        ri += (size_t)nbSchemes * im.n;
+    }
+    int rc = RSFM_OK;
+    for (size_t b = 0; b < imgs.size() && rc == RSFM_OK; b += kRerankMaxTasks + 1) {
+        const size_t n = std::min<size_t>(kRerankMaxTasks + 1, imgs.size() - b);
+        RecBatch rb{};
+        uint32_t maxN = 0;
+        for (size_t i = 0; i < n; i++) {
+            rb.im[i] = imgs[b + i];
+            maxN = std::max(maxN, imgs[b + i].n);
+        }
+        if (maxN == 0) continue;
+        rerank_recs_kernel<<<dim3(div_up(nbSchemes * maxN, 256), (uint32_t)n), 256, 0, stream>>>(rb, nbSchemes);
+        rc = check_launch("find_best_match_rerank records");
+    }
     // every task's query direction, then (where asked) its train direction
     std::vector<RerankTask> all;
-    RerankImage t{};
-    RSFM_CHECK(image(*train, t));
+    const RerankImage& t = imgs[0];
     for (uint32_t i = 0; i < nbTasks; i++) {
-        RerankImage q{};
-        RSFM_CHECK(image(queries[i], q));
-        RSFM_RETURN_IF(q.n > 0 && results[i].query == nullptr, RSFM_ERR_INVALID,
-                       "find_best_match_rerank: results.query is null");
+        const RerankImage& q = imgs[1 + i];
         if (q.n > 0) all.push_back(RerankTask{q, t, reinterpret_cast<uint64_t*>(results[i].query)});
         if (results[i].train != nullptr && t.n > 0)
             all.push_back(RerankTask{t, q, reinterpret_cast<uint64_t*>(results[i].train)});
     }
-    for (size_t b = 0; b < all.size(); b += kRerankMaxTasks) {
+    for (size_t b = 0; b < all.size() && rc == RSFM_OK; b += kRerankMaxTasks) {
         const size_t n = std::min<size_t>(kRerankMaxTasks, all.size() - b);
         RerankBatch batch{};
         uint32_t maxN = 0;
@@ -817,7 +874,8 @@ extern "C" int rsfm_cashash_find_best_match_rerank(const rsfm_cashash_rerank_ima
         if (keep == 8) rerank_kernel<8><<<grid, kRerankWaves * 64, 0, stream>>>(batch, nbSchemes);
         else if (keep == 16) rerank_kernel<16><<<grid, kRerankWaves * 64, 0, stream>>>(batch, nbSchemes);
         else rerank_kernel<32><<<grid, kRerankWaves * 64, 0, stream>>>(batch, nbSchemes);
-        RSFM_CHECK(check_launch("find_best_match_rerank"));
+        rc = check_launch("find_best_match_rerank");
     }
-    return RSFM_OK;
+    const int rf = check_hip(hipFreeAsync(ws, stream), "find_best_match_rerank: records release");
+    return rc != RSFM_OK ? rc : rf;
 }
