@@ -81,15 +81,6 @@ __device__ inline uint64_t umax64(uint64_t a, uint64_t b) { return a > b ? a : b
 
 // (a << 9) + b: one v_lshl_add_u32
 __device__ inline int32_t shl9_add(int32_t a, int32_t b) { return (int32_t)(((uint32_t)a << 9) + (uint32_t)b); }
-// The same as asm. The row and the column key shift the same accumulator; with
-// both in plain code the compiler shares one shift and adds twice (three
-// instructions for two keys), so the column key uses this form.
-__device__ inline int32_t shl9_add_asm(int32_t a, int32_t b)
-{
-    int32_t r;
-    asm("v_lshl_add_u32 %0, %1, 9, %2" : "=v"(r) : "v"(a), "v"(b));
-    return r;
-}
 // middle of three: one v_med3_i32
 __device__ inline int32_t med3(int32_t a, int32_t b, int32_t c) { return imax(imin(a, b), imin(imax(a, b), c)); }
 // Fold two new keys x0, x1 into a running top-2 (m1 < m2). The smallest of

@@ -16,12 +16,13 @@
 //    already in the code-byte / remap-word bit order.
 //  * histogram, bounds and buckets use LDS atomics and one global reservation
 //    per (scheme, bucket) per workgroup.
-//  * matching packs each (scheme, bucket) candidate block into 32 x 32 MFMA
-//    tiles (average bucket size at 8K descriptors is 32, so a tile is mostly
-//    full, against 128 x 128 tiles in the reference); one wave per
-//    (task, scheme, bucket); the distance tile is transposed through LDS and
-//    each lane owns one query row or one train column, merged with 64-bit
-//    atomicMin on (distance, global index) keys.
+//  * matching cuts each (scheme, bucket) candidate set into 64 x 64 blocks
+//    on v_mfma_i32_16x16x64_i8 (against 128 x 128 tiles in the reference),
+//    balanced over all waves of the launch; row and column minima are folded
+//    in registers with 32-bit (distance, position) keys inside a block and
+//    merged across blocks with 64-bit atomicMin on (distance, global index)
+//    keys: once per query row and run of blocks, once per train column and
+//    block.
 #include "common.hpp"
 #include <algorithm>
 #include <vector>
@@ -290,186 +291,304 @@ struct MatchBatch {
     DevImage t;
 };
 
-__device__ inline void bucket_range(const DevImage& im, uint32_t s, uint32_t b, uint32_t& beg, uint32_t& cnt)
-{
-    const uint32_t e = im.ends[s * 256 + b];
-    beg = b == 0 ? 0u : im.ends[s * 256 + b - 1];
-    cnt = e - beg;
-}
+// Work split. Bucket sizes are far from uniform (with the reference's
+// projection construction the largest of 256 buckets can hold half an
+// image), so a task is cut into blocks of at most 64 query x 64 train
+// members of one bucket, ordered (scheme, bucket, query chunk, train chunk),
+// and every wave of the launch takes an even slice of its task's blocks. A
+// wave keeps its query chunk (operands in registers, row minima as
+// (distance, index) keys) while its consecutive blocks share it and merges
+// each row into the result once per run; each block's train-side minima are
+// merged once per train member. Per block, on v_mfma_i32_16x16x64_i8 (lane
+// l: column class l % 16, row group q = l / 16; products of rows
+// rb * 16 + 4 q + j and columns cb * 16 + l % 16):
+//  * both chunks' member indices are sorted in the wave (bitonic over the 64
+//    lanes), so a position inside a chunk orders like the global index and
+//    32-bit keys (distance << 6 | position) break ties as the reference does;
+//  * operands are gathered straight into registers: u = 127 - a for query
+//    rows, w = b - 128 for train columns (zero for positions past the
+//    chunk), L2^2 = NA' + NB' + 2 u.w with NA' = |a - 128|^2 and
+//    NB' = |b - 127|^2 - 128;
+//  * row key (acc << 7) + (NB' << 6 | column) and column key
+//    (acc << 7) + (NA' << 6 | row), one v_lshl_add each; a row's minimum over
+//    the lane's 4 columns goes through a per-wave LDS transpose, after which
+//    lane L takes row L's 16 class minima and adds NA'; a column's minimum
+//    over the lane's 16 rows, two lane swaps that leave lane L with column L,
+//    then NB' added.
+constexpr int kMatchWaves = 4;
+constexpr int kRowStride = 17;
+constexpr int32_t kPastChunk = 0x00FC0000;   // norm of a position past the chunk: key 0x3F000000 | position
+constexpr int32_t kFoundLimit = 1 << 29;     // real keys: distance << 6 | position, distance < 2^23
 
-// 16-byte chunk of the (index-gathered) descriptor, operand-transformed
-__device__ inline v4i gather16(const DevImage& im, uint32_t s, uint32_t beg, uint32_t idx, uint32_t cnt, int off, int xorMask,
-                               bool& valid, uint32_t& gid)
+__device__ inline uint32_t wave_sort_u32(uint32_t v, int lane)
 {
-    valid = idx < cnt;
-    v4i x = {0, 0, 0, 0};
-    gid = valid ? im.buckets[(size_t)s * im.n + beg + idx] : 0u;
-    if (valid) {
-        x = *reinterpret_cast<const v4i*>(im.desc + (size_t)gid * 128 + off);
 #pragma unroll
-        for (int q = 0; q < 4; q++) x[q] ^= xorMask;
-    }
-    return x;
+    for (int k = 2; k <= 64; k <<= 1)
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            const uint32_t o = (uint32_t)__shfl_xor((int)v, j);
+            const bool keepMin = ((lane & j) == 0) == ((lane & k) == 0);
+            v = keepMin ? min(v, o) : max(v, o);
+        }
+    return v;
 }
 
-// Work distribution: bucket sizes are far from uniform (with the reference's
-// projection construction the largest of 256 buckets holds ~15% of an image),
-// so a (task, scheme) is split by 32 x 32 tiles, not by bucket. gridDim.x
-// wavefronts share one (task, scheme): each computes the per-bucket tile
-// prefix (ceil(q/32) * ceil(t/32) per bucket) in LDS and takes an even slice
-// of the scheme's tile range, tiles ordered (bucket, query tile, train tile),
-// reusing the gathered query tile while it stays the same.
-__global__ __launch_bounds__(64) void match_kernel(const MatchBatch batch, uint32_t nSchemes)
+// a value the whole wave holds (an LDS word read at a wave-uniform address)
+__device__ inline uint32_t uniform(uint32_t x) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)x); }
+
+// (a << 7) + b in plain code: a is an MFMA accumulator, and only compiled
+// code gets the wait states a VALU read of an MFMA result needs (inline asm
+// reading it could see a stale register)
+__device__ inline int32_t shl7_add(int32_t a, int32_t b) { return (int32_t)(((uint32_t)a << 7) + (uint32_t)b); }
+
+// __launch_bounds__(256, 3): at least three waves per SIMD (at most 168 VGPRs)
+__global__ __launch_bounds__(kMatchWaves * 64, 3) void match_kernel(const MatchBatch batch, uint32_t nSchemes)
 {
-    const uint32_t w = blockIdx.x, s = blockIdx.y, task = blockIdx.z, nW = gridDim.x;
+    const uint32_t task = blockIdx.z;
     const DevImage& Q = batch.q[task];
     const DevImage& T = batch.t;
     uint64_t* resQ = batch.resQ[task];
     uint64_t* resT = batch.resT[task];
-    if (s >= nSchemes || Q.n == 0 || T.n == 0) return;
-    __shared__ uint32_t sPre[kBuckets + 1];
-    __shared__ uint32_t sNT[kBuckets];
-    __shared__ uint32_t sD[32][33];
-    __shared__ uint32_t sQid[32], sTid[32];
-    const int lane = threadIdx.x, l32 = lane & 31, hi = lane >> 5;
+    if (Q.n == 0 || T.n == 0) return;
+    // flat bucket index fb = scheme * 256 + bucket
+    __shared__ uint32_t sPre[kSchemes * kBuckets + 1];  // first block of each bucket
+    __shared__ uint32_t sQE[kSchemes * kBuckets], sTE[kSchemes * kBuckets];  // inclusive bucket ends
+    __shared__ uint32_t sWave[kSchemes][kMatchWaves];
+    __shared__ int32_t sRow[kMatchWaves][64 * kRowStride];  // per wave: [row][column class]
+    const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int c16 = lane & 15, q = lane >> 4;
+    static_assert(kMatchWaves * 64 == kBuckets, "one thread per bucket");
     {
-        constexpr int kPer = kBuckets / 64;
-        uint32_t cnt[kPer], sum = 0;
+        // blocks per bucket, ceil(q / 64) * ceil(t / 64), and their exclusive
+        // prefix over all (scheme, bucket) in order; every load issued first
+        uint32_t qe[kSchemes], qb[kSchemes], te[kSchemes], tb[kSchemes];
 #pragma unroll
-        for (int j = 0; j < kPer; j++) {
-            uint32_t qb, qc, tb, tc;
-            const uint32_t b = lane * kPer + j;
-            bucket_range(Q, s, b, qb, qc);
-            bucket_range(T, s, b, tb, tc);
-            const uint32_t nt = (tc + 31) / 32;
-            sNT[b] = nt;
-            cnt[j] = ((qc + 31) / 32) * nt;
-            sum += cnt[j];
+        for (uint32_t s = 0; s < kSchemes; s++) {
+            if (s >= nSchemes) continue;
+            const uint32_t o = s * kBuckets + (uint32_t)tid;
+            qe[s] = Q.ends[o];
+            te[s] = T.ends[o];
+            qb[s] = tid ? Q.ends[o - 1] : 0u;
+            tb[s] = tid ? T.ends[o - 1] : 0u;
         }
-        // inclusive scan of the lane sums
-        uint32_t inc = sum;
+        uint32_t cnt[kSchemes], inc[kSchemes];
 #pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-            const uint32_t v = __shfl_up(inc, off);
-            if (lane >= off) inc += v;
-        }
-        uint32_t run = inc - sum;
+        for (uint32_t s = 0; s < kSchemes; s++) {
+            if (s >= nSchemes) continue;
+            sQE[s * kBuckets + tid] = qe[s];
+            sTE[s * kBuckets + tid] = te[s];
+            cnt[s] = ((qe[s] - qb[s] + 63) >> 6) * ((te[s] - tb[s] + 63) >> 6);
+            inc[s] = cnt[s];
 #pragma unroll
-        for (int j = 0; j < kPer; j++) {
-            sPre[lane * kPer + j] = run;
-            run += cnt[j];
+            for (int off = 1; off < 64; off <<= 1) {
+                const uint32_t v = (uint32_t)__shfl_up((int)inc[s], off);
+                if (lane >= off) inc[s] += v;
+            }
+            if (lane == 63) sWave[s][wave] = inc[s];
         }
-        if (lane == 63) sPre[kBuckets] = inc;
+        __syncthreads();
+        uint32_t base = 0;
+#pragma unroll
+        for (uint32_t s = 0; s < kSchemes; s++) {
+            if (s >= nSchemes) continue;
+            uint32_t before = 0, all = 0;
+#pragma unroll
+            for (int w = 0; w < kMatchWaves; w++) {
+                before += w < wave ? sWave[s][w] : 0u;
+                all += sWave[s][w];
+            }
+            sPre[s * kBuckets + tid] = base + before + inc[s] - cnt[s];
+            base += all;
+        }
+        if (tid == 0) sPre[nSchemes * kBuckets] = base;
     }
     __syncthreads();
-    const uint32_t total = sPre[kBuckets];
-    const uint32_t kBeg = (uint32_t)(((uint64_t)w * total) / nW);
-    const uint32_t kEnd = (uint32_t)(((uint64_t)(w + 1) * total) / nW);
-    uint32_t curB = ~0u, curQt = ~0u;
-    v4i aop[4];
-    bool qv = false;
-    uint32_t qid = 0;
-    int32_t na = 0;
-    // query-row minimum over the consecutive tiles of the current query block
-    // (lanes 0..31), merged into resQ when the block changes and at the end
-    uint64_t rowBest = kNoMatch;
-    uint32_t rowQid = ~0u;
-    auto flush_row = [&]() {
-        if (hi == 0 && rowQid != ~0u && rowBest != kNoMatch)
-            atomicMin(reinterpret_cast<unsigned long long*>(resQ + rowQid), (unsigned long long)rowBest);
-        rowBest = kNoMatch;
-    };
-    for (uint32_t k = kBeg; k < kEnd; k++) {
-        // bucket of tile k: the largest b with sPre[b] <= k
-        uint32_t lo = 0, hiB = kBuckets;
-        while (hiB - lo > 1) {
-            const uint32_t mid = (lo + hiB) >> 1;
-            if (sPre[mid] <= k) lo = mid;
+    const uint32_t nFb = nSchemes * kBuckets;
+    const uint32_t total = uniform(sPre[nFb]);
+    const uint32_t gw = blockIdx.x * kMatchWaves + (uint32_t)wave, nW = gridDim.x * kMatchWaves;
+    uint32_t k = (uint32_t)(((uint64_t)gw * total) / nW);
+    const uint32_t kEnd = (uint32_t)(((uint64_t)(gw + 1) * total) / nW);
+    if (k >= kEnd) return;  // no barrier follows
+    // bucket of block k: the largest fb with sPre[fb] <= k
+    uint32_t fb = 0;
+    {
+        uint32_t hiB = nFb;
+        while (hiB - fb > 1) {
+            const uint32_t mid = (fb + hiB) >> 1;
+            if (uniform(sPre[mid]) <= k) fb = mid;
             else hiB = mid;
         }
-        const uint32_t b = lo;
-        const uint32_t local = k - sPre[b];
-        const uint32_t nTT = sNT[b];
-        const uint32_t qt = local / nTT, tt = local - qt * nTT;
-        uint32_t qBeg, qCnt, tBeg, tCnt;
-        bucket_range(Q, s, b, qBeg, qCnt);
-        bucket_range(T, s, b, tBeg, tCnt);
-        if (b != curB || qt != curQt) {
-            flush_row();
-            // query rows: u = 127 - a, NA' = |a - 128|^2
-            int32_t nap = 0;
+    }
+    uint32_t qBeg, qCnt, tBeg, tCnt;
+    auto bucket = [&]() {
+        const bool first = (fb & (kBuckets - 1)) == 0;
+        qBeg = first ? 0u : uniform(sQE[fb - 1]);
+        tBeg = first ? 0u : uniform(sTE[fb - 1]);
+        qCnt = uniform(sQE[fb]) - qBeg;
+        tCnt = uniform(sTE[fb]) - tBeg;
+    };
+    bucket();
+    uint32_t nT = (tCnt + 63) >> 6, nQ = (qCnt + 63) >> 6;
+    const uint32_t kIn = k - uniform(sPre[fb]);
+    uint32_t qt = kIn / nT, tt = kIn - qt * nT;
+    const bool doCols = resT != nullptr;
+    const uint8_t* qDesc = Q.desc;
+    const uint8_t* tDesc = T.desc;
+
+    // query chunk state; lane L owns the chunk's row at position L
+    uint32_t qgid = ~0u;      // lane L: the L-th smallest member index of the chunk
+    v4i aop[4][2];            // row block rb, K-step ks: row rb * 16 + c16, bytes [64 ks + 16 q, +16)
+    int32_t cA[4][4];         // NA' << 6 | position of row rb * 16 + 4 q + j
+    int32_t naOwn = 0;        // NA' << 6 of row L
+    uint64_t rowBest = kNoMatch;
+    int32_t* sR = &sRow[wave][0];
+    auto flush_rows = [&]() {
+        if (rowBest != kNoMatch && qgid != ~0u)
+            atomicMin(reinterpret_cast<unsigned long long*>(resQ + qgid), (unsigned long long)rowBest);
+        rowBest = kNoMatch;
+    };
+    bool newQ = true;
+    for (;;) {
+        if (newQ) {
+            flush_rows();
+            const uint32_t i = qt * 64 + (uint32_t)lane;
+            const uint32_t g = i < qCnt ? Q.buckets[(size_t)(fb >> 8) * Q.n + qBeg + i] : ~0u;
+            qgid = wave_sort_u32(g, lane);
+            int32_t na[4];
 #pragma unroll
-            for (int kc = 0; kc < 4; kc++) {
-                aop[kc] = gather16(Q, s, qBeg, qt * 32 + l32, qCnt, kc * 32 + hi * 16, 0x7F7F7F7F, qv, qid);
+            for (int rb = 0; rb < 4; rb++) {
+                const uint32_t gr = (uint32_t)__shfl((int)qgid, rb * 16 + c16);
+                const bool valid = gr != ~0u;
+                int32_t nap = 0;
 #pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    const int xa = aop[kc][q] ^ (int)0xFFFFFFFFu;  // (a ^ 0x7F) ^ 0xFF = a ^ 0x80
-                    nap = qv ? __builtin_amdgcn_sdot4(xa, xa, nap, false) : 0;
+                for (int ks = 0; ks < 2; ks++) {
+                    // positions past the chunk read member 0 and are zeroed
+                    const v4i x = *reinterpret_cast<const v4i*>(qDesc + (size_t)(valid ? gr : 0u) * 128 + ks * 64 + q * 16);
+#pragma unroll
+                    for (int kk = 0; kk < 4; kk++) {
+                        const int xa = x[kk] ^ (int)0x80808080u;
+                        nap = __builtin_amdgcn_sdot4(xa, xa, nap, false);
+                        aop[rb][ks][kk] = valid ? (x[kk] ^ 0x7F7F7F7F) : 0;
+                    }
                 }
+                nap += __shfl_xor(nap, 16);
+                nap += __shfl_xor(nap, 32);
+                na[rb] = valid ? nap : kPastChunk;  // lanes with c16 = r hold row rb * 16 + r
             }
-            na = nap + __shfl_xor(nap, 32);
-            curB = b;
-            curQt = qt;
+#pragma unroll
+            for (int rb = 0; rb < 4; rb++)
+#pragma unroll
+                for (int j = 0; j < 4; j++)
+                    cA[rb][j] = (__shfl(na[rb], 4 * q + j) << 6) | (rb * 16 + 4 * q + j);
+            naOwn = (q == 0 ? na[0] : q == 1 ? na[1] : q == 2 ? na[2] : na[3]) << 6;
         }
-        const uint32_t t0 = tt * 32;
+        // train chunk
+        const uint32_t i = tt * 64 + (uint32_t)lane;
+        const uint32_t g = i < tCnt ? T.buckets[(size_t)(fb >> 8) * T.n + tBeg + i] : ~0u;
+        const uint32_t tgid = wave_sort_u32(g, lane);
+        v4i bop[4][2];
+        int32_t nb[4], rB[4];
+#pragma unroll
+        for (int cb = 0; cb < 4; cb++) {
+            const uint32_t gc = (uint32_t)__shfl((int)tgid, cb * 16 + c16);
+            const bool valid = gc != ~0u;
+            int32_t npb = 0;
+#pragma unroll
+            for (int ks = 0; ks < 2; ks++) {
+                const v4i x = *reinterpret_cast<const v4i*>(tDesc + (size_t)(valid ? gc : 0u) * 128 + ks * 64 + q * 16);
+#pragma unroll
+                for (int kk = 0; kk < 4; kk++) {
+                    const int xw = x[kk] ^ (int)0x80808080u;
+                    npb = __builtin_amdgcn_sdot4(xw, xw, npb, false);
+                    npb = __builtin_amdgcn_sdot4(xw, 0x02020202, npb, false);
+                    bop[cb][ks][kk] = valid ? xw : 0;
+                }
+            }
+            npb += __shfl_xor(npb, 16);
+            npb += __shfl_xor(npb, 32);
+            nb[cb] = valid ? npb : kPastChunk;  // NB' of column cb * 16 + c16
+            rB[cb] = (nb[cb] << 6) | (cb * 16 + c16);
+        }
+        // the 64 rows in two halves of 32 (rb = 2h, 2h + 1): a half's products
+        // are folded into the row keys and the column minima before the next
+        // half's MFMAs, so only 32 accumulator registers are live
+        // row rb * 16 + 4 q + j: minimum over the lane's 4 columns, to sR
+        int32_t cm[4] = {INT32_MAX, INT32_MAX, INT32_MAX, INT32_MAX};  // column cb * 16 + c16 over the lane's rows
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            v4i acc[2][4];
+#pragma unroll
+            for (int ks = 0; ks < 2; ks++)
+#pragma unroll
+                for (int r = 0; r < 2; r++)
+#pragma unroll
+                    for (int cb = 0; cb < 4; cb++)
+                        acc[r][cb] = __builtin_amdgcn_mfma_i32_16x16x64_i8(aop[2 * h + r][ks], bop[cb][ks],
+                                                                          ks == 0 ? v4i{0, 0, 0, 0} : acc[r][cb], 0, 0, 0);
+#pragma unroll
+            for (int r = 0; r < 2; r++)
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const int rb = 2 * h + r;
+                    const int32_t r0 = shl7_add(acc[r][0][j], rB[0]), r1 = shl7_add(acc[r][1][j], rB[1]);
+                    const int32_t r2 = shl7_add(acc[r][2][j], rB[2]), r3 = shl7_add(acc[r][3][j], rB[3]);
+                    sR[(rb * 16 + 4 * q + j) * kRowStride + c16] = min(min(r0, r1), min(r2, r3));
+                }
+            if (doCols) {
+#pragma unroll
+                for (int cb = 0; cb < 4; cb++)
+#pragma unroll
+                    for (int r = 0; r < 2; r++)
+#pragma unroll
+                        for (int j = 0; j < 4; j++) cm[cb] = min(cm[cb], shl7_add(acc[r][cb][j], cA[2 * h + r][j]));
+            }
+        }
+        // rows: lane L takes row L's 16 column-class minima back from sR
+        // (the wave's own LDS traffic completes in issue order)
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
         {
-            v4i bop[4];
-            bool tv;
-            uint32_t tid;
-            int32_t nbp = 0;
+            int32_t m = sR[lane * kRowStride];
 #pragma unroll
-            for (int kc = 0; kc < 4; kc++) {
-                bop[kc] = gather16(T, s, tBeg, t0 + l32, tCnt, kc * 32 + hi * 16, (int)0x80808080u, tv, tid);
-#pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    nbp = __builtin_amdgcn_sdot4(bop[kc][q], bop[kc][q], nbp, false);
-                    nbp = __builtin_amdgcn_sdot4(bop[kc][q], 0x02020202, nbp, false);
-                }
+            for (int c = 1; c < 16; c++) m = min(m, sR[lane * kRowStride + c]);
+            const int32_t key = m + naOwn;  // distance << 6 | column position of row L
+            const uint32_t tg = (uint32_t)__shfl((int)tgid, key & 63);
+            if (key < kFoundLimit) {
+                const uint64_t cand = make_key((uint32_t)(key >> 6), tg);
+                rowBest = cand < rowBest ? cand : rowBest;
             }
-            const int32_t nb = nbp + __shfl_xor(nbp, 32);  // NB' = |b - 127|^2 - 128
-            v16i acc = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-#pragma unroll
-            for (int kc = 0; kc < 4; kc++) acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(aop[kc], bop[kc], acc, 0, 0, 0);
-            __syncthreads();
-            if (hi == 0) {
-                sQid[l32] = qv ? qid : ~0u;
-                sTid[l32] = tv ? tid : ~0u;
-            }
-            // lane holds column l32 (train), rows r(v); NA' of row r from lane r
-#pragma unroll
-            for (int v = 0; v < 16; v++) {
-                const int r = 8 * (v >> 2) + 4 * hi + (v & 3);
-                const int32_t nar = __shfl(na, r);
-                sD[r][l32] = (uint32_t)(nar + nb + 2 * acc[v]);
-            }
-            __syncthreads();
-            if (hi == 0) {
-                // lane l32 < 32: query row l32 -> best train column of this tile,
-                // folded into the running minimum of the query block
-                rowQid = sQid[l32];
-                if (rowQid != ~0u) {
-                    for (int c = 0; c < 32; c++) {
-                        const uint32_t gt = sTid[c];
-                        if (gt == ~0u) continue;
-                        rowBest = min(rowBest, make_key(sD[l32][c], gt));
-                    }
-                }
-            } else if (resT) {
-                // lane 32 + c: train column c -> best query row
-                const uint32_t gt = sTid[l32];
-                if (gt != ~0u) {
-                    uint64_t best = kNoMatch;
-                    for (int r = 0; r < 32; r++) {
-                        const uint32_t gq = sQid[r];
-                        if (gq == ~0u) continue;
-                        best = min(best, make_key(sD[r][l32], gq));
-                    }
-                    if (best != kNoMatch) atomicMin(reinterpret_cast<unsigned long long*>(resT + gt), (unsigned long long)best);
-                }
+        }
+        if (doCols) {
+            // lane L ends with column L's minimum over the chunk's rows
+            const auto sa = __builtin_amdgcn_permlane32_swap((uint32_t)cm[0], (uint32_t)cm[2], false, false);
+            const auto sb = __builtin_amdgcn_permlane32_swap((uint32_t)cm[1], (uint32_t)cm[3], false, false);
+            const int32_t t02 = min((int32_t)sa[0], (int32_t)sa[1]);
+            const int32_t t13 = min((int32_t)sb[0], (int32_t)sb[1]);
+            const auto sc = __builtin_amdgcn_permlane16_swap((uint32_t)t02, (uint32_t)t13, false, false);
+            const int32_t nbOwn = q == 0 ? nb[0] : q == 1 ? nb[1] : q == 2 ? nb[2] : nb[3];
+            const int32_t key = min((int32_t)sc[0], (int32_t)sc[1]) + (nbOwn << 6);
+            const uint32_t qg = (uint32_t)__shfl((int)qgid, key & 63);
+            if (tgid != ~0u && key < kFoundLimit)
+                atomicMin(reinterpret_cast<unsigned long long*>(resT + tgid),
+                          (unsigned long long)make_key((uint32_t)(key >> 6), qg));
+        }
+        // next block
+        if (++k >= kEnd) break;
+        newQ = ++tt == nT;
+        if (newQ) {
+            tt = 0;
+            if (++qt == nQ) {
+                qt = 0;
+                do {
+                    fb++;
+                } while (uniform(sPre[fb + 1]) == uniform(sPre[fb]));
+                bucket();
+                nT = (tCnt + 63) >> 6;
+                nQ = (qCnt + 63) >> 6;
             }
         }
     }
-    flush_row();
+    flush_rows();
 }
 
 __global__ void match_init_kernel(const MatchBatch batch)
@@ -781,12 +900,10 @@ extern "C" int rsfm_cashash_find_best_match(const rsfm_cashash_image* queries, c
         if (maxN == 0) continue;
         match_init_kernel<<<dim3(div_up(maxN, 256), 1, n), 256, 0, stream>>>(batch);
         RSFM_CHECK(check_launch("find_best_match init"));
-        // wavefronts per (task, scheme): about 16384 over the launch
-        // about 8K wavefronts per launch (1366 per scheme for one pair of 6
-        // schemes, about two tiles each at 8K keypoints): 16K was 7% slower,
-        // 32K 15% slower, 4K 7% slower
-        const uint32_t perScheme = std::min<uint32_t>(2048u, std::max<uint32_t>(16u, div_up(8192u, n * nbSchemes)));
-        match_kernel<<<dim3(perScheme, nbSchemes, n), 64, 0, stream>>>(batch, nbSchemes);
+        // about 512 workgroups (2048 waves) per launch, each wave a few
+        // 64 x 64 blocks at 8K keypoints
+        const uint32_t perTask = std::max<uint32_t>(8u, div_up(512u, n));
+        match_kernel<<<dim3(perTask, 1, n), kMatchWaves * 64, 0, stream>>>(batch, nbSchemes);
         RSFM_CHECK(check_launch("find_best_match"));
     }
     return RSFM_OK;
