@@ -34,7 +34,10 @@ constexpr int kFinThreads = 256;
 constexpr int kFinWaves = kFinThreads / 64;
 constexpr int kFinPer = 4;
 constexpr uint32_t kFinChunk = kFinThreads * kFinPer;
-constexpr int kFinMaxTasks = 16;
+// pairs per launch of the C-ABI entry point: one workgroup each, so a launch
+// lasts about as long as its slowest pair's refit (a chain of FP64 steps);
+// 32 tasks of 88 bytes stay well inside the kernel-argument limit
+constexpr int kFinMaxTasks = 32;
 constexpr int kNormEntries = 45;  // upper triangle of the 9x9 normal matrix
 
 // log2 of x in (0, 1] (0 -> -inf, negative -> NaN): x = m 2^e with m in
@@ -92,6 +95,7 @@ __host__ __device__ inline uint32_t required_iters(uint32_t best, uint32_t n, do
 }
 
 typedef ByValue<FinishTask, kFinMaxTasks> FinishBatch;
+static_assert(sizeof(FinishBatch) + sizeof(FinishParams) <= 4096, "kernel arguments are limited to 4 KiB");
 
 struct FinishSmem {
     double part[kNormEntries][kFinThreads];

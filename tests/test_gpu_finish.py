@@ -109,3 +109,18 @@ def test_finish_edge_cases(rs, orc, dev):
         _check(o, p, orc.f8_finish(p, i, c, None, 1.0, 1, 0.99, 5000))
     assert _u32(outs[0]["result"]).tolist() == [0xFFFFFFFF, 0, 0, 0]
     assert _u32(outs[1]["result"])[0] == 0xFFFFFFFF
+
+
+def test_finish_many_tasks(rs, orc, dev):
+    """40 problems in one call: one launch of 32 workgroups and one of 8."""
+    _say("finish, 40 problems in one call")
+    rng = np.random.default_rng(4)
+    probs, refs = [], []
+    for k in range(40):
+        n = int(rng.integers(300, 1500))
+        pts, idx, counts, models = _problem(orc, 200 + k, n, n // 3, int(rng.integers(64, 512)), 1.0, 1)
+        probs.append((_d(pts, dev), _d(idx.view(np.int32), dev), _d(counts.view(np.int32), dev), _d(models, dev)))
+        refs.append((pts, orc.f8_finish(pts, idx, counts, models, 1.0, 1, 0.99, 5000)))
+    outs = rs.ransac_f8_finish(probs, 1.0, 1, confidence=0.99)
+    for o, (pts, ref) in zip(outs, refs):
+        _check(o, pts, ref)
