@@ -104,16 +104,17 @@ static int null_vector9(const double N[9][9], double v[9])
     for (int a = 0; a < 9; a++) tr = tr + N[a][a];
     if (!(tr > 0.0) || !isfinite(tr)) return 0;
     const double sigma = kShift * tr;
-    double L[9][9];
+    double L[9][9], rd[9]; /* rd: reciprocals of the diagonal, as the GPU */
     for (int j = 0; j < 9; j++) {
         double s = N[j][j] + sigma;
         for (int k = 0; k < j; k++) s = s - L[j][k] * L[j][k];
         if (!(s > 0.0)) return 0;
         L[j][j] = sqrt(s);
+        rd[j] = 1.0 / L[j][j];
         for (int i = j + 1; i < 9; i++) {
             double t = N[i][j];
             for (int k = 0; k < j; k++) t = t - L[i][k] * L[j][k];
-            L[i][j] = t / L[j][j];
+            L[i][j] = t * rd[j];
         }
     }
     for (int i = 0; i < 9; i++) v[i] = 1.0;
@@ -122,12 +123,12 @@ static int null_vector9(const double N[9][9], double v[9])
         for (int i = 0; i < 9; i++) {
             double t = v[i];
             for (int k = 0; k < i; k++) t = t - L[i][k] * y[k];
-            y[i] = t / L[i][i];
+            y[i] = t * rd[i];
         }
         for (int i = 8; i >= 0; i--) {
             double t = y[i];
             for (int k = i + 1; k < 9; k++) t = t - L[k][i] * z[k];
-            z[i] = t / L[i][i];
+            z[i] = t * rd[i];
         }
         double nrm2 = 0.0;
         for (int i = 0; i < 9; i++) nrm2 = nrm2 + z[i] * z[i];
@@ -658,9 +659,23 @@ uint32_t or_ransac_adaptive_cut(const uint32_t* counts, uint32_t H, uint32_t n, 
 
 #define FIN_THREADS 256u
 
+/* an entry's 256 thread partials in the GPU's order: group g (0..15) adds
+ * the partials of threads g, g + 16, ..., g + 240 in that order, then the 16
+ * group sums are added in group order */
+static double sum_partials(const double part[FIN_THREADS])
+{
+    double tot = 0.0;
+    for (uint32_t g = 0; g < 16; g++) {
+        double a = 0.0;
+        for (uint32_t k = 0; k < 16; k++) a = a + part[k * 16 + g];
+        tot = tot + a;
+    }
+    return tot;
+}
+
 /* least-squares F on the inliers of F0 (m of them) in the GPU's summation
  * order: thread t of 256 sums points t, t + 256, ... in index order, the 256
- * partials are added in thread order */
+ * partials are added by sum_partials */
 static int refit_all(const float* pts, uint32_t n, const float F0[9], float t, int mode, uint32_t m, float G[9])
 {
     const double dm = (double)m;
@@ -675,11 +690,7 @@ static int refit_all(const float* pts, uint32_t n, const float F0[9], float t, i
         }
         for (int k = 0; k < 4; k++) part[k][th] = a[k];
     }
-    for (int k = 0; k < 4; k++) {
-        double a = 0.0;
-        for (uint32_t th = 0; th < FIN_THREADS; th++) a = a + part[k][th];
-        tot[k] = a;
-    }
+    for (int k = 0; k < 4; k++) tot[k] = sum_partials(part[k]);
     dg_norm nm;
     for (int k = 0; k < 4; k++) nm.c[k] = tot[k] / dm;
     for (uint32_t th = 0; th < FIN_THREADS; th++) {
@@ -695,11 +706,7 @@ static int refit_all(const float* pts, uint32_t n, const float F0[9], float t, i
         part[0][th] = d0;
         part[1][th] = d1;
     }
-    for (int k = 0; k < 2; k++) {
-        double a = 0.0;
-        for (uint32_t th = 0; th < FIN_THREADS; th++) a = a + part[k][th];
-        tot[k] = a;
-    }
+    for (int k = 0; k < 2; k++) tot[k] = sum_partials(part[k]);
     const double n0 = tot[0] / dm, n1 = tot[1] / dm;
     if (!(n0 > 0.0) || !(n1 > 0.0)) return 0;
     nm.s0 = kSqrt2 / n0;
@@ -726,8 +733,7 @@ static int refit_all(const float* pts, uint32_t n, const float F0[9], float t, i
     int e = 0;
     for (int u = 0; u < 9; u++)
         for (int v = u; v < 9; v++) {
-            double a = 0.0;
-            for (uint32_t th = 0; th < FIN_THREADS; th++) a = a + part[e][th];
+            const double a = sum_partials(part[e]);
             N[u][v] = a;
             N[v][u] = a;
             e++;

@@ -103,7 +103,7 @@ __device__ inline bool null_vector9(const double N[9][9], double v[9])
     for (int a = 0; a < 9; a++) tr = tr + N[a][a];
     if (!(tr > 0.0) || !isfinite(tr)) return false;
     const double sigma = kShift * tr;
-    double L[9][9];
+    double L[9][9], rd[9];  // rd: reciprocals of the diagonal, so every later step multiplies
 #pragma unroll
     for (int j = 0; j < 9; j++) {
         double s = N[j][j] + sigma;
@@ -111,12 +111,13 @@ __device__ inline bool null_vector9(const double N[9][9], double v[9])
         for (int k = 0; k < j; k++) s = s - L[j][k] * L[j][k];
         if (!(s > 0.0)) return false;
         L[j][j] = sqrt(s);
+        rd[j] = 1.0 / L[j][j];
 #pragma unroll
         for (int i = j + 1; i < 9; i++) {
             double t = N[i][j];
 #pragma unroll
             for (int k = 0; k < j; k++) t = t - L[i][k] * L[j][k];
-            L[i][j] = t / L[j][j];
+            L[i][j] = t * rd[j];
         }
     }
 #pragma unroll
@@ -128,14 +129,14 @@ __device__ inline bool null_vector9(const double N[9][9], double v[9])
             double t = v[i];
 #pragma unroll
             for (int k = 0; k < i; k++) t = t - L[i][k] * y[k];
-            y[i] = t / L[i][i];
+            y[i] = t * rd[i];
         }
 #pragma unroll
         for (int i = 8; i >= 0; i--) {
             double t = y[i];
 #pragma unroll
             for (int k = i + 1; k < 9; k++) t = t - L[k][i] * z[k];
-            z[i] = t / L[i][i];
+            z[i] = t * rd[i];
         }
         double nrm2 = 0.0;
 #pragma unroll

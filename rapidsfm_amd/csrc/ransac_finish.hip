@@ -18,10 +18,13 @@
 //  * Refit: three passes over the points, each thread taking points t,
 //    t + 256, ... in order (inlier test of the best F, then FP64 sums):
 //    centroids, mean distances to them (Hartley scales), the 45 entries of
-//    the normal matrix of the normalised rows. Thread partials are summed in
-//    thread order by one lane per entry, so the CPU restatement repeats every
-//    rounding. The null vector, rank-2 projection and denormalisation are the
-//    FP64 single-lane routines of hgeom.hpp.
+//    the normal matrix of the normalised rows. An entry's 256 thread
+//    partials are summed in a fixed two-level order (16 groups of the
+//    threads congruent mod 16, each in thread order, then the group sums in
+//    group order), so the CPU restatement repeats every rounding with a
+//    dependency chain of 32 additions instead of 256. The null vector,
+//    rank-2 projection and denormalisation are the FP64 single-lane
+//    routines of hgeom.hpp.
 #include "internal.hpp"
 #include "fgeom.hpp"
 #include "hgeom.hpp"
@@ -97,13 +100,17 @@ __host__ __device__ inline uint32_t required_iters(uint32_t best, uint32_t n, do
 typedef ByValue<FinishTask, kFinMaxTasks> FinishBatch;
 static_assert(sizeof(FinishBatch) + sizeof(FinishParams) <= 4096, "kernel arguments are limited to 4 KiB");
 
+constexpr int kSumGroups = 16;  // kFinThreads = kSumGroups * kSumGroups
+static_assert(kSumGroups * kSumGroups == kFinThreads, "two-level sums of the thread partials");
+
 struct FinishSmem {
     double part[kNormEntries][kFinThreads];
+    double grp[kNormEntries][kSumGroups];
     double tot[kNormEntries];
     uint64_t waveKey[kFinWaves];
     uint32_t waveCnt[kFinWaves];
     int32_t waveLast[kFinWaves];
-    uint32_t cnt[kFinThreads];
+    uint32_t firstStop;
     uint32_t stop;
     uint64_t stopKey;
     float F[9];
@@ -124,31 +131,48 @@ __device__ inline T wave_scan_incl(T x, Op op)
     return x;
 }
 
-// inliers of s.F (s.ok) among the n points; the count is exact, the order free
-template <int kMode>
-__device__ uint32_t count_pts(FinishSmem& s, const float4* pts, uint32_t n, float t2)
+// sum of the workgroup's per-thread counts (integers: any order is exact)
+__device__ inline uint32_t block_sum_u32(FinishSmem& s, uint32_t c)
 {
-    float F[9];
 #pragma unroll
-    for (int q = 0; q < 9; q++) F[q] = s.F[q];
-    uint32_t c = 0;
-    if (s.ok)
-        for (uint32_t i = threadIdx.x; i < n; i += kFinThreads) c += fgeom::inlier<kMode>(F, pts[i], t2) ? 1u : 0u;
-    s.cnt[threadIdx.x] = c;
+    for (int o = 32; o > 0; o >>= 1) c += (uint32_t)__shfl_xor((int)c, o);
+    if ((threadIdx.x & 63) == 0) s.waveCnt[threadIdx.x >> 6] = c;
     __syncthreads();
     uint32_t total = 0;
-    for (int k = 0; k < kFinThreads; k++) total += s.cnt[k];
+#pragma unroll
+    for (int w = 0; w < kFinWaves; w++) total += s.waveCnt[w];
     __syncthreads();
     return total;
 }
 
-// the first `entries` partials summed in thread order into s.tot
+// inliers of model F (ok) among the n points
+template <int kMode>
+__device__ uint32_t count_model(FinishSmem& s, const float F[9], bool ok, const float4* pts, uint32_t n, float t2)
+{
+    uint32_t c = 0;
+    if (ok)
+        for (uint32_t i = threadIdx.x; i < n; i += kFinThreads) c += fgeom::inlier<kMode>(F, pts[i], t2) ? 1u : 0u;
+    return block_sum_u32(s, c);
+}
+
+// the first `entries` rows of s.part summed into s.tot: group g adds the
+// partials of threads g, g + 16, ..., g + 240 in that order, then the 16
+// group sums are added in group order
 __device__ inline void sum_parts(FinishSmem& s, int entries)
 {
     __syncthreads();
+    for (int task = threadIdx.x; task < entries * kSumGroups; task += kFinThreads) {
+        const int e = task / kSumGroups, g = task % kSumGroups;
+        double a = 0.0;
+#pragma unroll
+        for (int k = 0; k < kSumGroups; k++) a = a + s.part[e][k * kSumGroups + g];
+        s.grp[e][g] = a;
+    }
+    __syncthreads();
     if ((int)threadIdx.x < entries) {
         double a = 0.0;
-        for (int k = 0; k < kFinThreads; k++) a = a + s.part[threadIdx.x][k];
+#pragma unroll
+        for (int g = 0; g < kSumGroups; g++) a = a + s.grp[threadIdx.x][g];
         s.tot[threadIdx.x] = a;
     }
     __syncthreads();
@@ -330,11 +354,11 @@ __device__ void adaptive_cut(FinishSmem& s, const uint32_t* counts, uint32_t H, 
             }
         }
         __syncthreads();  // wave totals read by everyone
-        if (threadIdx.x == 0) s.cnt[0] = ~0u;
+        if (threadIdx.x == 0) s.firstStop = ~0u;
         __syncthreads();
-        if (myStop != ~0u) atomicMin(&s.cnt[0], myStop);
+        if (myStop != ~0u) atomicMin(&s.firstStop, myStop);
         __syncthreads();
-        const uint32_t first = s.cnt[0];
+        const uint32_t first = s.firstStop;
         if (first != ~0u) {
             if (myStop == first) {
                 s.stop = first + 1;
@@ -420,7 +444,13 @@ __global__ __launch_bounds__(kFinThreads) void finish_kernel(const Src batch, co
         s.ok = ok;
     }
     __syncthreads();
-    const uint32_t c0 = count_pts<kMode>(s, T.pts, n, t2);
+    uint32_t c0;
+    {
+        float F[9];
+#pragma unroll
+        for (int q = 0; q < 9; q++) F[q] = s.F[q];
+        c0 = count_model<kMode>(s, F, s.ok != 0, T.pts, n, t2);
+    }
     uint32_t cFinal = c0;
     // ---- refit on the inliers
     if (prm.refit && c0 >= 8 && s.ok) {
@@ -428,13 +458,7 @@ __global__ __launch_bounds__(kFinThreads) void finish_kernel(const Src batch, co
             float G[9];
 #pragma unroll
             for (int q = 0; q < 9; q++) G[q] = s.G[q];
-            uint32_t c = 0;
-            for (uint32_t i = t; i < n; i += kFinThreads) c += fgeom::inlier<kMode>(G, T.pts[i], t2) ? 1u : 0u;
-            s.cnt[t] = c;
-            __syncthreads();
-            uint32_t c1 = 0;
-            for (int k = 0; k < kFinThreads; k++) c1 += s.cnt[k];
-            __syncthreads();
+            const uint32_t c1 = count_model<kMode>(s, G, true, T.pts, n, t2);
             if (c1 > c0) {
                 cFinal = c1;
                 if (t < 9) s.F[t] = G[t];
