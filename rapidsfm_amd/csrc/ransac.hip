@@ -287,85 +287,140 @@ __device__ inline int fit_hypothesis7(const DevTask& T, uint32_t n, uint32_t h, 
     return fgeom::fit7(p, Fs);
 }
 
-// one lane per 7-point sample, workgroups as score_kernel: wavefront 0 solves
-// the samples and shares up to three F per lane through LDS; every point is
-// tested against all three (a missing model is all zeros and never counts:
-// 0 < 0 is false in both tests)
+// 7-point scoring. A workgroup takes kS7Samples samples, one per lane of its
+// first two wavefronts: every such lane solves its sample (up to three F) and
+// the workgroup's M models are listed in LDS in sample order (a scan of the
+// model counts). Lane l of every wavefront then holds models l, l + 64, ...
+// (R = ceil(M / 64) of them, at most six) in registers and tests each point
+// of its quarter of the correspondences against all R, as score_kernel walks
+// them; the four partial counts are added with LDS atomics. A sample with one
+// real root thus takes one model slot instead of three (about 2.5 models per
+// sample with 8K-keypoint pairs: R = 5 instead of six slots per lane), and
+// every point load feeds R tests. Each sample then reports its best model
+// (most inliers, the first on ties) and that count.
+constexpr uint32_t kS7Samples = 128;
+constexpr int kS7MaxRounds = (int)(kS7Samples * kMaxF7 / 64);
+constexpr uint32_t kS7MaxModels = kS7Samples * kMaxF7;
+static_assert(kS7Samples <= kScoreBlock && kS7Samples % 64 == 0, "samples are fitted one per lane");
+
+// inliers of the lane's R models among P[0, m): the next four points are in
+// flight while the current four are tested (four, not eight: the R models
+// already hold up to 54 registers, and the kernel keeps four waves per SIMD)
+template <int kMode, int R>
+__device__ inline void count7(const float (&F)[kS7MaxRounds][9], const float4* __restrict__ P, uint32_t m, float t2,
+                              uint32_t (&c)[kS7MaxRounds])
+{
+    auto test = [&](const float4& p) {
+#pragma unroll
+        for (int r = 0; r < R; r++) c[r] += fgeom::inlier<kMode>(F[r], p, t2) ? 1u : 0u;
+    };
+    uint32_t i = 0;
+    if (m >= 4) {
+        float4 q0 = P[0], q1 = P[1], q2 = P[2], q3 = P[3];
+        for (; i + 8 <= m; i += 4) {
+            const float4* __restrict__ Q = P + i + 4;
+            const float4 n0 = Q[0], n1 = Q[1], n2 = Q[2], n3 = Q[3];
+            test(q0);
+            test(q1);
+            test(q2);
+            test(q3);
+            q0 = n0;
+            q1 = n1;
+            q2 = n2;
+            q3 = n3;
+        }
+        test(q0);
+        test(q1);
+        test(q2);
+        test(q3);
+        i += 4;
+    }
+    for (; i < m; i++) test(P[i]);
+}
+
 template <int kMode, class Src>
 __global__ __launch_bounds__(kScoreBlock) void score7_kernel(const Src batch, float t2, float s0, float s1)
 {
     const Wg3 wg = xcd_local_block();
     const DevTask& T = batch[wg.y];
-    if (wg.x * kScoreThreads >= T.H) return;
-    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const uint32_t h = wg.x * kScoreThreads + lane;
+    if (wg.x * kS7Samples >= T.H) return;
+    const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const uint32_t h = wg.x * kS7Samples + tid;
     const uint32_t n = task_n(T);
-    __shared__ float sF[kMaxF7][9][kScoreThreads];
-    __shared__ int sNm[kScoreThreads];
-    __shared__ uint32_t sCnt[kScoreWaves][kMaxF7][kScoreThreads];
-    if (w == 0) {
-        float Fw[kMaxF7][9];  // written by the solver with a running index
-        const int nm = fit_hypothesis7(T, n, h, s0, s1, Fw);
+    __shared__ float sF[9][kS7MaxModels];
+    __shared__ uint32_t sCnt[kS7MaxModels];
+    __shared__ uint32_t sWaveModels[kS7Samples / 64];
+    for (uint32_t k = tid; k < kS7MaxModels; k += kScoreBlock) sCnt[k] = 0u;
+    float Fw[kMaxF7][9];  // written by the solver with a running index
+    const int nm = tid < kS7Samples ? fit_hypothesis7(T, n, h, s0, s1, Fw) : 0;
+    // the models' place in the list: prefix of nm in lane order (the lanes
+    // past the samples add nothing)
+    uint32_t inc = (uint32_t)nm;
 #pragma unroll
-        for (int m = 0; m < kMaxF7; m++)
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = (uint32_t)__shfl_up((int)inc, o);
+        if (lane >= (uint32_t)o) inc += y;
+    }
+    if (lane == 63 && w < kS7Samples / 64) sWaveModels[w] = inc;
+    __syncthreads();
+    uint32_t off = inc - (uint32_t)nm;
 #pragma unroll
-            for (int q = 0; q < 9; q++) sF[m][q][lane] = m < nm ? Fw[m][q] : 0.f;
-        sNm[lane] = nm;
+    for (uint32_t k = 0; k < kS7Samples / 64; k++) off += k < w ? sWaveModels[k] : 0u;
+#pragma unroll
+    for (int mi = 0; mi < kMaxF7; mi++)
+        if (mi < nm)
+#pragma unroll
+            for (int q = 0; q < 9; q++) sF[q][off + mi] = Fw[mi][q];
+    __syncthreads();
+    uint32_t M = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kS7Samples / 64; k++) M += sWaveModels[k];
+    const int R = (int)((M + 63) / 64);
+    float F[kS7MaxRounds][9];  // model r * 64 + lane; zero (never an inlier) past the list
+#pragma unroll
+    for (int r = 0; r < kS7MaxRounds; r++) {
+        const uint32_t k = (uint32_t)r * 64 + lane;
+#pragma unroll
+        for (int q = 0; q < 9; q++) F[r][q] = k < M ? sF[q][k] : 0.f;
+    }
+    uint32_t c[kS7MaxRounds] = {0u, 0u, 0u, 0u, 0u, 0u};
+    uint32_t beg, end;
+    wave_range(n, w, beg, end);
+    // the offset is hidden from the uniformity analysis so the loads take the
+    // vector memory path (see score_kernel)
+    uint32_t base = beg;
+    asm volatile("" : "+v"(base));
+    const float4* P = T.pts + base;
+    const uint32_t m = end - beg;
+    switch (R) {
+        case 1: count7<kMode, 1>(F, P, m, t2, c); break;
+        case 2: count7<kMode, 2>(F, P, m, t2, c); break;
+        case 3: count7<kMode, 3>(F, P, m, t2, c); break;
+        case 4: count7<kMode, 4>(F, P, m, t2, c); break;
+        case 5: count7<kMode, 5>(F, P, m, t2, c); break;
+        case 6: count7<kMode, 6>(F, P, m, t2, c); break;
+        default: break;
+    }
+#pragma unroll
+    for (int r = 0; r < kS7MaxRounds; r++) {
+        const uint32_t k = (uint32_t)r * 64 + lane;
+        if (r < R && k < M) atomicAdd(&sCnt[k], c[r]);
     }
     __syncthreads();
-    const int nm = sNm[lane];
-    float Fs[kMaxF7][9];  // register copy for the scoring loop
-#pragma unroll
-    for (int m = 0; m < kMaxF7; m++)
-#pragma unroll
-        for (int q = 0; q < 9; q++) Fs[m][q] = sF[m][q][lane];
-    uint32_t c[kMaxF7] = {0u, 0u, 0u};
-    if (nm > 0) {
-        uint32_t beg, end;
-        wave_range(n, w, beg, end);
-        const float4* __restrict__ P = T.pts + beg;
-        const uint32_t m = end - beg;
-        auto test = [&](const float4& p) {
-#pragma unroll
-            for (int k = 0; k < kMaxF7; k++) c[k] += fgeom::inlier<kMode>(Fs[k], p, t2) ? 1u : 0u;
-        };
-        uint32_t i = 0;
-        for (; i + 4 <= m; i += 4) {
-            const float4 p0 = P[i], p1 = P[i + 1], p2 = P[i + 2], p3 = P[i + 3];
-            test(p0);
-            test(p1);
-            test(p2);
-            test(p3);
-        }
-        for (; i < m; i++) test(P[i]);
-    }
-#pragma unroll
-    for (int k = 0; k < kMaxF7; k++) sCnt[w][k][lane] = c[k];
-    __syncthreads();
-    if (w != 0 || h >= T.H) return;
-#pragma unroll
-    for (int k = 0; k < kMaxF7; k++) {
-        uint32_t t = 0;
-#pragma unroll
-        for (int j = 0; j < kScoreWaves; j++) t += sCnt[j][k][lane];
-        c[k] = t;
-    }
+    if (tid >= kS7Samples || h >= T.H) return;
     uint32_t best = 0;
     int bm = 0;
-    for (int m = 0; m < nm; m++)
-        if (m == 0 || c[m] > best) {
-            best = c[m];
-            bm = m;
+    for (int mi = 0; mi < nm; mi++) {
+        const uint32_t cm = sCnt[off + mi];
+        if (mi == 0 || cm > best) {
+            best = cm;
+            bm = mi;
         }
+    }
     T.counts[h] = nm ? best : 0u;
     if (T.models) {
 #pragma unroll
-        for (int q = 0; q < 9; q++) {
-            float v = Fs[0][q];
-            if (bm == 1) v = Fs[1][q];
-            if (bm == 2) v = Fs[2][q];
-            T.models[(size_t)h * 9 + q] = nm ? v : 0.f;
-        }
+        for (int q = 0; q < 9; q++) T.models[(size_t)h * 9 + q] = nm ? sF[q][off + bm] : 0.f;
     }
 }
 
@@ -662,7 +717,7 @@ int launch_score7_table(const DevTask* tasks, uint32_t nTasks, uint32_t maxH, fl
     const float t2 = threshold * threshold;
     const InTable<DevTask> src{tasks};
     if (maxH > 0) {
-        const dim3 grid(div_up(maxH, kScoreThreads), nTasks);
+        const dim3 grid(div_up(maxH, kS7Samples), nTasks);
         if (mode == 0) score7_kernel<0><<<grid, kScoreBlock, 0, stream>>>(src, t2, minSepSq0, minSepSq1);
         else score7_kernel<1><<<grid, kScoreBlock, 0, stream>>>(src, t2, minSepSq0, minSepSq1);
         RSFM_CHECK(check_launch("ransac7 score table"));
@@ -832,7 +887,7 @@ extern "C" int rsfm_ransac_f7(const rsfm_ransac_task* tasks, size_t nbTasks, flo
             maxH = std::max(maxH, t.nbHyp);
         }
         if (maxH > 0) {
-            const dim3 grid(div_up(maxH, kScoreThreads), (uint32_t)n);
+            const dim3 grid(div_up(maxH, kS7Samples), (uint32_t)n);
             if (mode == 0) score7_kernel<0><<<grid, kScoreBlock, 0, stream>>>(batch, t2, minSepSq0, minSepSq1);
             else score7_kernel<1><<<grid, kScoreBlock, 0, stream>>>(batch, t2, minSepSq0, minSepSq1);
             RSFM_CHECK(check_launch("ransac7 score"));
