@@ -191,7 +191,10 @@ int rsfm_cashash_find_best_match(const rsfm_cashash_image* queries, const uint32
  * the 128-bit remap hashes (ties: lower index), and the first `keep` (8, 16 or
  * 32) are re-ranked by exact L2^2 (ties: lower index). results.train, when
  * set, gets the same search with the roles of the images swapped. No
- * candidate -> {~0u, UINT32_MAX}. At most 2^24 descriptors per image. */
+ * candidate -> {~0u, UINT32_MAX}. At most 2^24 descriptors per image. Device
+ * memory: 20 bytes per member and table of every image (the members' indices
+ * and remap hashes in bucket order) from the stream-ordered allocator on
+ * `stream`, released on it before the call returns. */
 typedef struct rsfm_cashash_rerank_image {
     const void* desc;           /* u8 [nbDesc][128], 16-byte aligned */
     const uint8_t* codes;       /* [nbDesc][8] bucket codes, 8-byte aligned */
