@@ -145,13 +145,31 @@ __device__ inline uint32_t block_sum_u32(FinishSmem& s, uint32_t c)
     return total;
 }
 
+// fn(p) for each inlier p of model F among this thread's points t, t + 256,
+// ... in index order; the loads of four points are issued before any test
+constexpr int kFinUnroll = 4;
+template <int kMode, class Fn>
+__device__ inline void for_inliers(const float F[9], const float4* pts, uint32_t n, float t2, Fn fn)
+{
+    for (uint32_t i0 = threadIdx.x; i0 < n; i0 += kFinUnroll * kFinThreads) {
+        float4 p[kFinUnroll];
+#pragma unroll
+        for (int u = 0; u < kFinUnroll; u++) {
+            const uint32_t i = i0 + u * kFinThreads;
+            p[u] = i < n ? pts[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+#pragma unroll
+        for (int u = 0; u < kFinUnroll; u++)
+            if (i0 + u * kFinThreads < n && fgeom::inlier<kMode>(F, p[u], t2)) fn(p[u]);
+    }
+}
+
 // inliers of model F (ok) among the n points
 template <int kMode>
 __device__ uint32_t count_model(FinishSmem& s, const float F[9], bool ok, const float4* pts, uint32_t n, float t2)
 {
     uint32_t c = 0;
-    if (ok)
-        for (uint32_t i = threadIdx.x; i < n; i += kFinThreads) c += fgeom::inlier<kMode>(F, pts[i], t2) ? 1u : 0u;
+    if (ok) for_inliers<kMode>(F, pts, n, t2, [&](const float4&) { c++; });
     return block_sum_u32(s, c);
 }
 
@@ -188,14 +206,12 @@ __device__ bool refit_f(FinishSmem& s, const float4* pts, uint32_t n, float t2, 
     const uint32_t t = threadIdx.x;
     // centroids
     double a[4] = {0.0, 0.0, 0.0, 0.0};
-    for (uint32_t i = t; i < n; i += kFinThreads) {
-        const float4 p = pts[i];
-        if (!fgeom::inlier<kMode>(F, p, t2)) continue;
+    for_inliers<kMode>(F, pts, n, t2, [&](const float4& p) {
         a[0] = a[0] + (double)p.x;
         a[1] = a[1] + (double)p.y;
         a[2] = a[2] + (double)p.z;
         a[3] = a[3] + (double)p.w;
-    }
+    });
 #pragma unroll
     for (int k = 0; k < 4; k++) s.part[k][t] = a[k];
     sum_parts(s, 4);
@@ -205,14 +221,12 @@ __device__ bool refit_f(FinishSmem& s, const float4* pts, uint32_t n, float t2, 
     for (int k = 0; k < 4; k++) c[k] = s.tot[k] / dm;
     // mean distances to the centroids
     double d0 = 0.0, d1 = 0.0;
-    for (uint32_t i = t; i < n; i += kFinThreads) {
-        const float4 p = pts[i];
-        if (!fgeom::inlier<kMode>(F, p, t2)) continue;
+    for_inliers<kMode>(F, pts, n, t2, [&](const float4& p) {
         const double dx0 = (double)p.x - c[0], dy0 = (double)p.y - c[1];
         const double dx1 = (double)p.z - c[2], dy1 = (double)p.w - c[3];
         d0 = d0 + sqrt(dx0 * dx0 + dy0 * dy0);
         d1 = d1 + sqrt(dx1 * dx1 + dy1 * dy1);
-    }
+    });
     s.part[0][t] = d0;
     s.part[1][t] = d1;
     sum_parts(s, 2);
@@ -223,9 +237,7 @@ __device__ bool refit_f(FinishSmem& s, const float4* pts, uint32_t n, float t2, 
     double acc[kNormEntries];
 #pragma unroll
     for (int e = 0; e < kNormEntries; e++) acc[e] = 0.0;
-    for (uint32_t i = t; i < n; i += kFinThreads) {
-        const float4 p = pts[i];
-        if (!fgeom::inlier<kMode>(F, p, t2)) continue;
+    for_inliers<kMode>(F, pts, n, t2, [&](const float4& p) {
         const double x0 = ((double)p.x - c[0]) * s0, y0 = ((double)p.y - c[1]) * s0;
         const double x1 = ((double)p.z - c[2]) * s1, y1 = ((double)p.w - c[3]) * s1;
         const double r[9] = {x1 * x0, y1 * x0, x0, x1 * y0, y1 * y0, y0, x1, y1, 1.0};
@@ -237,7 +249,7 @@ __device__ bool refit_f(FinishSmem& s, const float4* pts, uint32_t n, float t2, 
                 acc[e] = acc[e] + r[u] * r[v];
                 e++;
             }
-    }
+    });
 #pragma unroll
     for (int e = 0; e < kNormEntries; e++) s.part[e][t] = acc[e];
     sum_parts(s, kNormEntries);
@@ -472,8 +484,19 @@ __global__ __launch_bounds__(kFinThreads) void finish_kernel(const Src batch, co
     for (int q = 0; q < 9; q++) F[q] = s.F[q];
     const bool ok = s.ok != 0;
     if (T.mask)
-        for (uint32_t i = t; i < T.nHost; i += kFinThreads)
-            T.mask[i] = (ok && i < n) ? (uint8_t)fgeom::inlier<kMode>(F, T.pts[i], t2) : (uint8_t)0;
+        for (uint32_t i0 = t; i0 < T.nHost; i0 += kFinUnroll * kFinThreads) {
+            float4 p[kFinUnroll];
+#pragma unroll
+            for (int u = 0; u < kFinUnroll; u++) {
+                const uint32_t i = i0 + u * kFinThreads;
+                p[u] = (ok && i < n) ? T.pts[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+            }
+#pragma unroll
+            for (int u = 0; u < kFinUnroll; u++) {
+                const uint32_t i = i0 + u * kFinThreads;
+                if (i < T.nHost) T.mask[i] = (ok && i < n) ? (uint8_t)fgeom::inlier<kMode>(F, p[u], t2) : (uint8_t)0;
+            }
+        }
     if (t < 9) T.F[t] = F[t];
     if (t == 0) {
         T.result[0] = bestH;
