@@ -689,23 +689,36 @@ __global__ __launch_bounds__(kRerankWaves * 64) void rerank_kernel(const RerankB
         const uint32_t beg = code == 0 ? 0u : T.t.ends[s * 256 + code - 1];
         const uint4* recR = T.t.recRemap + (size_t)s * T.t.n;
         const uint32_t* recI = T.t.recIdx + (size_t)s * T.t.n;
-        for (uint32_t p = beg + lane; p < end; p += 64) {
-            const uint32_t j = recI[p];
-            const uint4 r = recR[p];
-            const uint32_t h = (uint32_t)(__popc(r.x ^ qr.x) + __popc(r.y ^ qr.y) + __popc(r.z ^ qr.z) +
-                                          __popc(r.w ^ qr.w));
-            uint32_t key = (h << 24) | j;
-            if (key >= top[kKeep - 1]) continue;
-            // an item met in an earlier table (same key) is listed once
-            bool dup = false;
+        // four records per lane in flight: their loads are issued before the
+        // first is tested
+        for (uint32_t p0 = beg + lane; p0 < end; p0 += 4 * 64) {
+            uint32_t jj[4];
+            uint4 rr[4];
 #pragma unroll
-            for (int k = 0; k < kKeep; k++) dup = dup || top[k] == key;
-            if (dup) continue;
+            for (int u = 0; u < 4; u++) {
+                const uint32_t p = p0 + u * 64;
+                jj[u] = p < end ? recI[p] : ~0u;
+                rr[u] = p < end ? recR[p] : make_uint4(0u, 0u, 0u, 0u);
+            }
 #pragma unroll
-            for (int k = 0; k < kKeep; k++) {
-                const uint32_t lo = min(top[k], key);
-                key = max(top[k], key);
-                top[k] = lo;
+            for (int u = 0; u < 4; u++) {
+                if (p0 + u * 64 >= end) continue;
+                const uint4 r = rr[u];
+                const uint32_t h = (uint32_t)(__popc(r.x ^ qr.x) + __popc(r.y ^ qr.y) + __popc(r.z ^ qr.z) +
+                                              __popc(r.w ^ qr.w));
+                uint32_t key = (h << 24) | jj[u];
+                if (key >= top[kKeep - 1]) continue;
+                // an item met in an earlier table (same key) is listed once
+                bool dup = false;
+#pragma unroll
+                for (int k = 0; k < kKeep; k++) dup = dup || top[k] == key;
+                if (dup) continue;
+#pragma unroll
+                for (int k = 0; k < kKeep; k++) {
+                    const uint32_t lo = min(top[k], key);
+                    key = max(top[k], key);
+                    top[k] = lo;
+                }
             }
         }
     }
