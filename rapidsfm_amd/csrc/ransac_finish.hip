@@ -39,7 +39,7 @@ constexpr int kFinPer = 4;
 constexpr uint32_t kFinChunk = kFinThreads * kFinPer;
 // pairs per launch of the C-ABI entry point: one workgroup each, so a launch
 // lasts about as long as its slowest pair's refit (a chain of FP64 steps);
-// 32 tasks of 88 bytes stay well inside the kernel-argument limit
+// 32 tasks of 88 bytes are 2.8 KiB of kernel arguments
 constexpr int kFinMaxTasks = 32;
 constexpr int kNormEntries = 45;  // upper triangle of the 9x9 normal matrix
 
@@ -98,7 +98,7 @@ __host__ __device__ inline uint32_t required_iters(uint32_t best, uint32_t n, do
 }
 
 typedef ByValue<FinishTask, kFinMaxTasks> FinishBatch;
-static_assert(sizeof(FinishBatch) + sizeof(FinishParams) <= 4096, "kernel arguments are limited to 4 KiB");
+static_assert(sizeof(FinishBatch) + sizeof(FinishParams) <= 4096, "the batch is kept within 4 KiB of kernel arguments");
 
 constexpr int kSumGroups = 16;  // kFinThreads = kSumGroups * kSumGroups
 static_assert(kSumGroups * kSumGroups == kFinThreads, "two-level sums of the thread partials");
