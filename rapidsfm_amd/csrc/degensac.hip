@@ -145,15 +145,28 @@ __device__ uint32_t count_pass(Smem& s, const float4* pts, uint32_t n, const flo
     if (threadIdx.x == 0) s.acc = 0;
     __syncthreads();
     uint32_t c = 0;
-    for (uint32_t base = 0; base < n; base += kThreads) {
-        const uint32_t i = base + threadIdx.x;
-        const bool in = i < n && is_inlier<kKind>(m, pts[i], t2);
-        const uint64_t b = __ballot(in);
-        if (lane == 0) {
-            const uint32_t w = (base >> 5) + 2 * wave;
-            dst[w] = (uint32_t)b;
-            dst[w + 1] = (uint32_t)(b >> 32);
-            c += (uint32_t)__popcll(b);
+    // four rows of 256 points per trip, their loads issued before the tests
+    constexpr uint32_t kU = 4;
+    for (uint32_t base0 = 0; base0 < n; base0 += kU * kThreads) {
+        float4 q[kU];
+#pragma unroll
+        for (uint32_t u = 0; u < kU; u++) {
+            const uint32_t i = base0 + u * kThreads + threadIdx.x;
+            q[u] = i < n ? pts[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kU; u++) {
+            const uint32_t base = base0 + u * kThreads;
+            if (base >= n) break;
+            const uint32_t i = base + threadIdx.x;
+            const bool in = i < n && is_inlier<kKind>(m, q[u], t2);
+            const uint64_t b = __ballot(in);
+            if (lane == 0) {
+                const uint32_t w = (base >> 5) + 2 * wave;
+                dst[w] = (uint32_t)b;
+                dst[w + 1] = (uint32_t)(b >> 32);
+                c += (uint32_t)__popcll(b);
+            }
         }
     }
     if (lane == 0) atomicAdd(&s.acc, c);
