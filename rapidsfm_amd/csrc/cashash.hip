@@ -634,10 +634,11 @@ struct RerankTask {
     RerankImage q, t;  // queries matched against the train image t
     uint64_t* res;     // [q.n] (distance << 32 | train index)
 };
-constexpr int kRerankMaxTasks = 32;
+constexpr int kRerankMaxTasks = 30;
 struct RerankBatch {
     RerankTask t[kRerankMaxTasks];
 };
+static_assert(sizeof(RerankBatch) + sizeof(uint32_t) <= 4096, "the batch is kept within 4 KiB of kernel arguments");
 constexpr int kRerankWaves = 4;
 
 __device__ inline uint32_t wave_min_u32(uint32_t x)

@@ -184,3 +184,25 @@ def test_rerank_many_tasks_and_edges(ch, orc, dev):
     for (gq, gt_), (rq, rt) in zip(outs, refs):
         assert np.array_equal(_h(gq, np.uint32), rq)
         assert np.array_equal(_h(gt_, np.uint32), rt)
+
+
+def test_rerank_batches_of_directions(ch, orc, dev):
+    """20 query images, both directions: 40 searches in one call, more than one
+    launch batch holds."""
+    _say("re-rank: 20 queries vs one train, both directions, keep 8")
+    w, base = orc.cashash_gaussian_weights(5, 192)
+    train, _, _ = orc.gen_sift_pair(6, 1500, 10, 0, 0.05)
+    acc = orc.cashash_accumulate(train)
+    bias = orc.cashash_compute_bias(w, base, acc, 1500)
+    ti, tc, tr = _register(ch, orc, train, dev, w, bias)
+    qimgs, refs = [], []
+    for k in range(20):
+        _, q, _ = orc.gen_sift_pair(6, 1500, 200 + 37 * k, 60 + k, 0.05)
+        qi, qc, qr = _register(ch, orc, q, dev, w, bias)
+        qimgs.append(qi)
+        refs.append((orc.cashash_find_best_rerank(q, qc, qr, train, tc, tr, 6, 8),
+                     orc.cashash_find_best_rerank(train, tc, tr, q, qc, qr, 6, 8)))
+    outs = ch.find_best_match_rerank(qimgs, ti, 6, 8)
+    for (gq, gt_), (rq, rt) in zip(outs, refs):
+        assert np.array_equal(_h(gq, np.uint32), rq)
+        assert np.array_equal(_h(gt_, np.uint32), rt)
