@@ -157,7 +157,7 @@ __device__ __forceinline__ void match_tile_top2(const DevTask& T, uint32_t rowBa
     constexpr int kChunkBytes = kBN * 128;
     constexpr int kRing = 4;  // ring slot of chunk c: c % 4
     constexpr int kPSlots = 8;
-    constexpr int kColSlots = 4;
+    constexpr int kColSlots = 8;
     constexpr int kCS = 72;
     constexpr int kTailStride = 17;
     constexpr int kUnionBytes = kRing * kChunkBytes;
@@ -247,22 +247,9 @@ __device__ __forceinline__ void match_tile_top2(const DevTask& T, uint32_t rowBa
     };
     const bool doCols = T.bestB != nullptr;
 
-    // chunk c: the trip's first chunk copies chunks c+2 and c+3 (their slots
-    // were last read before the previous trip's barrier) and flushes the
-    // previous trip's column partials; the chunk's 16 MFMAs (operand reads of
-    // one K-step at a time); keys of chunk c; the trip's barrier waits for
-    // the copies
-    auto body = [&](uint32_t c, bool sync, bool flushPrev, bool issue) {
-        if (issue) {
-            if (c + 2 < nChunks) stage_dma(c + 2);
-            if (c + 3 < nChunks) stage_dma(c + 3);
-        }
-        if (flushPrev) {
-            flush(c - 2);
-            flush(c - 1);
-        }
+    // chunk c's 16 MFMAs (operand reads of one K-step at a time)
+    auto mfma_chunk = [&](uint32_t c, v4i (&acc)[2][4]) {
         const uint8_t* sb = sU + (c % kRing) * kChunkBytes;
-        v4i acc[2][4];
 #pragma unroll
         for (int ks = 0; ks < 2; ks++) {
             v4i b[4];
@@ -278,6 +265,9 @@ __device__ __forceinline__ void match_tile_top2(const DevTask& T, uint32_t rowBa
                     acc[rb][cb] = __builtin_amdgcn_mfma_i32_16x16x64_i8(aop[rb][ks], b[cb], ks == 0 ? Hv[rb] : acc[rb][cb], 0, 0, 0);
             __builtin_amdgcn_sched_barrier(0);
         }
+    };
+    // chunk c's keys: row top-2 folds, column minima to sCol
+    auto keys_chunk = [&](uint32_t c, const v4i (&acc)[2][4]) {
         int32_t P[4];
 #pragma unroll
         for (int cb = 0; cb < 4; cb++) P[cb] = sP[c % kPSlots][cb * 16 + c16];
@@ -293,8 +283,6 @@ __device__ __forceinline__ void match_tile_top2(const DevTask& T, uint32_t rowBa
 #pragma unroll
                     for (int cb = 0; cb < 4; cb++) r[h][cb] = shl9_add(acc[rb][cb][j], P[cb]);
                     if constexpr (kTop2) {
-                        // two pairs into the running top-2: m2 takes the
-                        // second of each pair's fold in one v_min3
                         const int32_t t1 = med3(m1[rb][j], r[h][0], r[h][1]);
                         asm("v_min3_i32 %0, %0, %1, %2" : "+v"(m1[rb][j]) : "v"(r[h][0]), "v"(r[h][1]));
                         const int32_t t2 = med3(m1[rb][j], r[h][2], r[h][3]);
@@ -305,7 +293,6 @@ __device__ __forceinline__ void match_tile_top2(const DevTask& T, uint32_t rowBa
                         asm("v_min3_i32 %0, %0, %1, %2" : "+v"(m1[rb][j]) : "v"(r[h][2]), "v"(r[h][3]));
                     }
                 }
-                // column minima over the lane's rows, two rows per v_min3
 #pragma unroll
                 for (int cb = 0; cb < 4; cb++) {
                     if (rb == 0 && jp == 0)
@@ -315,10 +302,6 @@ __device__ __forceinline__ void match_tile_top2(const DevTask& T, uint32_t rowBa
                 }
             }
         sCol[c % kColSlots][wave][lane] = col_reduce(cm);
-        if (sync) {
-            wait_vm0();
-            __syncthreads();
-        }
     };
 
     stage_dma(0);
@@ -332,17 +315,34 @@ __device__ __forceinline__ void match_tile_top2(const DevTask& T, uint32_t rowBa
         for (int j = 0; j < 4; j++) Hv[rb][j] = q4[j] >> 1;
     }
 
-    // two chunks per trip, barrier after the odd one; the column partials of a
-    // trip's chunks are complete after its barrier and flushed at the next trip
+    // Two chunks per trip, one barrier per trip. The waves of the upper half
+    // (the second wave on each SIMD) run the keys of a trip's second chunk at
+    // the start of the next trip, so on every SIMD one wave issues MFMAs while
+    // the other runs key epilogues. A trip's first chunk copies chunks c+2 and
+    // c+3 (their slots were last read before the previous barrier); a chunk's
+    // column partials are complete one trip after its own and flushed then.
+    const bool lagging = wave >= kWaves / 2;
+    v4i accA[2][4], accB[2][4];
     uint32_t c = 0;
     for (; c + 2 <= nChunks; c += 2) {
-        body(c, false, doCols && c >= 2, true);
-        body(c + 1, true, false, false);
+        if (lagging && c >= 2) keys_chunk(c - 1, accB);
+        if (c + 2 < nChunks) stage_dma(c + 2);
+        if (c + 3 < nChunks) stage_dma(c + 3);
+        if (doCols && c >= 2) flush(c - 2);
+        if (doCols && c >= 4) flush(c - 3);
+        mfma_chunk(c, accA);
+        keys_chunk(c, accA);
+        mfma_chunk(c + 1, accB);
+        if (!lagging) keys_chunk(c + 1, accB);
+        wait_vm0();
+        __syncthreads();
     }
-    uint32_t firstUnflushed = c >= 2 ? c - 2 : 0;
+    const uint32_t firstUnflushed = c >= 3 ? c - 3 : 0;
+    if (lagging && c >= 2) keys_chunk(c - 1, accB);
     if (c < nChunks) {
-        body(c, false, doCols && c >= 2, true);
-        firstUnflushed = c;
+        // the odd last chunk: its copy was issued by the last trip (or before the loop)
+        mfma_chunk(c, accA);
+        keys_chunk(c, accA);
     }
     __syncthreads();
     if (doCols)
