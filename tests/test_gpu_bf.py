@@ -67,6 +67,20 @@ def test_config0_top2_ratio_crosscheck(rs, orc, dev):
     assert np.array_equal(out2[:n2].cpu().numpy().view(np.uint32), ref_cc)
 
 
+@pytest.mark.parametrize("nB", [50, 100, 150, 300, 4160, 8257])
+def test_top2_chunk_counts(rs, orc, dev, nB):
+    """Top-2 over train ranges of 1, 2, 3 and 5 chunks of 64 columns (odd
+    chunk counts end with a lone chunk after the last trip) and over several
+    4096-column segments: 4160 = 4096 + 64 and 8257 = 2 x 4096 + 65 columns."""
+    _say("top2 chunk counts, nB", nB)
+    train, query, _ = orc.gen_sift_pair(12, nB, 700, min(nB, 700) // 3, 0.05)
+    ra, sa, rb = orc.bf_match_top2_u8(query, train)
+    (ga, gs, gb), = rs.bf_match_top2([(_to_dev(query, dev), _to_dev(train, dev))])
+    assert np.array_equal(_np(ga), ra)
+    assert np.array_equal(_np(gs), sa)
+    assert np.array_equal(_np(gb), rb)
+
+
 @pytest.mark.parametrize("nA,nB", [(1, 1), (31, 65), (897, 1000), (257, 63), (300, 2049), (5, 9000)])
 @pytest.mark.parametrize("impl", ["mfma", "idp"])
 def test_ragged_full_range(rs, orc, dev, nA, nB, impl):
