@@ -416,9 +416,20 @@ __global__ __launch_bounds__(kFinThreads) void finish_kernel(const Src batch, co
     if (prm.conf > 0.f) {
         adaptive_cut(s, T.counts, T.H, n, prm.conf, prm.maxIter);
     } else {
+        // keys are distinct, so their maximum does not depend on the order;
+        // four counts are loaded per trip before they are compared
         uint64_t key = 0;
-        for (uint32_t h = t; h < T.H; h += kFinThreads)
-            if (T.counts[h] > 0) key = max(key, ((uint64_t)T.counts[h] << 32) | (uint32_t)~h);
+        for (uint32_t h0 = t; h0 < T.H; h0 += kFinUnroll * kFinThreads) {
+            uint32_t cnt[kFinUnroll];
+#pragma unroll
+            for (int u = 0; u < kFinUnroll; u++) {
+                const uint32_t h = h0 + u * kFinThreads;
+                cnt[u] = h < T.H ? T.counts[h] : 0u;
+            }
+#pragma unroll
+            for (int u = 0; u < kFinUnroll; u++)
+                if (cnt[u] > 0) key = max(key, ((uint64_t)cnt[u] << 32) | (uint32_t)~(h0 + u * kFinThreads));
+        }
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) {
             const uint64_t k = __shfl_xor(key, o);
