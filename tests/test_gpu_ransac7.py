@@ -36,6 +36,10 @@ def test_f7_bit_exact(orc, dev, mode, t, seed, n, n_out, H):
         mask = orc.f_inlier_f32(F, pts, t, mode).astype(np.uint8)
         assert np.array_equal(o["mask"].cpu().numpy(), mask)
         assert int(mask.sum()) == ref_c[hb]
+    # without stored models the best sample is refitted: same best and mask
+    (o2,) = ops.ransac_f7([(pt, it)], t, mode=mode, want_models=False)
+    assert np.array_equal(o2["best"].cpu().numpy(), o["best"].cpu().numpy())
+    assert np.array_equal(o2["mask"].cpu().numpy(), o["mask"].cpu().numpy())
 
 
 def test_f7_gate_and_edge_cases(orc, dev):
