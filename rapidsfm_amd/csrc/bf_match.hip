@@ -137,7 +137,10 @@ __device__ inline void wait_vm0() { __builtin_amdgcn_s_waitcnt(0x0F70); }
 //    NB' << 8 | tag per column, written once per call by prep_train_kernel);
 //    a chunk is 4 column blocks of 16, so a chunk's 16 MFMAs (2 row blocks x
 //    4 column blocks x 2 K-steps) give the lane 32 products: rows
-//    rb * 16 + 4 q + j (j < 4) of columns cb * 16 + l % 16;
+//    rb * 16 + 4 q + j (j < 4) of columns cb * 16 + l % 16; two chunks per
+//    trip between barriers, the upper four waves running the keys of a
+//    trip's second chunk half a trip late (on each SIMD one wave issues
+//    MFMAs while the other runs keys);
 //  * the MFMA chains start at H_i = NA'_i >> 1, so the single key
 //    R = (acc << 9) + (NB'_j << 8 | tag) = 256 (d - p_i) + tag
 //    (p_i = NA'_i & 1, tag = 4 chunk + cb within the segment, hence at most
