@@ -120,6 +120,16 @@ __device__ inline void dma4(const void* src, void* lds)
         reinterpret_cast<const __attribute__((address_space(1))) void*>(reinterpret_cast<uintptr_t>(src)),
         reinterpret_cast<__attribute__((address_space(3))) void*>(reinterpret_cast<uintptr_t>(lds)), 4, 0, 0);
 }
+// a wave-uniform address as an opaque scalar value: a per-lane offset added to
+// it is not reassociated into a per-lane pointer advanced every trip, so a
+// copy's address costs one 64-bit vector add (the chunk step stays scalar)
+__device__ inline const uint8_t* uniform_ptr(const void* p)
+{
+    const uint64_t v = reinterpret_cast<uint64_t>(p);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return reinterpret_cast<const uint8_t*>(((uint64_t)hi << 32) | lo);
+}
 // s_waitcnt vmcnt(0) (gfx9 encoding: expcnt and lgkmcnt left at their maxima)
 __device__ inline void wait_vm0() { __builtin_amdgcn_s_waitcnt(0x0F70); }
 
@@ -214,11 +224,13 @@ __device__ __forceinline__ void match_tile_top2(const DevTask& T, uint32_t rowBa
     // lane L of wave w fills column w*8 + L/8, slot L%8 of the chunk's ring
     // slot, i.e. it reads source slot (L%8) ^ swz(column); wave 0 also copies
     // the chunk's 64 column offsets
+    const uint8_t* const segW = T.W + (size_t)colBeg * 128;
+    const int32_t* const segNB = T.NB + colBeg;
+    const uint32_t dmaOff = (uint32_t)((wave * 8 + (lane >> 3)) * 128 + (((lane & 7) ^ swz(wave * 8 + (lane >> 3))) << 4));
     auto stage_dma = [&](uint32_t c) {
-        const int col = wave * 8 + (lane >> 3);
-        const uint8_t* src = T.W + (size_t)(colBeg + c * kBN + col) * 128 + (((lane & 7) ^ swz(col)) << 4);
-        dma16(src, sU + (c % kRing) * kChunkBytes + wave * 1024);
-        if (wave == 0) dma4(T.NB + colBeg + c * kBN + lane, &sP[c % kPSlots][0]);
+        const uint8_t* chunkW = uniform_ptr(segW + (size_t)c * (kBN * 128));
+        dma16(chunkW + dmaOff, sU + (c % kRing) * kChunkBytes + wave * 1024);
+        if (wave == 0) dma4(uniform_ptr(segNB + (size_t)c * kBN) + lane * 4, &sP[c % kPSlots][0]);
     };
     // wave k % 8 reduces chunk k's 8 wave minima per column (one lane per
     // column): E = Rmin >> 8 to T.colMin by atomicMin, and to the block's
